@@ -1,0 +1,146 @@
+"""ctypes binding of the FP64 CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, as the checker.  The product never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError(f"oracle not built: {path} (run make -C oracle)")
+        L = C.CDLL(path)
+        L.ho_oracle_create.restype = C.c_void_p
+        L.ho_oracle_create.argtypes = [C.c_int, C.c_int, C.c_int, _dp, C.c_int, _ip, C.c_int, _ip, _ip, C.c_int,
+                                       C.c_char_p, C.c_int, C.c_double, C.c_double, C.c_int, C.c_int,
+                                       C.c_double, C.c_int]
+        L.ho_oracle_destroy.argtypes = [C.c_void_p]
+        L.ho_oracle_last_error.restype = C.c_char_p
+        L.ho_oracle_last_error.argtypes = [C.c_void_p]
+        for nm in ("ho_oracle_dofs", "ho_oracle_n_bedges"):
+            getattr(L, nm).argtypes = [C.c_void_p]
+            getattr(L, nm).restype = C.c_int
+        for nm in ("ho_oracle_set_vorticity", "ho_oracle_get_vorticity", "ho_oracle_get_streamfunction",
+                   "ho_oracle_set_streamfunction", "ho_oracle_get_velocity", "ho_oracle_get_boundary_source"):
+            getattr(L, nm).argtypes = [C.c_void_p, _dp]
+        L.ho_oracle_time.argtypes = [C.c_void_p]
+        L.ho_oracle_time.restype = C.c_double
+        L.ho_oracle_step.argtypes = [C.c_void_p, C.c_int]
+        L.ho_oracle_stage.argtypes = [C.c_void_p, _dp, C.c_double, _dp]
+        L.ho_oracle_poisson.argtypes = [C.c_void_p, _dp, C.c_double, _dp]
+        L.ho_oracle_last_solve.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_double)]
+        L.ho_oracle_nnz.argtypes = [C.c_void_p]
+        L.ho_oracle_nnz.restype = C.c_int64
+        L.ho_oracle_get_csr.argtypes = [C.c_void_p, _ip, _ip, _dp]
+        L.ho_oracle_table.argtypes = [C.c_void_p, C.c_char_p, C.c_void_p, C.c_int64]
+        L.ho_oracle_table.restype = C.c_int64
+        L.ho_oracle_apply_laplacian_mf.argtypes = [C.c_void_p, _dp, _dp]
+        _LIB = L
+    return _LIB
+
+
+class Oracle:
+    """FP64 CPU restatement of the HO_2D hot path for one mesh/problem."""
+
+    def __init__(self, mesh, K: int, problem_type: int = 1, Re: float = 100.0, dt: float = 1e-3,
+                 time_integration_type: int = 4, advection_type: int = 1, tol: float = 1e-15,
+                 maxiter: int = 150):
+        L = lib()
+        self.mesh, self.K = mesh, K
+        names = "\n".join(mesh.bnames).encode()
+        self.h = L.ho_oracle_create(K, mesh.lnod, mesh.n_nodes, np.ascontiguousarray(mesh.xy, dtype=np.float64).ravel(),
+                                    mesh.n_elem, np.ascontiguousarray(mesh.elem, dtype=np.int32).ravel(),
+                                    mesh.n_bedges, np.ascontiguousarray(mesh.bedges, dtype=np.int32).ravel(),
+                                    np.ascontiguousarray(mesh.bedge_bdry, dtype=np.int32),
+                                    len(mesh.bnames), names, problem_type, Re, dt, time_integration_type,
+                                    advection_type, tol, maxiter)
+        if not self.h:
+            raise RuntimeError("oracle create failed: " + L.ho_oracle_last_error(None).decode())
+        self.D = L.ho_oracle_dofs(self.h)
+        self.n_bedges = L.ho_oracle_n_bedges(self.h)
+
+    def __del__(self):
+        if getattr(self, "h", None) and _LIB is not None:
+            _LIB.ho_oracle_destroy(self.h)
+            self.h = None
+
+    def _chk(self, rc):
+        if rc != 0:
+            raise RuntimeError(lib().ho_oracle_last_error(self.h).decode())
+
+    def vorticity(self):
+        out = np.zeros(self.D); lib().ho_oracle_get_vorticity(self.h, out); return out
+
+    def set_vorticity(self, w):
+        lib().ho_oracle_set_vorticity(self.h, np.ascontiguousarray(w, dtype=np.float64))
+
+    def streamfunction(self):
+        out = np.zeros(self.D); lib().ho_oracle_get_streamfunction(self.h, out); return out
+
+    def set_streamfunction(self, p):
+        lib().ho_oracle_set_streamfunction(self.h, np.ascontiguousarray(p, dtype=np.float64))
+
+    def velocity(self):
+        out = np.zeros(2 * self.D); lib().ho_oracle_get_velocity(self.h, out); return out.reshape(-1, 2)
+
+    def time(self):
+        return lib().ho_oracle_time(self.h)
+
+    def step(self, n=1):
+        self._chk(lib().ho_oracle_step(self.h, n))
+
+    def stage(self, w, t=0.0):
+        out = np.zeros(self.D)
+        self._chk(lib().ho_oracle_stage(self.h, np.ascontiguousarray(w, dtype=np.float64), t, out))
+        return out
+
+    def poisson(self, w, t=0.0):
+        out = np.zeros(self.D)
+        self._chk(lib().ho_oracle_poisson(self.h, np.ascontiguousarray(w, dtype=np.float64), t, out))
+        return out
+
+    def last_solve(self):
+        it, rel = C.c_int(), C.c_double()
+        lib().ho_oracle_last_solve(self.h, C.byref(it), C.byref(rel))
+        return it.value, rel.value
+
+    def csr(self):
+        L = lib()
+        nnz = L.ho_oracle_nnz(self.h)
+        ptr = np.zeros(self.D + 1, dtype=np.int32)
+        col = np.zeros(nnz, dtype=np.int32)
+        val = np.zeros(nnz)
+        L.ho_oracle_get_csr(self.h, ptr, col, val)
+        return ptr, col, val
+
+    def boundary_source(self):
+        out = np.zeros(self.n_bedges * self.K ** 3)
+        lib().ho_oracle_get_boundary_source(self.h, out)
+        return out.reshape(self.n_bedges, self.K * self.K, self.K)
+
+    def table(self, name):
+        L = lib()
+        n = L.ho_oracle_table(self.h, name.encode(), None, 0)
+        if n < 0:
+            raise KeyError(name)
+        out = np.zeros(max(n, 1))
+        L.ho_oracle_table(self.h, name.encode(), out.ctypes.data, n)
+        return out[:n]
+
+    def laplacian_mf(self, x):
+        y = np.zeros(self.D)
+        self._chk(lib().ho_oracle_apply_laplacian_mf(self.h, np.ascontiguousarray(x, dtype=np.float64), y))
+        return y
