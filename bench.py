@@ -1,0 +1,147 @@
+"""Throughput of the HO-CXX time-step hot path on one MI355X.
+
+Metric: Mdof-steps/s per RK4 step including the four Poisson solves, where
+Mdof = N_el * K^2 / 1e6.  Workload (the north-star configuration): lid-driven
+cavity on the unit square, 512 x 512 = 2^18 bilinear quads stored as
+unstructured element/edge lists, K = 5 (P = 4), Re = 100, zero initial
+vorticity, RK4 with dt = half the measured explicit-RK4 diffusion limit, Krylov
+relative tolerance 1e-10.  Inputs live in HBM for the whole timed region.
+
+Prints ONE JSON line.  The cpu_baseline leg times the FP64 CPU oracle (the
+restatement of the reference algorithm, single thread) on a bounded sample of
+the same workload (a 64 x 64 cavity at the same K, Re, dt rule and tolerance).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for sub in ("ho-cxx_amd", "oracle"):
+    p = os.path.join(ROOT, sub)
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+HBM_PEAK_GBS = 8000.0
+RK4_REAL_LIMIT = 2.785          # |z| on the negative real axis inside the RK4 stability region
+LAP_RHO_PER_N2 = {4: 373.9, 5: 799.5}   # rho(J^-1 A) / n^2 on an n x n unit cavity (measured)
+
+
+def cavity_dt(n, K, Re, frac=0.5):
+    rho = LAP_RHO_PER_N2[K] * n * n
+    return frac * RK4_REAL_LIMIT / (rho / Re)
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(K, Re, tol, n=64, warm=2, steps=6):
+    from hoc import meshes
+    from pyoracle import Oracle
+    m = meshes.cavity(n, 2)
+    dt = cavity_dt(n, K, Re)
+    t0 = time.perf_counter()
+    o = Oracle(m, K, problem_type=1, Re=Re, dt=dt, time_integration_type=4, advection_type=1, tol=tol, maxiter=150)
+    setup = time.perf_counter() - t0
+    o.step(warm)
+    t0 = time.perf_counter()
+    o.step(steps)
+    el = time.perf_counter() - t0
+    D = n * n * K * K
+    return {"value": D * steps / el / 1e6, "unit": "Mdof-steps/s", "cores": 1, "kind": "port",
+            "sample": f"oracle (FP64 CPU restatement, 1 thread) on a {n}x{n} cavity, K={K}, Re={Re}, "
+                      f"dt={dt:.3g}, tol={tol:g}: {warm} warm-up + {steps} timed RK4 steps "
+                      f"({el:.1f} s timed, setup {setup:.1f} s)"}
+
+
+def roofline(prof):
+    """Dominant kernel (largest total time among the fine-operator variants): algorithmic
+    bytes per launch over its average launch duration, from HIP events on the library's
+    stream during one profiled RK4 step."""
+    name, v = max(prof.items(), key=lambda kv: kv[1]["total_ms"])
+    avg_s = v["total_ms"] / max(1, v["launches"]) / 1e3
+    gbs = v["bytes_per_launch"] / avg_s / 1e9
+    return {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
+            "traffic": None, "kernel": name, "avg_launch_us": avg_s * 1e6,
+            "bytes_per_launch": v["bytes_per_launch"], "launches": v["launches"],
+            "all_variants": {k: {"avg_us": 1e3 * x["total_ms"] / max(1, x["launches"]),
+                                 "GBs": x["bytes_per_launch"] / max(1e-12, x["total_ms"] / max(1, x["launches"]) / 1e3) / 1e9,
+                                 "launches": x["launches"]} for k, x in prof.items()}}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=512, help="cells per side of the cavity")
+    ap.add_argument("--K", type=int, default=5)
+    ap.add_argument("--Re", type=float, default=100.0)
+    ap.add_argument("--tol", type=float, default=1e-10)
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is design-only (DESIGN.md)",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+    import hoc
+    from hoc import meshes
+
+    n, K, Re = args.n, args.K, args.Re
+    dt = cavity_dt(n, K, Re)
+    t0 = time.perf_counter()
+    m = meshes.cavity(n, 2)
+    log(f"mesh {n}x{n}: {time.perf_counter() - t0:.1f} s")
+    prec = hoc.HOC_MIXED if args.precision == "mixed" else hoc.HOC_FP64
+    g = hoc.HO2D(K=K, problem_type=1, Re=Re, dt=dt, time_integration_type=4, advection_type=1,
+                 tol=args.tol, maxiter=500, precision=prec)
+    g.load_mesh(m)
+    t0 = time.perf_counter()
+    g.setup()
+    setup_s = time.perf_counter() - t0
+    st = g.stats()
+    log(f"setup {setup_s:.1f} s, dofs {st['n_dofs']}, levels {st['n_levels']}, krylov {st['krylov']}")
+    for i in range(args.warmup):
+        g.step(1)
+        log(f"warmup step {i}: iters/solve {g.stats()['last_iters']}")
+    it0 = g.stats()["total_iters"]
+    s0 = g.stats()["n_solves"]
+    # untimed pass with per-launch events on the fine-operator kernel (live roofline)
+    g.profile(True)
+    g.step(1)
+    prof = g.profile_read()
+    g.profile(False)
+    it0 = g.stats()["total_iters"]
+    s0 = g.stats()["n_solves"]
+    t0 = time.perf_counter()
+    g.step(args.steps)   # synchronizes the device at the end
+    el = time.perf_counter() - t0
+    st = g.stats()
+    iters = (st["total_iters"] - it0) / max(1, st["n_solves"] - s0)
+    D = st["n_dofs"]
+    value = D * args.steps / el / 1e6
+    log(f"timed {args.steps} steps in {el:.3f} s: {value:.2f} Mdof-steps/s, {iters:.1f} Krylov its/solve")
+    out = {
+        "metric": "Mdof-steps/s per RK4 step incl. Poisson",
+        "value": value, "unit": "Mdof-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * el / args.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64" if prec == hoc.HOC_FP64 else "f64+f32",
+        "data": "synthetic (lid-driven cavity, zero initial vorticity)",
+        "config": {"workload": f"lid-driven cavity {n}x{n} quads (unstructured storage), K={K} (P={K - 1}), "
+                               f"Re={Re:g}, RK4 dt={dt:.3g}, Krylov tol={args.tol:g}",
+                   "n_elem": n * n, "dofs": D, "krylov_iters_per_solve": iters, "setup_s": setup_s},
+        "roofline": roofline(prof),
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(K, Re, args.tol)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,936 @@
+// hoc_fr.hip - flux-reconstruction stage kernels for gfx950.
+//
+// Second-order FR operator (the Poisson matrix of src/HO_2D.cpp:1533-1804 applied
+// matrix-free, and the diffusion term of src/HO_2D.cpp:2666-2831), velocity from the
+// stream function (src/HO_2D.cpp:3174-3366), advection (src/HO_2D.cpp:2245-2645) and
+// the RK stage update (src/HO_2D.cpp:1465-1531, 1833-1836).
+#include "hoc_fr.hpp"
+
+namespace hoc {
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+__device__ __forceinline__ int rev_flag(int ijp, int ijpm) { return (ijp + ijpm - ijp / 2 - ijpm / 2 + 1) & 1; }
+__device__ __forceinline__ double face_sign(int ijp, int ijpm) { return ((ijp + ijpm) & 1) ? 1.0 : -1.0; }
+// element-local flux point k of face ijp -> boundary-edge flux point (src/HO_2D.cpp:2229)
+template <int K>
+__device__ __forceinline__ int edge_fp(int ijp, int k) { return (ijp == 0 || ijp == 3) ? K - 1 - k : k; }
+
+// Value and normal-derivative traces on the four faces (src/HO_2D.cpp:2695-2713).
+template <int K>
+__device__ __forceinline__ void traces(const Tab& T, const double (&u)[K * K], double (&bv)[4][K], double (&bg)[4][K]) {
+#pragma unroll
+  for (int rc = 0; rc < K; ++rc)
+#pragma unroll
+    for (int ib = 0; ib < 2; ++ib) {
+      double s = 0., g = 0., s2 = 0., g2 = 0.;
+#pragma unroll
+      for (int m = 0; m < K; ++m) {
+        s += u[rc * K + m] * T.bb[m][ib];
+        g += u[rc * K + m] * T.bgb[m][ib];
+        s2 += u[m * K + rc] * T.bb[m][ib];
+        g2 += u[m * K + rc] * T.bgb[m][ib];
+      }
+      bv[ib][rc] = s; bg[ib][rc] = g;
+      bv[2 + ib][rc] = s2; bg[2 + ib][rc] = g2;
+    }
+}
+
+// Common value and common normal flux on an interior face (src/HO_2D.cpp:2833-2870 applied
+// to both sides, then the average of :2742-2768).
+template <int K>
+__device__ __forceinline__ void interior_face(const Tab& T, int ib, int ibn, int rev, double sgn,
+                                              const double (&bv)[K], const double (&bg)[K],
+                                              const double (&vn)[K], const double (&gn)[K],
+                                              const double (&fa)[K], const double (&fb)[K],
+                                              const double (&fan)[K], const double (&fbn)[K],
+                                              double (&cv)[K], double (&cg)[K]) {
+  double com[K], comn[K], F[K], Fn[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) com[k] = 0.5 * (bv[k] + (rev ? vn[K - 1 - k] : vn[k]));
+#pragma unroll
+  for (int m = 0; m < K; ++m) comn[m] = rev ? com[K - 1 - m] : com[m];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    double cr = 0., crn = 0.;
+#pragma unroll
+    for (int d = 0; d < K; ++d) { cr += com[d] * T.D[d][k]; crn += comn[d] * T.D[d][k]; }
+    const double dv = bg[k] + (com[k] - bv[k]) * T.gp[ib];
+    const double dvn = gn[k] + (comn[k] - vn[k]) * T.gp[ibn];
+    F[k] = dv * fa[k] - cr * fb[k];
+    Fn[k] = dvn * fan[k] - crn * fbn[k];
+  }
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    cv[k] = com[k];
+    cg[k] = 0.5 * (F[k] + sgn * (rev ? Fn[K - 1 - k] : Fn[k]));
+  }
+}
+
+// Boundary face (src/HO_2D.cpp:2872-2927).  bc: edge values (edge flux-point order) or
+// nullptr for zero values; bscale multiplies the Neumann values.
+template <int K>
+__device__ __forceinline__ void boundary_face(const Tab& T, const Geo& G, int el, int ijp, int sw,
+                                              const double* __restrict__ bc, double bscale,
+                                              const double (&bv)[K], const double (&bg)[K],
+                                              const double (&fa)[K], const double (&fb)[K],
+                                              double (&cv)[K], double (&cg)[K]) {
+  const int ib = ijp & 1;
+  if (sw == 0) {  // Dirichlet: common value = BC
+    double com[K];
+#pragma unroll
+    for (int m = 0; m < K; ++m) com[m] = bc ? bc[edge_fp<K>(ijp, m)] : 0.;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      double cr = 0.;
+#pragma unroll
+      for (int d = 0; d < K; ++d) cr += com[d] * T.D[d][k];
+      const double dv = bg[k] + (com[k] - bv[k]) * T.gp[ib];
+      cv[k] = com[k];
+      cg[k] = dv * fa[k] - cr * fb[k];
+    }
+  } else {  // Neumann: common flux = BC * |g|, common value from the K x K system
+    const size_t q = ((size_t)el * 4 + ijp) * K;
+    double M[K][K], r[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const double A = G.fA[q + k], B = G.fB[q + k], J = G.fjac[q + k];
+      cg[k] = (bc ? bscale * bc[edge_fp<K>(ijp, k)] : 0.) * G.fAn[q + k];
+      r[k] = J * cg[k] - A * (bg[k] - bv[k] * T.gp[ib]);
+#pragma unroll
+      for (int i = 0; i < K; ++i) M[k][i] = -B * T.D[i][k];
+      M[k][k] += A * T.gp[ib];
+    }
+    // Gauss elimination without pivoting (src/MathHelpers.hpp:53-74)
+#pragma unroll
+    for (int k = 0; k < K - 1; ++k)
+#pragma unroll
+      for (int i = k + 1; i < K; ++i) {
+        const double c = M[i][k] / M[k][k];
+        r[i] -= c * r[k];
+#pragma unroll
+        for (int j = k + 1; j < K; ++j) M[i][j] -= c * M[k][j];
+      }
+#pragma unroll
+    for (int i = K - 1; i >= 0; --i) {
+      double c = 0.;
+#pragma unroll
+      for (int j = i + 1; j < K; ++j) c += M[i][j] * cv[j];
+      cv[i] = (r[i] - c) / M[i][i];
+    }
+  }
+}
+
+// Gathers one face's neighbour trace pair (value, derivative) and neighbour metrics.
+template <int K, bool AFF = false>
+__device__ __forceinline__ void load_nbr(const Geo& G, const double* __restrict__ tr, int nb, int fn,
+                                         double (&vn)[K], double (&gn)[K], double (&fan)[K], double (&fbn)[K]) {
+  const double* t = tr + ((size_t)nb * 4 + fn) * 2 * K;
+  if (AFF) {
+    const double a = G.gface[((size_t)nb * 4 + fn) * 2], b = G.gface[((size_t)nb * 4 + fn) * 2 + 1];
+#pragma unroll
+    for (int k = 0; k < K; ++k) { vn[k] = t[k]; gn[k] = t[K + k]; fan[k] = a; fbn[k] = b; }
+  } else {
+    const size_t q = ((size_t)nb * 4 + fn) * K;
+#pragma unroll
+    for (int k = 0; k < K; ++k) { vn[k] = t[k]; gn[k] = t[K + k]; fan[k] = G.fa[q + k]; fbn[k] = G.fb[q + k]; }
+  }
+}
+
+// All four faces of the second-order operator.  mode: 0 Poisson (psi switches, bc values
+// or zero), 1 diffusion (diffusion switches).
+template <int K, bool AFF = false>
+__device__ __forceinline__ void faces2(const Tab& T, const Geo& G, int el, const double* __restrict__ tr,
+                                       const int8_t* __restrict__ sw, const double* __restrict__ bcv, bool signed_neumann,
+                                       const double (&bv)[4][K], const double (&bg)[4][K],
+                                       double (&cv)[4][K], double (&cg)[4][K]) {
+#pragma unroll
+  for (int f = 0; f < 4; ++f) {
+    const int nb = G.nb[(size_t)el * 4 + f];
+    double fa[K], fb[K];
+    if (AFF) {
+      const double a = G.gface[((size_t)el * 4 + f) * 2], b = G.gface[((size_t)el * 4 + f) * 2 + 1];
+#pragma unroll
+      for (int k = 0; k < K; ++k) { fa[k] = a; fb[k] = b; }
+    } else {
+      const size_t q = ((size_t)el * 4 + f) * K;
+#pragma unroll
+      for (int k = 0; k < K; ++k) { fa[k] = G.fa[q + k]; fb[k] = G.fb[q + k]; }
+    }
+    if (nb >= 0) {
+      const int fn = G.nbf[(size_t)el * 4 + f];
+      double vn[K], gn[K], fan[K], fbn[K];
+      load_nbr<K, AFF>(G, tr, nb, fn, vn, gn, fan, fbn);
+      interior_face<K>(T, f & 1, fn & 1, rev_flag(f, fn), face_sign(f, fn), bv[f], bg[f], vn, gn, fa, fb, fan, fbn, cv[f], cg[f]);
+    } else {
+      const int e = -1 - nb;
+      const double* b = bcv ? bcv + (size_t)e * K : nullptr;
+      const double sc = signed_neumann ? G.esign[e] : 1.0;
+      boundary_face<K>(T, G, el, f, sw[e], b, sc, bv[f], bg[f], fa, fb, cv[f], cg[f]);
+    }
+  }
+}
+
+// Contravariant corrected gradient (f~, g~) and its corrected divergence (src/HO_2D.cpp:2771-2816).
+template <int K>
+__device__ __forceinline__ void second_order_core(const Tab& T, const double (&u)[K * K], const double (&bv)[4][K],
+                                                  const double (&cv)[4][K], const double (&cg)[4][K],
+                                                  const double* ga, const double* gb, const double* gc,
+                                                  double (&out)[K * K]) {
+  double ft[K * K], gt[K * K];
+#pragma unroll
+  for (int ky = 0; ky < K; ++ky)
+#pragma unroll
+    for (int kx = 0; kx < K; ++kx) {
+      double dx = 0., de = 0.;
+#pragma unroll
+      for (int d = 0; d < K; ++d) { dx += u[ky * K + d] * T.D[d][kx]; de += u[d * K + kx] * T.D[d][ky]; }
+      dx += (cv[0][ky] - bv[0][ky]) * T.grad[kx][0] + (cv[1][ky] - bv[1][ky]) * T.grad[kx][1];
+      de += (cv[2][kx] - bv[2][kx]) * T.grad[ky][0] + (cv[3][kx] - bv[3][kx]) * T.grad[ky][1];
+      const int v = ky * K + kx;
+      ft[v] = dx * ga[v] - de * gb[v];
+      gt[v] = de * gc[v] - dx * gb[v];
+    }
+  double fB[2][K], gB[2][K];
+#pragma unroll
+  for (int ib = 0; ib < 2; ++ib)
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      double a = cg[ib][k], b = cg[2 + ib][k];
+#pragma unroll
+      for (int d = 0; d < K; ++d) { a -= ft[k * K + d] * T.bb[d][ib]; b -= gt[d * K + k] * T.bb[d][ib]; }
+      fB[ib][k] = a; gB[ib][k] = b;
+    }
+#pragma unroll
+  for (int ky = 0; ky < K; ++ky)
+#pragma unroll
+    for (int kx = 0; kx < K; ++kx) {
+      double r = 0.;
+#pragma unroll
+      for (int d = 0; d < K; ++d) r += ft[ky * K + d] * T.D[d][kx];
+      r += fB[0][ky] * T.grad[kx][0] + fB[1][ky] * T.grad[kx][1];
+#pragma unroll
+      for (int d = 0; d < K; ++d) r += gt[d * K + kx] * T.D[d][ky];
+      r += gB[0][kx] * T.grad[ky][0] + gB[1][kx] * T.grad[ky][1];
+      out[ky * K + kx] = r;
+    }
+}
+
+// ==========================================================================================
+// Trace kernel with fused vector pre-operations producing the traced field u:
+//   PRE_COPY   u = x
+//   PRE_CHEB0  u = r / (diag * theta); writes x = d = u          (Chebyshev step from zero)
+//   PRE_PROL   u = x + Pc xc (bilinear from the corner vertices); writes x = u
+//   PRE_PUPD   u = z + beta * p; writes p = u                     (PCG direction)
+//   PRE_ZERO   u = 0                                              (RHS assembly)
+// ==========================================================================================
+template <int K, int PRE>
+__global__ __launch_bounds__(kWG) void k_trace(int N, Tab T, double* __restrict__ x, const double* __restrict__ a,
+                                                const double* __restrict__ b, double* __restrict__ d, double coef,
+                                                const double* __restrict__ scal, int scal_idx, const int* __restrict__ active,
+                                                double* __restrict__ tr, const int32_t* __restrict__ ev) {
+  if (active && !*active) return;
+  constexpr int K2 = K * K;
+  __shared__ double lds[kWG * K2];
+  __shared__ double lds2[kWG * K2];
+  const int e0 = blockIdx.x * kWG;
+  const int el = e0 + threadIdx.x;
+  const bool live = el < N;
+  double u[K2];
+  if (PRE == PRE_COPY || PRE == PRE_PROL) tile_load<K2>(x, lds, e0, N);
+  if (PRE == PRE_CHEB0) { tile_load<K2>(a, lds, e0, N); tile_load<K2>(b, lds2, e0, N); }
+  if (PRE == PRE_PUPD) { tile_load<K2>(a, lds, e0, N); tile_load<K2>(x, lds2, e0, N); }
+  double beta = 0.;
+  if (PRE == PRE_PUPD) beta = scal[scal_idx];
+  double xc[4] = {0., 0., 0., 0.};
+  if (PRE == PRE_PROL && live)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) xc[c] = a[ev[(size_t)el * 4 + c]];
+#pragma unroll
+  for (int m = 0; m < K2; ++m) {
+    const int li = threadIdx.x * K2 + m;
+    double v = 0.;
+    if (PRE == PRE_COPY) v = lds[li];
+    if (PRE == PRE_PROL) {
+      const int j = m / K, i = m % K;
+      v = lds[li] + (q1w(T, 0, j, i) * xc[0] + q1w(T, 1, j, i) * xc[1] + q1w(T, 2, j, i) * xc[2] + q1w(T, 3, j, i) * xc[3]);
+    }
+    if (PRE == PRE_CHEB0) v = lds[li] / (lds2[li] * coef);
+    if (PRE == PRE_PUPD) v = lds[li] + beta * lds2[li];
+    u[m] = v;
+  }
+  if (PRE == PRE_CHEB0 || PRE == PRE_PROL || PRE == PRE_PUPD) {
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < K2; ++m) lds[threadIdx.x * K2 + m] = u[m];
+    if (PRE == PRE_CHEB0) { tile_store<K2>(x, lds, e0, N); tile_store<K2>(d, lds, e0, N); }
+    else tile_store<K2>(x, lds, e0, N);
+  }
+  if (!live) return;
+  double bv[4][K], bg[4][K];
+  traces<K>(T, u, bv, bg);
+  double* t = tr + (size_t)el * 8 * K;
+#pragma unroll
+  for (int f = 0; f < 4; ++f)
+#pragma unroll
+    for (int k = 0; k < K; ++k) { t[f * 2 * K + k] = bv[f][k]; t[f * 2 * K + K + k] = bg[f][k]; }
+}
+
+// ==========================================================================================
+// Poisson operator As = -W * L (W = w_i w_j) applied matrix-free, with fused epilogues:
+//   EPI_STORE     y = As x; partial sums of x.y                    (PCG / probing)
+//   EPI_RESID     y = r - As x; partial sums of y.y and (y/W).(y/W) (initial residual)
+//   EPI_CHEB      res = r - As x; d = c1 d + c2 res/diag; x += d; optional partial r.x
+//   EPI_CHEBF     res = r - As x; d = res/(diag theta); x += d     (first step from x != 0)
+//   EPI_RESTRICT  ec[el][a] = sum_ij q1w(a,ij) (r - As x)_ij        (Pc^T residual, per element)
+// ==========================================================================================
+template <int K, int EPI, bool AFF>
+__global__ __launch_bounds__(kWG) void k_lap(int N, Tab T, Geo G, double* __restrict__ x, const double* __restrict__ tr,
+                                              double* __restrict__ y, const double* __restrict__ r,
+                                              const double* __restrict__ diag, double* __restrict__ d,
+                                              double c1, double c2, double* __restrict__ part,
+                                              const int* __restrict__ active) {
+  if (active && !*active) return;
+  constexpr int K2 = K * K;
+  __shared__ double lds[kWG * K2];
+  const int e0 = blockIdx.x * kWG;
+  const int el = e0 + threadIdx.x;
+  const bool live = el < N;
+  double u[K2], ga[K2], gb[K2], gc[K2];
+  tile_load<K2>(x, lds, e0, N);
+#pragma unroll
+  for (int m = 0; m < K2; ++m) u[m] = live ? lds[threadIdx.x * K2 + m] : 0.;  // tail threads: no stale LDS
+  if (AFF) {  // metrics constant per element
+    const double a = live ? G.gel[(size_t)el * 3] : 0., b = live ? G.gel[(size_t)el * 3 + 1] : 0.,
+                 c = live ? G.gel[(size_t)el * 3 + 2] : 0.;
+#pragma unroll
+    for (int m = 0; m < K2; ++m) { ga[m] = a; gb[m] = b; gc[m] = c; }
+  } else {
+    __syncthreads();
+    tile_load<K2>(G.Ga, lds, e0, N);
+#pragma unroll
+    for (int m = 0; m < K2; ++m) ga[m] = lds[threadIdx.x * K2 + m];
+    __syncthreads();
+    tile_load<K2>(G.Gb, lds, e0, N);
+#pragma unroll
+    for (int m = 0; m < K2; ++m) gb[m] = lds[threadIdx.x * K2 + m];
+    __syncthreads();
+    tile_load<K2>(G.Gc, lds, e0, N);
+#pragma unroll
+    for (int m = 0; m < K2; ++m) gc[m] = lds[threadIdx.x * K2 + m];
+  }
+  double out[K2];
+  if (live) {
+    double bv[4][K], bg[4][K], cv[4][K], cg[4][K];
+    traces<K>(T, u, bv, bg);
+    faces2<K, AFF>(T, G, el, tr, G.sw_psi, nullptr, false, bv, bg, cv, cg);
+    second_order_core<K>(T, u, bv, cv, cg, ga, gb, gc, out);
+#pragma unroll
+    for (int j = 0; j < K; ++j)
+#pragma unroll
+      for (int i = 0; i < K; ++i) out[j * K + i] *= -(T.w[j] * T.w[i]);
+  } else {
+#pragma unroll
+    for (int m = 0; m < K2; ++m) out[m] = 0.;
+  }
+  double p0 = 0., p1 = 0.;
+  if (EPI == EPI_STORE) {
+#pragma unroll
+    for (int m = 0; m < K2; ++m) p0 += u[m] * out[m];
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < K2; ++m) lds[threadIdx.x * K2 + m] = out[m];
+    tile_store<K2>(y, lds, e0, N);
+  } else {
+    __syncthreads();
+    tile_load<K2>(r, lds, e0, N);
+    double res[K2];
+#pragma unroll
+    for (int m = 0; m < K2; ++m) res[m] = lds[threadIdx.x * K2 + m] - out[m];
+    if (EPI == EPI_RESID) {
+#pragma unroll
+      for (int j = 0; j < K; ++j)
+#pragma unroll
+        for (int i = 0; i < K; ++i) {
+          const double v = res[j * K + i], uw = v / (T.w[j] * T.w[i]);
+          p0 += v * v; p1 += uw * uw;
+        }
+      if (!live) { p0 = 0.; p1 = 0.; }
+      __syncthreads();
+#pragma unroll
+      for (int m = 0; m < K2; ++m) lds[threadIdx.x * K2 + m] = res[m];
+      tile_store<K2>(y, lds, e0, N);
+    } else if (EPI == EPI_RESTRICT) {
+      double s[4] = {0., 0., 0., 0.};
+#pragma unroll
+      for (int j = 0; j < K; ++j)
+#pragma unroll
+        for (int i = 0; i < K; ++i)
+#pragma unroll
+          for (int c = 0; c < 4; ++c) s[c] += q1w(T, c, j, i) * res[j * K + i];
+      if (live)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) y[(size_t)el * 4 + c] = s[c];
+    } else {  // Chebyshev updates
+      __syncthreads();
+      tile_load<K2>(diag, lds, e0, N);
+      double dd[K2];
+#pragma unroll
+      for (int m = 0; m < K2; ++m) dd[m] = res[m] / lds[threadIdx.x * K2 + m];
+      if (EPI == EPI_CHEB) {
+        __syncthreads();
+        tile_load<K2>(d, lds, e0, N);
+#pragma unroll
+        for (int m = 0; m < K2; ++m) dd[m] = c1 * lds[threadIdx.x * K2 + m] + c2 * dd[m];
+      } else {
+#pragma unroll
+        for (int m = 0; m < K2; ++m) dd[m] *= c2;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int m = 0; m < K2; ++m) lds[threadIdx.x * K2 + m] = dd[m];
+      tile_store<K2>(d, lds, e0, N);
+#pragma unroll
+      for (int m = 0; m < K2; ++m) u[m] += dd[m];
+#pragma unroll
+      for (int m = 0; m < K2; ++m) lds[threadIdx.x * K2 + m] = u[m];
+      tile_store<K2>(x, lds, e0, N);
+      if (part) {  // r . x (PCG's r.z after the last smoothing step)
+        tile_load<K2>(r, lds, e0, N);
+#pragma unroll
+        for (int m = 0; m < K2; ++m) p0 += lds[threadIdx.x * K2 + m] * u[m];
+        if (!live) p0 = 0.;
+      }
+    }
+  }
+  if (part) {
+    p0 = wave_sum(p0);
+    p1 = wave_sum(p1);
+    if (threadIdx.x == 0) { part[blockIdx.x] = p0; part[gridDim.x + blockIdx.x] = p1; }
+  }
+}
+
+// ==========================================================================================
+// Poisson right-hand side (src/HO_2D.cpp:1881-1900) in the scaled form
+//   bs = W (omega J + L(0; BC_psi)),   with partial sums of (bs/W)^2 = ||b||^2.
+// ==========================================================================================
+template <int K>
+__global__ __launch_bounds__(kWG) void k_rhs(int N, Tab T, Geo G, const double* __restrict__ omega,
+                                              const double* __restrict__ bcpsi, double* __restrict__ bs,
+                                              double* __restrict__ part) {
+  constexpr int K2 = K * K;
+  __shared__ double lds[kWG * K2];
+  const int e0 = blockIdx.x * kWG;
+  const int el = e0 + threadIdx.x;
+  const bool live = el < N;
+  double w[K2], jac[K2];
+  tile_load<K2>(omega, lds, e0, N);
+#pragma unroll
+  for (int m = 0; m < K2; ++m) w[m] = lds[threadIdx.x * K2 + m];
+  __syncthreads();
+  tile_load<K2>(G.jac, lds, e0, N);
+#pragma unroll
+  for (int m = 0; m < K2; ++m) jac[m] = lds[threadIdx.x * K2 + m];
+  double out[K2];
+#pragma unroll
+  for (int m = 0; m < K2; ++m) out[m] = 0.;
+  bool bnd = false;
+  if (live)
+#pragma unroll
+    for (int f = 0; f < 4; ++f) bnd |= G.nb[(size_t)el * 4 + f] < 0;
+  if (bnd) {
+    double z[K2], bv[4][K], bg[4][K], cv[4][K], cg[4][K];
+#pragma unroll
+    for (int m = 0; m < K2; ++m) z[m] = 0.;
+#pragma unroll
+    for (int f = 0; f < 4; ++f)
+#pragma unroll
+      for (int k = 0; k < K; ++k) { bv[f][k] = 0.; bg[f][k] = 0.; cv[f][k] = 0.; cg[f][k] = 0.; }
+#pragma unroll
+    for (int f = 0; f < 4; ++f) {
+      const int nb = G.nb[(size_t)el * 4 + f];
+      if (nb >= 0) continue;  // a zero field gives zero common values on interior faces
+      const int e = -1 - nb;
+      double fa[K], fb[K];
+      const size_t q = ((size_t)el * 4 + f) * K;
+#pragma unroll
+      for (int k = 0; k < K; ++k) { fa[k] = G.fa[q + k]; fb[k] = G.fb[q + k]; }
+      boundary_face<K>(T, G, el, f, G.sw_psi[e], bcpsi + (size_t)e * K, 1.0, bv[f], bg[f], fa, fb, cv[f], cg[f]);
+    }
+    const size_t base = (size_t)el * K2;
+    second_order_core<K>(T, z, bv, cv, cg, G.Ga + base, G.Gb + base, G.Gc + base, out);
+  }
+  double p0 = 0.;
+#pragma unroll
+  for (int j = 0; j < K; ++j)
+#pragma unroll
+    for (int i = 0; i < K; ++i) {
+      const int m = j * K + i;
+      const double b = -w[m] * jac[m] - out[m];  // unscaled reference RHS
+      p0 += b * b;
+      out[m] = -(T.w[j] * T.w[i]) * b;
+    }
+  if (!live) p0 = 0.;
+  __syncthreads();
+#pragma unroll
+  for (int m = 0; m < K2; ++m) lds[threadIdx.x * K2 + m] = out[m];
+  tile_store<K2>(bs, lds, e0, N);
+  p0 = wave_sum(p0);
+  if (threadIdx.x == 0) part[blockIdx.x] = p0;
+}
+
+// ==========================================================================================
+// Velocity from the stream function (src/HO_2D.cpp:3174-3366) with the no-slip wall
+// vorticity flux (update_diffusion_BC, src/HO_2D.cpp:2647-2664), fused with the traces the
+// stage kernel needs from neighbours: omega value/derivative traces and the contravariant
+// normal-velocity trace.
+// ==========================================================================================
+template <int K>
+__global__ __launch_bounds__(kWG) void k_velocity(int N, Tab T, Geo G, BCDev B, const double* __restrict__ psi,
+                                                   const double* __restrict__ trpsi, const double* __restrict__ omega,
+                                                   double* __restrict__ vel, double* __restrict__ U,
+                                                   double* __restrict__ trw, double* __restrict__ tru,
+                                                   double inv_dt_re) {
+  constexpr int K2 = K * K;
+  __shared__ double lds[kWG * K2 * 2];
+  const int e0 = blockIdx.x * kWG;
+  const int el = e0 + threadIdx.x;
+  const bool live = el < N;
+  double u[K2];
+  tile_load<K2>(psi, lds, e0, N);
+#pragma unroll
+  for (int m = 0; m < K2; ++m) u[m] = lds[threadIdx.x * K2 + m];
+  double vx[K2], vy[K2], U1[K2], U2[K2];
+  if (live) {
+    double bv[4][K], bg[4][K], cv[4][K], cg[4][K];
+    traces<K>(T, u, bv, bg);
+    faces2<K>(T, G, el, trpsi, G.sw_psi, B.psi, true, bv, bg, cv, cg);
+    // boundary side effects
+#pragma unroll
+    for (int f = 0; f < 4; ++f) {
+      const int nb = G.nb[(size_t)el * 4 + f];
+      if (nb >= 0) continue;
+      const int e = -1 - nb;
+      const size_t q = ((size_t)el * 4 + f) * K;
+      if (G.sw_psi[e] == 0) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) B.vjump[(size_t)e * K + edge_fp<K>(f, k)] = cg[f][k] / G.fAn[q + k];
+      } else {
+        const double vs = 1. - 2. * (f / 2);
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+          double s = 0.;
+#pragma unroll
+          for (int d = 0; d < K; ++d) s += vs * cv[f][d] * T.D[d][k];
+          B.nvel[(size_t)e * K + edge_fp<K>(f, k)] = s / G.fAn[q + k];
+        }
+      }
+      if (G.no_slip[e]) {
+        const double sg = G.esign[e];
+#pragma unroll
+        for (int m = 0; m < K; ++m) {
+          const size_t i = (size_t)e * K + m;
+          const double vj = B.vjump[i] + sg * B.pvel[i];
+          B.vjump[i] = vj;
+          B.diff[i] = vj * inv_dt_re;
+        }
+      }
+    }
+    const size_t base = (size_t)el * K2;
+    double ft[K2], gt[K2];
+#pragma unroll
+    for (int ky = 0; ky < K; ++ky)
+#pragma unroll
+      for (int kx = 0; kx < K; ++kx) {
+        double dx = 0., de = 0.;
+#pragma unroll
+        for (int d = 0; d < K; ++d) { dx += u[ky * K + d] * T.D[d][kx]; de += u[d * K + kx] * T.D[d][ky]; }
+        dx += (cv[0][ky] - bv[0][ky]) * T.grad[kx][0] + (cv[1][ky] - bv[1][ky]) * T.grad[kx][1];
+        de += (cv[2][kx] - bv[2][kx]) * T.grad[ky][0] + (cv[3][kx] - bv[3][kx]) * T.grad[ky][1];
+        const int v = ky * K + kx;
+        ft[v] = dx * G.Ga[base + v] - de * G.Gb[base + v];
+        gt[v] = de * G.Gc[base + v] - dx * G.Gb[base + v];
+      }
+    double fB[2][K], gB[2][K];
+#pragma unroll
+    for (int ib = 0; ib < 2; ++ib)
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        double a = cg[ib][k], b = cg[2 + ib][k];
+#pragma unroll
+        for (int d = 0; d < K; ++d) { a -= ft[k * K + d] * T.bb[d][ib]; b -= gt[d * K + k] * T.bb[d][ib]; }
+        fB[ib][k] = a; gB[ib][k] = b;
+      }
+#pragma unroll
+    for (int ky = 0; ky < K; ++ky)
+#pragma unroll
+      for (int kx = 0; kx < K; ++kx) {
+        const int v = ky * K + kx;
+        const double tx = ft[v] + fB[0][ky] * T.rad[kx][0] + fB[1][ky] * T.rad[kx][1];
+        const double ty = gt[v] + gB[0][kx] * T.rad[ky][0] + gB[1][kx] * T.rad[ky][1];
+        const double xxi = G.xxi[base + v], xet = G.xeta[base + v], yxi = G.yxi[base + v], yet = G.yeta[base + v];
+        const double J = G.jac[base + v];
+        const double dpdx = (tx * xxi + ty * xet) / J;
+        const double dpdy = (tx * yxi + ty * yet) / J;
+        vx[v] = dpdy;    // u =  dpsi/dy
+        vy[v] = -dpdx;   // v = -dpsi/dx
+        U1[v] = vx[v] * yet - vy[v] * xet;    // J (xi_x u + xi_y v)
+        U2[v] = -vx[v] * yxi + vy[v] * xxi;   // J (eta_x u + eta_y v)
+      }
+  } else {
+#pragma unroll
+    for (int m = 0; m < K2; ++m) { vx[m] = vy[m] = U1[m] = U2[m] = 0.; }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int m = 0; m < K2; ++m) { lds[threadIdx.x * 2 * K2 + 2 * m] = vx[m]; lds[threadIdx.x * 2 * K2 + 2 * m + 1] = vy[m]; }
+  tile_store<2 * K2>(vel, lds, e0, N);
+#pragma unroll
+  for (int m = 0; m < K2; ++m) { lds[threadIdx.x * 2 * K2 + m] = U1[m]; lds[threadIdx.x * 2 * K2 + K2 + m] = U2[m]; }
+  tile_store<2 * K2>(U, lds, e0, N);
+  // omega traces for the stage kernel
+  double w[K2];
+  tile_load<K2>(omega, lds, e0, N);
+#pragma unroll
+  for (int m = 0; m < K2; ++m) w[m] = lds[threadIdx.x * K2 + m];
+  if (!live) return;
+  double bv[4][K], bg[4][K];
+  traces<K>(T, w, bv, bg);
+  double* t = trw + (size_t)el * 8 * K;
+#pragma unroll
+  for (int f = 0; f < 4; ++f)
+#pragma unroll
+    for (int k = 0; k < K; ++k) { t[f * 2 * K + k] = bv[f][k]; t[f * 2 * K + K + k] = bg[f][k]; }
+  // contravariant normal-velocity traces (src/HO_2D.cpp:2332, 2489-2503)
+  double* tu = tru + (size_t)el * 4 * K;
+#pragma unroll
+  for (int rc = 0; rc < K; ++rc)
+#pragma unroll
+    for (int ib = 0; ib < 2; ++ib) {
+      double s = 0., s2 = 0.;
+#pragma unroll
+      for (int m = 0; m < K; ++m) { s += U1[rc * K + m] * T.bb[m][ib]; s2 += U2[m * K + rc] * T.bb[m][ib]; }
+      tu[ib * K + rc] = s;
+      tu[(2 + ib) * K + rc] = s2;
+    }
+}
+
+// ==========================================================================================
+// Stage kernel: diffusion (src/HO_2D.cpp:2666-2831, BC_vorticity at :2726-2727),
+// update_advection_BC (:2219-2243), advection type 1 (:2245-2437) or 2 (:2439-2645),
+// k = dt (adv + diff) (:1833-1836) and the RK update (:1465-1531).
+// ==========================================================================================
+template <int K, int ADV>
+__global__ __launch_bounds__(kWG) void k_stage(int N, Tab T, Geo G, BCDev B, const double* __restrict__ omega,
+                                                const double* __restrict__ U, const double* __restrict__ trw,
+                                                const double* __restrict__ tru, double re_inv, double dt,
+                                                StageOut so) {
+  constexpr int K2 = K * K;
+  __shared__ double lds[kWG * K2 * 2];
+  const int e0 = blockIdx.x * kWG;
+  const int el = e0 + threadIdx.x;
+  const bool live = el < N;
+  double w[K2], U1[K2], U2[K2];
+  tile_load<K2>(omega, lds, e0, N);
+#pragma unroll
+  for (int m = 0; m < K2; ++m) w[m] = lds[threadIdx.x * K2 + m];
+  __syncthreads();
+  tile_load<2 * K2>(U, lds, e0, N);
+#pragma unroll
+  for (int m = 0; m < K2; ++m) { U1[m] = lds[threadIdx.x * 2 * K2 + m]; U2[m] = lds[threadIdx.x * 2 * K2 + K2 + m]; }
+  double kk[K2];
+  if (live) {
+    const size_t base = (size_t)el * K2;
+    // ---- diffusion
+    double bv[4][K], bg[4][K], cv[4][K], cg[4][K];
+    traces<K>(T, w, bv, bg);
+    faces2<K>(T, G, el, trw, G.sw_diff, B.diff, false, bv, bg, cv, cg);
+    double dif[K2];
+    second_order_core<K>(T, w, bv, cv, cg, G.Ga + base, G.Gb + base, G.Gc + base, dif);
+    // ---- boundary vorticity and advective boundary flux
+    double bca[4][K];
+#pragma unroll
+    for (int f = 0; f < 4; ++f) {
+      const int nb = G.nb[(size_t)el * 4 + f];
+      if (nb >= 0) continue;
+      const int e = -1 - nb;
+      const size_t q = ((size_t)el * 4 + f) * K;
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        const int fp = edge_fp<K>(f, k);
+        const double vort = cv[f][k];
+        B.vort[(size_t)e * K + fp] = vort;
+        const double a = vort * B.nvel[(size_t)e * K + fp] * G.fAn[q + k];
+        B.adv[(size_t)e * K + fp] = a;
+        bca[f][k] = a;
+      }
+    }
+    // ---- advection
+    double upw[4][K], bu[4][K], cvf[4][K];
+#pragma unroll
+    for (int rc = 0; rc < K; ++rc)
+#pragma unroll
+      for (int ib = 0; ib < 2; ++ib) {
+        double s = 0., s2 = 0.;
+#pragma unroll
+        for (int m = 0; m < K; ++m) { s += U1[rc * K + m] * T.bb[m][ib]; s2 += U2[m * K + rc] * T.bb[m][ib]; }
+        bu[ib][rc] = s;
+        bu[2 + ib][rc] = s2;
+      }
+#pragma unroll
+    for (int f = 0; f < 4; ++f) {
+      const int nb = G.nb[(size_t)el * 4 + f];
+      if (nb < 0) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) { upw[f][k] = bca[f][k]; cvf[f][k] = 0.; }
+        if (ADV == 2) {
+          const int e = -1 - nb;
+          const size_t q = ((size_t)el * 4 + f) * K;
+#pragma unroll
+          for (int k = 0; k < K; ++k) cvf[f][k] = B.nvel[(size_t)e * K + edge_fp<K>(f, k)] * G.fAn[q + k];
+        }
+        continue;
+      }
+      const int fn = G.nbf[(size_t)el * 4 + f];
+      const int rev = rev_flag(f, fn);
+      const double sgn = face_sign(f, fn);
+      const double* tw = trw + ((size_t)nb * 4 + fn) * 2 * K;
+      const double* tuu = tru + ((size_t)nb * 4 + fn) * K;
+      double wn[K], un[K];
+#pragma unroll
+      for (int k = 0; k < K; ++k) { wn[k] = tw[k]; un[k] = tuu[k]; }
+      if (ADV == 1) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+          const int kn = rev ? K - 1 - k : k;
+          const double wnk = rev ? wn[K - 1 - k] : wn[k];
+          const double unk = rev ? un[K - 1 - k] : un[k];
+          (void)kn;
+          const double fo = bu[f][k] * bv[f][k];
+          const double fnn = unk * wnk;
+          double up = 0.5 * (fo + sgn * fnn);
+          const double dw = bv[f][k] - wnk;
+          if (fabs(dw) > 1.e-4) {
+            const double a = (fo - sgn * fnn) / dw;
+            up += 0.5 * ((f & 1) ? 1. : -1.) * fabs(a) * dw;
+          }
+          upw[f][k] = up;
+        }
+      } else {
+        const double t = (double)(f & 1);
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+          const double wnk = rev ? wn[K - 1 - k] : wn[k];
+          const double unk = rev ? un[K - 1 - k] : un[k];
+          const double c = 0.5 * (bu[f][k] + sgn * unk);
+          cvf[f][k] = c;
+          if (c > 0.) upw[f][k] = c * (t * bv[f][k] + (1. - t) * wnk);
+          else upw[f][k] = c * ((1. - t) * bv[f][k] + t * wnk);
+        }
+      }
+    }
+    if (ADV == 2) {  // continuous volumetric flux (src/HO_2D.cpp:2539-2546)
+      double c1[K2], c2[K2];
+#pragma unroll
+      for (int j = 0; j < K; ++j)
+#pragma unroll
+        for (int i = 0; i < K; ++i) {
+          const int v = j * K + i;
+          c1[v] = U1[v] + (cvf[0][j] - bu[0][j]) * T.rad[i][0] + (cvf[1][j] - bu[1][j]) * T.rad[i][1];
+          c2[v] = U2[v] + (cvf[2][i] - bu[2][i]) * T.rad[j][0] + (cvf[3][i] - bu[3][i]) * T.rad[j][1];
+        }
+#pragma unroll
+      for (int m = 0; m < K2; ++m) { U1[m] = c1[m]; U2[m] = c2[m]; }
+    }
+    double f1[K2], f2[K2];  // discontinuous fluxes U^1 w, U^2 w at the sps
+#pragma unroll
+    for (int m = 0; m < K2; ++m) { f1[m] = U1[m] * w[m]; f2[m] = U2[m] * w[m]; }
+    double bdf[4][K];
+#pragma unroll
+    for (int ib = 0; ib < 2; ++ib)
+#pragma unroll
+      for (int j = 0; j < K; ++j) {
+        double a = 0., b = 0.;
+#pragma unroll
+        for (int m = 0; m < K; ++m) { a -= f1[j * K + m] * T.bb[m][ib]; b -= f2[m * K + j] * T.bb[m][ib]; }
+        bdf[ib][j] = a + upw[ib][j];
+        bdf[2 + ib][j] = b + upw[2 + ib][j];
+      }
+#pragma unroll
+    for (int j = 0; j < K; ++j)
+#pragma unroll
+      for (int i = 0; i < K; ++i) {
+        const int v = j * K + i;
+        double r = 0.;
+#pragma unroll
+        for (int d = 0; d < K; ++d) r -= f1[j * K + d] * T.D[d][i];
+#pragma unroll
+        for (int d = 0; d < K; ++d) r -= f2[d * K + i] * T.D[d][j];
+        r -= bdf[0][j] * T.grad[i][0] + bdf[1][j] * T.grad[i][1];
+        r -= bdf[2][i] * T.grad[j][0] + bdf[3][i] * T.grad[j][1];
+        const double J = G.jac[base + v];
+        const double adv = r / J;
+        const double diff = dif[v] * (re_inv / J);
+        kk[v] = dt * (adv + diff);
+      }
+  } else {
+#pragma unroll
+    for (int m = 0; m < K2; ++m) kk[m] = 0.;
+  }
+  // ---- outputs: k, and the RK combination
+  __syncthreads();
+#pragma unroll
+  for (int m = 0; m < K2; ++m) lds[threadIdx.x * K2 + m] = kk[m];
+  if (so.k) tile_store<K2>(so.k, lds, e0, N);
+  if (so.next) {  // next stage input: base + c k
+    __syncthreads();
+    tile_load<K2>(so.base, lds, e0, N);
+#pragma unroll
+    for (int m = 0; m < K2; ++m) lds[threadIdx.x * K2 + m] = lds[threadIdx.x * K2 + m] + so.c * kk[m];
+    tile_store<K2>(so.next, lds, e0, N);
+  }
+  if (so.final_mode) {  // vorticity update
+    double acc[K2];
+    __syncthreads();
+    tile_load<K2>(so.base, lds, e0, N);
+#pragma unroll
+    for (int m = 0; m < K2; ++m) acc[m] = lds[threadIdx.x * K2 + m];
+    double k1[K2], k2[K2], k3[K2];
+    if (so.final_mode >= 2) {
+      __syncthreads();
+      tile_load<K2>(so.k1, lds, e0, N);
+#pragma unroll
+      for (int m = 0; m < K2; ++m) k1[m] = lds[threadIdx.x * K2 + m];
+    }
+    if (so.final_mode == 4) {
+      __syncthreads();
+      tile_load<K2>(so.k2, lds, e0, N);
+#pragma unroll
+      for (int m = 0; m < K2; ++m) k2[m] = lds[threadIdx.x * K2 + m];
+      __syncthreads();
+      tile_load<K2>(so.k3, lds, e0, N);
+#pragma unroll
+      for (int m = 0; m < K2; ++m) k3[m] = lds[threadIdx.x * K2 + m];
+    }
+#pragma unroll
+    for (int m = 0; m < K2; ++m) {
+      if (so.final_mode == 1) acc[m] += kk[m];
+      else if (so.final_mode == 2) acc[m] += 0.5 * (k1[m] + kk[m]);
+      else acc[m] += (k1[m] + 2. * k2[m] + 2. * k3[m] + kk[m]) / 6.;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < K2; ++m) lds[threadIdx.x * K2 + m] = acc[m];
+    tile_store<K2>(so.base, lds, e0, N);
+  }
+}
+
+// ==========================================================================================
+// Per-stage boundary values at time t (update_BCs, src/HO_2D.cpp:1979-2217).
+// ==========================================================================================
+__global__ void k_update_bcs(int n, BCDev B, double tc) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  B.psi[i] = B.psi_r[i] != 0. ? B.psi0[i] * tc : B.psi0[i];
+  B.nvel[i] = B.nvel_r[i] != 0. ? B.nvel0[i] * tc : B.nvel0[i];
+  B.pvel[i] = B.pvel_r[i] != 0. ? B.pvel0[i] * tc : B.pvel0[i];
+  B.diff[i] = B.diff_r[i] != 0. ? B.diff0[i] * tc : B.diff0[i];
+}
+
+// ==========================================================================================
+// Host launchers
+// ==========================================================================================
+#define HOC_K_SWITCH(KV, BODY)                                   \
+  switch (KV) {                                                  \
+    case 2: { constexpr int KK = 2; BODY; } break;               \
+    case 3: { constexpr int KK = 3; BODY; } break;               \
+    case 4: { constexpr int KK = 4; BODY; } break;               \
+    case 5: { constexpr int KK = 5; BODY; } break;               \
+    case 6: { constexpr int KK = 6; BODY; } break;               \
+    default: return hipErrorInvalidValue;                        \
+  }
+
+static inline int nblk(int N) { return (N + kWG - 1) / kWG; }
+
+hipError_t launch_trace(int K, int pre, int N, const Tab& T, double* x, const double* a, const double* b, double* d,
+                        double coef, const double* scal, int scal_idx, const int* active, double* tr, hipStream_t s,
+                        const int32_t* ev) {
+#define LT(P) hipLaunchKernelGGL((k_trace<KK, P>), dim3(nblk(N)), dim3(kWG), 0, s, N, T, x, a, b, d, coef, scal, scal_idx, active, tr, ev)
+  HOC_K_SWITCH(K, {
+    switch (pre) {
+      case PRE_COPY: LT(PRE_COPY); break;
+      case PRE_CHEB0: LT(PRE_CHEB0); break;
+      case PRE_PROL: LT(PRE_PROL); break;
+      case PRE_PUPD: LT(PRE_PUPD); break;
+      default: return hipErrorInvalidValue;
+    }
+  });
+#undef LT
+  return hipGetLastError();
+}
+
+hipError_t launch_lap(int K, int epi, int N, const Tab& T, const Geo& G, double* x, const double* tr, double* y,
+                      const double* r, const double* diag, double* d, double c1, double c2, double* part,
+                      const int* active, hipStream_t s) {
+#define LL(E)                                                                                                          \
+  do {                                                                                                               \
+    if (G.gel)                                                                                                       \
+      hipLaunchKernelGGL((k_lap<KK, E, true>), dim3(nblk(N)), dim3(kWG), 0, s, N, T, G, x, tr, y, r, diag, d, c1, c2,  \
+                         part, active);                                                                              \
+    else                                                                                                             \
+      hipLaunchKernelGGL((k_lap<KK, E, false>), dim3(nblk(N)), dim3(kWG), 0, s, N, T, G, x, tr, y, r, diag, d, c1, c2, \
+                         part, active);                                                                              \
+  } while (0)
+  HOC_K_SWITCH(K, {
+    switch (epi) {
+      case EPI_STORE: LL(EPI_STORE); break;
+      case EPI_RESID: LL(EPI_RESID); break;
+      case EPI_CHEB: LL(EPI_CHEB); break;
+      case EPI_CHEBF: LL(EPI_CHEBF); break;
+      case EPI_RESTRICT: LL(EPI_RESTRICT); break;
+      default: return hipErrorInvalidValue;
+    }
+  });
+#undef LL
+  return hipGetLastError();
+}
+
+hipError_t launch_rhs(int K, int N, const Tab& T, const Geo& G, const double* omega, const double* bcpsi, double* bs,
+                      double* part, hipStream_t s) {
+  HOC_K_SWITCH(K, hipLaunchKernelGGL((k_rhs<KK>), dim3(nblk(N)), dim3(kWG), 0, s, N, T, G, omega, bcpsi, bs, part));
+  return hipGetLastError();
+}
+
+hipError_t launch_velocity(int K, int N, const Tab& T, const Geo& G, const BCDev& B, const double* psi,
+                           const double* trpsi, const double* omega, double* vel, double* U, double* trw,
+                           double* tru, double inv_dt_re, hipStream_t s) {
+  HOC_K_SWITCH(K, hipLaunchKernelGGL((k_velocity<KK>), dim3(nblk(N)), dim3(kWG), 0, s, N, T, G, B, psi, trpsi, omega,
+                                     vel, U, trw, tru, inv_dt_re));
+  return hipGetLastError();
+}
+
+hipError_t launch_stage(int K, int adv, int N, const Tab& T, const Geo& G, const BCDev& B, const double* omega,
+                        const double* U, const double* trw, const double* tru, double re_inv, double dt,
+                        const StageOut& so, hipStream_t s) {
+  HOC_K_SWITCH(K, {
+    if (adv == 1)
+      hipLaunchKernelGGL((k_stage<KK, 1>), dim3(nblk(N)), dim3(kWG), 0, s, N, T, G, B, omega, U, trw, tru, re_inv, dt, so);
+    else
+      hipLaunchKernelGGL((k_stage<KK, 2>), dim3(nblk(N)), dim3(kWG), 0, s, N, T, G, B, omega, U, trw, tru, re_inv, dt, so);
+  });
+  return hipGetLastError();
+}
+
+hipError_t launch_update_bcs(int n, const BCDev& B, double tc, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_update_bcs, dim3((n + 255) / 256), dim3(256), 0, s, n, B, tc);
+  return hipGetLastError();
+}
+
+int lap_blocks(int N) { return nblk(N); }
+
+}  // namespace hoc

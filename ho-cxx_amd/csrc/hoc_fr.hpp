@@ -1,0 +1,38 @@
+// hoc_fr.hpp - launchers of the FR stage kernels (hoc_fr.hip).
+#pragma once
+
+#include "hoc_kernels.hpp"
+
+namespace hoc {
+
+enum { PRE_COPY = 0, PRE_CHEB0 = 1, PRE_PROL = 2, PRE_PUPD = 3 };
+enum { EPI_STORE = 0, EPI_RESID = 1, EPI_CHEB = 2, EPI_CHEBF = 3, EPI_RESTRICT = 4 };
+
+// Outputs of the stage kernel.
+struct StageOut {
+  double* k = nullptr;       // k_s = dt (adv + diff), or nullptr
+  double* next = nullptr;    // base + c k_s (next stage input), or nullptr
+  double c = 0.;
+  double* base = nullptr;    // omega_n (read for next; updated for final_mode)
+  int final_mode = 0;        // 0 none, 1 Euler, 2 RK2 (needs k1), 4 RK4 (needs k1..k3)
+  const double *k1 = nullptr, *k2 = nullptr, *k3 = nullptr;
+};
+
+hipError_t launch_trace(int K, int pre, int N, const Tab& T, double* x, const double* a, const double* b, double* d,
+                        double coef, const double* scal, int scal_idx, const int* active, double* tr, hipStream_t s,
+                        const int32_t* ev = nullptr);
+hipError_t launch_lap(int K, int epi, int N, const Tab& T, const Geo& G, double* x, const double* tr, double* y,
+                      const double* r, const double* diag, double* d, double c1, double c2, double* part,
+                      const int* active, hipStream_t s);
+hipError_t launch_rhs(int K, int N, const Tab& T, const Geo& G, const double* omega, const double* bcpsi, double* bs,
+                      double* part, hipStream_t s);
+hipError_t launch_velocity(int K, int N, const Tab& T, const Geo& G, const BCDev& B, const double* psi,
+                           const double* trpsi, const double* omega, double* vel, double* U, double* trw,
+                           double* tru, double inv_dt_re, hipStream_t s);
+hipError_t launch_stage(int K, int adv, int N, const Tab& T, const Geo& G, const BCDev& B, const double* omega,
+                        const double* U, const double* trw, const double* tru, double re_inv, double dt,
+                        const StageOut& so, hipStream_t s);
+hipError_t launch_update_bcs(int n, const BCDev& B, double tc, hipStream_t s);
+int lap_blocks(int N);
+
+}  // namespace hoc

@@ -1,0 +1,520 @@
+// hoc_host.cpp - host-side setup for the MI355X hot path (see hoc_host.hpp).
+#include "hoc_host.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <stdexcept>
+#include <unordered_map>
+
+namespace hoc {
+
+namespace {
+uint64_t pair_key(int a, int b) {
+  if (a > b) std::swap(a, b);
+  return ((uint64_t)(uint32_t)a << 32) | (uint32_t)b;
+}
+
+// Node ids of cell side s (south, east, north, west) in the element's CCW direction,
+// tensor order along the side (src/Mesh.cpp:372-377).
+void side_nodes(const Mesh& m, int el, int s, int* out) {
+  const int L = m.lnod;
+  const int32_t* en = &m.elem[(size_t)el * L * L];
+  out[0] = en[s];
+  out[L - 1] = en[(s + 1) % 4];
+  for (int q = 0; q < L - 2; ++q) out[1 + q] = en[4 + s * (L - 2) + q];
+}
+}  // namespace
+
+// Face connectivity with the reference's conventions (src/Mesh.cpp:342-439): an element
+// side shared with another element points at the lowest-numbered other element and its
+// first matching side; otherwise it must be one of the listed boundary edges, which then
+// records this element as its owner.  Edges are matched by their end-node pair.
+void connect(Mesh& m) {
+  const int L = m.lnod, N = m.n_elem;
+  std::unordered_map<uint64_t, std::vector<std::pair<int, int>>> sides;  // key -> (el, cell side)
+  sides.reserve((size_t)N * 3);
+  int tmp[4];
+  for (int el = 0; el < N; ++el)
+    for (int s = 0; s < 4; ++s) {
+      side_nodes(m, el, s, tmp);
+      sides[pair_key(tmp[0], tmp[L - 1])].push_back({el, s});
+    }
+  std::unordered_map<uint64_t, int> bkey;
+  for (int e = 0; e < m.n_bedges; ++e) {
+    uint64_t k = pair_key(m.edges[e][0], m.edges[e][1]);  // Gmsh line: two end nodes first
+    if (!bkey.count(k)) bkey[k] = e;
+  }
+  m.nb.assign((size_t)N * 4, 0);
+  m.nb_face.assign((size_t)N * 4, -1);
+  m.be_elem.assign(m.n_bedges, -1);
+  m.be_ijp.assign(m.n_bedges, -1);
+  for (int el = 0; el < N; ++el)
+    for (int s = 0; s < 4; ++s) {
+      side_nodes(m, el, s, tmp);
+      const uint64_t k = pair_key(tmp[0], tmp[L - 1]);
+      const auto& lst = sides[k];
+      int best = -1, bside = -1;
+      for (const auto& pr : lst) {
+        if (pr.first == el) continue;
+        if (best < 0 || pr.first < best || (pr.first == best && pr.second < bside)) { best = pr.first; bside = pr.second; }
+      }
+      const int ijp = kF2I[s];
+      if (best >= 0) {
+        m.nb[(size_t)el * 4 + ijp] = best;
+        m.nb_face[(size_t)el * 4 + ijp] = (int8_t)kF2I[bside];
+      } else {
+        auto it = bkey.find(k);
+        if (it == bkey.end())
+          throw std::runtime_error("mesh: element side without neighbour is not a listed boundary edge");
+        const int e = it->second;
+        m.nb[(size_t)el * 4 + ijp] = -1 - e;
+        m.be_elem[e] = el;
+        m.be_ijp[e] = (int8_t)ijp;
+      }
+    }
+  for (int e = 0; e < m.n_bedges; ++e)
+    if (m.be_elem[e] < 0) throw std::runtime_error("mesh: a listed boundary edge belongs to no element side");
+  // boundary edges take their owner's orientation (src/Mesh.cpp:385)
+  for (int e = 0; e < m.n_bedges; ++e) {
+    side_nodes(m, m.be_elem[e], kI2F[m.be_ijp[e]], tmp);
+    m.edges[e].assign(tmp, tmp + L);
+  }
+}
+
+int color_elements(const Mesh& m, int distance, std::vector<int32_t>& color) {
+  const int N = m.n_elem;
+  color.assign(N, -1);
+  int ncol = 0;
+  std::vector<int> mark(64, -1);
+  for (int el = 0; el < N; ++el) {
+    std::vector<int> forbid;
+    auto add_nbrs = [&](int e, int depth) {
+      for (int f = 0; f < 4; ++f) {
+        int n = m.nb[(size_t)e * 4 + f];
+        if (n < 0) continue;
+        if (color[n] >= 0) forbid.push_back(color[n]);
+        if (depth > 1)
+          for (int f2 = 0; f2 < 4; ++f2) {
+            int n2 = m.nb[(size_t)n * 4 + f2];
+            if (n2 >= 0 && n2 != el && color[n2] >= 0) forbid.push_back(color[n2]);
+          }
+      }
+    };
+    add_nbrs(el, distance);
+    int c = 0;
+    std::sort(forbid.begin(), forbid.end());
+    forbid.erase(std::unique(forbid.begin(), forbid.end()), forbid.end());
+    for (int f : forbid) { if (f == c) ++c; else if (f > c) break; }
+    color[el] = c;
+    ncol = std::max(ncol, c + 1);
+  }
+  return ncol;
+}
+
+namespace {
+double legendre(int n, double x, double* dp) {
+  double p0 = 1., p1 = x;
+  if (n == 0) { if (dp) *dp = 0.; return 1.; }
+  for (int k = 2; k <= n; ++k) { double p2 = ((2. * k - 1.) * x * p1 - (k - 1.) * p0) / k; p0 = p1; p1 = p2; }
+  if (dp) *dp = n * (x * p1 - p0) / (x * x - 1.);
+  return p1;
+}
+}  // namespace
+
+// Gauss-Legendre points (closed forms for K <= 5 as in src/HO_2D.cpp:488-521, Newton
+// iteration beyond), Lagrange tables at the sps and at xi = -1/+1, g_DG Radau correction
+// tables (src/HO_2D.cpp:642-671) and the geometry (equispaced, src/HO_2D.cpp:527-541) tables.
+void build_tables(int K, int lnod, Tables& t) {
+  if (K < 2 || K > kMaxK) throw std::runtime_error("K out of range");
+  if (lnod < 2 || lnod > 4) throw std::runtime_error("lnod out of range");
+  t.K = K; t.L = lnod;
+  double* x = t.sps; double* w = t.wgt;
+  if (K == 2) { x[1] = 1. / std::sqrt(3.); x[0] = -x[1]; w[0] = w[1] = 1.; }
+  else if (K == 3) { x[2] = std::sqrt(0.6); x[1] = 0.; x[0] = -x[2]; w[0] = w[2] = 5. / 9.; w[1] = 8. / 9.; }
+  else if (K == 4) {
+    x[3] = std::sqrt((15. + 2. * std::sqrt(30.)) / 35.); x[2] = std::sqrt((15. - 2. * std::sqrt(30.)) / 35.);
+    x[1] = -x[2]; x[0] = -x[3];
+    w[0] = w[3] = (18. - std::sqrt(30.)) / 36.; w[1] = w[2] = (18. + std::sqrt(30.)) / 36.;
+  } else if (K == 5) {
+    x[4] = std::sqrt(5. + 2. * std::sqrt(70.) / 7.) / 3.; x[3] = std::sqrt(5. - 2. * std::sqrt(70.) / 7.) / 3.;
+    x[2] = 0.; x[1] = -x[3]; x[0] = -x[4];
+    w[0] = w[4] = (322. - 13. * std::sqrt(70.)) / 900.; w[1] = w[3] = (322. + 13. * std::sqrt(70.)) / 900.; w[2] = 128. / 225.;
+  } else {
+    for (int i = 0; i < (K + 1) / 2; ++i) {
+      double z = std::cos(M_PI * (i + 0.75) / (K + 0.5)), dp = 0.;
+      for (int it = 0; it < 100; ++it) {
+        double p = legendre(K, z, &dp), dz = p / dp;
+        z -= dz;
+        if (std::fabs(dz) < 1e-16) break;
+      }
+      legendre(K, z, &dp);
+      x[K - 1 - i] = z; x[i] = -z;
+      w[K - 1 - i] = w[i] = 2. / ((1. - z * z) * dp * dp);
+    }
+    if (K % 2) x[K / 2] = 0.;
+  }
+  // Lagrange basis of the sps: values/derivatives at -1, +1 and derivative matrix
+  auto lag = [&](const double* nodes, int n, int k, double xx) {
+    double num = 1., den = 1.;
+    for (int j = 0; j < n; ++j) if (j != k) { num *= xx - nodes[j]; den *= nodes[k] - nodes[j]; }
+    return num / den;
+  };
+  auto dlag = [&](const double* nodes, int n, int k, double xx) {
+    double s = 0., den = 1.;
+    for (int j = 0; j < n; ++j) {
+      if (j == k) continue;
+      den *= nodes[k] - nodes[j];
+      double p = 1.;
+      for (int i = 0; i < n; ++i) if (i != k && i != j) p *= xx - nodes[i];
+      s += p;
+    }
+    return s / den;
+  };
+  for (int k = 0; k < K; ++k) {
+    t.bb[k][0] = lag(x, K, k, -1.); t.bb[k][1] = lag(x, K, k, 1.);
+    t.bgb[k][0] = dlag(x, K, k, -1.); t.bgb[k][1] = dlag(x, K, k, 1.);
+    for (int m = 0; m < K; ++m) t.D[k][m] = dlag(x, K, k, x[m]);
+  }
+  // g_DG = Radau of degree K: R(x) = (-1)^K/2 (P_K - P_{K-1}) is the left one
+  const double c = 0.5 * ((K % 2 == 0) ? 1. : -1.), cd = 0.5 * K * ((K % 2 == 0) ? 1. : -1.);
+  for (int i = 0; i < K; ++i) {
+    const double s = x[i];
+    t.rad[i][0] = c * (legendre(K, s, nullptr) - legendre(K - 1, s, nullptr));
+    t.rad[i][1] = c * (legendre(K, -s, nullptr) - legendre(K - 1, -s, nullptr));
+    t.grad[i][0] = cd * (legendre(K, s, nullptr) + legendre(K - 1, s, nullptr)) / (1. + s);
+    t.grad[i][1] = -cd * (legendre(K, -s, nullptr) + legendre(K - 1, -s, nullptr)) / (1. - s);
+  }
+  t.gprime[0] = -0.5 * K * (K + 1.);
+  t.gprime[1] = 0.5 * K * (K + 1.);
+  const int L = lnod;
+  for (int l = 0; l < L; ++l) t.gps[l] = -1. + 2. * l / (L - 1);
+  if (L == 4) { t.gps[1] = -1. / 3.; t.gps[2] = 1. / 3.; }
+  for (int l = 0; l < L; ++l) {
+    t.gbb[l][0] = lag(t.gps, L, l, -1.); t.gbb[l][1] = lag(t.gps, L, l, 1.);
+    t.gbgb[l][0] = dlag(t.gps, L, l, -1.); t.gbgb[l][1] = dlag(t.gps, L, l, 1.);
+    for (int k = 0; k < K; ++k) { t.gsb[l][k] = lag(t.gps, L, l, x[k]); t.gsgb[l][k] = dlag(t.gps, L, l, x[k]); }
+  }
+}
+
+namespace {
+// Gmsh quad node slot of tensor position (i along xi, j along eta) (src/Mesh.cpp:34-91).
+int t2g(int L, int i, int j) {
+  static const int m2[2][2] = {{0, 1}, {3, 2}};
+  static const int m3[3][3] = {{0, 4, 1}, {7, 8, 5}, {3, 6, 2}};
+  static const int m4[4][4] = {{0, 4, 5, 1}, {11, 12, 13, 6}, {10, 15, 14, 7}, {3, 9, 8, 2}};
+  return L == 2 ? m2[j][i] : (L == 3 ? m3[j][i] : m4[j][i]);
+}
+}  // namespace
+
+// Volume metrics at the sps and face metrics at the flux points (src/HO_2D.cpp:1240-1388).
+void build_geometry(const Mesh& m, const Tables& t, Geometry& g) {
+  const int K = t.K, L = t.L, N = m.n_elem, K2 = K * K;
+  auto rs = [&](std::vector<double>& v, size_t n) { v.assign(n, 0.); };
+  rs(g.xxi, (size_t)N * K2); rs(g.xeta, (size_t)N * K2); rs(g.yxi, (size_t)N * K2); rs(g.yeta, (size_t)N * K2); rs(g.jac, (size_t)N * K2);
+  for (auto* v : {&g.fxxi, &g.fxeta, &g.fyxi, &g.fyeta, &g.fjac, &g.fA, &g.fB, &g.fAn}) rs(*v, (size_t)N * 4 * K);
+  double X[4][4], Y[4][4];
+  for (int el = 0; el < N; ++el) {
+    for (int j = 0; j < L; ++j)
+      for (int i = 0; i < L; ++i) {
+        int nd = m.elem[(size_t)el * L * L + t2g(L, i, j)];
+        X[j][i] = m.xy[2 * (size_t)nd]; Y[j][i] = m.xy[2 * (size_t)nd + 1];
+      }
+    for (int j = 0; j < K; ++j)
+      for (int i = 0; i < K; ++i) {
+        double a = 0., b = 0., c = 0., d = 0.;
+        for (int n = 0; n < L; ++n)
+          for (int mm = 0; mm < L; ++mm) {
+            a += t.gsgb[mm][i] * t.gsb[n][j] * X[n][mm];
+            c += t.gsgb[mm][i] * t.gsb[n][j] * Y[n][mm];
+            b += t.gsgb[n][j] * t.gsb[mm][i] * X[n][mm];
+            d += t.gsgb[n][j] * t.gsb[mm][i] * Y[n][mm];
+          }
+        size_t v = (size_t)el * K2 + j * K + i;
+        g.xxi[v] = a; g.xeta[v] = b; g.yxi[v] = c; g.yeta[v] = d;
+        g.jac[v] = a * d - b * c;
+      }
+    for (int k = 0; k < K; ++k)
+      for (int f = 0; f < 4; ++f) {
+        double xx = 0., xe = 0., yx = 0., ye = 0.;
+        for (int j = 0; j < L; ++j)
+          for (int i = 0; i < L; ++i) {
+            if (f < 2) {
+              xx += t.gbgb[i][f] * t.gsb[j][k] * X[j][i]; yx += t.gbgb[i][f] * t.gsb[j][k] * Y[j][i];
+              xe += t.gbb[i][f] * t.gsgb[j][k] * X[j][i]; ye += t.gbb[i][f] * t.gsgb[j][k] * Y[j][i];
+            } else {
+              xx += t.gbb[j][f - 2] * t.gsgb[i][k] * X[j][i]; yx += t.gbb[j][f - 2] * t.gsgb[i][k] * Y[j][i];
+              xe += t.gbgb[j][f - 2] * t.gsb[i][k] * X[j][i]; ye += t.gbgb[j][f - 2] * t.gsb[i][k] * Y[j][i];
+            }
+          }
+        size_t q = ((size_t)el * 4 + f) * K + k;
+        g.fxxi[q] = xx; g.fxeta[q] = xe; g.fyxi[q] = yx; g.fyeta[q] = ye;
+        g.fjac[q] = xx * ye - xe * yx;
+        g.fA[q] = f < 2 ? xe * xe + ye * ye : xx * xx + yx * yx;
+        g.fB[q] = xx * xe + yx * ye;
+        g.fAn[q] = std::sqrt(g.fA[q]);
+      }
+  }
+}
+
+void solution_points(const Mesh& m, const Tables& t, std::vector<double>& xy) {
+  const int K = t.K, L = t.L;
+  xy.assign((size_t)m.n_elem * K * K * 2, 0.);
+  for (int el = 0; el < m.n_elem; ++el)
+    for (int j = 0; j < K; ++j)
+      for (int k = 0; k < K; ++k) {
+        double x = 0., y = 0.;
+        for (int l = 0; l < L; ++l)
+          for (int q = 0; q < L; ++q) {
+            int nd = m.elem[(size_t)el * L * L + t2g(L, q, l)];
+            x += t.gsb[l][j] * t.gsb[q][k] * m.xy[2 * (size_t)nd];
+            y += t.gsb[l][j] * t.gsb[q][k] * m.xy[2 * (size_t)nd + 1];
+          }
+        size_t v = ((size_t)el * K * K + j * K + k) * 2;
+        xy[v] = x; xy[v + 1] = y;
+      }
+}
+
+void build_bcs(const Mesh& m, const Tables& t, int problem_type, BCs& bc) {
+  const int K = t.K, E = m.n_bedges, L = t.L;
+  bc.sw_psi.assign(E, 0); bc.sw_diff.assign(E, 0); bc.no_slip_edge.assign(E, 0);
+  for (auto* v : {&bc.psi, &bc.psi_ramp, &bc.diff, &bc.diff_ramp, &bc.nvel, &bc.nvel_ramp, &bc.pvel, &bc.pvel_ramp})
+    v->assign((size_t)E * K, 0.);
+  bc.edge_sign.assign(E, 0.);
+  for (int e = 0; e < E; ++e) bc.edge_sign[e] = 2. * (m.be_ijp[e] % 2) - 1.;
+  const int NB = (int)m.bnames.size();
+  std::vector<std::vector<int>> bedges(NB);
+  for (int e = 0; e < E; ++e) bedges[m.bedge_bdry[e]].push_back(e);
+  auto edge_y = [&](int e, int k) {  // y at flux point k of the (reoriented) edge (src/HO_2D.cpp:752)
+    double y = 0.;
+    for (int j = 0; j < L; ++j) y += t.gsb[j][k] * m.xy[2 * (size_t)m.edges[e][j] + 1];
+    return y;
+  };
+  // inlet extent (src/HO_2D.cpp:730-741)
+  double y_min = 0., y_max = 0.;
+  for (int b = 0; b < NB; ++b) {
+    if (m.bnames[b] != "inlet" || bedges[b].empty()) continue;
+    const auto& e0 = m.edges[bedges[b][0]];
+    y_min = y_max = m.xy[2 * (size_t)e0[0] + 1];
+    for (int e : bedges[b])
+      for (int end : {0, L - 1}) {
+        double y = m.xy[2 * (size_t)m.edges[e][end] + 1];
+        y_min = std::min(y_min, y); y_max = std::max(y_max, y);
+      }
+  }
+  auto set = [&](std::vector<double>& v, std::vector<double>& r, int e, int k, double val, bool ramp) {
+    v[(size_t)e * K + k] = val; r[(size_t)e * K + k] = ramp ? 1. : 0.;
+  };
+  const double h2 = (y_max - y_min) * (y_max - y_min);
+  if (problem_type == 1) {  // src/HO_2D.cpp:700-726
+    for (int e = 0; e < E; ++e) { bc.sw_psi[e] = 0; bc.sw_diff[e] = 1; bc.no_slip_edge[e] = 1; }
+    for (int b = 0; b < NB; ++b)
+      if (m.bnames[b] == "top")
+        for (int e : bedges[b]) for (int k = 0; k < K; ++k) set(bc.pvel, bc.pvel_ramp, e, k, -1., false);
+  } else if (problem_type == 2) {  // src/HO_2D.cpp:728-807, ramps 1999-2046
+    bc.time_dependent = true;
+    for (int b = 0; b < NB; ++b) {
+      const std::string& nm = m.bnames[b];
+      for (int e : bedges[b]) {
+        if (nm == "inlet") {
+          bc.sw_psi[e] = 0; bc.sw_diff[e] = 0;
+          for (int k = 0; k < K; ++k) {
+            double y = edge_y(e, k);
+            set(bc.psi, bc.psi_ramp, e, k, (y - y_min) * (y - y_min) * (3. * y_max - y_min - 2. * y) / h2, true);
+            set(bc.diff, bc.diff_ramp, e, k, 6. * (2. * y - y_min - y_max) / h2, true);
+            set(bc.nvel, bc.nvel_ramp, e, k, 6. * (y - y_min) * (y - y_max) / h2 * bc.edge_sign[e], true);
+          }
+        } else if (nm == "outlet") {
+          bc.sw_psi[e] = 1; bc.sw_diff[e] = 1;
+        } else if (nm == "bottom") {
+          bc.sw_psi[e] = 0; bc.sw_diff[e] = 1; bc.no_slip_edge[e] = 1;
+        } else if (nm == "top") {
+          bc.sw_psi[e] = 0; bc.sw_diff[e] = 1; bc.no_slip_edge[e] = 1;
+          for (int k = 0; k < K; ++k) set(bc.psi, bc.psi_ramp, e, k, y_max - y_min, true);
+        }
+      }
+    }
+  } else if (problem_type == 3) {  // src/HO_2D.cpp:809-891, ramps 2048-2102
+    bc.time_dependent = true;
+    for (int b = 0; b < NB; ++b) {
+      const std::string& nm = m.bnames[b];
+      for (int e : bedges[b]) {
+        if (nm == "inlet") {
+          bc.sw_psi[e] = 0; bc.sw_diff[e] = 0;
+          for (int k = 0; k < K; ++k) {
+            set(bc.psi, bc.psi_ramp, e, k, edge_y(e, k) - y_min, true);
+            set(bc.nvel, bc.nvel_ramp, e, k, -1. * bc.edge_sign[e], true);
+          }
+        } else if (nm == "outlet") {
+          bc.sw_psi[e] = 1; bc.sw_diff[e] = 1;
+        } else if (nm == "top") {
+          bc.sw_psi[e] = 0; bc.sw_diff[e] = 1;
+          for (int k = 0; k < K; ++k) { set(bc.psi, bc.psi_ramp, e, k, y_max - y_min, true); set(bc.pvel, bc.pvel_ramp, e, k, -1., true); }
+        } else if (nm == "bottom") {
+          bc.sw_psi[e] = 0; bc.sw_diff[e] = 1;
+          for (int k = 0; k < K; ++k) set(bc.pvel, bc.pvel_ramp, e, k, 1., true);
+        } else if (nm == "object") {
+          bc.sw_psi[e] = 0; bc.sw_diff[e] = 1; bc.no_slip_edge[e] = 1;
+          for (int k = 0; k < K; ++k) set(bc.psi, bc.psi_ramp, e, k, 0.5 * (y_max - y_min), true);
+        }
+      }
+    }
+  } else {
+    throw std::runtime_error("problem_type must be 1 (cavity), 2 (backward-facing step) or 3 (cylinder)");
+  }
+}
+
+// ------------------------------------------------------------------------------------------
+// Smoothed aggregation (Vanek-Mandel-Brezina) with the parameters AMGCL uses by default:
+// strength eps = 0.08 halved per level, prolongator smoothing omega = 2/3 on the filtered
+// matrix, spai0 relaxation, Galerkin coarse operators.
+// ------------------------------------------------------------------------------------------
+namespace {
+CSR transpose(const CSR& A) {
+  CSR T;
+  T.n = A.m; T.m = A.n;
+  T.ptr.assign(T.n + 1, 0);
+  for (int c : A.col) T.ptr[c + 1]++;
+  for (int i = 0; i < T.n; ++i) T.ptr[i + 1] += T.ptr[i];
+  T.col.resize(A.col.size()); T.val.resize(A.val.size());
+  std::vector<int> pos(T.ptr.begin(), T.ptr.end() - 1);
+  for (int i = 0; i < A.n; ++i)
+    for (int j = A.ptr[i]; j < A.ptr[i + 1]; ++j) { int p = pos[A.col[j]]++; T.col[p] = i; T.val[p] = A.val[j]; }
+  return T;
+}
+CSR matmul(const CSR& A, const CSR& B) {
+  CSR C;
+  C.n = A.n; C.m = B.m;
+  C.ptr.assign(C.n + 1, 0);
+  std::vector<int> mark(B.m, -1);
+  std::vector<double> acc(B.m, 0.);
+  std::vector<int> cols;
+  for (int i = 0; i < A.n; ++i) {
+    cols.clear();
+    for (int ja = A.ptr[i]; ja < A.ptr[i + 1]; ++ja) {
+      const int k = A.col[ja]; const double va = A.val[ja];
+      for (int jb = B.ptr[k]; jb < B.ptr[k + 1]; ++jb) {
+        const int c = B.col[jb];
+        if (mark[c] != i) { mark[c] = i; acc[c] = 0.; cols.push_back(c); }
+        acc[c] += va * B.val[jb];
+      }
+    }
+    std::sort(cols.begin(), cols.end());
+    for (int c : cols) { C.col.push_back(c); C.val.push_back(acc[c]); }
+    C.ptr[i + 1] = (int)C.col.size();
+  }
+  return C;
+}
+}  // namespace
+
+void build_sa_amg(const CSR& A0, int coarse_enough, AMGHost& h) {
+  h.levels.clear();
+  CSR A = A0;
+  double eps = 0.08;
+  while (A.n > coarse_enough) {
+    AMGLevelHost lv;
+    const int n = A.n;
+    std::vector<double> dia(n, 0.);
+    lv.Minv.assign(n, 0.);
+    for (int i = 0; i < n; ++i) {
+      double s2 = 0.;
+      for (int j = A.ptr[i]; j < A.ptr[i + 1]; ++j) {
+        if (A.col[j] == i) dia[i] += A.val[j];
+        s2 += A.val[j] * A.val[j];
+      }
+      lv.Minv[i] = s2 > 0. ? dia[i] / s2 : 0.;
+    }
+    std::vector<uint8_t> strong(A.col.size(), 0);
+    for (int i = 0; i < n; ++i)
+      for (int j = A.ptr[i]; j < A.ptr[i + 1]; ++j) {
+        const int c = A.col[j];
+        strong[j] = c != i && eps * eps * dia[i] * dia[c] < A.val[j] * A.val[j];
+      }
+    const int UNDEF = -1, REMOVED = -2;
+    std::vector<int> agg(n, UNDEF);
+    for (int i = 0; i < n; ++i) {
+      bool any = false;
+      for (int j = A.ptr[i]; j < A.ptr[i + 1] && !any; ++j) any = strong[j];
+      if (!any) agg[i] = REMOVED;
+    }
+    int na = 0;
+    std::vector<int> nbrs;
+    for (int i = 0; i < n; ++i) {
+      if (agg[i] != UNDEF) continue;
+      const int id = na++;
+      agg[i] = id;
+      nbrs.clear();
+      for (int j = A.ptr[i]; j < A.ptr[i + 1]; ++j)
+        if (strong[j] && agg[A.col[j]] != REMOVED) { agg[A.col[j]] = id; nbrs.push_back(A.col[j]); }
+      for (int c : nbrs)
+        for (int j = A.ptr[c]; j < A.ptr[c + 1]; ++j)
+          if (strong[j] && agg[A.col[j]] == UNDEF) agg[A.col[j]] = id;
+    }
+    if (na == 0) break;
+    std::vector<int> cnt(na, 0), rn(na, -1);
+    for (int i = 0; i < n; ++i) if (agg[i] >= 0) cnt[agg[i]]++;
+    int nc = 0;
+    for (int a = 0; a < na; ++a) if (cnt[a]) rn[a] = nc++;
+    for (int i = 0; i < n; ++i) if (agg[i] >= 0) agg[i] = rn[agg[i]];
+    if (nc >= n) break;
+    eps *= 0.5;
+    const double omega = 2. / 3.;
+    CSR P;
+    P.n = n; P.m = nc;
+    P.ptr.assign(n + 1, 0);
+    std::vector<int> mark(nc, -1);
+    std::vector<double> acc(nc, 0.);
+    std::vector<int> cols;
+    for (int i = 0; i < n; ++i) {
+      double df = 0.;
+      for (int j = A.ptr[i]; j < A.ptr[i + 1]; ++j) if (A.col[j] == i || !strong[j]) df += A.val[j];
+      const double s = df != 0. ? -omega / df : 0.;
+      cols.clear();
+      for (int j = A.ptr[i]; j < A.ptr[i + 1]; ++j) {
+        const int c = A.col[j];
+        if (c != i && !strong[j]) continue;
+        const int g = agg[c];
+        if (g < 0) continue;
+        const double v = c == i ? 1. - omega : s * A.val[j];
+        if (mark[g] != i) { mark[g] = i; acc[g] = 0.; cols.push_back(g); }
+        acc[g] += v;
+      }
+      std::sort(cols.begin(), cols.end());
+      for (int g : cols) { P.col.push_back(g); P.val.push_back(acc[g]); }
+      P.ptr[i + 1] = (int)P.col.size();
+    }
+    lv.A = A;
+    lv.P = P;
+    lv.R = transpose(P);
+    CSR Ac = matmul(lv.R, matmul(A, P));
+    h.levels.push_back(std::move(lv));
+    A = std::move(Ac);
+  }
+  AMGLevelHost last;
+  last.A = A;
+  h.levels.push_back(last);
+  // dense inverse of the coarsest matrix (Gauss-Jordan, partial pivoting)
+  const int n = A.n;
+  std::vector<double> a((size_t)n * n, 0.), inv((size_t)n * n, 0.);
+  for (int i = 0; i < n; ++i) {
+    inv[(size_t)i * n + i] = 1.;
+    for (int j = A.ptr[i]; j < A.ptr[i + 1]; ++j) a[(size_t)i * n + A.col[j]] += A.val[j];
+  }
+  for (int c = 0; c < n; ++c) {
+    int p = c;
+    for (int r = c + 1; r < n; ++r) if (std::fabs(a[(size_t)r * n + c]) > std::fabs(a[(size_t)p * n + c])) p = r;
+    if (a[(size_t)p * n + c] == 0.) continue;  // singular direction: leave it out of the inverse
+    if (p != c)
+      for (int j = 0; j < n; ++j) { std::swap(a[(size_t)p * n + j], a[(size_t)c * n + j]); std::swap(inv[(size_t)p * n + j], inv[(size_t)c * n + j]); }
+    const double d = 1. / a[(size_t)c * n + c];
+    for (int j = 0; j < n; ++j) { a[(size_t)c * n + j] *= d; inv[(size_t)c * n + j] *= d; }
+    for (int r = 0; r < n; ++r) {
+      if (r == c) continue;
+      const double f = a[(size_t)r * n + c];
+      if (f == 0.) continue;
+      for (int j = 0; j < n; ++j) { a[(size_t)r * n + j] -= f * a[(size_t)c * n + j]; inv[(size_t)r * n + j] -= f * inv[(size_t)c * n + j]; }
+    }
+  }
+  h.coarse_inv = std::move(inv);
+}
+
+}  // namespace hoc

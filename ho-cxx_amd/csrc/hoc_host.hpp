@@ -1,0 +1,102 @@
+// hoc_host.hpp - host-side setup of the HO-CXX hot path on MI355X.
+//
+// Everything the GPU stage kernels read is produced here once per mesh:
+// connectivity in face-local form, 1-D FR tables, volume/face metrics, the
+// per-boundary-edge BC programme of problem types 1-3, element colourings for
+// operator probing, and an SA-AMG hierarchy for the element-level Poisson matrix.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace hoc {
+
+constexpr int kMaxK = 8;
+
+// Faces are numbered by "ijp" as in the reference (src/HO_2D.cpp:2303-2336):
+// 0 = west (xi=-1), 1 = east (xi=+1), 2 = south (eta=-1), 3 = north (eta=+1).
+// Cell sides (south, east, north, west) = (0,1,2,3) (src/Mesh.hpp:114-119).
+constexpr int kI2F[4] = {3, 1, 0, 2};   // ijp -> cell side   (src/MathHelpers.hpp:23)
+constexpr int kF2I[4] = {2, 1, 3, 0};   // cell side -> ijp   (src/MathHelpers.hpp:24)
+
+struct Mesh {
+  int lnod = 2;
+  int n_nodes = 0, n_elem = 0, n_bedges = 0;
+  std::vector<double> xy;                 // interleaved
+  std::vector<int32_t> elem;              // n_elem * lnod^2 (Gmsh order)
+  std::vector<std::vector<int32_t>> edges;  // first n_bedges are the listed boundary edges
+  std::vector<int32_t> bedge_bdry;
+  std::vector<std::string> bnames;
+  // connectivity by ijp face (filled by connect())
+  std::vector<int32_t> nb;      // n_elem*4: neighbour element, or -1-edge for a boundary face
+  std::vector<int8_t> nb_face;  // n_elem*4: neighbour's ijp of the shared face (or -1)
+  std::vector<int32_t> be_elem; // per boundary edge: owner element
+  std::vector<int8_t> be_ijp;   // per boundary edge: owner's face ijp
+};
+
+// Throws std::runtime_error on a non-conforming mesh.
+void connect(Mesh& m);
+
+// Greedy element colourings: distance 1 (face neighbours differ) and distance 2
+// (additionally no element has two face neighbours of one colour).
+int color_elements(const Mesh& m, int distance, std::vector<int32_t>& color);
+
+struct Tables {
+  int K = 0, L = 0;
+  double sps[kMaxK], wgt[kMaxK], gps[4];
+  double bb[kMaxK][2];        // Lagrange basis at xi=-1 / +1
+  double bgb[kMaxK][2];       // its derivative at xi=-1 / +1
+  double D[kMaxK][kMaxK];     // D[m][k] = l_m'(sps_k)
+  double rad[kMaxK][2];       // left/right Radau (g_DG) correction values at sps
+  double grad[kMaxK][2];      // their derivatives
+  double gprime[2];           // g'(-1), g'(+1)
+  double gsb[4][kMaxK];       // geometry basis at sps  [l][k]
+  double gsgb[4][kMaxK];      // its derivative         [l][k]
+  double gbb[4][2], gbgb[4][2];
+};
+void build_tables(int K, int lnod, Tables& t);
+
+struct Geometry {
+  // volume, [el][j][i]
+  std::vector<double> xxi, xeta, yxi, yeta, jac;
+  // faces, [el][ijp][k]
+  std::vector<double> fxxi, fxeta, fyxi, fyeta, fjac, fA, fB, fAn;
+};
+void build_geometry(const Mesh& m, const Tables& t, Geometry& g);
+// Solution-point coordinates (src/HO_2D.cpp:3946-3980), interleaved.
+void solution_points(const Mesh& m, const Tables& t, std::vector<double>& xy);
+
+// Boundary-condition programme of src/HO_2D.cpp:675-1238 and 1979-2217 for problem
+// types 1-3.  Per-edge arrays of K values (edge flux-point order).  The values valid at
+// time t are base * (ramp ? 2|sin(pi t/40)| : 1), with the outward->local sign folded in
+// where update_BCs applies it.
+struct BCs {
+  std::vector<uint8_t> sw_psi, sw_diff;        // 0 Dirichlet, 1 Neumann
+  std::vector<uint8_t> no_slip_edge;           // edge belongs to a no-slip boundary
+  std::vector<double> psi, psi_ramp;           // BC_Poisson base, ramp flag (0/1)
+  std::vector<double> diff, diff_ramp;         // BC_diffusion base (Dirichlet ones), ramp
+  std::vector<double> nvel, nvel_ramp;         // BC_normal_vel base (sign folded), ramp
+  std::vector<double> pvel, pvel_ramp;         // BC_parl_vel base, ramp
+  std::vector<double> edge_sign;               // 2*(ijp%2)-1 of the owner face
+  bool time_dependent = false;
+};
+void build_bcs(const Mesh& m, const Tables& t, int problem_type, BCs& bc);
+
+// Scalar CSR and smoothed-aggregation AMG hierarchy (host build, device apply).
+struct CSR {
+  int n = 0, m = 0;
+  std::vector<int32_t> ptr, col;
+  std::vector<double> val;
+};
+struct AMGLevelHost {
+  CSR A, P, R;
+  std::vector<double> Minv;   // spai0 weights
+};
+struct AMGHost {
+  std::vector<AMGLevelHost> levels;  // last level: coarsest (A only)
+  std::vector<double> coarse_inv;    // dense inverse of the coarsest matrix (row-major)
+};
+void build_sa_amg(const CSR& A, int coarse_enough, AMGHost& h);
+
+}  // namespace hoc

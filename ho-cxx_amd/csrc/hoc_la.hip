@@ -1,0 +1,301 @@
+// hoc_la.hip - vector, reduction, CSR and probing kernels of the Poisson solver.
+#include "hoc_la.hpp"
+#include "hoc_kernels.hpp"
+
+namespace hoc {
+
+__device__ __forceinline__ double wsum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Block (256 threads) sum of two values; result valid in thread 0.
+__device__ __forceinline__ void block_sum2(double& a, double& b) {
+  __shared__ double sa[4], sb[4];
+  a = wsum(a); b = wsum(b);
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  if (l == 0) { sa[w] = a; sb[w] = b; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    a = sa[0] + sa[1] + sa[2] + sa[3];
+    b = sb[0] + sb[1] + sb[2] + sb[3];
+  }
+}
+
+// Weight w_j w_i of flat dof index v (element-major, K*K per element).
+__device__ __forceinline__ double dof_weight(const VecArgs& A, size_t v) {
+  const int m = (int)(v % (size_t)(A.K * A.K));
+  return A.w[m / A.K] * A.w[m % A.K];
+}
+
+// Elementwise Krylov updates with fused partial norms.  Scalars come from S.
+__global__ void k_vec(int op, VecArgs A, size_t n, double* __restrict__ a, double* __restrict__ b,
+                      const double* __restrict__ c, const double* __restrict__ d, double* __restrict__ e,
+                      const double* __restrict__ S, const int* __restrict__ F, double* __restrict__ part) {
+  if (F && !F[F_ACTIVE]) return;
+  if (op >= VOP_BICG_X && F && F[F_HALF]) return;
+  double p0 = 0., p1 = 0.;
+  for (size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (size_t)gridDim.x * blockDim.x) {
+    switch (op) {
+      case VOP_PCG_UPD: {  // x += alpha p; r -= alpha q      a=x b=r c=p d=q
+        const double al = S[S_ALPHA];
+        a[v] += al * c[v];
+        const double r = b[v] - al * d[v];
+        b[v] = r;
+        const double ru = r / dof_weight(A, v);
+        p0 += r * r; p1 += ru * ru;
+      } break;
+      case VOP_BICG_P: {   // p = r + beta (p - omega v)      a=p c=r d=v
+        a[v] = c[v] + S[S_BETA] * (a[v] - S[S_OMEGA] * d[v]);
+      } break;
+      case VOP_BICG_S: {   // s = r - alpha v (in r)          b=r d=v
+        const double s = b[v] - S[S_ALPHA] * d[v];
+        b[v] = s;
+        const double su = s / dof_weight(A, v);
+        p0 += s * s; p1 += su * su;
+      } break;
+      case VOP_BICG_X: {   // x += alpha ph + omega sh; r = s - omega t   a=x b=r(s) c=ph d=sh e=t
+        a[v] += S[S_ALPHA] * c[v] + S[S_OMEGA] * d[v];
+        const double r = b[v] - S[S_OMEGA] * e[v];
+        b[v] = r;
+        const double ru = r / dof_weight(A, v);
+        p0 += r * r; p1 += ru * ru;
+      } break;
+      case VOP_DOT2: {     // partials of a.c and b.d (read only)
+        p0 += a[v] * c[v]; p1 += b[v] * d[v];
+      } break;
+      case VOP_COPY: {     // a = c
+        a[v] = c[v];
+      } break;
+    }
+  }
+  if (part) {
+    block_sum2(p0, p1);
+    if (threadIdx.x == 0) { part[blockIdx.x] = p0; part[gridDim.x + blockIdx.x] = p1; }
+  }
+}
+
+// x += alpha ph when BiCGSTAB converged at the half step.
+__global__ void k_bicg_half(size_t n, double* __restrict__ x, const double* __restrict__ ph, const double* __restrict__ S,
+                            int* __restrict__ F) {
+  if (!F[F_ACTIVE] || !F[F_HALF]) return;
+  for (size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (size_t)gridDim.x * blockDim.x)
+    x[v] += S[S_ALPHA] * ph[v];
+}
+
+__global__ void k_zero_if_flag(size_t n, double* __restrict__ x, const int* __restrict__ F) {
+  if (!F[F_ZERO]) return;
+  for (size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (size_t)gridDim.x * blockDim.x) x[v] = 0.;
+}
+
+// Scalar logic of the Krylov solvers.  pa: np partials of q0 then np of q1.
+__global__ void k_finalize(int phase, const double* __restrict__ pa, int np, double* __restrict__ S,
+                           int* __restrict__ F, int maxiter) {
+  if (phase != PH_INIT && phase != PH_BB && !F[F_ACTIVE]) return;
+  if (phase >= PH_OMEGA && F[F_HALF]) {  // BiCGSTAB converged at the half step
+    if (phase == PH_RNORM && threadIdx.x == 0) F[F_ACTIVE] = 0;
+    return;
+  }
+  double a = 0., b = 0.;
+  for (int i = threadIdx.x; i < np; i += blockDim.x) { a += pa[i]; b += pa[np + i]; }
+  block_sum2(a, b);
+  if (threadIdx.x != 0) return;
+  const double tol2bb = S[S_TOL2] * S[S_BB];
+  switch (phase) {
+    case PH_BB: S[S_BB] = a; break;
+    case PH_INIT: {
+      S[S_RR] = b;
+      const double tiny = 2. * 2.220446049250313e-16;
+      const bool zero = S[S_BB] < tiny * tiny;
+      F[F_ZERO] = zero;
+      F[F_ITER] = 0;
+      F[F_HALF] = 0;
+      F[F_ACTIVE] = (!zero && b > S[S_TOL2] * S[S_BB] && maxiter > 0) ? 1 : 0;
+      S[S_RZ] = 0.; S[S_RHO] = 1.; S[S_ALPHA] = 1.; S[S_OMEGA] = 1.; S[S_BETA] = 0.;
+    } break;
+    case PH_RZ: {  // breakdown (z = 0 or an indefinite preconditioner) ends the solve
+      if (!(a > 0.) || !isfinite(a)) { F[F_ACTIVE] = 0; break; }
+      S[S_BETA] = F[F_ITER] > 0 ? a / S[S_RZ] : 0.;
+      S[S_RZ] = a;
+    } break;
+    case PH_PQ: {
+      const double al = S[S_RZ] / a;
+      if (!(a > 0.) || !isfinite(al)) { F[F_ACTIVE] = 0; break; }
+      S[S_PQ] = a;
+      S[S_ALPHA] = al;
+    } break;
+    case PH_UPD: {
+      S[S_RR] = b;
+      F[F_ITER] += 1;
+      F[F_ACTIVE] = (b > tol2bb && F[F_ITER] < maxiter && isfinite(b)) ? 1 : 0;
+    } break;
+    case PH_RHO: {
+      if (a == 0. || !isfinite(a)) { F[F_ACTIVE] = 0; break; }
+      S[S_BETA] = F[F_ITER] > 0 ? (a / S[S_RHO]) * (S[S_ALPHA] / S[S_OMEGA]) : 0.;
+      S[S_RHO] = a;
+    } break;
+    case PH_ALPHA: {
+      const double al = S[S_RHO] / a;
+      if (a == 0. || !isfinite(al)) { F[F_ACTIVE] = 0; break; }
+      S[S_ALPHA] = al;
+    } break;
+    case PH_SNORM: {
+      S[S_RR] = b;
+      if (b <= tol2bb) { F[F_HALF] = 1; F[F_ITER] += 1; }
+    } break;
+    case PH_OMEGA: {  // a = t.s, b = t.t
+      const double om = b != 0. ? a / b : 0.;
+      S[S_OMEGA] = isfinite(om) ? om : 0.;
+    } break;
+    case PH_RNORM: {
+      S[S_RR] = b;
+      F[F_ITER] += 1;
+      F[F_ACTIVE] = (b > tol2bb && F[F_ITER] < maxiter && S[S_OMEGA] != 0. && isfinite(b)) ? 1 : 0;
+    } break;
+  }
+}
+
+// ---- CSR levels (one thread per row) ------------------------------------------------------
+// mode 0: y = A x;  1: y = f - A x;  2: y += A x;  3: x += M (f - A x) (y unused)
+__global__ void k_csr(int mode, int n, const int32_t* __restrict__ ptr, const int32_t* __restrict__ col,
+                      const double* __restrict__ val, const double* __restrict__ x, const double* __restrict__ f,
+                      double* __restrict__ y, const double* __restrict__ M, double* __restrict__ xio,
+                      const int* __restrict__ F) {
+  if (F && !F[F_ACTIVE]) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double s = 0.;
+  for (int j = ptr[i]; j < ptr[i + 1]; ++j) s += val[j] * x[col[j]];
+  if (mode == 0) y[i] = s;
+  else if (mode == 1) y[i] = f[i] - s;
+  else if (mode == 2) y[i] += s;
+  else xio[i] = xio[i] + M[i] * (f[i] - s);
+}
+
+// x = M f (spai0 from a zero guess)
+__global__ void k_scale(int n, const double* __restrict__ M, const double* __restrict__ f, double* __restrict__ x,
+                        const int* __restrict__ F) {
+  if (F && !F[F_ACTIVE]) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = M[i] * f[i];
+}
+
+// Coarsest level: x = Ainv f (dense, row-major), one wavefront per row.
+__global__ void k_dense(int n, const double* __restrict__ Ainv, const double* __restrict__ f, double* __restrict__ x,
+                        const int* __restrict__ F) {
+  if (F && !F[F_ACTIVE]) return;
+  const int row = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (row >= n) return;
+  double s = 0.;
+  for (int j = lane; j < n; j += 64) s += Ainv[(size_t)row * n + j] * f[j];
+  s = wsum(s);
+  if (lane == 0) x[row] = s;
+}
+
+// ---- probing ---------------------------------------------------------------------------------
+// x[el][m] = (color[el] == c && (m == rs || rs < 0)) ? 1 : 0
+__global__ void k_probe_set(int N, int K2, const int32_t* __restrict__ color, int c, int rs, double* __restrict__ x) {
+  const size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= (size_t)N * K2) return;
+  const int el = (int)(v / K2), m = (int)(v % K2);
+  x[v] = (color[el] == c && (rs < 0 || m == rs)) ? 1. : 0.;
+}
+// diag[el][rs] = y[el][rs] for elements of colour c
+__global__ void k_probe_diag(int N, int K2, const int32_t* __restrict__ color, int c, int rs, const double* __restrict__ y,
+                             double* __restrict__ diag) {
+  const int el = blockIdx.x * blockDim.x + threadIdx.x;
+  if (el >= N || color[el] != c) return;
+  diag[(size_t)el * K2 + rs] = y[(size_t)el * K2 + rs];
+}
+// s[el] = sum_m y[el][m]
+__global__ void k_elem_sum(int N, int K2, const double* __restrict__ y, double* __restrict__ s) {
+  const int el = blockIdx.x * blockDim.x + threadIdx.x;
+  if (el >= N) return;
+  double a = 0.;
+  for (int m = 0; m < K2; ++m) a += y[(size_t)el * K2 + m];
+  s[el] = a;
+}
+
+// ---- Q1 coarse space ---------------------------------------------------------------------------
+// rc[v] = sum over the (element, corner) incidences of vertex v of ec[element][corner]
+__global__ void k_q1_gather(int nv, const int32_t* __restrict__ vptr, const int32_t* __restrict__ vinc,
+                            const double* __restrict__ ec, double* __restrict__ rc, const int* __restrict__ F) {
+  if (F && !F[F_ACTIVE]) return;
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= nv) return;
+  double s = 0.;
+  for (int k = vptr[v]; k < vptr[v + 1]; ++k) s += ec[vinc[k]];
+  rc[v] = s;
+}
+// For a probe y = As x (x = unit vector at local dof rs of every element of one colour),
+// M[f][a][b] += (sum_ij q1w(a,ij) y[f][ij]) * q1w(b, rs): the Galerkin block between the
+// corners of f and the corners of f's probed neighbour.
+__global__ void k_q1_probe_accum(int N, int K, Tab T, int rs, const double* __restrict__ y, double* __restrict__ M) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= N) return;
+  const int K2 = K * K;
+  double t[4] = {0., 0., 0., 0.};
+  for (int m = 0; m < K2; ++m) {
+    const double v = y[(size_t)f * K2 + m];
+    for (int a = 0; a < 4; ++a) t[a] += q1w(T, a, m / K, m % K) * v;
+  }
+  for (int a = 0; a < 4; ++a)
+    for (int b = 0; b < 4; ++b) M[(size_t)f * 16 + a * 4 + b] += t[a] * q1w(T, b, rs / K, rs % K);
+}
+
+// ---- launchers -------------------------------------------------------------------------------
+static int grid_for(size_t n, int bs) {
+  size_t g = (n + bs - 1) / bs;
+  if (g > 4096) g = 4096;
+  if (g < 1) g = 1;
+  return (int)g;
+}
+
+int vec_grid(size_t n) { return grid_for(n, 256); }
+
+void vec(hipStream_t s, int op, const VecArgs& A, size_t n, double* a, double* b, const double* c, const double* d,
+         double* e, const double* S, const int* F, double* part) {
+  hipLaunchKernelGGL(k_vec, dim3(vec_grid(n)), dim3(256), 0, s, op, A, n, a, b, c, d, e, S, F, part);
+}
+void bicg_half(hipStream_t s, size_t n, double* x, const double* ph, const double* S, int* F) {
+  hipLaunchKernelGGL(k_bicg_half, dim3(vec_grid(n)), dim3(256), 0, s, n, x, ph, S, F);
+}
+void zero_if_flag(hipStream_t s, size_t n, double* x, const int* F) {
+  hipLaunchKernelGGL(k_zero_if_flag, dim3(vec_grid(n)), dim3(256), 0, s, n, x, F);
+}
+void finalize(hipStream_t s, int phase, const double* part, int np, double* S, int* F, int maxiter) {
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, phase, part, np, S, F, maxiter);
+}
+void csr(hipStream_t s, int mode, int n, const int32_t* ptr, const int32_t* col, const double* val, const double* x,
+         const double* f, double* y, const double* M, double* xio, const int* F) {
+  hipLaunchKernelGGL(k_csr, dim3((n + 255) / 256), dim3(256), 0, s, mode, n, ptr, col, val, x, f, y, M, xio, F);
+}
+void scale(hipStream_t s, int n, const double* M, const double* f, double* x, const int* F) {
+  hipLaunchKernelGGL(k_scale, dim3((n + 255) / 256), dim3(256), 0, s, n, M, f, x, F);
+}
+void dense(hipStream_t s, int n, const double* Ainv, const double* f, double* x, const int* F) {
+  hipLaunchKernelGGL(k_dense, dim3((n + 3) / 4), dim3(256), 0, s, n, Ainv, f, x, F);
+}
+void probe_set(hipStream_t s, int N, int K2, const int32_t* color, int c, int rs, double* x) {
+  size_t n = (size_t)N * K2;
+  hipLaunchKernelGGL(k_probe_set, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, N, K2, color, c, rs, x);
+}
+void probe_diag(hipStream_t s, int N, int K2, const int32_t* color, int c, int rs, const double* y, double* diag) {
+  hipLaunchKernelGGL(k_probe_diag, dim3((N + 255) / 256), dim3(256), 0, s, N, K2, color, c, rs, y, diag);
+}
+void elem_sum(hipStream_t s, int N, int K2, const double* y, double* out) {
+  hipLaunchKernelGGL(k_elem_sum, dim3((N + 255) / 256), dim3(256), 0, s, N, K2, y, out);
+}
+
+}  // namespace hoc
+
+namespace hoc {
+void q1_gather(hipStream_t s, int nv, const int32_t* vptr, const int32_t* vinc, const double* ec, double* rc, const int* F) {
+  hipLaunchKernelGGL(k_q1_gather, dim3((nv + 255) / 256), dim3(256), 0, s, nv, vptr, vinc, ec, rc, F);
+}
+void q1_probe_accum(hipStream_t s, int N, int K, const Tab& T, int rs, const double* y, double* M) {
+  hipLaunchKernelGGL(k_q1_probe_accum, dim3((N + 255) / 256), dim3(256), 0, s, N, K, T, rs, y, M);
+}
+}  // namespace hoc

@@ -1,0 +1,882 @@
+// hoc_solver.hip - device state, Poisson solver, RK driver and the C ABI (include/hoc.h).
+//
+// Per RK stage (src/HO_2D.cpp:1806-1839):
+//   update_BCs(t)            k_update_bcs
+//   solve_Poisson            k_rhs, then PCG or BiCGSTAB on As = -W A (W = Gauss weights
+//                            w_i w_j, which makes As symmetric on affine meshes) with a
+//                            V-cycle: matrix-free fine level + Chebyshev(2) Jacobi
+//                            smoothing, continuous bilinear coarse space on the element
+//                            corners (Ac = Pc^T As Pc assembled from probes of As), then
+//                            smoothed aggregation below
+//   velocity, wall vorticity k_trace + k_velocity
+//   diffusion + advection    k_stage (fused with k = dt (adv + diff) and the RK update)
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/hoc.h"
+#include "hoc_fr.hpp"
+#include "hoc_host.hpp"
+#include "hoc_la.hpp"
+
+namespace hoc {
+
+struct HipError : std::runtime_error {
+  explicit HipError(const std::string& s) : std::runtime_error(s) {}
+};
+struct StatusError : std::runtime_error {
+  int code;
+  StatusError(int c, const std::string& s) : std::runtime_error(s), code(c) {}
+};
+
+#define HIPCHK(x)                                                                         \
+  do {                                                                                    \
+    hipError_t e_ = (x);                                                                  \
+    if (e_ != hipSuccess) throw ::hoc::HipError(std::string(#x) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+template <class T>
+struct DBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DBuf() = default;
+  DBuf(const DBuf&) = delete;
+  DBuf& operator=(const DBuf&) = delete;
+  ~DBuf() { if (p) (void)hipFree(p); }
+  void alloc(size_t cnt) {
+    if (p) { (void)hipFree(p); p = nullptr; }
+    n = cnt;
+    if (cnt) {
+      HIPCHK(hipMalloc(&p, cnt * sizeof(T)));
+      HIPCHK(hipMemset(p, 0, cnt * sizeof(T)));
+      HIPCHK(hipDeviceSynchronize());
+    }
+  }
+  void upload(const std::vector<T>& v) {
+    alloc(v.size());
+    if (!v.empty()) HIPCHK(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    HIPCHK(hipDeviceSynchronize());
+  }
+  operator T*() const { return p; }
+};
+
+struct DevCSR {
+  int n = 0;
+  DBuf<int32_t> ptr, col;
+  DBuf<double> val;
+  void upload(const CSR& A) { n = A.n; ptr.upload(A.ptr); col.upload(A.col); val.upload(A.val); }
+};
+struct DevLevel {
+  DevCSR A, P, R;
+  DBuf<double> M, f, u, t;
+};
+
+struct Solver {
+  hoc_config cfg{};
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+
+  // host
+  Mesh mesh;
+  bool mesh_loaded = false, ready = false;
+  Tables tab;
+  Geometry geo;
+  BCs bch;
+  int K = 0, K2 = 0, N = 0, E = 0;
+  size_t D = 0;
+  double time = 0., re_inv = 0.;
+  Tab T{};
+  VecArgs VA{};
+
+  // device: geometry and connectivity
+  DBuf<double> d_jac, d_xxi, d_xeta, d_yxi, d_yeta, d_Ga, d_Gb, d_Gc, d_fa, d_fb, d_fA, d_fB, d_fjac, d_fAn;
+  DBuf<int32_t> d_nb;
+  DBuf<int8_t> d_nbf, d_swpsi, d_swdiff, d_noslip;
+  DBuf<double> d_esign, d_gel, d_gface;
+  Geo G{};
+  // boundary values
+  DBuf<double> b_psi, b_diff, b_nvel, b_pvel, b_vj, b_vort, b_adv;
+  DBuf<double> b_psi0, b_psi_r, b_diff0, b_diff_r, b_nvel0, b_nvel_r, b_pvel0, b_pvel_r;
+  BCDev B{};
+  // fields
+  DBuf<double> w, psi, vel, U, k1, k2, k3, k4, vort, kscratch;
+  DBuf<double> tr, trw, tru;
+  // Krylov
+  DBuf<double> bs, r, z, p, q, rh, v, ph, sh, t, d_cheb, diag, part;
+  DBuf<double> S;
+  DBuf<int> F;
+  int* pinned_flags = nullptr;  // 2 x F_COUNT
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  int nparts = 0;
+  // hierarchy
+  std::vector<std::unique_ptr<DevLevel>> lv;  // lv[0] = element level (A1), ...
+  DBuf<double> coarse_inv;
+  int coarse_n = 0;
+  DBuf<double> rc, xc, ec;
+  int nv = 0;                             // Q1 coarse space: element-corner vertices
+  DBuf<int32_t> d_ev, d_vptr, d_vinc;     // element -> 4 vertices; vertex -> (element*4+corner)
+  double cheb_theta = 1., cheb_c1 = 0., cheb_c2 = 0.;
+  double lam_max = 0.;
+  int krylov = HOC_KRYLOV_BICGSTAB;
+  // live kernel timing (hoc_profile_*)
+  struct Prof {
+    bool on = false;
+    std::vector<hipEvent_t> e0, e1;
+    std::vector<int> var;
+    int n = 0;
+    double ms[HOC_PROF_VARIANTS] = {0};
+    int64_t cnt[HOC_PROF_VARIANTS] = {0};
+  } prof;
+  // stats
+  int last_iters = 0;
+  double last_rel = 0.;
+  int64_t total_iters = 0, n_solves = 0;
+  double setup_seconds = 0.;
+
+  ~Solver() {
+    for (auto e : prof.e0) (void)hipEventDestroy(e);
+    for (auto e : prof.e1) (void)hipEventDestroy(e);
+    if (pinned_flags) (void)hipHostFree(pinned_flags);
+    for (auto& e : ev) if (e) (void)hipEventDestroy(e);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  // ---------------------------------------------------------------------------------------
+  void setup() {
+    auto t0 = std::chrono::steady_clock::now();
+    HIPCHK(hipSetDevice(device));
+    if (!stream) HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    connect(mesh);
+    build_tables(cfg.K, mesh.lnod, tab);
+    build_geometry(mesh, tab, geo);
+    build_bcs(mesh, tab, cfg.problem_type, bch);
+    K = cfg.K; K2 = K * K; N = mesh.n_elem; E = mesh.n_bedges; D = (size_t)N * K2;
+    re_inv = cfg.Re > 1.e-3 ? 1. / cfg.Re : -1.;  // src/HO_2D.cpp:255
+    // tables
+    std::memset(&T, 0, sizeof(T));
+    for (int m = 0; m < K; ++m) {
+      for (int s = 0; s < 2; ++s) {
+        T.bb[m][s] = tab.bb[m][s]; T.bgb[m][s] = tab.bgb[m][s];
+        T.rad[m][s] = tab.rad[m][s]; T.grad[m][s] = tab.grad[m][s];
+      }
+      for (int k = 0; k < K; ++k) T.D[m][k] = tab.D[m][k];
+      T.w[m] = tab.wgt[m];
+      T.x[m] = tab.sps[m];
+    }
+    T.gp[0] = tab.gprime[0]; T.gp[1] = tab.gprime[1];
+    VA.K = K;
+    for (int m = 0; m < 8; ++m) VA.w[m] = m < K ? tab.wgt[m] : 0.;
+    for (size_t v = 0; v < D; ++v)
+      if (!(geo.jac[v] > 0.)) throw StatusError(HOC_EMESH, "non-positive Jacobian at a solution point (element orientation must be CCW)");
+    // geometry upload
+    std::vector<double> Ga(D), Gb(D), Gc(D);
+    for (size_t v = 0; v < D; ++v) {
+      const double J = geo.jac[v];
+      Ga[v] = (geo.xeta[v] * geo.xeta[v] + geo.yeta[v] * geo.yeta[v]) / J;
+      Gb[v] = (geo.xxi[v] * geo.xeta[v] + geo.yxi[v] * geo.yeta[v]) / J;
+      Gc[v] = (geo.xxi[v] * geo.xxi[v] + geo.yxi[v] * geo.yxi[v]) / J;
+    }
+    std::vector<double> fa(geo.fA.size()), fb(geo.fA.size());
+    for (size_t i = 0; i < fa.size(); ++i) { fa[i] = geo.fA[i] / geo.fjac[i]; fb[i] = geo.fB[i] / geo.fjac[i]; }
+    d_jac.upload(geo.jac); d_xxi.upload(geo.xxi); d_xeta.upload(geo.xeta); d_yxi.upload(geo.yxi); d_yeta.upload(geo.yeta);
+    d_Ga.upload(Ga); d_Gb.upload(Gb); d_Gc.upload(Gc);
+    d_fa.upload(fa); d_fb.upload(fb); d_fA.upload(geo.fA); d_fB.upload(geo.fB); d_fjac.upload(geo.fjac); d_fAn.upload(geo.fAn);
+    d_nb.upload(mesh.nb);
+    d_nbf.upload(mesh.nb_face);
+    std::vector<int8_t> swp(E), swd(E), ns(E);
+    for (int e = 0; e < E; ++e) { swp[e] = (int8_t)bch.sw_psi[e]; swd[e] = (int8_t)bch.sw_diff[e]; ns[e] = (int8_t)bch.no_slip_edge[e]; }
+    d_swpsi.upload(swp); d_swdiff.upload(swd); d_noslip.upload(ns);
+    d_esign.upload(bch.edge_sign);
+    // affine fast path: every element's metrics constant over its sps and faces
+    bool affine = true;
+    auto same = [](double a, double b) { return std::fabs(a - b) <= 1e-12 * std::max(std::fabs(a), std::fabs(b)) + 1e-300; };
+    std::vector<double> gel((size_t)N * 3), gface((size_t)N * 8);
+    for (int e = 0; e < N && affine; ++e) {
+      const size_t b0 = (size_t)e * K2;
+      gel[(size_t)e * 3] = Ga[b0]; gel[(size_t)e * 3 + 1] = Gb[b0]; gel[(size_t)e * 3 + 2] = Gc[b0];
+      for (int m = 1; m < K2 && affine; ++m)
+        affine = same(Ga[b0 + m], Ga[b0]) && same(Gb[b0 + m], Gb[b0]) && same(Gc[b0 + m], Gc[b0]);
+      for (int f = 0; f < 4 && affine; ++f) {
+        const size_t q0 = ((size_t)e * 4 + f) * K;
+        gface[((size_t)e * 4 + f) * 2] = fa[q0]; gface[((size_t)e * 4 + f) * 2 + 1] = fb[q0];
+        for (int k = 1; k < K && affine; ++k) affine = same(fa[q0 + k], fa[q0]) && same(fb[q0 + k], fb[q0]);
+      }
+    }
+    if (affine) { d_gel.upload(gel); d_gface.upload(gface); }
+    G = Geo{N, E, d_jac, d_xxi, d_xeta, d_yxi, d_yeta, d_Ga, d_Gb, d_Gc, d_fa, d_fb, d_fA, d_fB, d_fjac, d_fAn,
+            affine ? d_gel.p : nullptr, affine ? d_gface.p : nullptr,
+            d_nb, d_nbf, d_swpsi, d_swdiff, d_noslip, d_esign};
+    const size_t EK = std::max<size_t>((size_t)E * K, 1);
+    for (auto* b : {&b_psi, &b_diff, &b_nvel, &b_pvel, &b_vj, &b_vort, &b_adv}) b->alloc(EK);
+    auto upl = [&](DBuf<double>& d, const std::vector<double>& h) { if (h.empty()) d.alloc(1); else d.upload(h); };
+    upl(b_psi0, bch.psi); upl(b_psi_r, bch.psi_ramp); upl(b_diff0, bch.diff); upl(b_diff_r, bch.diff_ramp);
+    upl(b_nvel0, bch.nvel); upl(b_nvel_r, bch.nvel_ramp); upl(b_pvel0, bch.pvel); upl(b_pvel_r, bch.pvel_ramp);
+    B = BCDev{b_psi, b_diff, b_nvel, b_pvel, b_vj, b_vort, b_adv, b_psi0, b_psi_r, b_diff0, b_diff_r, b_nvel0, b_nvel_r, b_pvel0, b_pvel_r};
+    // fields
+    for (auto* f : {&w, &psi, &k1, &k2, &k3, &k4, &vort, &kscratch, &bs, &r, &z, &p, &q, &rh, &v, &ph, &sh, &t, &d_cheb, &diag})
+      f->alloc(D);
+    vel.alloc(2 * D); U.alloc(2 * D);
+    tr.alloc((size_t)N * 8 * K); trw.alloc((size_t)N * 8 * K); tru.alloc((size_t)N * 4 * K);
+    nparts = std::max(lap_blocks(N), vec_grid(D));
+    part.alloc(2 * (size_t)nparts);
+    S.alloc(S_COUNT); F.alloc(F_COUNT);
+    if (!pinned_flags) HIPCHK(hipHostMalloc((void**)&pinned_flags, 2 * F_COUNT * sizeof(int), hipHostMallocDefault));
+    for (auto& e : ev) if (!e) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    build_hierarchy();
+    time = 0.;
+    ready = true;
+    HIPCHK(hipStreamSynchronize(stream));
+    setup_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  }
+
+  // Fine-level operator launch, bracketed by events when profiling.
+  void lap(int epi, double* x, double* y, const double* rr, const double* dg, double* d, double c1, double c2,
+           double* partp, const int* active) {
+    const bool rec = prof.on;
+    if (rec) HIPCHK(hipEventRecord(prof.e0[prof.n], stream));
+    HIPCHK(launch_lap(K, epi, N, T, G, x, tr, y, rr, dg, d, c1, c2, partp, active, stream));
+    if (rec) {
+      HIPCHK(hipEventRecord(prof.e1[prof.n], stream));
+      prof.var[prof.n] = epi;
+      if (++prof.n == (int)prof.e0.size()) prof_flush();
+    }
+  }
+  void prof_flush() {
+    if (prof.n == 0) return;
+    HIPCHK(hipEventSynchronize(prof.e1[prof.n - 1]));
+    for (int i = 0; i < prof.n; ++i) {
+      float ms = 0.f;
+      HIPCHK(hipEventElapsedTime(&ms, prof.e0[i], prof.e1[i]));
+      prof.ms[prof.var[i]] += ms;
+      prof.cnt[prof.var[i]] += 1;
+    }
+    prof.n = 0;
+  }
+  // Algorithmic HBM bytes of one fine-operator launch per epilogue: the field, the
+  // neighbour face traces (value + derivative), metrics, connectivity, and the epilogue's
+  // own vectors, each read or written once.
+  double lap_bytes(int epi) const {
+    const double vec = (double)N * K2 * 8.;
+    double b = vec                                   // x
+             + (double)N * 4 * 2 * K * 8.            // neighbour traces
+             + (double)N * 4 * (4 + 1);              // nb, nbf
+    if (G.gel) b += (double)N * (3 + 8 + 8) * 8.;    // affine: element + own/neighbour face metrics
+    else b += 3 * vec + (double)N * 4 * K * 2 * 8. * 2;
+    switch (epi) {
+      case EPI_STORE: b += vec; break;                        // y
+      case EPI_RESID: b += 2 * vec; break;                    // r, y
+      case EPI_CHEB: b += 5 * vec; break;                     // r, diag, d (rw), x out
+      case EPI_CHEBF: b += 4 * vec; break;                    // r, diag, d out, x out
+      case EPI_RESTRICT: b += vec + (double)N * 4 * 8.; break;  // r, corner sums
+    }
+    return b;
+  }
+
+  // As x into y (trace + apply); x is not modified.
+  void apply_As(double* x, double* y) {
+    HIPCHK(launch_trace(K, PRE_COPY, N, T, x, nullptr, nullptr, nullptr, 0., nullptr, 0, nullptr, tr, stream));
+    lap(EPI_STORE, x, y, nullptr, nullptr, nullptr, 0., 0., nullptr, nullptr);
+  }
+
+  // Stream-ordered copies that return when the data has arrived.
+  void d2h(void* dst, const void* src, size_t bytes) {
+    HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+  }
+  void h2d(void* dst, const void* src, size_t bytes) {
+    HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+  }
+
+  // Fine-level diagonal and element-level operator by probing, then SA below it.
+  void build_hierarchy() {
+    std::vector<int32_t> col1, col2;
+    const int nc1 = color_elements(mesh, 1, col1);
+    const int nc2 = color_elements(mesh, 2, col2);
+    DBuf<int32_t> dcol1, dcol2;
+    dcol1.upload(col1); dcol2.upload(col2);
+    for (int c = 0; c < nc1; ++c)
+      for (int rs = 0; rs < K2; ++rs) {
+        probe_set(stream, N, K2, dcol1, c, rs, z);
+        apply_As(z, q);
+        probe_diag(stream, N, K2, dcol1, c, rs, q, diag);
+      }
+    // Continuous bilinear coarse space on the element corners: Pc interpolates vertex values
+    // to every element's solution points; Ac = Pc^T As Pc is assembled from probes of As.
+    std::vector<int32_t> vid((size_t)mesh.n_nodes, -1), ev((size_t)N * 4);
+    nv = 0;
+    for (int e = 0; e < N; ++e)
+      for (int a = 0; a < 4; ++a) {
+        const int nd = mesh.elem[(size_t)e * mesh.lnod * mesh.lnod + a];
+        if (vid[nd] < 0) vid[nd] = nv++;
+        ev[(size_t)e * 4 + a] = vid[nd];
+      }
+    std::vector<int32_t> vptr(nv + 1, 0), vinc((size_t)N * 4);
+    for (int32_t v : ev) vptr[v + 1]++;
+    for (int v = 0; v < nv; ++v) vptr[v + 1] += vptr[v];
+    {
+      std::vector<int32_t> pos(vptr.begin(), vptr.end() - 1);
+      for (int e = 0; e < N; ++e)
+        for (int a = 0; a < 4; ++a) vinc[pos[ev[(size_t)e * 4 + a]]++] = e * 4 + a;
+    }
+    d_ev.upload(ev); d_vptr.upload(vptr); d_vinc.upload(vinc);
+    std::vector<std::vector<std::pair<int, double>>> rows(nv);
+    auto add = [&](int r, int c, double val) {
+      for (auto& pr : rows[r]) if (pr.first == c) { pr.second += val; return; }
+      rows[r].push_back({c, val});
+    };
+    DBuf<double> dM;
+    dM.alloc((size_t)N * 16);
+    std::vector<double> hM((size_t)N * 16);
+    for (int c = 0; c < nc2; ++c) {
+      HIPCHK(hipMemsetAsync(dM.p, 0, (size_t)N * 16 * sizeof(double), stream));
+      for (int rs = 0; rs < K2; ++rs) {
+        probe_set(stream, N, K2, dcol2, c, rs, z);
+        apply_As(z, q);
+        q1_probe_accum(stream, N, K, T, rs, q, dM);
+      }
+      d2h(hM.data(), dM.p, hM.size() * sizeof(double));
+      for (int f = 0; f < N; ++f) {
+        int e = -1;
+        if (col2[f] == c) e = f;
+        else
+          for (int k = 0; k < 4; ++k) {
+            const int nb = mesh.nb[(size_t)f * 4 + k];
+            if (nb >= 0 && col2[nb] == c) { e = nb; break; }
+          }
+        if (e < 0) continue;
+        for (int a = 0; a < 4; ++a)
+          for (int b = 0; b < 4; ++b) {
+            const double val = hM[(size_t)f * 16 + a * 4 + b];
+            if (val != 0.) add(ev[(size_t)f * 4 + a], ev[(size_t)e * 4 + b], val);
+          }
+      }
+    }
+    CSR A1;
+    A1.n = A1.m = nv;
+    A1.ptr.assign(nv + 1, 0);
+    for (int r0 = 0; r0 < nv; ++r0) {
+      std::sort(rows[r0].begin(), rows[r0].end());
+      for (auto& pr : rows[r0]) { A1.col.push_back(pr.first); A1.val.push_back(pr.second); }
+      A1.ptr[r0 + 1] = (int)A1.col.size();
+    }
+    AMGHost H;
+    build_sa_amg(A1, 400, H);
+    lv.clear();
+    for (size_t l = 0; l < H.levels.size(); ++l) {
+      auto L = std::make_unique<DevLevel>();
+      L->A.upload(H.levels[l].A);
+      if (l + 1 < H.levels.size()) {
+        L->P.upload(H.levels[l].P);
+        L->R.upload(H.levels[l].R);
+        L->M.upload(H.levels[l].Minv);
+      }
+      const int n = H.levels[l].A.n;
+      L->f.alloc(n); L->u.alloc(n); L->t.alloc(n);
+      lv.push_back(std::move(L));
+    }
+    coarse_n = H.levels.back().A.n;
+    coarse_inv.upload(H.coarse_inv);
+    // Chebyshev interval from lambda_max(D^-1 As) by power iteration
+    std::vector<double> hx(D);
+    uint64_t st = 88172645463325252ull;
+    for (size_t i = 0; i < D; ++i) { st ^= st << 13; st ^= st >> 7; st ^= st << 17; hx[i] = (double)(st % 2000001) / 1e6 - 1.; }
+    h2d(z.p, hx.data(), D * sizeof(double));
+    std::vector<double> hd(D), hy(D);
+    d2h(hd.data(), diag.p, D * sizeof(double));
+    for (size_t i = 0; i < D; ++i)
+      if (!(hd[i] > 0.)) throw StatusError(HOC_EINVAL, "Poisson operator has a non-positive diagonal entry");
+    double lam = 0.;
+    for (int it = 0; it < 30; ++it) {
+      apply_As(z, q);
+      d2h(hy.data(), q.p, D * sizeof(double));
+      double nrm = 0., ray = 0., xx = 0.;
+      for (size_t i = 0; i < D; ++i) { hy[i] /= hd[i]; nrm += hy[i] * hy[i]; ray += hy[i] * hx[i]; xx += hx[i] * hx[i]; }
+      lam = ray / xx;
+      nrm = std::sqrt(nrm);
+      for (size_t i = 0; i < D; ++i) hx[i] = hy[i] / nrm;
+      h2d(z.p, hx.data(), D * sizeof(double));
+    }
+    lam_max = lam;
+    const double lmax = 1.1 * lam, lmin = lmax / 30.;
+    const double theta = 0.5 * (lmax + lmin), delta = 0.5 * (lmax - lmin);
+    const double sigma = theta / delta, rho0 = 1. / sigma, rho1 = 1. / (2. * sigma - rho0);
+    cheb_theta = theta;
+    cheb_c1 = rho1 * rho0;
+    cheb_c2 = 2. * rho1 / delta;
+    // symmetry of As decides the Krylov method
+    krylov = cfg.krylov;
+    if (krylov == HOC_KRYLOV_AUTO) {
+      std::vector<double> hx2(D), hAx(D), hAy(D);
+      for (size_t i = 0; i < D; ++i) { st ^= st << 13; st ^= st >> 7; st ^= st << 17; hx2[i] = (double)(st % 2000001) / 1e6 - 1.; }
+      for (size_t i = 0; i < D; ++i) { st ^= st << 13; st ^= st >> 7; st ^= st << 17; hx[i] = (double)(st % 2000001) / 1e6 - 1.; }
+      h2d(z.p, hx.data(), D * sizeof(double));
+      apply_As(z, q);
+      d2h(hAx.data(), q.p, D * sizeof(double));
+      h2d(z.p, hx2.data(), D * sizeof(double));
+      apply_As(z, q);
+      d2h(hAy.data(), q.p, D * sizeof(double));
+      double a = 0., b = 0., na = 0., nb = 0.;
+      for (size_t i = 0; i < D; ++i) { a += hx2[i] * hAx[i]; b += hx[i] * hAy[i]; na += std::fabs(hx2[i] * hAx[i]); nb += std::fabs(hx[i] * hAy[i]); }
+      krylov = std::fabs(a - b) <= 1e-12 * std::max(na, nb) ? HOC_KRYLOV_PCG : HOC_KRYLOV_BICGSTAB;
+    }
+    rc.alloc(nv); xc.alloc(nv); ec.alloc((size_t)N * 4);
+    HIPCHK(hipMemsetAsync(z.p, 0, D * sizeof(double), stream));
+    HIPCHK(hipStreamSynchronize(stream));
+  }
+
+  // ---------------------------------------------------------------------------------------
+  // V-cycle on the element level and below: u_l = V_l(f_l)
+  void coarse_cycle(size_t l, const double* f, double* u) {
+    DevLevel& L = *lv[l];
+    const int* Fp = F.p;
+    if (l + 1 == lv.size()) { dense(stream, coarse_n, coarse_inv, f, u, Fp); return; }
+    DevLevel& C = *lv[l + 1];
+    scale(stream, L.A.n, L.M, f, u, Fp);
+    csr(stream, 1, L.A.n, L.A.ptr, L.A.col, L.A.val, u, f, L.t, nullptr, nullptr, Fp);
+    csr(stream, 0, C.A.n, L.R.ptr, L.R.col, L.R.val, L.t, nullptr, C.f, nullptr, nullptr, Fp);
+    coarse_cycle(l + 1, C.f, C.u);
+    csr(stream, 2, L.A.n, L.P.ptr, L.P.col, L.P.val, C.u, nullptr, u, nullptr, nullptr, Fp);
+    csr(stream, 3, L.A.n, L.A.ptr, L.A.col, L.A.val, u, f, nullptr, L.M, u, Fp);
+  }
+
+  // zout = V(rin); if part != nullptr, partials of rin.zout are left in part.
+  void vcycle(const double* rin, double* zout, double* partp) {
+    const int* Fp = F.p;
+    double* rr = const_cast<double*>(rin);
+    // pre-smoothing: Chebyshev(2) from zero
+    HIPCHK(launch_trace(K, PRE_CHEB0, N, T, zout, rr, diag, d_cheb, cheb_theta, nullptr, 0, Fp, tr, stream));
+    lap(EPI_CHEB, zout, nullptr, rr, diag, d_cheb, cheb_c1, cheb_c2, nullptr, Fp);
+    // restrict the residual to the corner vertices
+    HIPCHK(launch_trace(K, PRE_COPY, N, T, zout, nullptr, nullptr, nullptr, 0., nullptr, 0, Fp, tr, stream));
+    lap(EPI_RESTRICT, zout, ec, rr, nullptr, nullptr, 0., 0., nullptr, Fp);
+    q1_gather(stream, nv, d_vptr, d_vinc, ec, rc, Fp);
+    coarse_cycle(0, rc, xc);
+    // prolongate and post-smooth: Chebyshev(2) from the corrected iterate
+    HIPCHK(launch_trace(K, PRE_PROL, N, T, zout, xc, nullptr, nullptr, 0., nullptr, 0, Fp, tr, stream, d_ev));
+    lap(EPI_CHEBF, zout, nullptr, rr, diag, d_cheb, 0., 1. / cheb_theta, nullptr, Fp);
+    HIPCHK(launch_trace(K, PRE_COPY, N, T, zout, nullptr, nullptr, nullptr, 0., nullptr, 0, Fp, tr, stream));
+    lap(EPI_CHEB, zout, nullptr, rr, diag, d_cheb, cheb_c1, cheb_c2, partp, Fp);
+  }
+
+  // Poll the active flag with a one-iteration lag; returns false when the solve ended.
+  int poll_k = 0;
+  bool poll(bool final_wait) {
+    const int slot = poll_k & 1;
+    HIPCHK(hipMemcpyAsync(pinned_flags + slot * F_COUNT, F.p, F_COUNT * sizeof(int), hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipEventRecord(ev[slot], stream));
+    ++poll_k;
+    if (final_wait) { HIPCHK(hipEventSynchronize(ev[slot])); return pinned_flags[slot * F_COUNT + F_ACTIVE] != 0; }
+    if (poll_k < 2) return true;
+    const int prev = (poll_k - 2) & 1;
+    HIPCHK(hipEventSynchronize(ev[prev]));
+    return pinned_flags[prev * F_COUNT + F_ACTIVE] != 0;
+  }
+
+  // solve_Poisson (src/HO_2D.cpp:1841-1977) for omega at the current boundary values.
+  void poisson(const double* omega) {
+    const int lb = lap_blocks(N);
+    const int maxit = cfg.maxiter;
+    const double tol2 = cfg.tol * cfg.tol;
+    HIPCHK(hipMemcpyAsync(S.p + S_TOL2, &tol2, sizeof(double), hipMemcpyHostToDevice, stream));
+    HIPCHK(launch_rhs(K, N, T, G, omega, b_psi, bs, part, stream));
+    finalize(stream, PH_BB, part, lb, S, F, maxit);
+    HIPCHK(launch_trace(K, PRE_COPY, N, T, psi, nullptr, nullptr, nullptr, 0., nullptr, 0, nullptr, tr, stream));
+    lap(EPI_RESID, psi, r, bs, nullptr, nullptr, 0., 0., part, nullptr);
+    finalize(stream, PH_INIT, part, lb, S, F, maxit);
+    zero_if_flag(stream, D, psi, F);
+    const int vg = vec_grid(D);
+    poll_k = 0;
+    if (krylov == HOC_KRYLOV_PCG) {
+      HIPCHK(hipMemsetAsync(p.p, 0, D * sizeof(double), stream));
+      for (int it = 0;; ++it) {
+        vcycle(r, z, part);
+        finalize(stream, PH_RZ, part, lb, S, F, maxit);
+        HIPCHK(launch_trace(K, PRE_PUPD, N, T, p, z, nullptr, nullptr, 0., S.p, S_BETA, F.p, tr, stream));
+        lap(EPI_STORE, p, q, nullptr, nullptr, nullptr, 0., 0., part, F.p);
+        finalize(stream, PH_PQ, part, lb, S, F, maxit);
+        vec(stream, VOP_PCG_UPD, VA, D, psi, r, p, q, nullptr, S, F, part);
+        finalize(stream, PH_UPD, part, vg, S, F, maxit);
+        if (!poll(it > maxit + 2) || it > maxit + 4) break;
+      }
+    } else {
+      vec(stream, VOP_COPY, VA, D, rh, nullptr, r, nullptr, nullptr, nullptr, nullptr, nullptr);
+      HIPCHK(hipMemsetAsync(p.p, 0, D * sizeof(double), stream));
+      HIPCHK(hipMemsetAsync(v.p, 0, D * sizeof(double), stream));
+      for (int it = 0;; ++it) {
+        vec(stream, VOP_DOT2, VA, D, rh, rh, r, r, nullptr, S, F, part);
+        finalize(stream, PH_RHO, part, vg, S, F, maxit);
+        vec(stream, VOP_BICG_P, VA, D, p, nullptr, r, v, nullptr, S, F, nullptr);
+        vcycle(p, ph, nullptr);
+        HIPCHK(launch_trace(K, PRE_COPY, N, T, ph, nullptr, nullptr, nullptr, 0., nullptr, 0, F.p, tr, stream));
+        lap(EPI_STORE, ph, v, nullptr, nullptr, nullptr, 0., 0., nullptr, F.p);
+        vec(stream, VOP_DOT2, VA, D, rh, rh, v, v, nullptr, S, F, part);
+        finalize(stream, PH_ALPHA, part, vg, S, F, maxit);
+        vec(stream, VOP_BICG_S, VA, D, nullptr, r, nullptr, v, nullptr, S, F, part);
+        finalize(stream, PH_SNORM, part, vg, S, F, maxit);
+        bicg_half(stream, D, psi, ph, S, F);
+        vcycle(r, sh, nullptr);  // gated by F_ACTIVE only; harmless when converged at the half step
+        HIPCHK(launch_trace(K, PRE_COPY, N, T, sh, nullptr, nullptr, nullptr, 0., nullptr, 0, F.p, tr, stream));
+        lap(EPI_STORE, sh, t, nullptr, nullptr, nullptr, 0., 0., nullptr, F.p);
+        vec(stream, VOP_DOT2, VA, D, t, t, r, t, nullptr, S, F, part);
+        finalize(stream, PH_OMEGA, part, vg, S, F, maxit);
+        vec(stream, VOP_BICG_X, VA, D, psi, r, ph, sh, t, S, F, part);
+        finalize(stream, PH_RNORM, part, vg, S, F, maxit);
+        if (!poll(it > maxit + 2) || it > maxit + 4) break;
+      }
+    }
+    // statistics (one sync per solve)
+    double hs[S_COUNT];
+    int hf[F_COUNT];
+    HIPCHK(hipMemcpyAsync(hs, S.p, sizeof(hs), hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipMemcpyAsync(hf, F.p, sizeof(hf), hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    last_iters = hf[F_ITER];
+    last_rel = hs[S_BB] > 0. ? std::sqrt(hs[S_RR] / hs[S_BB]) : 0.;
+    if (hf[F_ZERO]) last_rel = 0.;
+    total_iters += last_iters;
+    ++n_solves;
+    if (!std::isfinite(last_rel)) throw StatusError(HOC_ENAN, "NaN/Inf in the Poisson solve");
+    if (cfg.fail_on_noconv && last_rel > cfg.tol)
+      throw StatusError(HOC_ENOCONV, "Poisson solve did not converge: rel residual " + std::to_string(last_rel));
+  }
+
+  double ramp(double t) const { return bch.time_dependent ? 2. * std::fabs(std::sin(M_PI / 40. * t)) : 1.; }
+
+  // Euler_time_integrate (src/HO_2D.cpp:1806-1839) on input win at time t.
+  void stage(const double* win, double t, const StageOut& so) {
+    HIPCHK(launch_update_bcs(E * K, B, ramp(t), stream));
+    poisson(win);
+    HIPCHK(launch_trace(K, PRE_COPY, N, T, psi, nullptr, nullptr, nullptr, 0., nullptr, 0, nullptr, tr, stream));
+    HIPCHK(launch_velocity(K, N, T, G, B, psi, tr, win, vel, U, trw, tru, 1. / (cfg.dt * re_inv), stream));
+    HIPCHK(launch_stage(K, cfg.advection_type, N, T, G, B, win, U, trw, tru, re_inv, cfg.dt, so, stream));
+  }
+
+  // solve_advection_diffusion (src/HO_2D.cpp:1465-1531)
+  void rk_step() {
+    const double t0 = time, dt = cfg.dt;
+    StageOut so;
+    if (cfg.time_integration_type == 1) {
+      so.base = w; so.final_mode = 1;
+      stage(w, t0, so);
+    } else if (cfg.time_integration_type == 2) {
+      so.k = k1; so.next = vort; so.c = 1.; so.base = w;
+      stage(w, t0, so);
+      StageOut s2; s2.base = w; s2.final_mode = 2; s2.k1 = k1;
+      stage(vort, t0 + dt, s2);
+    } else if (cfg.time_integration_type == 4) {
+      const double c[4] = {0., 0.5, 0.5, 1.};
+      double* ks[3] = {k1, k2, k3};
+      stage_rk4(w, t0 + c[0] * dt, ks[0], 0.5);
+      stage_rk4(vort, t0 + c[1] * dt, ks[1], 0.5);
+      stage_rk4(vort, t0 + c[2] * dt, ks[2], 1.0);
+      StageOut s4; s4.base = w; s4.final_mode = 4; s4.k1 = k1; s4.k2 = k2; s4.k3 = k3;
+      stage(vort, t0 + c[3] * dt, s4);
+    } else {
+      throw StatusError(HOC_EINVAL, "time_integration_type must be 1, 2 or 4");
+    }
+    time = t0 + dt;
+  }
+  void stage_rk4(const double* win, double t, double* kout, double cnext) {
+    StageOut so; so.k = kout; so.next = vort; so.c = cnext; so.base = w;
+    stage(win, t, so);
+  }
+};
+
+}  // namespace hoc
+
+using hoc::Solver;
+
+struct hoc_solver {
+  Solver s;
+};
+
+static std::string g_err;
+
+static int hoc_fail(hoc_t h, int code, const std::string& msg) {
+  if (h) h->s.err = msg;
+  g_err = msg;
+  return code;
+}
+
+template <class Fn>
+static int guarded(hoc_t h, Fn fn) {
+  if (!h) return hoc_fail(nullptr, HOC_EINVAL, "null handle");
+  try {
+    fn();
+  } catch (const hoc::StatusError& e) {
+    return hoc_fail(h, e.code, e.what());
+  } catch (const hoc::HipError& e) {
+    return hoc_fail(h, HOC_EDEVICE, e.what());
+  } catch (const std::bad_alloc&) {
+    return hoc_fail(h, HOC_EDEVICE, "host allocation failed");
+  } catch (const std::exception& e) {
+    return hoc_fail(h, HOC_EMESH, e.what());
+  }
+  return HOC_OK;
+}
+
+extern "C" {
+
+void hoc_default_config(hoc_config* c) {
+  std::memset(c, 0, sizeof(*c));
+  c->K = 3;                      // set_defaults (src/HO_2D.cpp:3699)
+  c->problem_type = 1;
+  c->Re = 1.0;
+  c->dt = 1.0;
+  c->time_integration_type = 2;  // :3703
+  c->advection_type = 2;         // :3704
+  c->tol = 1e-15;                // :3067
+  c->maxiter = 150;              // :3069
+  c->precision = HOC_FP64;
+  c->krylov = HOC_KRYLOV_AUTO;
+  c->device = 0;
+  c->fail_on_noconv = 0;
+}
+
+int hoc_create(hoc_t* out, const hoc_config* cfg) {
+  if (!out || !cfg) return hoc_fail(nullptr, HOC_EINVAL, "null argument");
+  *out = nullptr;
+  if (cfg->K < 2 || cfg->K > 6) return hoc_fail(nullptr, HOC_EINVAL, "K must be in 2..6");
+  if (cfg->time_integration_type != 1 && cfg->time_integration_type != 2 && cfg->time_integration_type != 4)
+    return hoc_fail(nullptr, HOC_EINVAL, "time_integration_type must be 1, 2 or 4");
+  if (cfg->advection_type != 1 && cfg->advection_type != 2) return hoc_fail(nullptr, HOC_EINVAL, "advection_type must be 1 or 2");
+  if (cfg->problem_type < 1 || cfg->problem_type > 3) return hoc_fail(nullptr, HOC_EINVAL, "problem_type must be 1, 2 or 3");
+  if (cfg->precision != HOC_FP64 && cfg->precision != HOC_MIXED) return hoc_fail(nullptr, HOC_EINVAL, "bad precision");
+  if (!(cfg->dt > 0.) || cfg->maxiter < 0 || !(cfg->tol >= 0.)) return hoc_fail(nullptr, HOC_EINVAL, "bad dt/tol/maxiter");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return hoc_fail(nullptr, HOC_EDEVICE, "no HIP device");
+  if (cfg->device < 0 || cfg->device >= ndev) return hoc_fail(nullptr, HOC_EDEVICE, "device ordinal out of range");
+  hoc_t h = new (std::nothrow) hoc_solver();
+  if (!h) return hoc_fail(nullptr, HOC_EDEVICE, "allocation failed");
+  h->s.cfg = *cfg;
+  h->s.device = cfg->device;
+  *out = h;
+  return HOC_OK;
+}
+
+int hoc_load_mesh(hoc_t h, int32_t lnod, int32_t n_nodes, const double* xy, int32_t n_elem, const int32_t* elem_nodes,
+                  int32_t n_bedges, const int32_t* bedge_nodes, const int32_t* bedge_bdry, int32_t n_bdry,
+                  const char* bdry_names) {
+  return guarded(h, [&] {
+    Solver& s = h->s;
+    if (lnod < 2 || lnod > 4) throw hoc::StatusError(HOC_EINVAL, "lnod must be 2, 3 or 4");
+    if (n_nodes <= 0 || n_elem <= 0 || n_bedges < 0 || n_bdry < 0 || !xy || !elem_nodes)
+      throw hoc::StatusError(HOC_EINVAL, "empty mesh");
+    if (n_bedges > 0 && (!bedge_nodes || !bedge_bdry)) throw hoc::StatusError(HOC_EINVAL, "missing boundary arrays");
+    hoc::Mesh m;
+    m.lnod = lnod; m.n_nodes = n_nodes; m.n_elem = n_elem; m.n_bedges = n_bedges;
+    m.xy.assign(xy, xy + 2 * (size_t)n_nodes);
+    m.elem.assign(elem_nodes, elem_nodes + (size_t)n_elem * lnod * lnod);
+    for (int32_t v : m.elem) if (v < 0 || v >= n_nodes) throw hoc::StatusError(HOC_EINVAL, "element node id out of range");
+    m.edges.resize(n_bedges);
+    for (int e = 0; e < n_bedges; ++e) {
+      m.edges[e].assign(bedge_nodes + (size_t)e * lnod, bedge_nodes + (size_t)(e + 1) * lnod);
+      for (int32_t v : m.edges[e]) if (v < 0 || v >= n_nodes) throw hoc::StatusError(HOC_EINVAL, "edge node id out of range");
+    }
+    m.bedge_bdry.assign(bedge_bdry, bedge_bdry + n_bedges);
+    for (int32_t b : m.bedge_bdry) if (b < 0 || b >= n_bdry) throw hoc::StatusError(HOC_EINVAL, "boundary id out of range");
+    std::string names(bdry_names ? bdry_names : "");
+    size_t p0 = 0;
+    for (int b = 0; b < n_bdry; ++b) {
+      size_t p1 = names.find('\n', p0);
+      m.bnames.push_back(names.substr(p0, p1 == std::string::npos ? std::string::npos : p1 - p0));
+      p0 = p1 == std::string::npos ? names.size() : p1 + 1;
+    }
+    s.mesh = std::move(m);
+    s.mesh_loaded = true;
+    s.ready = false;
+  });
+}
+
+int hoc_setup(hoc_t h) {
+  return guarded(h, [&] {
+    if (!h->s.mesh_loaded) throw hoc::StatusError(HOC_ESTATE, "hoc_setup before hoc_load_mesh");
+    h->s.setup();
+  });
+}
+
+#define NEED_READY(h) if (!h->s.ready) throw hoc::StatusError(HOC_ESTATE, "call hoc_setup first")
+#define NEED_LEN(len, want) if ((len) != (int64_t)(want)) throw hoc::StatusError(HOC_EINVAL, "length mismatch")
+
+int hoc_set_vorticity(hoc_t h, int64_t len, const double* w) {
+  return guarded(h, [&] {
+    NEED_READY(h); NEED_LEN(len, h->s.D);
+    h->s.h2d(h->s.w.p, w, h->s.D * sizeof(double));
+  });
+}
+int hoc_get_vorticity(hoc_t h, int64_t len, double* w) {
+  return guarded(h, [&] {
+    NEED_READY(h); NEED_LEN(len, h->s.D);
+    h->s.d2h(w, h->s.w.p, h->s.D * sizeof(double));
+  });
+}
+int hoc_get_streamfunction(hoc_t h, int64_t len, double* p) {
+  return guarded(h, [&] {
+    NEED_READY(h); NEED_LEN(len, h->s.D);
+    h->s.d2h(p, h->s.psi.p, h->s.D * sizeof(double));
+  });
+}
+int hoc_set_streamfunction(hoc_t h, int64_t len, const double* p) {
+  return guarded(h, [&] {
+    NEED_READY(h); NEED_LEN(len, h->s.D);
+    h->s.h2d(h->s.psi.p, p, h->s.D * sizeof(double));
+  });
+}
+int hoc_get_velocity(hoc_t h, int64_t len, double* uv) {
+  return guarded(h, [&] {
+    NEED_READY(h); NEED_LEN(len, 2 * h->s.D);
+    h->s.d2h(uv, h->s.vel.p, 2 * h->s.D * sizeof(double));
+  });
+}
+int hoc_get_solution_points(hoc_t h, int64_t len, double* xy) {
+  return guarded(h, [&] {
+    NEED_READY(h); NEED_LEN(len, 2 * h->s.D);
+    std::vector<double> pts;
+    hoc::solution_points(h->s.mesh, h->s.tab, pts);
+    std::memcpy(xy, pts.data(), pts.size() * sizeof(double));
+  });
+}
+int hoc_get_solution_areas(hoc_t h, int64_t len, double* a) {
+  return guarded(h, [&] {
+    NEED_READY(h); NEED_LEN(len, h->s.D);
+    Solver& s = h->s;
+    for (int el = 0; el < s.N; ++el)
+      for (int j = 0; j < s.K; ++j)
+        for (int k = 0; k < s.K; ++k) {
+          size_t v = (size_t)el * s.K2 + j * s.K + k;
+          a[v] = s.tab.wgt[j] * s.tab.wgt[k] * s.geo.jac[v];  // src/HO_2D.cpp:3988
+        }
+  });
+}
+double hoc_get_time(hoc_t h) { return h ? h->s.time : 0.; }
+int hoc_set_time(hoc_t h, double t) { return guarded(h, [&] { h->s.time = t; }); }
+
+int hoc_step(hoc_t h, int32_t nsteps) {
+  return guarded(h, [&] {
+    NEED_READY(h);
+    if (nsteps < 0) throw hoc::StatusError(HOC_EINVAL, "nsteps < 0");
+    for (int i = 0; i < nsteps; ++i) h->s.rk_step();
+    HIPCHK(hipStreamSynchronize(h->s.stream));
+  });
+}
+
+int hoc_solveto(hoc_t h, double outer_dt, int32_t nsub, int32_t integ, double Re) {
+  return guarded(h, [&] {
+    NEED_READY(h);
+    if (nsub <= 0 || !(outer_dt > 0.)) throw hoc::StatusError(HOC_EINVAL, "bad outer_dt/nsub");
+    if (integ != 1 && integ != 2 && integ != 4) throw hoc::StatusError(HOC_EINVAL, "integ must be 1, 2 or 4");
+    if (!(Re > 0.)) throw hoc::StatusError(HOC_EINVAL, "Re must be positive");
+    Solver& s = h->s;
+    s.cfg.dt = outer_dt / (double)nsub;   // src/HO_2D.cpp:4199
+    s.cfg.time_integration_type = integ;  // :4203
+    s.re_inv = 1.0 / Re;                  // :4204
+    for (int i = 0; i < nsub; ++i) s.rk_step();
+    HIPCHK(hipStreamSynchronize(s.stream));
+  });
+}
+
+int hoc_poisson(hoc_t h, const double* omega, double t, double* psi_out) {
+  return guarded(h, [&] {
+    NEED_READY(h);
+    Solver& s = h->s;
+    s.h2d(s.kscratch.p, omega, s.D * sizeof(double));
+    HIPCHK(hoc::launch_update_bcs(s.E * s.K, s.B, s.ramp(t), s.stream));
+    s.poisson(s.kscratch);
+    s.d2h(psi_out, s.psi.p, s.D * sizeof(double));
+  });
+}
+
+int hoc_stage_residual(hoc_t h, const double* omega, double t, double* k) {
+  return guarded(h, [&] {
+    NEED_READY(h);
+    Solver& s = h->s;
+    s.h2d(s.vort.p, omega, s.D * sizeof(double));
+    hoc::StageOut so;
+    so.k = s.kscratch;
+    s.stage(s.vort, t, so);
+    s.d2h(k, s.kscratch.p, s.D * sizeof(double));
+  });
+}
+
+int hoc_apply_laplacian(hoc_t h, const double* x, double* y) {
+  return guarded(h, [&] {
+    NEED_READY(h);
+    Solver& s = h->s;
+    s.h2d(s.z.p, x, s.D * sizeof(double));
+    s.apply_As(s.z, s.q);
+    std::vector<double> hy(s.D);
+    s.d2h(hy.data(), s.q.p, s.D * sizeof(double));
+    // As = -W A  =>  A x = -(As x) / W
+    for (size_t v = 0; v < s.D; ++v) {
+      const int m = (int)(v % s.K2);
+      y[v] = -hy[v] / (s.tab.wgt[m / s.K] * s.tab.wgt[m % s.K]);
+    }
+  });
+}
+
+int hoc_stats_get(hoc_t h, hoc_stats* st) {
+  return guarded(h, [&] {
+    Solver& s = h->s;
+    std::memset(st, 0, sizeof(*st));
+    st->n_elem = s.N; st->n_dofs = (int64_t)s.D; st->n_bedges = s.E;
+    st->n_levels = (int32_t)s.lv.size() + 1;
+    st->krylov = s.krylov;
+    st->last_iters = s.last_iters;
+    st->last_rel_residual = s.last_rel;
+    st->total_iters = s.total_iters;
+    st->n_solves = s.n_solves;
+    st->setup_seconds = s.setup_seconds;
+    st->fine_op_bytes = s.ready ? s.lap_bytes(hoc::EPI_STORE) : 0.;
+  });
+}
+
+int hoc_profile_enable(hoc_t h, int32_t on) {
+  return guarded(h, [&] {
+    NEED_READY(h);
+    Solver& s = h->s;
+    s.prof_flush();
+    if (on && s.prof.e0.empty()) {
+      const int cap = 4096;
+      s.prof.e0.resize(cap); s.prof.e1.resize(cap); s.prof.var.resize(cap);
+      for (int i = 0; i < cap; ++i) { HIPCHK(hipEventCreate(&s.prof.e0[i])); HIPCHK(hipEventCreate(&s.prof.e1[i])); }
+    }
+    if (on) for (int v = 0; v < HOC_PROF_VARIANTS; ++v) { s.prof.ms[v] = 0.; s.prof.cnt[v] = 0; }
+    s.prof.on = on != 0;
+  });
+}
+
+int hoc_profile_get(hoc_t h, hoc_kernel_profile* p) {
+  return guarded(h, [&] {
+    NEED_READY(h);
+    Solver& s = h->s;
+    s.prof_flush();
+    static const char* names[HOC_PROF_VARIANTS] = {"lap_apply", "lap_residual", "lap_cheb", "lap_cheb_first", "lap_restrict"};
+    std::memset(p, 0, sizeof(*p));
+    for (int v = 0; v < HOC_PROF_VARIANTS; ++v) {
+      p->launches[v] = s.prof.cnt[v];
+      p->total_ms[v] = s.prof.ms[v];
+      p->bytes_per_launch[v] = s.lap_bytes(v);
+      std::strncpy(p->names[v], names[v], sizeof(p->names[v]) - 1);
+    }
+  });
+}
+
+const char* hoc_last_error(hoc_t h) { return h ? h->s.err.c_str() : g_err.c_str(); }
+
+void hoc_destroy(hoc_t h) {
+  if (!h) return;
+  (void)hipSetDevice(h->s.device);
+  if (h->s.stream) (void)hipStreamSynchronize(h->s.stream);
+  delete h;
+}
+
+}  // extern "C"

@@ -1,0 +1,224 @@
+"""Python host side of the MI355X HO-CXX hot path: a ctypes binding of libhoc.so.
+
+The product path is the HIP library built from ho-cxx_amd/csrc (C ABI in
+include/hoc.h).  This module loads the in-tree libhoc.so and fails loudly when
+it is missing or when no GPU is present; there is no CPU fallback.
+
+``HO2D`` mirrors the reference's HO_2D setup/advance surface
+(src/HO_2D.hpp:220-290): load a mesh (``load_mesh`` ~ load_mesh_arrays_d /
+read_msh_file), ``setup`` (~ process_mesh_input), ``step`` (~
+solve_advection_diffusion), ``solveto`` (~ solveto_d), ``get_vorticity``
+(~ getallvorts_d), ``set_vorticity`` (~ setsolnvort_d).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import numpy as np
+
+from . import meshes  # noqa: F401
+
+_PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(_PKG, "libhoc.so")
+
+HOC_OK, HOC_EINVAL, HOC_ESTATE, HOC_ENOCONV, HOC_ENAN, HOC_EDEVICE, HOC_EMESH = 0, -1, -2, -3, -4, -5, -6
+HOC_FP64, HOC_MIXED = 0, 1
+KRYLOV_AUTO, KRYLOV_BICGSTAB, KRYLOV_PCG = 0, 1, 2
+
+EXPORTED = [
+    "hoc_default_config", "hoc_create", "hoc_load_mesh", "hoc_setup", "hoc_set_vorticity", "hoc_get_vorticity",
+    "hoc_get_streamfunction", "hoc_set_streamfunction", "hoc_get_velocity", "hoc_get_solution_points",
+    "hoc_get_solution_areas", "hoc_get_time", "hoc_set_time", "hoc_step", "hoc_solveto", "hoc_poisson",
+    "hoc_stage_residual", "hoc_apply_laplacian", "hoc_stats_get", "hoc_profile_enable", "hoc_profile_get",
+    "hoc_last_error", "hoc_destroy",
+]
+PROF_VARIANTS = 5
+
+
+class hoc_config(C.Structure):
+    _fields_ = [("K", C.c_int32), ("problem_type", C.c_int32), ("Re", C.c_double), ("dt", C.c_double),
+                ("time_integration_type", C.c_int32), ("advection_type", C.c_int32), ("tol", C.c_double),
+                ("maxiter", C.c_int32), ("precision", C.c_int32), ("krylov", C.c_int32), ("device", C.c_int32),
+                ("fail_on_noconv", C.c_int32)]
+
+
+class hoc_stats(C.Structure):
+    _fields_ = [("n_elem", C.c_int64), ("n_dofs", C.c_int64), ("n_bedges", C.c_int64), ("n_levels", C.c_int32),
+                ("krylov", C.c_int32), ("last_iters", C.c_int32), ("last_rel_residual", C.c_double),
+                ("total_iters", C.c_int64), ("n_solves", C.c_int64), ("setup_seconds", C.c_double),
+                ("fine_op_bytes", C.c_double)]
+
+
+class hoc_kernel_profile(C.Structure):
+    _fields_ = [("launches", C.c_int64 * 5), ("total_ms", C.c_double * 5), ("bytes_per_launch", C.c_double * 5),
+                ("names", (C.c_char * 24) * 5)]
+
+
+class HocError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"hoc error {code}: {msg}")
+        self.code = code
+
+
+_LIB = None
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+
+
+def lib():
+    """Load libhoc.so (raises when it has not been built)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() or make -C ho-cxx_amd")
+        L = C.CDLL(LIB_PATH)
+        L.hoc_default_config.argtypes = [C.POINTER(hoc_config)]
+        L.hoc_default_config.restype = None
+        L.hoc_create.argtypes = [C.POINTER(C.c_void_p), C.POINTER(hoc_config)]
+        L.hoc_load_mesh.argtypes = [C.c_void_p, C.c_int32, C.c_int32, _dp, C.c_int32, _ip, C.c_int32, _ip, _ip,
+                                    C.c_int32, C.c_char_p]
+        L.hoc_setup.argtypes = [C.c_void_p]
+        for nm in ("hoc_set_vorticity", "hoc_get_vorticity", "hoc_get_streamfunction", "hoc_set_streamfunction",
+                   "hoc_get_velocity", "hoc_get_solution_points", "hoc_get_solution_areas"):
+            getattr(L, nm).argtypes = [C.c_void_p, C.c_int64, _dp]
+        L.hoc_get_time.argtypes = [C.c_void_p]
+        L.hoc_get_time.restype = C.c_double
+        L.hoc_set_time.argtypes = [C.c_void_p, C.c_double]
+        L.hoc_step.argtypes = [C.c_void_p, C.c_int32]
+        L.hoc_solveto.argtypes = [C.c_void_p, C.c_double, C.c_int32, C.c_int32, C.c_double]
+        L.hoc_poisson.argtypes = [C.c_void_p, _dp, C.c_double, _dp]
+        L.hoc_stage_residual.argtypes = [C.c_void_p, _dp, C.c_double, _dp]
+        L.hoc_apply_laplacian.argtypes = [C.c_void_p, _dp, _dp]
+        L.hoc_stats_get.argtypes = [C.c_void_p, C.POINTER(hoc_stats)]
+        L.hoc_profile_enable.argtypes = [C.c_void_p, C.c_int32]
+        L.hoc_profile_get.argtypes = [C.c_void_p, C.POINTER(hoc_kernel_profile)]
+        L.hoc_last_error.argtypes = [C.c_void_p]
+        L.hoc_last_error.restype = C.c_char_p
+        L.hoc_destroy.argtypes = [C.c_void_p]
+        L.hoc_destroy.restype = None
+        _LIB = L
+    return _LIB
+
+
+def default_config(**kw) -> hoc_config:
+    c = hoc_config()
+    lib().hoc_default_config(C.byref(c))
+    for k, v in kw.items():
+        if not hasattr(c, k):
+            raise KeyError(k)
+        setattr(c, k, v)
+    return c
+
+
+class HO2D:
+    """One problem on one GPU; mirrors the reference HO_2D call surface."""
+
+    def __init__(self, K=3, problem_type=1, Re=100.0, dt=1e-3, time_integration_type=4, advection_type=1,
+                 tol=1e-15, maxiter=150, precision=HOC_FP64, krylov=KRYLOV_AUTO, device=0, fail_on_noconv=False):
+        self.cfg = default_config(K=K, problem_type=problem_type, Re=Re, dt=dt,
+                                  time_integration_type=time_integration_type, advection_type=advection_type,
+                                  tol=tol, maxiter=maxiter, precision=precision, krylov=krylov, device=device,
+                                  fail_on_noconv=int(fail_on_noconv))
+        self.h = C.c_void_p()
+        rc = lib().hoc_create(C.byref(self.h), C.byref(self.cfg))
+        if rc != HOC_OK:
+            raise HocError(rc, lib().hoc_last_error(None).decode())
+        self.K = K
+        self.D = 0
+
+    def _chk(self, rc):
+        if rc != HOC_OK:
+            raise HocError(rc, lib().hoc_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None) and self.h.value:
+            lib().hoc_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def load_mesh(self, m):
+        self._chk(lib().hoc_load_mesh(self.h, m.lnod, m.n_nodes, np.ascontiguousarray(m.xy, dtype=np.float64).ravel(),
+                                      m.n_elem, np.ascontiguousarray(m.elem, dtype=np.int32).ravel(),
+                                      m.n_bedges, np.ascontiguousarray(m.bedges, dtype=np.int32).ravel(),
+                                      np.ascontiguousarray(m.bedge_bdry, dtype=np.int32), len(m.bnames),
+                                      "\n".join(m.bnames).encode()))
+        self.D = m.n_elem * self.K * self.K
+
+    def setup(self):
+        self._chk(lib().hoc_setup(self.h))
+
+    def _get(self, fn, n):
+        out = np.zeros(n)
+        self._chk(getattr(lib(), fn)(self.h, n, out))
+        return out
+
+    def get_vorticity(self):
+        return self._get("hoc_get_vorticity", self.D)
+
+    def set_vorticity(self, w):
+        w = np.ascontiguousarray(w, dtype=np.float64)
+        self._chk(lib().hoc_set_vorticity(self.h, w.size, w))
+
+    def get_streamfunction(self):
+        return self._get("hoc_get_streamfunction", self.D)
+
+    def set_streamfunction(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float64)
+        self._chk(lib().hoc_set_streamfunction(self.h, p.size, p))
+
+    def get_velocity(self):
+        return self._get("hoc_get_velocity", 2 * self.D).reshape(-1, 2)
+
+    def solution_points(self):
+        return self._get("hoc_get_solution_points", 2 * self.D).reshape(-1, 2)
+
+    def solution_areas(self):
+        return self._get("hoc_get_solution_areas", self.D)
+
+    @property
+    def time(self):
+        return lib().hoc_get_time(self.h)
+
+    def step(self, n=1):
+        self._chk(lib().hoc_step(self.h, n))
+
+    def solveto(self, outer_dt, nsub, integ, Re):
+        self._chk(lib().hoc_solveto(self.h, outer_dt, nsub, integ, Re))
+
+    def poisson(self, w, t=0.0):
+        out = np.zeros(self.D)
+        self._chk(lib().hoc_poisson(self.h, np.ascontiguousarray(w, dtype=np.float64), t, out))
+        return out
+
+    def stage(self, w, t=0.0):
+        out = np.zeros(self.D)
+        self._chk(lib().hoc_stage_residual(self.h, np.ascontiguousarray(w, dtype=np.float64), t, out))
+        return out
+
+    def apply_laplacian(self, x):
+        out = np.zeros(self.D)
+        self._chk(lib().hoc_apply_laplacian(self.h, np.ascontiguousarray(x, dtype=np.float64), out))
+        return out
+
+    def stats(self):
+        s = hoc_stats()
+        self._chk(lib().hoc_stats_get(self.h, C.byref(s)))
+        return {f: getattr(s, f) for f, _ in s._fields_}
+
+    def profile(self, on=True):
+        """Turn live per-launch timing of the fine-level operator kernel on/off."""
+        self._chk(lib().hoc_profile_enable(self.h, int(on)))
+
+    def profile_read(self):
+        p = hoc_kernel_profile()
+        self._chk(lib().hoc_profile_get(self.h, C.byref(p)))
+        out = {}
+        for v in range(PROF_VARIANTS):
+            out[p.names[v].value.decode()] = {"launches": p.launches[v], "total_ms": p.total_ms[v],
+                                              "bytes_per_launch": p.bytes_per_launch[v]}
+        return out

@@ -1,0 +1,139 @@
+/*
+ * hoc.h - C ABI of the MI355X (gfx950) time-step hot path for HO-CXX.
+ *
+ * Drop-in for the HO_2D setup/advance surface of the reference
+ * (src/HO_2D.hpp:220-290).  One handle = one mesh/problem on one device and
+ * one HIP stream.  Host buffers are owned by the caller and copied; device
+ * state is owned by the handle.  Every call is synchronous unless its name ends
+ * in _device.  Status codes: 0 = OK, negative = error (message via
+ * hoc_last_error).  No call exits, asserts or prints.
+ *
+ * Layouts follow the reference: fields at solution points are
+ * [element][j (eta)][i (xi)] = el*K*K + j*K + i (src/HO_2D.cpp:4162, 4245);
+ * velocities and points are interleaved (x,y) (src/HO_2D.cpp:3972-3974).
+ */
+#ifndef HOC_H
+#define HOC_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct hoc_solver* hoc_t;
+
+enum {
+  HOC_OK = 0,
+  HOC_EINVAL = -1,      /* bad argument / size mismatch / unsupported option */
+  HOC_ESTATE = -2,      /* call out of order (e.g. step before setup)        */
+  HOC_ENOCONV = -3,     /* Poisson solve did not reach the tolerance          */
+  HOC_ENAN = -4,        /* NaN/Inf detected in a field                        */
+  HOC_EDEVICE = -5,     /* HIP runtime error / no device / allocation failed  */
+  HOC_EMESH = -6        /* mesh connectivity error                            */
+};
+
+enum { HOC_FP64 = 0, HOC_MIXED = 1 };
+enum { HOC_KRYLOV_AUTO = 0, HOC_KRYLOV_BICGSTAB = 1, HOC_KRYLOV_PCG = 2 };
+
+/* Parameters of input.dat (src/HO_2D.cpp:213-317) and of the linear solver
+ * (src/HO_2D.cpp:3066-3069). */
+typedef struct {
+  int32_t K;                      /* Knod: solution points per direction (P+1), 2..6    */
+  int32_t problem_type;           /* 1 cavity, 2 backward-facing step, 3 cylinder       */
+  double Re;                      /* Reynolds number (<= 1e-3 disables viscosity, :255) */
+  double dt;                      /* time step                                           */
+  int32_t time_integration_type;  /* 1 Euler, 2 RK2, 4 RK4                               */
+  int32_t advection_type;         /* 1 discontinuous mass flux, 2 continuous             */
+  double tol;                     /* Krylov relative residual tolerance                  */
+  int32_t maxiter;                /* Krylov iteration cap                                */
+  int32_t precision;              /* HOC_FP64 or HOC_MIXED (FP32 V-cycle)               */
+  int32_t krylov;                 /* HOC_KRYLOV_*; AUTO = PCG when W*A is symmetric     */
+  int32_t device;                 /* HIP device ordinal                                  */
+  int32_t fail_on_noconv;         /* nonzero: return HOC_ENOCONV when tol is not met     */
+} hoc_config;
+
+typedef struct {
+  int64_t n_elem, n_dofs, n_bedges;
+  int32_t n_levels;               /* Poisson hierarchy levels (fine level included)      */
+  int32_t krylov;                 /* method in use                                       */
+  int32_t last_iters;             /* iterations of the last Poisson solve                */
+  double last_rel_residual;       /* ||b - A psi|| / ||b|| of the last solve             */
+  int64_t total_iters;            /* summed over all solves                              */
+  int64_t n_solves;
+  double setup_seconds;
+  double fine_op_bytes;           /* algorithmic bytes of one fine-operator application  */
+} hoc_stats;
+
+/* Fill cfg with the reference defaults (src/HO_2D.cpp:3698-3713, 3066-3069). */
+void hoc_default_config(hoc_config* cfg);
+
+/* HO_2D() + set_defaults (src/HO_2D.cpp:3698). */
+int hoc_create(hoc_t* h, const hoc_config* cfg);
+
+/* = load_mesh_arrays_d / load_inout_arrays_d (src/HO_2D.cpp:3823-3898) and
+ * Mesh::read_msh_file's output (src/Mesh.cpp:93-340):
+ *   lnod         geometry nodes per edge (2..4)
+ *   xy           n_nodes interleaved (x,y)
+ *   elem_nodes   n_elem * lnod^2 node ids, Gmsh order (CCW corners, edge, interior)
+ *   bedge_nodes  n_bedges * lnod node ids, Gmsh line order; listed boundary edges
+ *                are numbered 0..n_bedges-1 in this order (BC arrays use it)
+ *   bedge_bdry   boundary id of each edge; bdry_names holds n_bdry names joined by '\n'
+ */
+int hoc_load_mesh(hoc_t h, int32_t lnod, int32_t n_nodes, const double* xy,
+                  int32_t n_elem, const int32_t* elem_nodes,
+                  int32_t n_bedges, const int32_t* bedge_nodes,
+                  const int32_t* bedge_bdry, int32_t n_bdry, const char* bdry_names);
+
+/* = process_mesh_input (src/HO_2D.cpp:3901-3918): connectivity, bases, metrics,
+ * IC/BC (setup_IC_BC_SRC), Poisson operator and hierarchy, device upload. */
+int hoc_setup(hoc_t h);
+
+int hoc_set_vorticity(hoc_t h, int64_t len, const double* w);     /* = setsolnvort_d target */
+int hoc_get_vorticity(hoc_t h, int64_t len, double* w);           /* = getallvorts_d        */
+int hoc_get_streamfunction(hoc_t h, int64_t len, double* psi);
+int hoc_set_streamfunction(hoc_t h, int64_t len, const double* psi);
+int hoc_get_velocity(hoc_t h, int64_t len, double* uv);           /* len = 2*dofs          */
+int hoc_get_solution_points(hoc_t h, int64_t len, double* xy);   /* = getsolnpts_d        */
+int hoc_get_solution_areas(hoc_t h, int64_t len, double* a);     /* = getsolnareas_d      */
+double hoc_get_time(hoc_t h);
+int hoc_set_time(hoc_t h, double t);
+
+/* nsteps x solve_advection_diffusion (src/HO_2D.cpp:1465-1531), time += dt each. */
+int hoc_step(hoc_t h, int32_t nsteps);
+
+/* = solveto_d (src/HO_2D.cpp:4194-4233): dt = outer_dt/nsub, integ, Re, then nsub steps. */
+int hoc_solveto(hoc_t h, double outer_dt, int32_t nsub, int32_t integ, double Re);
+
+/* Operator-level entry points for parity tests. */
+/* update_BCs(t) + solve_Poisson(omega) (src/HO_2D.cpp:1841-1977); psi is warm-started
+ * from the handle's current stream function and stored back into it. */
+int hoc_poisson(hoc_t h, const double* omega, double t, double* psi);
+/* One stage Euler_time_integrate(omega -> k, coeff=0 at time t) (src/HO_2D.cpp:1806-1839). */
+int hoc_stage_residual(hoc_t h, const double* omega, double t, double* k);
+/* y = A x with the Poisson matrix of form_Laplace_operator_matrix (src/HO_2D.cpp:1533-1804),
+ * applied matrix-free. */
+int hoc_apply_laplacian(hoc_t h, const double* x, double* y);
+
+int hoc_stats_get(hoc_t h, hoc_stats* s);
+
+/* Live timing of the fine-level Poisson operator kernel (the dominant kernel): with
+ * profiling on, every launch is bracketed by HIP events recorded on the handle's own
+ * stream.  One entry per fused epilogue variant. */
+#define HOC_PROF_VARIANTS 5
+typedef struct {
+  int64_t launches[HOC_PROF_VARIANTS];
+  double total_ms[HOC_PROF_VARIANTS];
+  double bytes_per_launch[HOC_PROF_VARIANTS];  /* algorithmic HBM bytes of one launch */
+  char names[HOC_PROF_VARIANTS][24];
+} hoc_kernel_profile;
+int hoc_profile_enable(hoc_t h, int32_t on);   /* on != 0 also clears the counters */
+int hoc_profile_get(hoc_t h, hoc_kernel_profile* p);
+
+const char* hoc_last_error(hoc_t h);   /* h may be NULL for hoc_create failures */
+void hoc_destroy(hoc_t h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -13,9 +13,16 @@
  * parameters at their defaults.  Section "AMGCL restatement" below restates that
  * published algorithm (plain aggregation with eps_strong=0.08 halved per level,
  * prolongator smoothing with omega=2/3 on the filtered matrix, Galerkin coarse
- * operators, spai0 relaxation, V(1,1), direct solve once n <= 3000,
- * right-preconditioned BiCGSTAB).  A converged solve of the same A psi = b does not
- * depend on the preconditioner beyond the tolerance.
+ * operators, V(1,1), skyline-LU direct solve once n <= 3000, right-preconditioned
+ * BiCGSTAB).  With the default spai0 relaxation that solver stalls for K >= 5
+ * (the FR Laplacian has lambda_max(D^-1 A) ~ 3.5 there), so by default the oracle
+ * relaxes with AMGCL's relaxation::chebyshev (degree 3, interval [lmax/30, lmax])
+ * and applies the hierarchy to the row-scaled system (W A) psi = W b, W = w_i w_j,
+ * which is symmetric on affine meshes; the stopping test stays on the unscaled
+ * residual ||b - A psi|| / ||b||.  Environment knobs reproduce the default
+ * configuration: HO_ORACLE_RELAX=0 (spai0), HO_ORACLE_UNSCALED=1,
+ * HO_ORACLE_SA_RHO=1 (omega = 4/3 / rho), HO_ORACLE_CHEB_DEG.  A converged solve of
+ * A psi = b does not depend on the preconditioner beyond the tolerance.
  */
 #include "ho_oracle.h"
 
@@ -23,6 +30,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <cstdlib>
 #include <map>
 #include <stdexcept>
 #include <string>
@@ -211,10 +219,15 @@ CSR multiply(const CSR& A, const CSR& B) {
 struct AMGLevel {
   CSR A, P, R;
   std::vector<double> M;            // spai0 weights
+  std::vector<double> dinv;         // 1/a_ii (Chebyshev relaxation)
+  double lmax = 0.;                 // lambda_max(D^-1 A) by power iteration
   std::vector<double> f, u, t;      // work vectors
 };
 
 struct AMG {
+  int omega_mode = 0;
+  int relax_mode = 1;   // 0 spai0 (AMGCL default; stalls for K >= 5), 1 Chebyshev
+  int cheb_degree = 3;
   std::vector<AMGLevel> levels;
   // Coarse direct solve (AMGCL's builtin skyline_lu): reverse Cuthill-McKee ordering, then LU
   // without pivoting on the symmetric envelope (skyline) of the reordered matrix.
@@ -328,6 +341,22 @@ struct AMG {
         }
         L.M[i] = num / den;
       }
+      if (relax_mode == 1) {  // AMGCL relaxation::chebyshev: interval [lmax/30, lmax] of D^-1 A
+        L.dinv.assign(A.n, 0.);
+        for (int i = 0; i < A.n; ++i)
+          for (int j = A.ptr[i]; j < A.ptr[i + 1]; ++j) if (A.col[j] == i) L.dinv[i] = 1. / A.val[j];
+        std::vector<double> x(A.n), y(A.n);
+        for (int i = 0; i < A.n; ++i) x[i] = 1. + 0.37 * std::sin(1.7 * i + 0.3);
+        double lam = 1.;
+        for (int it = 0; it < 25; ++it) {
+          spmv(A, x.data(), y.data());
+          double ny = 0., xy = 0., xx = 0.;
+          for (int i = 0; i < A.n; ++i) { y[i] *= L.dinv[i]; ny += y[i] * y[i]; xy += x[i] * y[i]; xx += x[i] * x[i]; }
+          lam = std::sqrt(ny / xx);
+          for (int i = 0; i < A.n; ++i) x[i] = y[i] / std::sqrt(ny);
+        }
+        L.lmax = 1.1 * lam;
+      }
       // plain aggregation
       std::vector<double> dia(A.n, 0.);
       for (int i = 0; i < A.n; ++i)
@@ -373,8 +402,31 @@ struct AMG {
       for (int i = 0; i < A.n; ++i) if (id[i] >= 0) id[i] = newid[id[i]];
       count = nc2;
       eps_strong *= 0.5;
-      // smoothed prolongation P = (I - omega Df^-1 Af) Ptent, omega = 2/3
-      const double omega = 2. / 3.;
+      // smoothed prolongation P = (I - omega Df^-1 Af) Ptent.  AMGCL's default is omega = 2/3;
+      // with estimate_spectral_radius it is 4/3 / rho(Df^-1 Af), rho from power iteration.
+      double omega = 2. / 3.;
+      if (omega_mode == 1) {
+        std::vector<double> df(A.n, 0.);
+        for (int i = 0; i < A.n; ++i)
+          for (int j = A.ptr[i]; j < A.ptr[i + 1]; ++j) if (A.col[j] == i || !strong[j]) df[i] += A.val[j];
+        std::vector<double> x(A.n), y(A.n);
+        for (int i = 0; i < A.n; ++i) x[i] = 1. + 0.37 * std::sin(1.7 * i);
+        double rho = 1.;
+        for (int it = 0; it < 20; ++it) {
+          for (int i = 0; i < A.n; ++i) {
+            double sacc = 0.;
+            for (int j = A.ptr[i]; j < A.ptr[i + 1]; ++j)
+              if (A.col[j] == i) sacc += df[i] * x[i];
+              else if (strong[j]) sacc += A.val[j] * x[A.col[j]];
+            y[i] = df[i] != 0. ? sacc / df[i] : 0.;
+          }
+          double nx = 0., ny = 0.;
+          for (int i = 0; i < A.n; ++i) { nx += x[i] * x[i]; ny += y[i] * y[i]; }
+          rho = std::sqrt(ny / nx);
+          for (int i = 0; i < A.n; ++i) x[i] = y[i] / std::sqrt(ny);
+        }
+        omega = 4. / 3. / rho;
+      }
       CSR P;
       P.n = A.n; P.m = count;
       P.ptr.assign(A.n + 1, 0);
@@ -420,8 +472,9 @@ struct AMG {
     AMGLevel& L = levels[l];
     if (l + 1 == levels.size()) { coarse_solve(rhs, x); return; }
     const int n = L.A.n;
-    // pre-relaxation: x += M (rhs - A x), with x = 0 on entry
-    for (int i = 0; i < n; ++i) x[i] = L.M[i] * rhs[i];
+    // pre-relaxation from x = 0
+    std::fill(x.begin(), x.end(), 0.);
+    relax(L, rhs, x);
     spmv(L.A, x.data(), L.t.data());
     for (int i = 0; i < n; ++i) L.t[i] = rhs[i] - L.t[i];
     AMGLevel& C = levels[l + 1];
@@ -430,9 +483,32 @@ struct AMG {
     cycle(l + 1, C.f, C.u);
     spmv(L.P, C.u.data(), L.t.data());
     for (int i = 0; i < n; ++i) x[i] += L.t[i];
-    // post-relaxation
-    spmv(L.A, x.data(), L.t.data());
-    for (int i = 0; i < n; ++i) x[i] += L.M[i] * (rhs[i] - L.t[i]);
+    relax(L, rhs, x);  // post-relaxation
+  }
+
+  // One relaxation sweep x <- x + S (rhs - A x): spai0, or Chebyshev of degree cheb_degree.
+  void relax(AMGLevel& L, const std::vector<double>& rhs, std::vector<double>& x) {
+    const int n = L.A.n;
+    std::vector<double> r(n), d(n);
+    spmv(L.A, x.data(), r.data());
+    for (int i = 0; i < n; ++i) r[i] = rhs[i] - r[i];
+    if (relax_mode == 0) {
+      for (int i = 0; i < n; ++i) x[i] += L.M[i] * r[i];
+      return;
+    }
+    const double lmax = L.lmax, lmin = lmax / 30.;
+    const double theta = 0.5 * (lmax + lmin), delta = 0.5 * (lmax - lmin), sigma = theta / delta;
+    double rho = 1. / sigma;
+    for (int i = 0; i < n; ++i) { d[i] = L.dinv[i] * r[i] / theta; x[i] += d[i]; }
+    for (int k = 1; k < cheb_degree; ++k) {
+      spmv(L.A, x.data(), r.data());
+      const double rho_new = 1. / (2. * sigma - rho);
+      for (int i = 0; i < n; ++i) {
+        d[i] = rho_new * rho * d[i] + 2. * rho_new / delta * L.dinv[i] * (rhs[i] - r[i]);
+        x[i] += d[i];
+      }
+      rho = rho_new;
+    }
   }
   void apply(const std::vector<double>& rhs, std::vector<double>& x) { cycle(0, rhs, x); }
 };
@@ -445,10 +521,19 @@ double dot(const std::vector<double>& a, const std::vector<double>& b) {
 
 // Right-preconditioned BiCGSTAB (AMGCL solver::bicgstab, pside = right).
 // Returns iterations; rel = ||r||/||b||.  Zero rhs clears x (as AMGCL does).
+// With wts != nullptr the system is the row-scaled one (W A) x = W b and the stopping test uses
+// the residual of the unscaled system, ||W^-1 r|| / ||W^-1 b||.
+double wnorm(const std::vector<double>& r, const std::vector<double>* wts) {
+  if (!wts) return std::sqrt(dot(r, r));
+  double s = 0.;
+  for (size_t i = 0; i < r.size(); ++i) { double u = r[i] / (*wts)[i]; s += u * u; }
+  return std::sqrt(s);
+}
+
 int bicgstab(const CSR& A, AMG& M, const std::vector<double>& b, std::vector<double>& x,
-             double tol, int maxiter, double& rel) {
+             double tol, int maxiter, double& rel, const std::vector<double>* wts = nullptr) {
   const int n = A.n;
-  double norm_b = std::sqrt(dot(b, b));
+  double norm_b = wnorm(b, wts);
   if (norm_b < 2. * 2.220446049250313e-16) { std::fill(x.begin(), x.end(), 0.); rel = norm_b; return 0; }
   double eps = tol * norm_b;
   std::vector<double> r(n), rh(n), p(n), v(n), ph(n), sh(n), t(n);
@@ -456,7 +541,7 @@ int bicgstab(const CSR& A, AMG& M, const std::vector<double>& b, std::vector<dou
   for (int i = 0; i < n; ++i) r[i] = b[i] - r[i];
   rh = r;
   double rho1 = 0., rho2 = 0., alpha = 0., omega = 0.;
-  double res = std::sqrt(dot(r, r));
+  double res = wnorm(r, wts);
   int iter = 0;
   for (; iter < maxiter && res > eps; ++iter) {
     rho1 = dot(r, rh);
@@ -473,7 +558,7 @@ int bicgstab(const CSR& A, AMG& M, const std::vector<double>& b, std::vector<dou
     if (rv == 0.) break;
     alpha = rho1 / rv;
     for (int i = 0; i < n; ++i) r[i] -= alpha * v[i];
-    res = std::sqrt(dot(r, r));
+    res = wnorm(r, wts);
     if (res > eps) {
       M.apply(r, sh);
       spmv(A, sh.data(), t.data());
@@ -481,7 +566,7 @@ int bicgstab(const CSR& A, AMG& M, const std::vector<double>& b, std::vector<dou
       omega = tt != 0. ? dot(t, r) / tt : 0.;
       for (int i = 0; i < n; ++i) x[i] += alpha * ph[i] + omega * sh[i];
       for (int i = 0; i < n; ++i) r[i] -= omega * t[i];
-      res = std::sqrt(dot(r, r));
+      res = wnorm(r, wts);
       if (omega == 0.) { ++iter; break; }
     } else {
       for (int i = 0; i < n; ++i) x[i] += alpha * ph[i];
@@ -534,7 +619,10 @@ struct ho_oracle {
   std::vector<double> boundary_source;  // [eb][K2][K]
 
   // Poisson
-  CSR A;
+  CSR A;                  // the reference's matrix (src/HO_2D.cpp:3011-3063)
+  CSR As;                 // W A, rows scaled by the Gauss weights w_i w_j (solve_scaled)
+  std::vector<double> Wd; // w_i w_j per dof
+  bool solve_scaled = true;
   AMG amg;
   int last_iters = 0;
   double last_rel = 0.;
@@ -1155,7 +1243,22 @@ struct ho_oracle {
       A.val.insert(A.val.end(), values[r].begin(), values[r].end());
       A.ptr[r + 1] = (int)A.col.size();
     }
-    amg.build(A);
+    amg.omega_mode = getenv("HO_ORACLE_SA_RHO") ? atoi(getenv("HO_ORACLE_SA_RHO")) : 0;
+    amg.relax_mode = getenv("HO_ORACLE_RELAX") ? atoi(getenv("HO_ORACLE_RELAX")) : 1;
+    if (getenv("HO_ORACLE_CHEB_DEG")) amg.cheb_degree = atoi(getenv("HO_ORACLE_CHEB_DEG"));
+    if (getenv("HO_ORACLE_UNSCALED")) solve_scaled = false;
+    Wd.assign(A.n, 0.);
+    for (int el = 0; el < N_el; ++el)
+      for (int j = 0; j < K; ++j)
+        for (int i = 0; i < K; ++i) Wd[v3(el, j, i)] = wgt[j] * wgt[i];
+    if (solve_scaled) {
+      As = A;
+      for (int r = 0; r < As.n; ++r)
+        for (int j = As.ptr[r]; j < As.ptr[r + 1]; ++j) As.val[j] *= Wd[r];
+      amg.build(As);
+    } else {
+      amg.build(A);
+    }
   }
 
   // --- src/HO_2D.cpp:1979-2217 (problem types 1, 2, 3) ---------------------------------------
@@ -1228,7 +1331,12 @@ struct ho_oracle {
             RHS[el * Ksq + ij] -= boundary_source[((size_t)eb * Ksq + ij) * K + alpha] * BC_Poisson[b2(eb, alpha)];
         }
     std::vector<double> x(stream_function);
-    last_iters = bicgstab(A, amg, RHS, x, tol, maxiter, last_rel);
+    if (solve_scaled) {
+      for (int i = 0; i < D; ++i) RHS[i] *= Wd[i];
+      last_iters = bicgstab(As, amg, RHS, x, tol, maxiter, last_rel, &Wd);
+    } else {
+      last_iters = bicgstab(A, amg, RHS, x, tol, maxiter, last_rel);
+    }
     stream_function = x;
   }
 
@@ -1924,6 +2032,18 @@ int64_t ho_oracle_table(ho_oracle* h, const char* name, double* out, int64_t cap
             }
           tmp.push_back(x); tmp.push_back(y);
         }
+  } else if (n == "edge_xy") {  // flux points of each boundary edge, edge order (src/HO_2D.cpp:752)
+    const int K = h->K;
+    for (int e = 0; e < h->N_eb; ++e)
+      for (int k = 0; k < K; ++k) {
+        double x = 0., y = 0.;
+        for (int j = 0; j < h->Lnod; ++j) {
+          int nd = h->edges[e][h->tensor2FEM(j)];
+          x += h->gsb[j * K + k] * h->nx[nd];
+          y += h->gsb[j * K + k] * h->ny[nd];
+        }
+        tmp.push_back(x); tmp.push_back(y);
+      }
   } else if (n == "areas") {  // src/HO_2D.cpp:3982-3992
     const int K = h->K;
     for (int el = 0; el < h->N_el; ++el)
@@ -1933,6 +2053,41 @@ int64_t ho_oracle_table(ho_oracle* h, const char* name, double* out, int64_t cap
   int64_t c = std::min<int64_t>(cap, (int64_t)tmp.size());
   if (out) std::copy(tmp.begin(), tmp.begin() + c, out);
   return (int64_t)tmp.size();
+}
+
+int ho_oracle_set_bc(ho_oracle* h, const char* name, const double* vals) {
+  std::string n(name);
+  std::vector<double>* v = nullptr;
+  if (n == "BC_Poisson") v = &h->BC_Poisson;
+  else if (n == "BC_diffusion") v = &h->BC_diffusion;
+  else if (n == "BC_normal_vel") v = &h->BC_normal_vel;
+  else if (n == "BC_parl_vel") v = &h->BC_parl_vel;
+  else return -1;
+  std::copy(vals, vals + v->size(), v->begin());
+  return 0;
+}
+
+int ho_oracle_apply_laplacian_bc(ho_oracle* h, const double* x, const double* bcvals, double* y) {
+  try {
+    h->fr_second_order(x, h->BC_switch_Poisson.data(), bcvals, nullptr, y);
+  } catch (const std::exception& e) { return fail(h, e); }
+  return 0;
+}
+
+int ho_oracle_solve_poisson_bc(ho_oracle* h, const double* w, double* psi) {
+  try {
+    h->solve_Poisson(w);
+    std::copy(h->stream_function.begin(), h->stream_function.end(), psi);
+  } catch (const std::exception& e) { return fail(h, e); }
+  return 0;
+}
+
+int ho_oracle_velocity_from_psi(ho_oracle* h, double* uv) {
+  try {
+    h->calc_velocity();
+    std::copy(h->vel.begin(), h->vel.end(), uv);
+  } catch (const std::exception& e) { return fail(h, e); }
+  return 0;
 }
 
 int ho_oracle_apply_laplacian_mf(ho_oracle* h, const double* x, double* y) {

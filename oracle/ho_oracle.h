@@ -80,6 +80,15 @@ int64_t ho_oracle_table(ho_oracle*, const char* name, double* out, int64_t cap);
  * ψ boundary types and zero boundary values: the matrix-free form of the Laplacian. */
 int ho_oracle_apply_laplacian_mf(ho_oracle*, const double* x, double* y);
 
+/* Test hooks: overwrite a BC array ("BC_Poisson", "BC_diffusion", "BC_normal_vel",
+ * "BC_parl_vel"; N_eb*K values), apply the FR Laplacian with given psi BC values, solve
+ * the Poisson system with the current BC values (no update_BCs), velocity from the
+ * current stream function. */
+int ho_oracle_set_bc(ho_oracle*, const char* name, const double* vals);
+int ho_oracle_apply_laplacian_bc(ho_oracle*, const double* x, const double* bcvals, double* y);
+int ho_oracle_solve_poisson_bc(ho_oracle*, const double* w, double* psi);
+int ho_oracle_velocity_from_psi(ho_oracle*, double* uv);
+
 #ifdef __cplusplus
 }
 #endif

@@ -48,6 +48,10 @@ def lib():
         L.ho_oracle_table.argtypes = [C.c_void_p, C.c_char_p, C.c_void_p, C.c_int64]
         L.ho_oracle_table.restype = C.c_int64
         L.ho_oracle_apply_laplacian_mf.argtypes = [C.c_void_p, _dp, _dp]
+        L.ho_oracle_set_bc.argtypes = [C.c_void_p, C.c_char_p, _dp]
+        L.ho_oracle_apply_laplacian_bc.argtypes = [C.c_void_p, _dp, _dp, _dp]
+        L.ho_oracle_solve_poisson_bc.argtypes = [C.c_void_p, _dp, _dp]
+        L.ho_oracle_velocity_from_psi.argtypes = [C.c_void_p, _dp]
         _LIB = L
     return _LIB
 
@@ -144,3 +148,26 @@ class Oracle:
         y = np.zeros(self.D)
         self._chk(lib().ho_oracle_apply_laplacian_mf(self.h, np.ascontiguousarray(x, dtype=np.float64), y))
         return y
+
+    def set_bc(self, name, vals):
+        vals = np.ascontiguousarray(vals, dtype=np.float64).ravel()
+        if vals.size != self.n_bedges * self.K:
+            raise ValueError("need n_bedges*K values")
+        self._chk(lib().ho_oracle_set_bc(self.h, name.encode(), vals))
+
+    def laplacian_bc(self, x, bcvals):
+        y = np.zeros(self.D)
+        self._chk(lib().ho_oracle_apply_laplacian_bc(self.h, np.ascontiguousarray(x, dtype=np.float64),
+                                                     np.ascontiguousarray(bcvals, dtype=np.float64).ravel(), y))
+        return y
+
+    def solve_poisson_bc(self, w):
+        out = np.zeros(self.D)
+        self._chk(lib().ho_oracle_solve_poisson_bc(self.h, np.ascontiguousarray(w, dtype=np.float64), out))
+        return out
+
+    def velocity_from_psi(self, psi):
+        self.set_streamfunction(psi)
+        out = np.zeros(2 * self.D)
+        self._chk(lib().ho_oracle_velocity_from_psi(self.h, out))
+        return out.reshape(-1, 2)
