@@ -1,0 +1,146 @@
+// HO_2D_GPU.hpp - header-only C++ facade with the reference's HO_2D method names
+// (src/HO_2D.hpp:220-290) over the C ABI of include/hoc.h.
+//
+// Existing callers of the reference class switch by type: the mesh/boundary loaders,
+// process_mesh_input, the solution getters/setters, solveto_d and getallvorts_d keep
+// their names, argument meaning and ownership (caller-owned buffers, copied).  Errors
+// that the reference handles with exit()/assert throw hoc_gpu_error instead; methods the
+// reference declares void still return nothing.  Problem types 1-3 are supported.
+#pragma once
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/hoc.h"
+
+class hoc_gpu_error : public std::runtime_error {
+ public:
+  hoc_gpu_error(int code, const std::string& msg) : std::runtime_error(msg), code(code) {}
+  int code;
+};
+
+class HO_2D_GPU {
+ public:
+  HO_2D_GPU() { hoc_default_config(&cfg_); }
+  ~HO_2D_GPU() { clean_up(); }
+  HO_2D_GPU(const HO_2D_GPU&) = delete;             // the reference's shallow copy is a double-free hazard
+  HO_2D_GPU& operator=(const HO_2D_GPU&) = delete;
+
+  // ---- configuration (src/HO_2D.cpp:3698-3721; input.dat fields :213-317) ----
+  void set_defaults() { hoc_default_config(&cfg_); }
+  void enable_hybrid() { hybrid_ = true; }
+  void set_elemorder(const int32_t k) { cfg_.K = k; }
+  void set_problem(int problem_type, double Re, double dt, int time_integration_type = 4, int advection_type = 1) {
+    cfg_.problem_type = problem_type; cfg_.Re = Re; cfg_.dt = dt;
+    cfg_.time_integration_type = time_integration_type; cfg_.advection_type = advection_type;
+  }
+  void set_linear_solver(double tol, int maxiter, int precision = HOC_FP64) {
+    cfg_.tol = tol; cfg_.maxiter = maxiter; cfg_.precision = precision;
+  }
+  hoc_config& config() { return cfg_; }
+
+  // ---- mesh input (src/HO_2D.cpp:3823-3898) ----
+  // _iorder: geometry order (Lnod-1); _nnodes: 2*number of nodes; _nelems: nodes*elements;
+  // _nwall/_nopen: Lnod*number of edges, edge node lists in Gmsh line order.
+  void load_mesh_arrays_d(const int32_t _iorder, const int32_t _nnodes, const double* _xy, const int32_t _nelems,
+                          const int32_t* _idxelems, const int32_t _nwall, const int32_t* _idxwall,
+                          const int32_t _nopen, const int32_t* _idxopen) {
+    lnod_ = _iorder + 1;
+    xy_.assign(_xy, _xy + _nnodes);
+    elem_.assign(_idxelems, _idxelems + _nelems);
+    bedge_.clear(); bbdry_.clear(); names_.clear();
+    add_boundary("wall", _nwall, _idxwall);
+    add_boundary("open", _nopen, _idxopen);
+  }
+  void load_inout_arrays_d(const int32_t, const double*, const int32_t _nin, const int32_t* _idxin,
+                           const int32_t, const double*, const int32_t _nout, const int32_t* _idxout) {
+    add_boundary("inlet", _nin, _idxin);
+    add_boundary("outlet", _nout, _idxout);
+  }
+  // Named boundaries as read from a Gmsh file (src/Mesh.cpp:121-253).
+  void add_boundary(const std::string& name, int32_t n_idx, const int32_t* idx) {
+    if (n_idx <= 0) return;
+    const int b = (int)names_.size();
+    names_.push_back(name);
+    for (int32_t i = 0; i < n_idx; ++i) bedge_.push_back(idx[i]);
+    for (int32_t e = 0; e < n_idx / lnod_; ++e) bbdry_.push_back(b);
+  }
+  void process_mesh_input() {
+    clean_up();
+    check(hoc_create(&h_, &cfg_), nullptr);
+    std::string joined;
+    for (size_t i = 0; i < names_.size(); ++i) joined += (i ? "\n" : "") + names_[i];
+    const int32_t L2 = lnod_ * lnod_;
+    check(hoc_load_mesh(h_, lnod_, (int32_t)(xy_.size() / 2), xy_.data(), (int32_t)(elem_.size() / L2), elem_.data(),
+                        (int32_t)bbdry_.size(), bedge_.data(), bbdry_.data(), (int32_t)names_.size(), joined.c_str()),
+          h_);
+    check(hoc_setup(h_), h_);
+  }
+
+  // ---- getters (src/HO_2D.cpp:3942-3992, 4238-4259) ----
+  int32_t getsolnptlen() { return (int32_t)(2 * dofs()); }
+  void getsolnpts_d(const int32_t n, double* xy) { check(hoc_get_solution_points(h_, n, xy), h_); }
+  void getsolnareas_d(const int32_t n, double* a) { check(hoc_get_solution_areas(h_, n, a), h_); }
+  void getallvorts_d(const int32_t n, double* w) { check(hoc_get_vorticity(h_, n, w), h_); }
+  void get_hoquad_weights_d(const int32_t n, double* out) {  // 0.25 w_j w_k (src/HO_2D.cpp:4252-4259)
+    const int K = cfg_.K;
+    if (n != K * K) throw hoc_gpu_error(HOC_EINVAL, "get_hoquad_weights_d: length must be K*K");
+    const double* w = gauss_weights(K);
+    for (int j = 0; j < K; ++j)
+      for (int k = 0; k < K; ++k) out[j * K + k] = 0.25 * w[j] * w[k];
+  }
+  void getstreamfunction_d(const int32_t n, double* p) { check(hoc_get_streamfunction(h_, n, p), h_); }
+  void getvelocity_d(const int32_t n, double* uv) { check(hoc_get_velocity(h_, n, uv), h_); }
+
+  // ---- setters / marching ----
+  void setsolnvort_d(const int32_t n, const double* w) { check(hoc_set_vorticity(h_, n, w), h_); }
+  void solveto_d(const double outer_dt, const int32_t nstep, const int32_t integ, const double Re,
+                 const int32_t /*region*/ = 0) {
+    check(hoc_solveto(h_, outer_dt, nstep, integ, Re), h_);
+  }
+  // solve_advection_diffusion (src/HO_2D.cpp:1465-1531), nsteps times
+  char solve_advection_diffusion(int32_t nsteps = 1) {
+    check(hoc_step(h_, nsteps), h_);
+    return 0;
+  }
+  double current_time() const { return h_ ? hoc_get_time(h_) : 0.; }
+  void clean_up() {
+    if (h_) hoc_destroy(h_);
+    h_ = nullptr;
+  }
+  hoc_t handle() const { return h_; }
+
+ private:
+  int64_t dofs() const { return (int64_t)(elem_.size() / (lnod_ * lnod_)) * cfg_.K * cfg_.K; }
+  static void check(int rc, hoc_t h) {
+    if (rc != HOC_OK) throw hoc_gpu_error(rc, hoc_last_error(h));
+  }
+  static const double* gauss_weights(int K) {  // src/HO_2D.cpp:488-521
+    static double w[7][6];
+    static bool init = false;
+    if (!init) {
+      const double s30 = 5.477225575051661, s70 = 8.366600265340756;
+      double t[7][6] = {{0}, {2.}, {1., 1.}, {5. / 9., 8. / 9., 5. / 9.},
+                        {(18. - s30) / 36., (18. + s30) / 36., (18. + s30) / 36., (18. - s30) / 36.},
+                        {(322. - 13. * s70) / 900., (322. + 13. * s70) / 900., 128. / 225., (322. + 13. * s70) / 900.,
+                         (322. - 13. * s70) / 900.},
+                        {0.17132449237917036, 0.36076157304813861, 0.46791393457269104, 0.46791393457269104,
+                         0.36076157304813861, 0.17132449237917036}};
+      for (int a = 0; a < 7; ++a)
+        for (int b = 0; b < 6; ++b) w[a][b] = t[a][b];
+      init = true;
+    }
+    if (K < 1 || K > 6) throw hoc_gpu_error(HOC_EINVAL, "K out of range");
+    return w[K];
+  }
+
+  hoc_config cfg_{};
+  hoc_t h_ = nullptr;
+  bool hybrid_ = false;
+  int32_t lnod_ = 2;
+  std::vector<double> xy_;
+  std::vector<int32_t> elem_, bedge_, bbdry_;
+  std::vector<std::string> names_;
+};

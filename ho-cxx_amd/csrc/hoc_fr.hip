@@ -293,7 +293,7 @@ __global__ __launch_bounds__(kWG) void k_lap(int N, Tab T, Geo G, double* __rest
                                               double* __restrict__ y, const double* __restrict__ r,
                                               const double* __restrict__ diag, double* __restrict__ d,
                                               double c1, double c2, double* __restrict__ part,
-                                              const int* __restrict__ active) {
+                                              const int* __restrict__ active, double* __restrict__ tr_out) {
   if (active && !*active) return;
   constexpr int K2 = K * K;
   __shared__ double lds[kWG * K2];
@@ -399,6 +399,15 @@ __global__ __launch_bounds__(kWG) void k_lap(int N, Tab T, Geo G, double* __rest
 #pragma unroll
       for (int m = 0; m < K2; ++m) lds[threadIdx.x * K2 + m] = u[m];
       tile_store<K2>(x, lds, e0, N);
+      if (tr_out && live) {  // traces of the updated iterate for the next operator application
+        double bv[4][K], bg[4][K];
+        traces<K>(T, u, bv, bg);
+        double* t = tr_out + (size_t)el * 8 * K;
+#pragma unroll
+        for (int f = 0; f < 4; ++f)
+#pragma unroll
+          for (int k = 0; k < K; ++k) { t[f * 2 * K + k] = bv[f][k]; t[f * 2 * K + K + k] = bg[f][k]; }
+      }
       if (part) {  // r . x (PCG's r.z after the last smoothing step)
         tile_load<K2>(r, lds, e0, N);
 #pragma unroll
@@ -875,15 +884,15 @@ hipError_t launch_trace(int K, int pre, int N, const Tab& T, double* x, const do
 
 hipError_t launch_lap(int K, int epi, int N, const Tab& T, const Geo& G, double* x, const double* tr, double* y,
                       const double* r, const double* diag, double* d, double c1, double c2, double* part,
-                      const int* active, hipStream_t s) {
+                      const int* active, hipStream_t s, double* tr_out) {
 #define LL(E)                                                                                                          \
   do {                                                                                                               \
     if (G.gel)                                                                                                       \
       hipLaunchKernelGGL((k_lap<KK, E, true>), dim3(nblk(N)), dim3(kWG), 0, s, N, T, G, x, tr, y, r, diag, d, c1, c2,  \
-                         part, active);                                                                              \
+                         part, active, tr_out);                                                                      \
     else                                                                                                             \
       hipLaunchKernelGGL((k_lap<KK, E, false>), dim3(nblk(N)), dim3(kWG), 0, s, N, T, G, x, tr, y, r, diag, d, c1, c2, \
-                         part, active);                                                                              \
+                         part, active, tr_out);                                                                      \
   } while (0)
   HOC_K_SWITCH(K, {
     switch (epi) {

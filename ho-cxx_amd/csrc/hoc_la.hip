@@ -173,6 +173,28 @@ __global__ void k_csr(int mode, int n, const int32_t* __restrict__ ptr, const in
   else xio[i] = xio[i] + M[i] * (f[i] - s);
 }
 
+// Same operations with LPR lanes per row (rows of many nonzeros): the lanes of a group
+// stride over the row together, so a wavefront's loads are contiguous.
+template <int LPR>
+__global__ void k_csr_vec(int mode, int n, const int32_t* __restrict__ ptr, const int32_t* __restrict__ col,
+                          const double* __restrict__ val, const double* __restrict__ x, const double* __restrict__ f,
+                          double* __restrict__ y, const double* __restrict__ M, double* __restrict__ xio,
+                          const int* __restrict__ F) {
+  if (F && !F[F_ACTIVE]) return;
+  const int lane = threadIdx.x % LPR;
+  const int i = (blockIdx.x * blockDim.x + threadIdx.x) / LPR;
+  double s = 0.;
+  if (i < n)
+    for (int j = ptr[i] + lane; j < ptr[i + 1]; j += LPR) s += val[j] * x[col[j]];
+#pragma unroll
+  for (int off = LPR / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, LPR);
+  if (i >= n || lane != 0) return;
+  if (mode == 0) y[i] = s;
+  else if (mode == 1) y[i] = f[i] - s;
+  else if (mode == 2) y[i] += s;
+  else xio[i] = xio[i] + M[i] * (f[i] - s);
+}
+
 // x = M f (spai0 from a zero guess)
 __global__ void k_scale(int n, const double* __restrict__ M, const double* __restrict__ f, double* __restrict__ x,
                         const int* __restrict__ F) {
@@ -269,8 +291,20 @@ void finalize(hipStream_t s, int phase, const double* part, int np, double* S, i
   hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, phase, part, np, S, F, maxiter);
 }
 void csr(hipStream_t s, int mode, int n, const int32_t* ptr, const int32_t* col, const double* val, const double* x,
-         const double* f, double* y, const double* M, double* xio, const int* F) {
-  hipLaunchKernelGGL(k_csr, dim3((n + 255) / 256), dim3(256), 0, s, mode, n, ptr, col, val, x, f, y, M, xio, F);
+         const double* f, double* y, const double* M, double* xio, const int* F, int lpr) {
+  const size_t threads = (size_t)n * lpr;
+  const dim3 grid((unsigned)((threads + 255) / 256));
+  switch (lpr) {
+    case 1: hipLaunchKernelGGL(k_csr, dim3((n + 255) / 256), dim3(256), 0, s, mode, n, ptr, col, val, x, f, y, M, xio, F); break;
+    case 2: hipLaunchKernelGGL(k_csr_vec<2>, grid, dim3(256), 0, s, mode, n, ptr, col, val, x, f, y, M, xio, F); break;
+    case 4: hipLaunchKernelGGL(k_csr_vec<4>, grid, dim3(256), 0, s, mode, n, ptr, col, val, x, f, y, M, xio, F); break;
+    case 8: hipLaunchKernelGGL(k_csr_vec<8>, grid, dim3(256), 0, s, mode, n, ptr, col, val, x, f, y, M, xio, F); break;
+    default: hipLaunchKernelGGL(k_csr_vec<16>, grid, dim3(256), 0, s, mode, n, ptr, col, val, x, f, y, M, xio, F); break;
+  }
+}
+int csr_lanes(int n, size_t nnz) {
+  const double avg = n ? (double)nnz / n : 0.;
+  return avg >= 24 ? 16 : avg >= 12 ? 8 : avg >= 6 ? 4 : avg >= 3 ? 2 : 1;
 }
 void scale(hipStream_t s, int n, const double* M, const double* f, double* x, const int* F) {
   hipLaunchKernelGGL(k_scale, dim3((n + 255) / 256), dim3(256), 0, s, n, M, f, x, F);

@@ -25,7 +25,8 @@ void bicg_half(hipStream_t s, size_t n, double* x, const double* ph, const doubl
 void zero_if_flag(hipStream_t s, size_t n, double* x, const int* F);
 void finalize(hipStream_t s, int phase, const double* part, int np, double* S, int* F, int maxiter);
 void csr(hipStream_t s, int mode, int n, const int32_t* ptr, const int32_t* col, const double* val, const double* x,
-         const double* f, double* y, const double* M, double* xio, const int* F);
+         const double* f, double* y, const double* M, double* xio, const int* F, int lanes_per_row);
+int csr_lanes(int n, size_t nnz);  // lanes per row suited to a matrix's average row length
 void scale(hipStream_t s, int n, const double* M, const double* f, double* x, const int* F);
 void dense(hipStream_t s, int n, const double* Ainv, const double* f, double* x, const int* F);
 void probe_set(hipStream_t s, int N, int K2, const int32_t* color, int c, int rs, double* x);

@@ -68,10 +68,13 @@ struct DBuf {
 };
 
 struct DevCSR {
-  int n = 0;
+  int n = 0, lanes = 1;
   DBuf<int32_t> ptr, col;
   DBuf<double> val;
-  void upload(const CSR& A) { n = A.n; ptr.upload(A.ptr); col.upload(A.col); val.upload(A.val); }
+  void upload(const CSR& A) {
+    n = A.n; ptr.upload(A.ptr); col.upload(A.col); val.upload(A.val);
+    lanes = csr_lanes(A.n, A.val.size());
+  }
 };
 struct DevLevel {
   DevCSR A, P, R;
@@ -108,7 +111,7 @@ struct Solver {
   BCDev B{};
   // fields
   DBuf<double> w, psi, vel, U, k1, k2, k3, k4, vort, kscratch;
-  DBuf<double> tr, trw, tru;
+  DBuf<double> tr, tr2, trw, tru;
   // Krylov
   DBuf<double> bs, r, z, p, q, rh, v, ph, sh, t, d_cheb, diag, part;
   DBuf<double> S;
@@ -196,18 +199,23 @@ struct Solver {
     d_swpsi.upload(swp); d_swdiff.upload(swd); d_noslip.upload(ns);
     d_esign.upload(bch.edge_sign);
     // affine fast path: every element's metrics constant over its sps and faces
+    // (tolerances relative to each element's metric scale: rounding leaves ~1e-16 noise
+    // in components that vanish on rectangles)
     bool affine = true;
-    auto same = [](double a, double b) { return std::fabs(a - b) <= 1e-12 * std::max(std::fabs(a), std::fabs(b)) + 1e-300; };
     std::vector<double> gel((size_t)N * 3), gface((size_t)N * 8);
     for (int e = 0; e < N && affine; ++e) {
       const size_t b0 = (size_t)e * K2;
+      const double sc = std::max(std::fabs(Ga[b0]), std::fabs(Gc[b0]));
       gel[(size_t)e * 3] = Ga[b0]; gel[(size_t)e * 3 + 1] = Gb[b0]; gel[(size_t)e * 3 + 2] = Gc[b0];
       for (int m = 1; m < K2 && affine; ++m)
-        affine = same(Ga[b0 + m], Ga[b0]) && same(Gb[b0 + m], Gb[b0]) && same(Gc[b0 + m], Gc[b0]);
+        affine = std::fabs(Ga[b0 + m] - Ga[b0]) <= 1e-12 * sc && std::fabs(Gb[b0 + m] - Gb[b0]) <= 1e-12 * sc &&
+                 std::fabs(Gc[b0 + m] - Gc[b0]) <= 1e-12 * sc;
       for (int f = 0; f < 4 && affine; ++f) {
         const size_t q0 = ((size_t)e * 4 + f) * K;
+        const double fs = std::max(std::fabs(fa[q0]), 1e-300);
         gface[((size_t)e * 4 + f) * 2] = fa[q0]; gface[((size_t)e * 4 + f) * 2 + 1] = fb[q0];
-        for (int k = 1; k < K && affine; ++k) affine = same(fa[q0 + k], fa[q0]) && same(fb[q0 + k], fb[q0]);
+        for (int k = 1; k < K && affine; ++k)
+          affine = std::fabs(fa[q0 + k] - fa[q0]) <= 1e-12 * fs && std::fabs(fb[q0 + k] - fb[q0]) <= 1e-12 * fs;
       }
     }
     if (affine) { d_gel.upload(gel); d_gface.upload(gface); }
@@ -224,7 +232,7 @@ struct Solver {
     for (auto* f : {&w, &psi, &k1, &k2, &k3, &k4, &vort, &kscratch, &bs, &r, &z, &p, &q, &rh, &v, &ph, &sh, &t, &d_cheb, &diag})
       f->alloc(D);
     vel.alloc(2 * D); U.alloc(2 * D);
-    tr.alloc((size_t)N * 8 * K); trw.alloc((size_t)N * 8 * K); tru.alloc((size_t)N * 4 * K);
+    tr.alloc((size_t)N * 8 * K); tr2.alloc((size_t)N * 8 * K); trw.alloc((size_t)N * 8 * K); tru.alloc((size_t)N * 4 * K);
     nparts = std::max(lap_blocks(N), vec_grid(D));
     part.alloc(2 * (size_t)nparts);
     S.alloc(S_COUNT); F.alloc(F_COUNT);
@@ -239,10 +247,10 @@ struct Solver {
 
   // Fine-level operator launch, bracketed by events when profiling.
   void lap(int epi, double* x, double* y, const double* rr, const double* dg, double* d, double c1, double c2,
-           double* partp, const int* active) {
+           double* partp, const int* active, const double* tin = nullptr, double* tout = nullptr) {
     const bool rec = prof.on;
     if (rec) HIPCHK(hipEventRecord(prof.e0[prof.n], stream));
-    HIPCHK(launch_lap(K, epi, N, T, G, x, tr, y, rr, dg, d, c1, c2, partp, active, stream));
+    HIPCHK(launch_lap(K, epi, N, T, G, x, tin ? tin : tr.p, y, rr, dg, d, c1, c2, partp, active, stream, tout));
     if (rec) {
       HIPCHK(hipEventRecord(prof.e1[prof.n], stream));
       prof.var[prof.n] = epi;
@@ -441,11 +449,11 @@ struct Solver {
     if (l + 1 == lv.size()) { dense(stream, coarse_n, coarse_inv, f, u, Fp); return; }
     DevLevel& C = *lv[l + 1];
     scale(stream, L.A.n, L.M, f, u, Fp);
-    csr(stream, 1, L.A.n, L.A.ptr, L.A.col, L.A.val, u, f, L.t, nullptr, nullptr, Fp);
-    csr(stream, 0, C.A.n, L.R.ptr, L.R.col, L.R.val, L.t, nullptr, C.f, nullptr, nullptr, Fp);
+    csr(stream, 1, L.A.n, L.A.ptr, L.A.col, L.A.val, u, f, L.t, nullptr, nullptr, Fp, L.A.lanes);
+    csr(stream, 0, C.A.n, L.R.ptr, L.R.col, L.R.val, L.t, nullptr, C.f, nullptr, nullptr, Fp, L.R.lanes);
     coarse_cycle(l + 1, C.f, C.u);
-    csr(stream, 2, L.A.n, L.P.ptr, L.P.col, L.P.val, C.u, nullptr, u, nullptr, nullptr, Fp);
-    csr(stream, 3, L.A.n, L.A.ptr, L.A.col, L.A.val, u, f, nullptr, L.M, u, Fp);
+    csr(stream, 2, L.A.n, L.P.ptr, L.P.col, L.P.val, C.u, nullptr, u, nullptr, nullptr, Fp, L.P.lanes);
+    csr(stream, 3, L.A.n, L.A.ptr, L.A.col, L.A.val, u, f, nullptr, L.M, u, Fp, L.A.lanes);
   }
 
   // zout = V(rin); if part != nullptr, partials of rin.zout are left in part.
@@ -454,17 +462,15 @@ struct Solver {
     double* rr = const_cast<double*>(rin);
     // pre-smoothing: Chebyshev(2) from zero
     HIPCHK(launch_trace(K, PRE_CHEB0, N, T, zout, rr, diag, d_cheb, cheb_theta, nullptr, 0, Fp, tr, stream));
-    lap(EPI_CHEB, zout, nullptr, rr, diag, d_cheb, cheb_c1, cheb_c2, nullptr, Fp);
+    lap(EPI_CHEB, zout, nullptr, rr, diag, d_cheb, cheb_c1, cheb_c2, nullptr, Fp, tr, tr2);
     // restrict the residual to the corner vertices
-    HIPCHK(launch_trace(K, PRE_COPY, N, T, zout, nullptr, nullptr, nullptr, 0., nullptr, 0, Fp, tr, stream));
-    lap(EPI_RESTRICT, zout, ec, rr, nullptr, nullptr, 0., 0., nullptr, Fp);
+    lap(EPI_RESTRICT, zout, ec, rr, nullptr, nullptr, 0., 0., nullptr, Fp, tr2);
     q1_gather(stream, nv, d_vptr, d_vinc, ec, rc, Fp);
     coarse_cycle(0, rc, xc);
     // prolongate and post-smooth: Chebyshev(2) from the corrected iterate
     HIPCHK(launch_trace(K, PRE_PROL, N, T, zout, xc, nullptr, nullptr, 0., nullptr, 0, Fp, tr, stream, d_ev));
-    lap(EPI_CHEBF, zout, nullptr, rr, diag, d_cheb, 0., 1. / cheb_theta, nullptr, Fp);
-    HIPCHK(launch_trace(K, PRE_COPY, N, T, zout, nullptr, nullptr, nullptr, 0., nullptr, 0, Fp, tr, stream));
-    lap(EPI_CHEB, zout, nullptr, rr, diag, d_cheb, cheb_c1, cheb_c2, partp, Fp);
+    lap(EPI_CHEBF, zout, nullptr, rr, diag, d_cheb, 0., 1. / cheb_theta, nullptr, Fp, tr, tr2);
+    lap(EPI_CHEB, zout, nullptr, rr, diag, d_cheb, cheb_c1, cheb_c2, partp, Fp, tr2);
   }
 
   // Poll the active flag with a one-iteration lag; returns false when the solve ended.
@@ -540,6 +546,10 @@ struct Solver {
     HIPCHK(hipMemcpyAsync(hf, F.p, sizeof(hf), hipMemcpyDeviceToHost, stream));
     HIPCHK(hipStreamSynchronize(stream));
     last_iters = hf[F_ITER];
+    if (!std::isfinite(hs[S_BB]) || !std::isfinite(hs[S_RR])) {
+      last_rel = std::nan("");
+      throw StatusError(HOC_ENAN, "NaN/Inf in the Poisson right-hand side or residual");
+    }
     last_rel = hs[S_BB] > 0. ? std::sqrt(hs[S_RR] / hs[S_BB]) : 0.;
     if (hf[F_ZERO]) last_rel = 0.;
     total_iters += last_iters;
