@@ -7,7 +7,11 @@ unstructured element/edge lists, K = 5 (P = 4), Re = 100, zero initial
 vorticity, RK4 with dt = half the measured explicit-RK4 diffusion limit, Krylov
 relative tolerance 1e-10.  Inputs live in HBM for the whole timed region.
 
-Prints ONE JSON line.  The cpu_baseline leg times the FP64 CPU oracle (the
+Prints ONE JSON line.  `value` is the plain FP64 path; by default the same
+workload is then run in mixed precision (FP64 iterative refinement around an FP32
+Krylov + V-cycle) from the same initial state, and the line carries its throughput,
+speed-up and relative L2 field error against the FP64 run after the same steps
+(valid at <= 1e-8).  The cpu_baseline leg times the FP64 CPU oracle (the
 restatement of the reference algorithm, single thread) on a bounded sample of
 the same workload (a 64 x 64 cavity at the same K, Re, dt rule and tolerance).
 """
@@ -58,11 +62,12 @@ def cpu_baseline(K, Re, tol, n=64, warm=2, steps=6):
                       f"({el:.1f} s timed, setup {setup:.1f} s)"}
 
 
-def roofline(prof):
-    """Dominant kernel (largest total time among the fine-operator variants): algorithmic
-    bytes per launch over its average launch duration, from HIP events on the library's
-    stream during one profiled RK4 step."""
-    name, v = max(prof.items(), key=lambda kv: kv[1]["total_ms"])
+def roofline(prof, suffix=""):
+    """Dominant kernel (largest total time among the fine-operator variants of one
+    precision): algorithmic bytes per launch over its average launch duration, from HIP
+    events on the library's stream during one profiled RK4 step."""
+    cand = {k: v for k, v in prof.items() if v["launches"] > 0 and k.endswith("_f32") == (suffix == "_f32")}
+    name, v = max(cand.items(), key=lambda kv: kv[1]["total_ms"])
     avg_s = v["total_ms"] / max(1, v["launches"]) / 1e3
     gbs = v["bytes_per_launch"] / avg_s / 1e9
     return {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
@@ -70,7 +75,41 @@ def roofline(prof):
             "bytes_per_launch": v["bytes_per_launch"], "launches": v["launches"],
             "all_variants": {k: {"avg_us": 1e3 * x["total_ms"] / max(1, x["launches"]),
                                  "GBs": x["bytes_per_launch"] / max(1e-12, x["total_ms"] / max(1, x["launches"]) / 1e3) / 1e9,
-                                 "launches": x["launches"]} for k, x in prof.items()}}
+                                 "launches": x["launches"]} for k, x in cand.items()}}
+
+
+def run(hoc, m, n, K, Re, dt, tol, precision, warmup, steps):
+    """Setup, warm-up, one profiled step and `steps` timed steps of one handle."""
+    g = hoc.HO2D(K=K, problem_type=1, Re=Re, dt=dt, time_integration_type=4, advection_type=1,
+                 tol=tol, maxiter=500, precision=precision)
+    g.load_mesh(m)
+    t0 = time.perf_counter()
+    g.setup()
+    setup_s = time.perf_counter() - t0
+    st = g.stats()
+    tag = "mixed" if precision == hoc.HOC_MIXED else "fp64"
+    log(f"[{tag}] setup {setup_s:.1f} s, dofs {st['n_dofs']}, levels {st['n_levels']}, krylov {st['krylov']}")
+    for i in range(warmup):
+        g.step(1)
+        log(f"[{tag}] warmup step {i}: iters/solve {g.stats()['last_iters']}")
+    # untimed pass with per-launch events on the fine-operator kernel (live roofline)
+    g.profile(True)
+    g.step(1)
+    prof = g.profile_read()
+    g.profile(False)
+    s0 = g.stats()
+    t0 = time.perf_counter()
+    g.step(steps)   # synchronizes the device at the end
+    el = time.perf_counter() - t0
+    st = g.stats()
+    ns = max(1, st["n_solves"] - s0["n_solves"])
+    iters = (st["total_iters"] - s0["total_iters"]) / ns
+    D = st["n_dofs"]
+    value = D * steps / el / 1e6
+    log(f"[{tag}] timed {steps} steps in {el:.3f} s: {value:.2f} Mdof-steps/s, {iters:.1f} Krylov its/solve, "
+        f"last solve: {st['last_refinements']} refinement passes, rel residual {st['last_rel_residual']:.2e}")
+    return g, {"value": value, "ms_per_step": 1e3 * el / steps, "krylov_iters_per_solve": iters, "setup_s": setup_s,
+               "last_refinements": st["last_refinements"], "last_rel_residual": st["last_rel_residual"]}, prof
 
 
 def main():
@@ -82,7 +121,9 @@ def main():
     ap.add_argument("--K", type=int, default=5)
     ap.add_argument("--Re", type=float, default=100.0)
     ap.add_argument("--tol", type=float, default=1e-10)
-    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--precision", choices=["fp64", "mixed", "both"], default="both",
+                    help="fp64: plain path only; both (default): plain FP64 line plus the mixed-precision "
+                         "run from the same initial state, with its field error against the FP64 run")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -97,47 +138,42 @@ def main():
     t0 = time.perf_counter()
     m = meshes.cavity(n, 2)
     log(f"mesh {n}x{n}: {time.perf_counter() - t0:.1f} s")
-    prec = hoc.HOC_MIXED if args.precision == "mixed" else hoc.HOC_FP64
-    g = hoc.HO2D(K=K, problem_type=1, Re=Re, dt=dt, time_integration_type=4, advection_type=1,
-                 tol=args.tol, maxiter=500, precision=prec)
-    g.load_mesh(m)
-    t0 = time.perf_counter()
-    g.setup()
-    setup_s = time.perf_counter() - t0
-    st = g.stats()
-    log(f"setup {setup_s:.1f} s, dofs {st['n_dofs']}, levels {st['n_levels']}, krylov {st['krylov']}")
-    for i in range(args.warmup):
-        g.step(1)
-        log(f"warmup step {i}: iters/solve {g.stats()['last_iters']}")
-    it0 = g.stats()["total_iters"]
-    s0 = g.stats()["n_solves"]
-    # untimed pass with per-launch events on the fine-operator kernel (live roofline)
-    g.profile(True)
-    g.step(1)
-    prof = g.profile_read()
-    g.profile(False)
-    it0 = g.stats()["total_iters"]
-    s0 = g.stats()["n_solves"]
-    t0 = time.perf_counter()
-    g.step(args.steps)   # synchronizes the device at the end
-    el = time.perf_counter() - t0
-    st = g.stats()
-    iters = (st["total_iters"] - it0) / max(1, st["n_solves"] - s0)
-    D = st["n_dofs"]
-    value = D * args.steps / el / 1e6
-    log(f"timed {args.steps} steps in {el:.3f} s: {value:.2f} Mdof-steps/s, {iters:.1f} Krylov its/solve")
+    primary = hoc.HOC_MIXED if args.precision == "mixed" else hoc.HOC_FP64
+    g, res, prof = run(hoc, m, n, K, Re, dt, args.tol, primary, args.warmup, args.steps)
+    D = g.stats()["n_dofs"]
     out = {
         "metric": "Mdof-steps/s per RK4 step incl. Poisson",
-        "value": value, "unit": "Mdof-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": 1e3 * el / args.steps, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f64" if prec == hoc.HOC_FP64 else "f64+f32",
+        "value": res["value"], "unit": "Mdof-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": res["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64" if primary == hoc.HOC_FP64 else "f64+f32",
         "data": "synthetic (lid-driven cavity, zero initial vorticity)",
         "config": {"workload": f"lid-driven cavity {n}x{n} quads (unstructured storage), K={K} (P={K - 1}), "
                                f"Re={Re:g}, RK4 dt={dt:.3g}, Krylov tol={args.tol:g}",
-                   "n_elem": n * n, "dofs": D, "krylov_iters_per_solve": iters, "setup_s": setup_s},
-        "roofline": roofline(prof),
+                   "n_elem": n * n, "dofs": D, "precision": "fp64" if primary == hoc.HOC_FP64 else "mixed",
+                   "krylov_iters_per_solve": res["krylov_iters_per_solve"], "setup_s": res["setup_s"]},
+        "roofline": roofline(prof, "" if primary == hoc.HOC_FP64 else "_f32"),
         "cpu_baseline": None,
     }
+    if args.precision == "both":
+        g2, res2, prof2 = run(hoc, m, n, K, Re, dt, args.tol, hoc.HOC_MIXED, args.warmup, args.steps)
+        a = g.solution_areas()
+        err = {}
+        for fld, get in (("omega", "get_vorticity"), ("psi", "get_streamfunction")):
+            x64, x32 = getattr(g, get)(), getattr(g2, get)()
+            dn = np.linalg.norm(x64)
+            qd = np.sqrt(np.sum(a * x64 * x64))
+            err[fld] = {"rel_l2": float(np.linalg.norm(x32 - x64) / dn) if dn > 0 else 0.0,
+                        "rel_l2_quadrature": float(np.sqrt(np.sum(a * (x32 - x64) ** 2)) / qd) if qd > 0 else 0.0}
+        worst = max(v for e in err.values() for v in e.values())
+        out["mixed"] = {"value": res2["value"], "unit": "Mdof-steps/s", "ms_per_step": res2["ms_per_step"],
+                        "speedup_vs_fp64": res2["value"] / res["value"],
+                        "field_error_vs_fp64": err, "valid_at_1e-8": worst <= 1e-8,
+                        "krylov_iters_per_solve": res2["krylov_iters_per_solve"],
+                        "refinements_last_solve": res2["last_refinements"],
+                        "roofline": roofline(prof2, "_f32"),
+                        "dtype": "f64 outer residual + f32 Krylov/V-cycle"}
+        log(f"mixed: {res2['value']:.2f} Mdof-steps/s ({res2['value'] / res['value']:.2f}x), field error {err}")
+        g2.close()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(K, Re, args.tol)
     print(json.dumps(out))

@@ -15,19 +15,20 @@ __device__ __forceinline__ double wave_sum(double v) {
 }
 
 __device__ __forceinline__ int rev_flag(int ijp, int ijpm) { return (ijp + ijpm - ijp / 2 - ijpm / 2 + 1) & 1; }
-__device__ __forceinline__ double face_sign(int ijp, int ijpm) { return ((ijp + ijpm) & 1) ? 1.0 : -1.0; }
+__device__ __forceinline__ int face_sign_i(int ijp, int ijpm) { return ((ijp + ijpm) & 1) ? 1 : -1; }
+__device__ __forceinline__ double face_sign(int ijp, int ijpm) { return (double)face_sign_i(ijp, ijpm); }
 // element-local flux point k of face ijp -> boundary-edge flux point (src/HO_2D.cpp:2229)
 template <int K>
 __device__ __forceinline__ int edge_fp(int ijp, int k) { return (ijp == 0 || ijp == 3) ? K - 1 - k : k; }
 
 // Value and normal-derivative traces on the four faces (src/HO_2D.cpp:2695-2713).
-template <int K>
-__device__ __forceinline__ void traces(const Tab& T, const double (&u)[K * K], double (&bv)[4][K], double (&bg)[4][K]) {
+template <int K, class R>
+__device__ __forceinline__ void traces(const TabT<R>& T, const R (&u)[K * K], R (&bv)[4][K], R (&bg)[4][K]) {
 #pragma unroll
   for (int rc = 0; rc < K; ++rc)
 #pragma unroll
     for (int ib = 0; ib < 2; ++ib) {
-      double s = 0., g = 0., s2 = 0., g2 = 0.;
+      R s = 0, g = 0, s2 = 0, g2 = 0;
 #pragma unroll
       for (int m = 0; m < K; ++m) {
         s += u[rc * K + m] * T.bb[m][ib];
@@ -42,64 +43,64 @@ __device__ __forceinline__ void traces(const Tab& T, const double (&u)[K * K], d
 
 // Common value and common normal flux on an interior face (src/HO_2D.cpp:2833-2870 applied
 // to both sides, then the average of :2742-2768).
-template <int K>
-__device__ __forceinline__ void interior_face(const Tab& T, int ib, int ibn, int rev, double sgn,
-                                              const double (&bv)[K], const double (&bg)[K],
-                                              const double (&vn)[K], const double (&gn)[K],
-                                              const double (&fa)[K], const double (&fb)[K],
-                                              const double (&fan)[K], const double (&fbn)[K],
-                                              double (&cv)[K], double (&cg)[K]) {
-  double com[K], comn[K], F[K], Fn[K];
+template <int K, class R>
+__device__ __forceinline__ void interior_face(const TabT<R>& T, int ib, int ibn, int rev, R sgn,
+                                              const R (&bv)[K], const R (&bg)[K],
+                                              const R (&vn)[K], const R (&gn)[K],
+                                              const R (&fa)[K], const R (&fb)[K],
+                                              const R (&fan)[K], const R (&fbn)[K],
+                                              R (&cv)[K], R (&cg)[K]) {
+  R com[K], comn[K], F[K], Fn[K];
 #pragma unroll
-  for (int k = 0; k < K; ++k) com[k] = 0.5 * (bv[k] + (rev ? vn[K - 1 - k] : vn[k]));
+  for (int k = 0; k < K; ++k) com[k] = R(0.5) * (bv[k] + (rev ? vn[K - 1 - k] : vn[k]));
 #pragma unroll
   for (int m = 0; m < K; ++m) comn[m] = rev ? com[K - 1 - m] : com[m];
 #pragma unroll
   for (int k = 0; k < K; ++k) {
-    double cr = 0., crn = 0.;
+    R cr = 0, crn = 0;
 #pragma unroll
     for (int d = 0; d < K; ++d) { cr += com[d] * T.D[d][k]; crn += comn[d] * T.D[d][k]; }
-    const double dv = bg[k] + (com[k] - bv[k]) * T.gp[ib];
-    const double dvn = gn[k] + (comn[k] - vn[k]) * T.gp[ibn];
+    const R dv = bg[k] + (com[k] - bv[k]) * T.gp[ib];
+    const R dvn = gn[k] + (comn[k] - vn[k]) * T.gp[ibn];
     F[k] = dv * fa[k] - cr * fb[k];
     Fn[k] = dvn * fan[k] - crn * fbn[k];
   }
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     cv[k] = com[k];
-    cg[k] = 0.5 * (F[k] + sgn * (rev ? Fn[K - 1 - k] : Fn[k]));
+    cg[k] = R(0.5) * (F[k] + sgn * (rev ? Fn[K - 1 - k] : Fn[k]));
   }
 }
 
 // Boundary face (src/HO_2D.cpp:2872-2927).  bc: edge values (edge flux-point order) or
 // nullptr for zero values; bscale multiplies the Neumann values.
-template <int K>
-__device__ __forceinline__ void boundary_face(const Tab& T, const Geo& G, int el, int ijp, int sw,
-                                              const double* __restrict__ bc, double bscale,
-                                              const double (&bv)[K], const double (&bg)[K],
-                                              const double (&fa)[K], const double (&fb)[K],
-                                              double (&cv)[K], double (&cg)[K]) {
+template <int K, class R>
+__device__ __forceinline__ void boundary_face(const TabT<R>& T, const GeoT<R>& G, int el, int ijp, int sw,
+                                              const R* __restrict__ bc, R bscale,
+                                              const R (&bv)[K], const R (&bg)[K],
+                                              const R (&fa)[K], const R (&fb)[K],
+                                              R (&cv)[K], R (&cg)[K]) {
   const int ib = ijp & 1;
   if (sw == 0) {  // Dirichlet: common value = BC
-    double com[K];
+    R com[K];
 #pragma unroll
-    for (int m = 0; m < K; ++m) com[m] = bc ? bc[edge_fp<K>(ijp, m)] : 0.;
+    for (int m = 0; m < K; ++m) com[m] = bc ? bc[edge_fp<K>(ijp, m)] : R(0);
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-      double cr = 0.;
+      R cr = 0;
 #pragma unroll
       for (int d = 0; d < K; ++d) cr += com[d] * T.D[d][k];
-      const double dv = bg[k] + (com[k] - bv[k]) * T.gp[ib];
+      const R dv = bg[k] + (com[k] - bv[k]) * T.gp[ib];
       cv[k] = com[k];
       cg[k] = dv * fa[k] - cr * fb[k];
     }
   } else {  // Neumann: common flux = BC * |g|, common value from the K x K system
     const size_t q = ((size_t)el * 4 + ijp) * K;
-    double M[K][K], r[K];
+    R M[K][K], r[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-      const double A = G.fA[q + k], B = G.fB[q + k], J = G.fjac[q + k];
-      cg[k] = (bc ? bscale * bc[edge_fp<K>(ijp, k)] : 0.) * G.fAn[q + k];
+      const R A = G.fA[q + k], B = G.fB[q + k], J = G.fjac[q + k];
+      cg[k] = (bc ? bscale * bc[edge_fp<K>(ijp, k)] : R(0)) * G.fAn[q + k];
       r[k] = J * cg[k] - A * (bg[k] - bv[k] * T.gp[ib]);
 #pragma unroll
       for (int i = 0; i < K; ++i) M[k][i] = -B * T.D[i][k];
@@ -110,14 +111,14 @@ __device__ __forceinline__ void boundary_face(const Tab& T, const Geo& G, int el
     for (int k = 0; k < K - 1; ++k)
 #pragma unroll
       for (int i = k + 1; i < K; ++i) {
-        const double c = M[i][k] / M[k][k];
+        const R c = M[i][k] / M[k][k];
         r[i] -= c * r[k];
 #pragma unroll
         for (int j = k + 1; j < K; ++j) M[i][j] -= c * M[k][j];
       }
 #pragma unroll
     for (int i = K - 1; i >= 0; --i) {
-      double c = 0.;
+      R c = 0;
 #pragma unroll
       for (int j = i + 1; j < K; ++j) c += M[i][j] * cv[j];
       cv[i] = (r[i] - c) / M[i][i];
@@ -126,12 +127,12 @@ __device__ __forceinline__ void boundary_face(const Tab& T, const Geo& G, int el
 }
 
 // Gathers one face's neighbour trace pair (value, derivative) and neighbour metrics.
-template <int K, bool AFF = false>
-__device__ __forceinline__ void load_nbr(const Geo& G, const double* __restrict__ tr, int nb, int fn,
-                                         double (&vn)[K], double (&gn)[K], double (&fan)[K], double (&fbn)[K]) {
-  const double* t = tr + ((size_t)nb * 4 + fn) * 2 * K;
+template <int K, bool AFF = false, class R>
+__device__ __forceinline__ void load_nbr(const GeoT<R>& G, const R* __restrict__ tr, int nb, int fn,
+                                         R (&vn)[K], R (&gn)[K], R (&fan)[K], R (&fbn)[K]) {
+  const R* t = tr + ((size_t)nb * 4 + fn) * 2 * K;
   if (AFF) {
-    const double a = G.gface[((size_t)nb * 4 + fn) * 2], b = G.gface[((size_t)nb * 4 + fn) * 2 + 1];
+    const R a = G.gface[((size_t)nb * 4 + fn) * 2], b = G.gface[((size_t)nb * 4 + fn) * 2 + 1];
 #pragma unroll
     for (int k = 0; k < K; ++k) { vn[k] = t[k]; gn[k] = t[K + k]; fan[k] = a; fbn[k] = b; }
   } else {
@@ -143,17 +144,17 @@ __device__ __forceinline__ void load_nbr(const Geo& G, const double* __restrict_
 
 // All four faces of the second-order operator.  mode: 0 Poisson (psi switches, bc values
 // or zero), 1 diffusion (diffusion switches).
-template <int K, bool AFF = false>
-__device__ __forceinline__ void faces2(const Tab& T, const Geo& G, int el, const double* __restrict__ tr,
-                                       const int8_t* __restrict__ sw, const double* __restrict__ bcv, bool signed_neumann,
-                                       const double (&bv)[4][K], const double (&bg)[4][K],
-                                       double (&cv)[4][K], double (&cg)[4][K]) {
+template <int K, bool AFF = false, class R>
+__device__ __forceinline__ void faces2(const TabT<R>& T, const GeoT<R>& G, int el, const R* __restrict__ tr,
+                                       const int8_t* __restrict__ sw, const R* __restrict__ bcv, bool signed_neumann,
+                                       const R (&bv)[4][K], const R (&bg)[4][K],
+                                       R (&cv)[4][K], R (&cg)[4][K]) {
 #pragma unroll
   for (int f = 0; f < 4; ++f) {
     const int nb = G.nb[(size_t)el * 4 + f];
-    double fa[K], fb[K];
+    R fa[K], fb[K];
     if (AFF) {
-      const double a = G.gface[((size_t)el * 4 + f) * 2], b = G.gface[((size_t)el * 4 + f) * 2 + 1];
+      const R a = G.gface[((size_t)el * 4 + f) * 2], b = G.gface[((size_t)el * 4 + f) * 2 + 1];
 #pragma unroll
       for (int k = 0; k < K; ++k) { fa[k] = a; fb[k] = b; }
     } else {
@@ -163,30 +164,31 @@ __device__ __forceinline__ void faces2(const Tab& T, const Geo& G, int el, const
     }
     if (nb >= 0) {
       const int fn = G.nbf[(size_t)el * 4 + f];
-      double vn[K], gn[K], fan[K], fbn[K];
+      R vn[K], gn[K], fan[K], fbn[K];
       load_nbr<K, AFF>(G, tr, nb, fn, vn, gn, fan, fbn);
-      interior_face<K>(T, f & 1, fn & 1, rev_flag(f, fn), face_sign(f, fn), bv[f], bg[f], vn, gn, fa, fb, fan, fbn, cv[f], cg[f]);
+      interior_face<K>(T, f & 1, fn & 1, rev_flag(f, fn), R(face_sign_i(f, fn)), bv[f], bg[f], vn, gn, fa, fb, fan, fbn,
+                       cv[f], cg[f]);
     } else {
       const int e = -1 - nb;
-      const double* b = bcv ? bcv + (size_t)e * K : nullptr;
-      const double sc = signed_neumann ? G.esign[e] : 1.0;
+      const R* b = bcv ? bcv + (size_t)e * K : nullptr;
+      const R sc = signed_neumann ? G.esign[e] : R(1);
       boundary_face<K>(T, G, el, f, sw[e], b, sc, bv[f], bg[f], fa, fb, cv[f], cg[f]);
     }
   }
 }
 
 // Contravariant corrected gradient (f~, g~) and its corrected divergence (src/HO_2D.cpp:2771-2816).
-template <int K>
-__device__ __forceinline__ void second_order_core(const Tab& T, const double (&u)[K * K], const double (&bv)[4][K],
-                                                  const double (&cv)[4][K], const double (&cg)[4][K],
-                                                  const double* ga, const double* gb, const double* gc,
-                                                  double (&out)[K * K]) {
-  double ft[K * K], gt[K * K];
+template <int K, class R>
+__device__ __forceinline__ void second_order_core(const TabT<R>& T, const R (&u)[K * K], const R (&bv)[4][K],
+                                                  const R (&cv)[4][K], const R (&cg)[4][K],
+                                                  const R* ga, const R* gb, const R* gc,
+                                                  R (&out)[K * K]) {
+  R ft[K * K], gt[K * K];
 #pragma unroll
   for (int ky = 0; ky < K; ++ky)
 #pragma unroll
     for (int kx = 0; kx < K; ++kx) {
-      double dx = 0., de = 0.;
+      R dx = 0, de = 0;
 #pragma unroll
       for (int d = 0; d < K; ++d) { dx += u[ky * K + d] * T.D[d][kx]; de += u[d * K + kx] * T.D[d][ky]; }
       dx += (cv[0][ky] - bv[0][ky]) * T.grad[kx][0] + (cv[1][ky] - bv[1][ky]) * T.grad[kx][1];
@@ -195,12 +197,12 @@ __device__ __forceinline__ void second_order_core(const Tab& T, const double (&u
       ft[v] = dx * ga[v] - de * gb[v];
       gt[v] = de * gc[v] - dx * gb[v];
     }
-  double fB[2][K], gB[2][K];
+  R fB[2][K], gB[2][K];
 #pragma unroll
   for (int ib = 0; ib < 2; ++ib)
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-      double a = cg[ib][k], b = cg[2 + ib][k];
+      R a = cg[ib][k], b = cg[2 + ib][k];
 #pragma unroll
       for (int d = 0; d < K; ++d) { a -= ft[k * K + d] * T.bb[d][ib]; b -= gt[d * K + k] * T.bb[d][ib]; }
       fB[ib][k] = a; gB[ib][k] = b;
@@ -209,7 +211,7 @@ __device__ __forceinline__ void second_order_core(const Tab& T, const double (&u
   for (int ky = 0; ky < K; ++ky)
 #pragma unroll
     for (int kx = 0; kx < K; ++kx) {
-      double r = 0.;
+      R r = 0;
 #pragma unroll
       for (int d = 0; d < K; ++d) r += ft[ky * K + d] * T.D[d][kx];
       r += fB[0][ky] * T.grad[kx][0] + fB[1][ky] * T.grad[kx][1];
@@ -228,32 +230,32 @@ __device__ __forceinline__ void second_order_core(const Tab& T, const double (&u
 //   PRE_PUPD   u = z + beta * p; writes p = u                     (PCG direction)
 //   PRE_ZERO   u = 0                                              (RHS assembly)
 // ==========================================================================================
-template <int K, int PRE>
-__global__ __launch_bounds__(kWG) void k_trace(int N, Tab T, double* __restrict__ x, const double* __restrict__ a,
-                                                const double* __restrict__ b, double* __restrict__ d, double coef,
+template <int K, int PRE, class R>
+__global__ __launch_bounds__(kWG) void k_trace(int N, TabT<R> T, R* __restrict__ x, const R* __restrict__ a,
+                                                const R* __restrict__ b, R* __restrict__ d, R coef,
                                                 const double* __restrict__ scal, int scal_idx, const int* __restrict__ active,
-                                                double* __restrict__ tr, const int32_t* __restrict__ ev) {
+                                                R* __restrict__ tr, const int32_t* __restrict__ ev) {
   if (active && !*active) return;
   constexpr int K2 = K * K;
-  __shared__ double lds[kWG * K2];
-  __shared__ double lds2[kWG * K2];
+  __shared__ __align__(16) R lds[kWG * K2];
+  __shared__ __align__(16) R lds2[kWG * K2];
   const int e0 = blockIdx.x * kWG;
   const int el = e0 + threadIdx.x;
   const bool live = el < N;
-  double u[K2];
+  R u[K2];
   if (PRE == PRE_COPY || PRE == PRE_PROL) tile_load<K2>(x, lds, e0, N);
   if (PRE == PRE_CHEB0) { tile_load<K2>(a, lds, e0, N); tile_load<K2>(b, lds2, e0, N); }
   if (PRE == PRE_PUPD) { tile_load<K2>(a, lds, e0, N); tile_load<K2>(x, lds2, e0, N); }
-  double beta = 0.;
-  if (PRE == PRE_PUPD) beta = scal[scal_idx];
-  double xc[4] = {0., 0., 0., 0.};
+  R beta = 0;
+  if (PRE == PRE_PUPD) beta = (R)scal[scal_idx];
+  R xc[4] = {0, 0, 0, 0};
   if (PRE == PRE_PROL && live)
 #pragma unroll
     for (int c = 0; c < 4; ++c) xc[c] = a[ev[(size_t)el * 4 + c]];
 #pragma unroll
   for (int m = 0; m < K2; ++m) {
     const int li = threadIdx.x * K2 + m;
-    double v = 0.;
+    R v = 0;
     if (PRE == PRE_COPY) v = lds[li];
     if (PRE == PRE_PROL) {
       const int j = m / K, i = m % K;
@@ -271,9 +273,9 @@ __global__ __launch_bounds__(kWG) void k_trace(int N, Tab T, double* __restrict_
     else tile_store<K2>(x, lds, e0, N);
   }
   if (!live) return;
-  double bv[4][K], bg[4][K];
+  R bv[4][K], bg[4][K];
   traces<K>(T, u, bv, bg);
-  double* t = tr + (size_t)el * 8 * K;
+  R* t = tr + (size_t)el * 8 * K;
 #pragma unroll
   for (int f = 0; f < 4; ++f)
 #pragma unroll
@@ -288,25 +290,25 @@ __global__ __launch_bounds__(kWG) void k_trace(int N, Tab T, double* __restrict_
 //   EPI_CHEBF     res = r - As x; d = res/(diag theta); x += d     (first step from x != 0)
 //   EPI_RESTRICT  ec[el][a] = sum_ij q1w(a,ij) (r - As x)_ij        (Pc^T residual, per element)
 // ==========================================================================================
-template <int K, int EPI, bool AFF>
-__global__ __launch_bounds__(kWG) void k_lap(int N, Tab T, Geo G, double* __restrict__ x, const double* __restrict__ tr,
-                                              double* __restrict__ y, const double* __restrict__ r,
-                                              const double* __restrict__ diag, double* __restrict__ d,
-                                              double c1, double c2, double* __restrict__ part,
-                                              const int* __restrict__ active, double* __restrict__ tr_out) {
+template <int K, int EPI, bool AFF, class R>
+__global__ __launch_bounds__(kWG) void k_lap(int N, TabT<R> T, GeoT<R> G, R* __restrict__ x, const R* __restrict__ tr,
+                                              R* __restrict__ y, const R* __restrict__ r,
+                                              const R* __restrict__ diag, R* __restrict__ d,
+                                              R c1, R c2, double* __restrict__ part,
+                                              const int* __restrict__ active, R* __restrict__ tr_out) {
   if (active && !*active) return;
   constexpr int K2 = K * K;
-  __shared__ double lds[kWG * K2];
+  __shared__ __align__(16) R lds[kWG * K2];
   const int e0 = blockIdx.x * kWG;
   const int el = e0 + threadIdx.x;
   const bool live = el < N;
-  double u[K2], ga[K2], gb[K2], gc[K2];
+  R u[K2], ga[K2], gb[K2], gc[K2];
   tile_load<K2>(x, lds, e0, N);
 #pragma unroll
-  for (int m = 0; m < K2; ++m) u[m] = live ? lds[threadIdx.x * K2 + m] : 0.;  // tail threads: no stale LDS
+  for (int m = 0; m < K2; ++m) u[m] = live ? lds[threadIdx.x * K2 + m] : R(0);  // tail threads: no stale LDS
   if (AFF) {  // metrics constant per element
-    const double a = live ? G.gel[(size_t)el * 3] : 0., b = live ? G.gel[(size_t)el * 3 + 1] : 0.,
-                 c = live ? G.gel[(size_t)el * 3 + 2] : 0.;
+    const R a = live ? G.gel[(size_t)el * 3] : R(0), b = live ? G.gel[(size_t)el * 3 + 1] : R(0),
+            c = live ? G.gel[(size_t)el * 3 + 2] : R(0);
 #pragma unroll
     for (int m = 0; m < K2; ++m) { ga[m] = a; gb[m] = b; gc[m] = c; }
   } else {
@@ -323,11 +325,11 @@ __global__ __launch_bounds__(kWG) void k_lap(int N, Tab T, Geo G, double* __rest
 #pragma unroll
     for (int m = 0; m < K2; ++m) gc[m] = lds[threadIdx.x * K2 + m];
   }
-  double out[K2];
+  R out[K2];
   if (live) {
-    double bv[4][K], bg[4][K], cv[4][K], cg[4][K];
+    R bv[4][K], bg[4][K], cv[4][K], cg[4][K];
     traces<K>(T, u, bv, bg);
-    faces2<K, AFF>(T, G, el, tr, G.sw_psi, nullptr, false, bv, bg, cv, cg);
+    faces2<K, AFF>(T, G, el, tr, G.sw_psi, static_cast<const R*>(nullptr), false, bv, bg, cv, cg);
     second_order_core<K>(T, u, bv, cv, cg, ga, gb, gc, out);
 #pragma unroll
     for (int j = 0; j < K; ++j)
@@ -335,7 +337,7 @@ __global__ __launch_bounds__(kWG) void k_lap(int N, Tab T, Geo G, double* __rest
       for (int i = 0; i < K; ++i) out[j * K + i] *= -(T.w[j] * T.w[i]);
   } else {
 #pragma unroll
-    for (int m = 0; m < K2; ++m) out[m] = 0.;
+    for (int m = 0; m < K2; ++m) out[m] = 0;
   }
   double p0 = 0., p1 = 0.;
   if (EPI == EPI_STORE) {
@@ -348,7 +350,7 @@ __global__ __launch_bounds__(kWG) void k_lap(int N, Tab T, Geo G, double* __rest
   } else {
     __syncthreads();
     tile_load<K2>(r, lds, e0, N);
-    double res[K2];
+    R res[K2];
 #pragma unroll
     for (int m = 0; m < K2; ++m) res[m] = lds[threadIdx.x * K2 + m] - out[m];
     if (EPI == EPI_RESID) {
@@ -356,7 +358,7 @@ __global__ __launch_bounds__(kWG) void k_lap(int N, Tab T, Geo G, double* __rest
       for (int j = 0; j < K; ++j)
 #pragma unroll
         for (int i = 0; i < K; ++i) {
-          const double v = res[j * K + i], uw = v / (T.w[j] * T.w[i]);
+          const double v = res[j * K + i], uw = v / ((double)T.w[j] * T.w[i]);
           p0 += v * v; p1 += uw * uw;
         }
       if (!live) { p0 = 0.; p1 = 0.; }
@@ -365,7 +367,7 @@ __global__ __launch_bounds__(kWG) void k_lap(int N, Tab T, Geo G, double* __rest
       for (int m = 0; m < K2; ++m) lds[threadIdx.x * K2 + m] = res[m];
       tile_store<K2>(y, lds, e0, N);
     } else if (EPI == EPI_RESTRICT) {
-      double s[4] = {0., 0., 0., 0.};
+      R s[4] = {0, 0, 0, 0};
 #pragma unroll
       for (int j = 0; j < K; ++j)
 #pragma unroll
@@ -378,7 +380,7 @@ __global__ __launch_bounds__(kWG) void k_lap(int N, Tab T, Geo G, double* __rest
     } else {  // Chebyshev updates
       __syncthreads();
       tile_load<K2>(diag, lds, e0, N);
-      double dd[K2];
+      R dd[K2];
 #pragma unroll
       for (int m = 0; m < K2; ++m) dd[m] = res[m] / lds[threadIdx.x * K2 + m];
       if (EPI == EPI_CHEB) {
@@ -400,9 +402,9 @@ __global__ __launch_bounds__(kWG) void k_lap(int N, Tab T, Geo G, double* __rest
       for (int m = 0; m < K2; ++m) lds[threadIdx.x * K2 + m] = u[m];
       tile_store<K2>(x, lds, e0, N);
       if (tr_out && live) {  // traces of the updated iterate for the next operator application
-        double bv[4][K], bg[4][K];
+        R bv[4][K], bg[4][K];
         traces<K>(T, u, bv, bg);
-        double* t = tr_out + (size_t)el * 8 * K;
+        R* t = tr_out + (size_t)el * 8 * K;
 #pragma unroll
         for (int f = 0; f < 4; ++f)
 #pragma unroll
@@ -432,7 +434,7 @@ __global__ __launch_bounds__(kWG) void k_rhs(int N, Tab T, Geo G, const double* 
                                               const double* __restrict__ bcpsi, double* __restrict__ bs,
                                               double* __restrict__ part) {
   constexpr int K2 = K * K;
-  __shared__ double lds[kWG * K2];
+  __shared__ __align__(16) double lds[kWG * K2];
   const int e0 = blockIdx.x * kWG;
   const int el = e0 + threadIdx.x;
   const bool live = el < N;
@@ -505,7 +507,7 @@ __global__ __launch_bounds__(kWG) void k_velocity(int N, Tab T, Geo G, BCDev B, 
                                                    double* __restrict__ trw, double* __restrict__ tru,
                                                    double inv_dt_re) {
   constexpr int K2 = K * K;
-  __shared__ double lds[kWG * K2 * 2];
+  __shared__ __align__(16) double lds[kWG * K2 * 2];
   const int e0 = blockIdx.x * kWG;
   const int el = e0 + threadIdx.x;
   const bool live = el < N;
@@ -639,7 +641,7 @@ __global__ __launch_bounds__(kWG) void k_stage(int N, Tab T, Geo G, BCDev B, con
                                                 const double* __restrict__ tru, double re_inv, double dt,
                                                 StageOut so) {
   constexpr int K2 = K * K;
-  __shared__ double lds[kWG * K2 * 2];
+  __shared__ __align__(16) double lds[kWG * K2 * 2];
   const int e0 = blockIdx.x * kWG;
   const int el = e0 + threadIdx.x;
   const bool live = el < N;
@@ -865,10 +867,12 @@ __global__ void k_update_bcs(int n, BCDev B, double tc) {
 
 static inline int nblk(int N) { return (N + kWG - 1) / kWG; }
 
-hipError_t launch_trace(int K, int pre, int N, const Tab& T, double* x, const double* a, const double* b, double* d,
-                        double coef, const double* scal, int scal_idx, const int* active, double* tr, hipStream_t s,
-                        const int32_t* ev) {
-#define LT(P) hipLaunchKernelGGL((k_trace<KK, P>), dim3(nblk(N)), dim3(kWG), 0, s, N, T, x, a, b, d, coef, scal, scal_idx, active, tr, ev)
+template <class R>
+static hipError_t launch_trace_t(int K, int pre, int N, const TabT<R>& T, R* x, const R* a, const R* b, R* d,
+                                 double coef, const double* scal, int scal_idx, const int* active, R* tr, hipStream_t s,
+                                 const int32_t* ev) {
+#define LT(P) hipLaunchKernelGGL((k_trace<KK, P, R>), dim3(nblk(N)), dim3(kWG), 0, s, N, T, x, a, b, d, (R)coef, scal, \
+                                 scal_idx, active, tr, ev)
   HOC_K_SWITCH(K, {
     switch (pre) {
       case PRE_COPY: LT(PRE_COPY); break;
@@ -882,17 +886,18 @@ hipError_t launch_trace(int K, int pre, int N, const Tab& T, double* x, const do
   return hipGetLastError();
 }
 
-hipError_t launch_lap(int K, int epi, int N, const Tab& T, const Geo& G, double* x, const double* tr, double* y,
-                      const double* r, const double* diag, double* d, double c1, double c2, double* part,
-                      const int* active, hipStream_t s, double* tr_out) {
-#define LL(E)                                                                                                          \
+template <class R>
+static hipError_t launch_lap_t(int K, int epi, int N, const TabT<R>& T, const GeoT<R>& G, R* x, const R* tr, R* y,
+                               const R* r, const R* diag, R* d, double c1, double c2, double* part, const int* active,
+                               hipStream_t s, R* tr_out) {
+#define LL(E)                                                                                                        \
   do {                                                                                                               \
     if (G.gel)                                                                                                       \
-      hipLaunchKernelGGL((k_lap<KK, E, true>), dim3(nblk(N)), dim3(kWG), 0, s, N, T, G, x, tr, y, r, diag, d, c1, c2,  \
-                         part, active, tr_out);                                                                      \
+      hipLaunchKernelGGL((k_lap<KK, E, true, R>), dim3(nblk(N)), dim3(kWG), 0, s, N, T, G, x, tr, y, r, diag, d,      \
+                         (R)c1, (R)c2, part, active, tr_out);                                                        \
     else                                                                                                             \
-      hipLaunchKernelGGL((k_lap<KK, E, false>), dim3(nblk(N)), dim3(kWG), 0, s, N, T, G, x, tr, y, r, diag, d, c1, c2, \
-                         part, active, tr_out);                                                                      \
+      hipLaunchKernelGGL((k_lap<KK, E, false, R>), dim3(nblk(N)), dim3(kWG), 0, s, N, T, G, x, tr, y, r, diag, d,     \
+                         (R)c1, (R)c2, part, active, tr_out);                                                        \
   } while (0)
   HOC_K_SWITCH(K, {
     switch (epi) {
@@ -906,6 +911,27 @@ hipError_t launch_lap(int K, int epi, int N, const Tab& T, const Geo& G, double*
   });
 #undef LL
   return hipGetLastError();
+}
+
+hipError_t launch_trace(int K, int pre, int N, const TabT<double>& T, double* x, const double* a, const double* b,
+                        double* d, double coef, const double* scal, int scal_idx, const int* active, double* tr,
+                        hipStream_t s, const int32_t* ev) {
+  return launch_trace_t<double>(K, pre, N, T, x, a, b, d, coef, scal, scal_idx, active, tr, s, ev);
+}
+hipError_t launch_trace(int K, int pre, int N, const TabT<float>& T, float* x, const float* a, const float* b,
+                        float* d, double coef, const double* scal, int scal_idx, const int* active, float* tr,
+                        hipStream_t s, const int32_t* ev) {
+  return launch_trace_t<float>(K, pre, N, T, x, a, b, d, coef, scal, scal_idx, active, tr, s, ev);
+}
+hipError_t launch_lap(int K, int epi, int N, const TabT<double>& T, const GeoT<double>& G, double* x, const double* tr,
+                      double* y, const double* r, const double* diag, double* d, double c1, double c2, double* part,
+                      const int* active, hipStream_t s, double* tr_out) {
+  return launch_lap_t<double>(K, epi, N, T, G, x, tr, y, r, diag, d, c1, c2, part, active, s, tr_out);
+}
+hipError_t launch_lap(int K, int epi, int N, const TabT<float>& T, const GeoT<float>& G, float* x, const float* tr,
+                      float* y, const float* r, const float* diag, float* d, double c1, double c2, double* part,
+                      const int* active, hipStream_t s, float* tr_out) {
+  return launch_lap_t<float>(K, epi, N, T, G, x, tr, y, r, diag, d, c1, c2, part, active, s, tr_out);
 }
 
 hipError_t launch_rhs(int K, int N, const Tab& T, const Geo& G, const double* omega, const double* bcpsi, double* bs,

@@ -1,6 +1,5 @@
 // hoc_la.hip - vector, reduction, CSR and probing kernels of the Poisson solver.
 #include "hoc_la.hpp"
-#include "hoc_kernels.hpp"
 
 namespace hoc {
 
@@ -30,40 +29,41 @@ __device__ __forceinline__ double dof_weight(const VecArgs& A, size_t v) {
 }
 
 // Elementwise Krylov updates with fused partial norms.  Scalars come from S.
-__global__ void k_vec(int op, VecArgs A, size_t n, double* __restrict__ a, double* __restrict__ b,
-                      const double* __restrict__ c, const double* __restrict__ d, double* __restrict__ e,
-                      const double* __restrict__ S, const int* __restrict__ F, double* __restrict__ part) {
+template <class R>
+__global__ void k_vec(int op, VecArgs A, size_t n, R* __restrict__ a, R* __restrict__ b, const R* __restrict__ c,
+                      const R* __restrict__ d, R* __restrict__ e, const double* __restrict__ S,
+                      const int* __restrict__ F, double* __restrict__ part) {
   if (F && !F[F_ACTIVE]) return;
   if (op >= VOP_BICG_X && F && F[F_HALF]) return;
   double p0 = 0., p1 = 0.;
   for (size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (size_t)gridDim.x * blockDim.x) {
     switch (op) {
       case VOP_PCG_UPD: {  // x += alpha p; r -= alpha q      a=x b=r c=p d=q
-        const double al = S[S_ALPHA];
+        const R al = (R)S[S_ALPHA];
         a[v] += al * c[v];
-        const double r = b[v] - al * d[v];
+        const R r = b[v] - al * d[v];
         b[v] = r;
         const double ru = r / dof_weight(A, v);
-        p0 += r * r; p1 += ru * ru;
+        p0 += (double)r * r; p1 += ru * ru;
       } break;
       case VOP_BICG_P: {   // p = r + beta (p - omega v)      a=p c=r d=v
-        a[v] = c[v] + S[S_BETA] * (a[v] - S[S_OMEGA] * d[v]);
+        a[v] = c[v] + (R)S[S_BETA] * (a[v] - (R)S[S_OMEGA] * d[v]);
       } break;
       case VOP_BICG_S: {   // s = r - alpha v (in r)          b=r d=v
-        const double s = b[v] - S[S_ALPHA] * d[v];
+        const R s = b[v] - (R)S[S_ALPHA] * d[v];
         b[v] = s;
         const double su = s / dof_weight(A, v);
-        p0 += s * s; p1 += su * su;
+        p0 += (double)s * s; p1 += su * su;
       } break;
       case VOP_BICG_X: {   // x += alpha ph + omega sh; r = s - omega t   a=x b=r(s) c=ph d=sh e=t
-        a[v] += S[S_ALPHA] * c[v] + S[S_OMEGA] * d[v];
-        const double r = b[v] - S[S_OMEGA] * e[v];
+        a[v] += (R)S[S_ALPHA] * c[v] + (R)S[S_OMEGA] * d[v];
+        const R r = b[v] - (R)S[S_OMEGA] * e[v];
         b[v] = r;
         const double ru = r / dof_weight(A, v);
-        p0 += r * r; p1 += ru * ru;
+        p0 += (double)r * r; p1 += ru * ru;
       } break;
       case VOP_DOT2: {     // partials of a.c and b.d (read only)
-        p0 += a[v] * c[v]; p1 += b[v] * d[v];
+        p0 += (double)a[v] * c[v]; p1 += (double)b[v] * d[v];
       } break;
       case VOP_COPY: {     // a = c
         a[v] = c[v];
@@ -77,16 +77,18 @@ __global__ void k_vec(int op, VecArgs A, size_t n, double* __restrict__ a, doubl
 }
 
 // x += alpha ph when BiCGSTAB converged at the half step.
-__global__ void k_bicg_half(size_t n, double* __restrict__ x, const double* __restrict__ ph, const double* __restrict__ S,
+template <class R>
+__global__ void k_bicg_half(size_t n, R* __restrict__ x, const R* __restrict__ ph, const double* __restrict__ S,
                             int* __restrict__ F) {
   if (!F[F_ACTIVE] || !F[F_HALF]) return;
   for (size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (size_t)gridDim.x * blockDim.x)
-    x[v] += S[S_ALPHA] * ph[v];
+    x[v] += (R)S[S_ALPHA] * ph[v];
 }
 
-__global__ void k_zero_if_flag(size_t n, double* __restrict__ x, const int* __restrict__ F) {
+template <class R>
+__global__ void k_zero_if_flag(size_t n, R* __restrict__ x, const int* __restrict__ F) {
   if (!F[F_ZERO]) return;
-  for (size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (size_t)gridDim.x * blockDim.x) x[v] = 0.;
+  for (size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (size_t)gridDim.x * blockDim.x) x[v] = 0;
 }
 
 // Scalar logic of the Krylov solvers.  pa: np partials of q0 then np of q1.
@@ -158,14 +160,14 @@ __global__ void k_finalize(int phase, const double* __restrict__ pa, int np, dou
 
 // ---- CSR levels (one thread per row) ------------------------------------------------------
 // mode 0: y = A x;  1: y = f - A x;  2: y += A x;  3: x += M (f - A x) (y unused)
+template <class R>
 __global__ void k_csr(int mode, int n, const int32_t* __restrict__ ptr, const int32_t* __restrict__ col,
-                      const double* __restrict__ val, const double* __restrict__ x, const double* __restrict__ f,
-                      double* __restrict__ y, const double* __restrict__ M, double* __restrict__ xio,
-                      const int* __restrict__ F) {
+                      const R* __restrict__ val, const R* __restrict__ x, const R* __restrict__ f, R* __restrict__ y,
+                      const R* __restrict__ M, R* __restrict__ xio, const int* __restrict__ F) {
   if (F && !F[F_ACTIVE]) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  double s = 0.;
+  R s = 0;
   for (int j = ptr[i]; j < ptr[i + 1]; ++j) s += val[j] * x[col[j]];
   if (mode == 0) y[i] = s;
   else if (mode == 1) y[i] = f[i] - s;
@@ -175,15 +177,14 @@ __global__ void k_csr(int mode, int n, const int32_t* __restrict__ ptr, const in
 
 // Same operations with LPR lanes per row (rows of many nonzeros): the lanes of a group
 // stride over the row together, so a wavefront's loads are contiguous.
-template <int LPR>
+template <int LPR, class R>
 __global__ void k_csr_vec(int mode, int n, const int32_t* __restrict__ ptr, const int32_t* __restrict__ col,
-                          const double* __restrict__ val, const double* __restrict__ x, const double* __restrict__ f,
-                          double* __restrict__ y, const double* __restrict__ M, double* __restrict__ xio,
-                          const int* __restrict__ F) {
+                          const R* __restrict__ val, const R* __restrict__ x, const R* __restrict__ f,
+                          R* __restrict__ y, const R* __restrict__ M, R* __restrict__ xio, const int* __restrict__ F) {
   if (F && !F[F_ACTIVE]) return;
   const int lane = threadIdx.x % LPR;
   const int i = (blockIdx.x * blockDim.x + threadIdx.x) / LPR;
-  double s = 0.;
+  R s = 0;
   if (i < n)
     for (int j = ptr[i] + lane; j < ptr[i + 1]; j += LPR) s += val[j] * x[col[j]];
 #pragma unroll
@@ -196,7 +197,8 @@ __global__ void k_csr_vec(int mode, int n, const int32_t* __restrict__ ptr, cons
 }
 
 // x = M f (spai0 from a zero guess)
-__global__ void k_scale(int n, const double* __restrict__ M, const double* __restrict__ f, double* __restrict__ x,
+template <class R>
+__global__ void k_scale(int n, const R* __restrict__ M, const R* __restrict__ f, R* __restrict__ x,
                         const int* __restrict__ F) {
   if (F && !F[F_ACTIVE]) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -204,15 +206,17 @@ __global__ void k_scale(int n, const double* __restrict__ M, const double* __res
 }
 
 // Coarsest level: x = Ainv f (dense, row-major), one wavefront per row.
-__global__ void k_dense(int n, const double* __restrict__ Ainv, const double* __restrict__ f, double* __restrict__ x,
+template <class R>
+__global__ void k_dense(int n, const R* __restrict__ Ainv, const R* __restrict__ f, R* __restrict__ x,
                         const int* __restrict__ F) {
   if (F && !F[F_ACTIVE]) return;
   const int row = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (row >= n) return;
-  double s = 0.;
+  R s = 0;
   for (int j = lane; j < n; j += 64) s += Ainv[(size_t)row * n + j] * f[j];
-  s = wsum(s);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
   if (lane == 0) x[row] = s;
 }
 
@@ -231,23 +235,15 @@ __global__ void k_probe_diag(int N, int K2, const int32_t* __restrict__ color, i
   if (el >= N || color[el] != c) return;
   diag[(size_t)el * K2 + rs] = y[(size_t)el * K2 + rs];
 }
-// s[el] = sum_m y[el][m]
-__global__ void k_elem_sum(int N, int K2, const double* __restrict__ y, double* __restrict__ s) {
-  const int el = blockIdx.x * blockDim.x + threadIdx.x;
-  if (el >= N) return;
-  double a = 0.;
-  for (int m = 0; m < K2; ++m) a += y[(size_t)el * K2 + m];
-  s[el] = a;
-}
-
 // ---- Q1 coarse space ---------------------------------------------------------------------------
 // rc[v] = sum over the (element, corner) incidences of vertex v of ec[element][corner]
+template <class R>
 __global__ void k_q1_gather(int nv, const int32_t* __restrict__ vptr, const int32_t* __restrict__ vinc,
-                            const double* __restrict__ ec, double* __restrict__ rc, const int* __restrict__ F) {
+                            const R* __restrict__ ec, R* __restrict__ rc, const int* __restrict__ F) {
   if (F && !F[F_ACTIVE]) return;
   const int v = blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= nv) return;
-  double s = 0.;
+  R s = 0;
   for (int k = vptr[v]; k < vptr[v + 1]; ++k) s += ec[vinc[k]];
   rc[v] = s;
 }
@@ -267,6 +263,32 @@ __global__ void k_q1_probe_accum(int N, int K, Tab T, int rs, const double* __re
     for (int b = 0; b < 4; ++b) M[(size_t)f * 16 + a * 4 + b] += t[a] * q1w(T, b, rs / K, rs % K);
 }
 
+// ---- mixed precision ---------------------------------------------------------------------------
+// r32 = (float) r64, e32 = 0; both partial slots receive ||r64 / W||^2 (the unscaled norm the
+// inner solve's tolerance is relative to).
+__global__ void k_mixed_demote(VecArgs A, size_t n, const double* __restrict__ r64, float* __restrict__ r32,
+                               float* __restrict__ e32, double* __restrict__ part, const int* __restrict__ F) {
+  if (F && !F[F_ACTIVE]) return;
+  double p0 = 0.;
+  for (size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (size_t)gridDim.x * blockDim.x) {
+    const double r = r64[v];
+    r32[v] = (float)r;
+    e32[v] = 0.f;
+    const double ru = r / dof_weight(A, v);
+    p0 += ru * ru;
+  }
+  double p1 = p0;
+  block_sum2(p0, p1);
+  if (threadIdx.x == 0) { part[blockIdx.x] = p0; part[gridDim.x + blockIdx.x] = p1; }
+}
+// x64 += (double) e32
+__global__ void k_mixed_promote_add(size_t n, double* __restrict__ x64, const float* __restrict__ e32,
+                                    const int* __restrict__ F) {
+  if (F && !F[F_ACTIVE]) return;
+  for (size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (size_t)gridDim.x * blockDim.x)
+    x64[v] += (double)e32[v];
+}
+
 // ---- launchers -------------------------------------------------------------------------------
 static int grid_for(size_t n, int bs) {
   size_t g = (n + bs - 1) / bs;
@@ -277,40 +299,57 @@ static int grid_for(size_t n, int bs) {
 
 int vec_grid(size_t n) { return grid_for(n, 256); }
 
-void vec(hipStream_t s, int op, const VecArgs& A, size_t n, double* a, double* b, const double* c, const double* d,
-         double* e, const double* S, const int* F, double* part) {
-  hipLaunchKernelGGL(k_vec, dim3(vec_grid(n)), dim3(256), 0, s, op, A, n, a, b, c, d, e, S, F, part);
+template <class R>
+void vec(hipStream_t s, int op, const VecArgs& A, size_t n, R* a, R* b, const R* c, const R* d, R* e, const double* S,
+         const int* F, double* part) {
+  hipLaunchKernelGGL(k_vec<R>, dim3(vec_grid(n)), dim3(256), 0, s, op, A, n, a, b, c, d, e, S, F, part);
 }
-void bicg_half(hipStream_t s, size_t n, double* x, const double* ph, const double* S, int* F) {
-  hipLaunchKernelGGL(k_bicg_half, dim3(vec_grid(n)), dim3(256), 0, s, n, x, ph, S, F);
+template <class R>
+void bicg_half(hipStream_t s, size_t n, R* x, const R* ph, const double* S, int* F) {
+  hipLaunchKernelGGL(k_bicg_half<R>, dim3(vec_grid(n)), dim3(256), 0, s, n, x, ph, S, F);
 }
-void zero_if_flag(hipStream_t s, size_t n, double* x, const int* F) {
-  hipLaunchKernelGGL(k_zero_if_flag, dim3(vec_grid(n)), dim3(256), 0, s, n, x, F);
+template <class R>
+void zero_if_flag(hipStream_t s, size_t n, R* x, const int* F) {
+  hipLaunchKernelGGL(k_zero_if_flag<R>, dim3(vec_grid(n)), dim3(256), 0, s, n, x, F);
 }
 void finalize(hipStream_t s, int phase, const double* part, int np, double* S, int* F, int maxiter) {
   hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, phase, part, np, S, F, maxiter);
 }
-void csr(hipStream_t s, int mode, int n, const int32_t* ptr, const int32_t* col, const double* val, const double* x,
-         const double* f, double* y, const double* M, double* xio, const int* F, int lpr) {
+template <class R>
+void csr(hipStream_t s, int mode, int n, const int32_t* ptr, const int32_t* col, const R* val, const R* x, const R* f,
+         R* y, const R* M, R* xio, const int* F, int lpr) {
   const size_t threads = (size_t)n * lpr;
   const dim3 grid((unsigned)((threads + 255) / 256));
   switch (lpr) {
-    case 1: hipLaunchKernelGGL(k_csr, dim3((n + 255) / 256), dim3(256), 0, s, mode, n, ptr, col, val, x, f, y, M, xio, F); break;
-    case 2: hipLaunchKernelGGL(k_csr_vec<2>, grid, dim3(256), 0, s, mode, n, ptr, col, val, x, f, y, M, xio, F); break;
-    case 4: hipLaunchKernelGGL(k_csr_vec<4>, grid, dim3(256), 0, s, mode, n, ptr, col, val, x, f, y, M, xio, F); break;
-    case 8: hipLaunchKernelGGL(k_csr_vec<8>, grid, dim3(256), 0, s, mode, n, ptr, col, val, x, f, y, M, xio, F); break;
-    default: hipLaunchKernelGGL(k_csr_vec<16>, grid, dim3(256), 0, s, mode, n, ptr, col, val, x, f, y, M, xio, F); break;
+    case 1: hipLaunchKernelGGL(k_csr<R>, dim3((n + 255) / 256), dim3(256), 0, s, mode, n, ptr, col, val, x, f, y, M, xio, F); break;
+    case 2: hipLaunchKernelGGL((k_csr_vec<2, R>), grid, dim3(256), 0, s, mode, n, ptr, col, val, x, f, y, M, xio, F); break;
+    case 4: hipLaunchKernelGGL((k_csr_vec<4, R>), grid, dim3(256), 0, s, mode, n, ptr, col, val, x, f, y, M, xio, F); break;
+    case 8: hipLaunchKernelGGL((k_csr_vec<8, R>), grid, dim3(256), 0, s, mode, n, ptr, col, val, x, f, y, M, xio, F); break;
+    default: hipLaunchKernelGGL((k_csr_vec<16, R>), grid, dim3(256), 0, s, mode, n, ptr, col, val, x, f, y, M, xio, F); break;
   }
 }
 int csr_lanes(int n, size_t nnz) {
   const double avg = n ? (double)nnz / n : 0.;
   return avg >= 24 ? 16 : avg >= 12 ? 8 : avg >= 6 ? 4 : avg >= 3 ? 2 : 1;
 }
-void scale(hipStream_t s, int n, const double* M, const double* f, double* x, const int* F) {
-  hipLaunchKernelGGL(k_scale, dim3((n + 255) / 256), dim3(256), 0, s, n, M, f, x, F);
+template <class R>
+void scale(hipStream_t s, int n, const R* M, const R* f, R* x, const int* F) {
+  hipLaunchKernelGGL(k_scale<R>, dim3((n + 255) / 256), dim3(256), 0, s, n, M, f, x, F);
 }
-void dense(hipStream_t s, int n, const double* Ainv, const double* f, double* x, const int* F) {
-  hipLaunchKernelGGL(k_dense, dim3((n + 3) / 4), dim3(256), 0, s, n, Ainv, f, x, F);
+template <class R>
+void dense(hipStream_t s, int n, const R* Ainv, const R* f, R* x, const int* F) {
+  hipLaunchKernelGGL(k_dense<R>, dim3((n + 3) / 4), dim3(256), 0, s, n, Ainv, f, x, F);
+}
+template <class R>
+void q1_gather(hipStream_t s, int nv, const int32_t* vptr, const int32_t* vinc, const R* ec, R* rc, const int* F) {
+  hipLaunchKernelGGL(k_q1_gather<R>, dim3((nv + 255) / 256), dim3(256), 0, s, nv, vptr, vinc, ec, rc, F);
+}
+void mixed_demote(hipStream_t s, const VecArgs& A, size_t n, const double* r64, float* r32, float* e32, double* part,
+                  const int* F) {
+  hipLaunchKernelGGL(k_mixed_demote, dim3(vec_grid(n)), dim3(256), 0, s, A, n, r64, r32, e32, part, F);
+}
+void mixed_promote_add(hipStream_t s, size_t n, double* x64, const float* e32, const int* F) {
+  hipLaunchKernelGGL(k_mixed_promote_add, dim3(vec_grid(n)), dim3(256), 0, s, n, x64, e32, F);
 }
 void probe_set(hipStream_t s, int N, int K2, const int32_t* color, int c, int rs, double* x) {
   size_t n = (size_t)N * K2;
@@ -319,17 +358,22 @@ void probe_set(hipStream_t s, int N, int K2, const int32_t* color, int c, int rs
 void probe_diag(hipStream_t s, int N, int K2, const int32_t* color, int c, int rs, const double* y, double* diag) {
   hipLaunchKernelGGL(k_probe_diag, dim3((N + 255) / 256), dim3(256), 0, s, N, K2, color, c, rs, y, diag);
 }
-void elem_sum(hipStream_t s, int N, int K2, const double* y, double* out) {
-  hipLaunchKernelGGL(k_elem_sum, dim3((N + 255) / 256), dim3(256), 0, s, N, K2, y, out);
-}
-
-}  // namespace hoc
-
-namespace hoc {
-void q1_gather(hipStream_t s, int nv, const int32_t* vptr, const int32_t* vinc, const double* ec, double* rc, const int* F) {
-  hipLaunchKernelGGL(k_q1_gather, dim3((nv + 255) / 256), dim3(256), 0, s, nv, vptr, vinc, ec, rc, F);
-}
 void q1_probe_accum(hipStream_t s, int N, int K, const Tab& T, int rs, const double* y, double* M) {
   hipLaunchKernelGGL(k_q1_probe_accum, dim3((N + 255) / 256), dim3(256), 0, s, N, K, T, rs, y, M);
 }
+
+#define HOC_LA_INST(R)                                                                                                 \
+  template void vec<R>(hipStream_t, int, const VecArgs&, size_t, R*, R*, const R*, const R*, R*, const double*,        \
+                       const int*, double*);                                                                           \
+  template void bicg_half<R>(hipStream_t, size_t, R*, const R*, const double*, int*);                                 \
+  template void zero_if_flag<R>(hipStream_t, size_t, R*, const int*);                                                 \
+  template void csr<R>(hipStream_t, int, int, const int32_t*, const int32_t*, const R*, const R*, const R*, R*,        \
+                       const R*, R*, const int*, int);                                                                 \
+  template void scale<R>(hipStream_t, int, const R*, const R*, R*, const int*);                                       \
+  template void dense<R>(hipStream_t, int, const R*, const R*, R*, const int*);                                       \
+  template void q1_gather<R>(hipStream_t, int, const int32_t*, const int32_t*, const R*, R*, const int*);
+HOC_LA_INST(double)
+HOC_LA_INST(float)
+#undef HOC_LA_INST
+
 }  // namespace hoc

@@ -1,9 +1,14 @@
 // hoc_la.hpp - vector/reduction/CSR/probing kernels of the Poisson solver (hoc_la.hip).
+//
+// Vector and level kernels are instantiated for double (FP64 solve) and float (the
+// mixed-precision inner solve); scalars, partial sums and flags are always double/int.
 #pragma once
 
 #include <hip/hip_runtime.h>
 #include <cstddef>
 #include <cstdint>
+
+#include "hoc_kernels.hpp"
 
 namespace hoc {
 
@@ -19,21 +24,34 @@ struct VecArgs {
 };
 
 int vec_grid(size_t n);
-void vec(hipStream_t s, int op, const VecArgs& A, size_t n, double* a, double* b, const double* c, const double* d,
-         double* e, const double* S, const int* F, double* part);
-void bicg_half(hipStream_t s, size_t n, double* x, const double* ph, const double* S, int* F);
-void zero_if_flag(hipStream_t s, size_t n, double* x, const int* F);
+template <class R>
+void vec(hipStream_t s, int op, const VecArgs& A, size_t n, R* a, R* b, const R* c, const R* d, R* e, const double* S,
+         const int* F, double* part);
+template <class R>
+void bicg_half(hipStream_t s, size_t n, R* x, const R* ph, const double* S, int* F);
+template <class R>
+void zero_if_flag(hipStream_t s, size_t n, R* x, const int* F);
 void finalize(hipStream_t s, int phase, const double* part, int np, double* S, int* F, int maxiter);
-void csr(hipStream_t s, int mode, int n, const int32_t* ptr, const int32_t* col, const double* val, const double* x,
-         const double* f, double* y, const double* M, double* xio, const int* F, int lanes_per_row);
+template <class R>
+void csr(hipStream_t s, int mode, int n, const int32_t* ptr, const int32_t* col, const R* val, const R* x, const R* f,
+         R* y, const R* M, R* xio, const int* F, int lanes_per_row);
 int csr_lanes(int n, size_t nnz);  // lanes per row suited to a matrix's average row length
-void scale(hipStream_t s, int n, const double* M, const double* f, double* x, const int* F);
-void dense(hipStream_t s, int n, const double* Ainv, const double* f, double* x, const int* F);
+template <class R>
+void scale(hipStream_t s, int n, const R* M, const R* f, R* x, const int* F);
+template <class R>
+void dense(hipStream_t s, int n, const R* Ainv, const R* f, R* x, const int* F);
+template <class R>
+void q1_gather(hipStream_t s, int nv, const int32_t* vptr, const int32_t* vinc, const R* ec, R* rc, const int* F);
+
+// Mixed precision: r32 = (float) r64, e32 = 0, partials of ||r64 / W||^2 (both slots);
+// and x64 += (double) e32.
+void mixed_demote(hipStream_t s, const VecArgs& A, size_t n, const double* r64, float* r32, float* e32, double* part,
+                  const int* F);
+void mixed_promote_add(hipStream_t s, size_t n, double* x64, const float* e32, const int* F);
+
+// Setup-time probing (FP64).
 void probe_set(hipStream_t s, int N, int K2, const int32_t* color, int c, int rs, double* x);
 void probe_diag(hipStream_t s, int N, int K2, const int32_t* color, int c, int rs, const double* y, double* diag);
-void elem_sum(hipStream_t s, int N, int K2, const double* y, double* out);
-struct Tab;
-void q1_gather(hipStream_t s, int nv, const int32_t* vptr, const int32_t* vinc, const double* ec, double* rc, const int* F);
 void q1_probe_accum(hipStream_t s, int N, int K, const Tab& T, int rs, const double* y, double* M);
 
 }  // namespace hoc

@@ -64,21 +64,68 @@ struct DBuf {
     if (!v.empty()) HIPCHK(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
     HIPCHK(hipDeviceSynchronize());
   }
+  // Upload with conversion (FP64 host data into an FP32 buffer).
+  template <class U>
+  void upload_as(const std::vector<U>& v) {
+    std::vector<T> c(v.begin(), v.end());
+    upload(c);
+  }
   operator T*() const { return p; }
 };
 
+template <class R>
 struct DevCSR {
   int n = 0, lanes = 1;
   DBuf<int32_t> ptr, col;
-  DBuf<double> val;
+  DBuf<R> val;
   void upload(const CSR& A) {
-    n = A.n; ptr.upload(A.ptr); col.upload(A.col); val.upload(A.val);
+    n = A.n; ptr.upload(A.ptr); col.upload(A.col); val.upload_as(A.val);
     lanes = csr_lanes(A.n, A.val.size());
   }
 };
+template <class R>
 struct DevLevel {
-  DevCSR A, P, R;
-  DBuf<double> M, f, u, t;
+  DevCSR<R> A, P, Rs;
+  DBuf<R> M, f, u, t;
+};
+
+// Preconditioned-Krylov state in working precision R: the fine operator's tables and
+// metrics, the Krylov and smoother vectors, face traces and the V-cycle hierarchy.
+// FP64 (R = double) serves the plain solve, the outer residual of the mixed solve and
+// setup probing; FP32 (R = float) is the mixed solve's inner Krylov + V-cycle.
+template <class R>
+struct Prec {
+  TabT<R> T{};
+  GeoT<R> G{};
+  DBuf<R> Ga, Gb, Gc, fa, fb, fA, fB, fjac, fAn, gel, gface;  // FP32 copies (unused for FP64)
+  DBuf<R> r, z, p, q, rh, v, ph, sh, t, x, dch, diag, tr, tr2, ec, rc, xc;
+  std::vector<std::unique_ptr<DevLevel<R>>> lv;  // Q1 vertex level and the SA levels below
+  DBuf<R> coarse_inv;
+  int coarse_n = 0;
+  int prof_off = 0;  // offset of this precision's entries in the live kernel profile
+  void alloc_vectors(size_t D, int N, int K) {
+    for (auto* f : {&r, &z, &p, &q, &rh, &v, &ph, &sh, &t, &x, &dch, &diag}) f->alloc(D);
+    tr.alloc((size_t)N * 8 * K); tr2.alloc((size_t)N * 8 * K);
+    ec.alloc((size_t)N * 4);
+  }
+  void upload_levels(const AMGHost& H) {
+    lv.clear();
+    for (size_t l = 0; l < H.levels.size(); ++l) {
+      auto L = std::make_unique<DevLevel<R>>();
+      L->A.upload(H.levels[l].A);
+      if (l + 1 < H.levels.size()) {
+        L->P.upload(H.levels[l].P);
+        L->Rs.upload(H.levels[l].R);
+        L->M.upload_as(H.levels[l].Minv);
+      }
+      const int n = H.levels[l].A.n;
+      L->f.alloc(n); L->u.alloc(n); L->t.alloc(n);
+      lv.push_back(std::move(L));
+    }
+    coarse_n = H.levels.back().A.n;
+    coarse_inv.upload_as(H.coarse_inv);
+    rc.alloc(H.levels[0].A.n); xc.alloc(H.levels[0].A.n);
+  }
 };
 
 struct Solver {
@@ -98,6 +145,7 @@ struct Solver {
   double time = 0., re_inv = 0.;
   Tab T{};
   VecArgs VA{};
+  bool mixed = false;
 
   // device: geometry and connectivity
   DBuf<double> d_jac, d_xxi, d_xeta, d_yxi, d_yeta, d_Ga, d_Gb, d_Gc, d_fa, d_fb, d_fA, d_fB, d_fjac, d_fAn;
@@ -110,20 +158,18 @@ struct Solver {
   DBuf<double> b_psi0, b_psi_r, b_diff0, b_diff_r, b_nvel0, b_nvel_r, b_pvel0, b_pvel_r;
   BCDev B{};
   // fields
-  DBuf<double> w, psi, vel, U, k1, k2, k3, k4, vort, kscratch;
-  DBuf<double> tr, tr2, trw, tru;
-  // Krylov
-  DBuf<double> bs, r, z, p, q, rh, v, ph, sh, t, d_cheb, diag, part;
-  DBuf<double> S;
-  DBuf<int> F;
+  DBuf<double> w, psi, vel, U, k1, k2, k3, k4, vort, kscratch, bs;
+  DBuf<double> trw, tru;
+  // Poisson solver: FP64 state (plain solve, outer residual, probing) and FP32 state
+  // (mixed-precision inner solve); S/F = scalars and flags of the FP64 (outer) solve,
+  // S2/F2 those of the inner solve.
+  Prec<double> P64;
+  Prec<float> P32;
+  DBuf<double> part, S, S2;
+  DBuf<int> F, F2;
   int* pinned_flags = nullptr;  // 2 x F_COUNT
   hipEvent_t ev[2] = {nullptr, nullptr};
   int nparts = 0;
-  // hierarchy
-  std::vector<std::unique_ptr<DevLevel>> lv;  // lv[0] = element level (A1), ...
-  DBuf<double> coarse_inv;
-  int coarse_n = 0;
-  DBuf<double> rc, xc, ec;
   int nv = 0;                             // Q1 coarse space: element-corner vertices
   DBuf<int32_t> d_ev, d_vptr, d_vinc;     // element -> 4 vertices; vertex -> (element*4+corner)
   double cheb_theta = 1., cheb_c1 = 0., cheb_c2 = 0.;
@@ -139,7 +185,7 @@ struct Solver {
     int64_t cnt[HOC_PROF_VARIANTS] = {0};
   } prof;
   // stats
-  int last_iters = 0;
+  int last_iters = 0, last_outer = 0;
   double last_rel = 0.;
   int64_t total_iters = 0, n_solves = 0;
   double setup_seconds = 0.;
@@ -229,13 +275,31 @@ struct Solver {
     upl(b_nvel0, bch.nvel); upl(b_nvel_r, bch.nvel_ramp); upl(b_pvel0, bch.pvel); upl(b_pvel_r, bch.pvel_ramp);
     B = BCDev{b_psi, b_diff, b_nvel, b_pvel, b_vj, b_vort, b_adv, b_psi0, b_psi_r, b_diff0, b_diff_r, b_nvel0, b_nvel_r, b_pvel0, b_pvel_r};
     // fields
-    for (auto* f : {&w, &psi, &k1, &k2, &k3, &k4, &vort, &kscratch, &bs, &r, &z, &p, &q, &rh, &v, &ph, &sh, &t, &d_cheb, &diag})
-      f->alloc(D);
+    for (auto* f : {&w, &psi, &k1, &k2, &k3, &k4, &vort, &kscratch, &bs}) f->alloc(D);
     vel.alloc(2 * D); U.alloc(2 * D);
-    tr.alloc((size_t)N * 8 * K); tr2.alloc((size_t)N * 8 * K); trw.alloc((size_t)N * 8 * K); tru.alloc((size_t)N * 4 * K);
+    trw.alloc((size_t)N * 8 * K); tru.alloc((size_t)N * 4 * K);
+    mixed = cfg.precision == HOC_MIXED;
+    P64.T = T; P64.G = G; P64.prof_off = 0;
+    P64.alloc_vectors(D, N, K);
+    if (mixed) {  // FP32 tables and metrics of the inner solve's fine operator
+      Prec<float>& P = P32;
+      static_assert(2 * sizeof(TabT<float>) == sizeof(TabT<double>), "table layout");
+      float* Tf = reinterpret_cast<float*>(&P.T);
+      const double* Td = reinterpret_cast<const double*>(&T);
+      for (size_t i = 0; i < sizeof(T) / sizeof(double); ++i) Tf[i] = (float)Td[i];
+      P.Ga.upload_as(Ga); P.Gb.upload_as(Gb); P.Gc.upload_as(Gc);
+      P.fa.upload_as(fa); P.fb.upload_as(fb); P.fA.upload_as(geo.fA); P.fB.upload_as(geo.fB);
+      P.fjac.upload_as(geo.fjac); P.fAn.upload_as(geo.fAn);
+      if (affine) { P.gel.upload_as(gel); P.gface.upload_as(gface); }
+      P.G = GeoT<float>{N, E, nullptr, nullptr, nullptr, nullptr, nullptr, P.Ga, P.Gb, P.Gc, P.fa, P.fb, P.fA, P.fB,
+                        P.fjac, P.fAn, affine ? P.gel.p : nullptr, affine ? P.gface.p : nullptr,
+                        d_nb, d_nbf, d_swpsi, d_swdiff, d_noslip, nullptr};
+      P.prof_off = HOC_PROF_VARIANTS / 2;
+      P.alloc_vectors(D, N, K);
+    }
     nparts = std::max(lap_blocks(N), vec_grid(D));
     part.alloc(2 * (size_t)nparts);
-    S.alloc(S_COUNT); F.alloc(F_COUNT);
+    S.alloc(S_COUNT); F.alloc(F_COUNT); S2.alloc(S_COUNT); F2.alloc(F_COUNT);
     if (!pinned_flags) HIPCHK(hipHostMalloc((void**)&pinned_flags, 2 * F_COUNT * sizeof(int), hipHostMallocDefault));
     for (auto& e : ev) if (!e) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     build_hierarchy();
@@ -245,15 +309,16 @@ struct Solver {
     setup_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   }
 
-  // Fine-level operator launch, bracketed by events when profiling.
-  void lap(int epi, double* x, double* y, const double* rr, const double* dg, double* d, double c1, double c2,
-           double* partp, const int* active, const double* tin = nullptr, double* tout = nullptr) {
+  // Fine-level operator launch in precision R, bracketed by events when profiling.
+  template <class R>
+  void lap(Prec<R>& P, int epi, R* x, R* y, const R* rr, const R* dg, R* d, double c1, double c2, double* partp,
+           const int* active, const R* tin = nullptr, R* tout = nullptr) {
     const bool rec = prof.on;
     if (rec) HIPCHK(hipEventRecord(prof.e0[prof.n], stream));
-    HIPCHK(launch_lap(K, epi, N, T, G, x, tin ? tin : tr.p, y, rr, dg, d, c1, c2, partp, active, stream, tout));
+    HIPCHK(launch_lap(K, epi, N, P.T, P.G, x, tin ? tin : P.tr.p, y, rr, dg, d, c1, c2, partp, active, stream, tout));
     if (rec) {
       HIPCHK(hipEventRecord(prof.e1[prof.n], stream));
-      prof.var[prof.n] = epi;
+      prof.var[prof.n] = P.prof_off + epi;
       if (++prof.n == (int)prof.e0.size()) prof_flush();
     }
   }
@@ -268,30 +333,30 @@ struct Solver {
     }
     prof.n = 0;
   }
-  // Algorithmic HBM bytes of one fine-operator launch per epilogue: the field, the
-  // neighbour face traces (value + derivative), metrics, connectivity, and the epilogue's
-  // own vectors, each read or written once.
-  double lap_bytes(int epi) const {
-    const double vec = (double)N * K2 * 8.;
+  // Algorithmic HBM bytes of one fine-operator launch per epilogue, for values of vb bytes:
+  // the field, the neighbour face traces (value + derivative), metrics, connectivity, and
+  // the epilogue's own vectors, each read or written once.
+  double lap_bytes(int epi, double vb = 8.) const {
+    const double vec = (double)N * K2 * vb;
     double b = vec                                   // x
-             + (double)N * 4 * 2 * K * 8.            // neighbour traces
+             + (double)N * 4 * 2 * K * vb            // neighbour traces
              + (double)N * 4 * (4 + 1);              // nb, nbf
-    if (G.gel) b += (double)N * (3 + 8 + 8) * 8.;    // affine: element + own/neighbour face metrics
-    else b += 3 * vec + (double)N * 4 * K * 2 * 8. * 2;
+    if (G.gel) b += (double)N * (3 + 8 + 8) * vb;    // affine: element + own/neighbour face metrics
+    else b += 3 * vec + (double)N * 4 * K * 2 * vb * 2;
     switch (epi) {
       case EPI_STORE: b += vec; break;                        // y
       case EPI_RESID: b += 2 * vec; break;                    // r, y
       case EPI_CHEB: b += 5 * vec; break;                     // r, diag, d (rw), x out
       case EPI_CHEBF: b += 4 * vec; break;                    // r, diag, d out, x out
-      case EPI_RESTRICT: b += vec + (double)N * 4 * 8.; break;  // r, corner sums
+      case EPI_RESTRICT: b += vec + (double)N * 4 * vb; break;  // r, corner sums
     }
     return b;
   }
 
-  // As x into y (trace + apply); x is not modified.
+  // As x into y (FP64 trace + apply); x is not modified.
   void apply_As(double* x, double* y) {
-    HIPCHK(launch_trace(K, PRE_COPY, N, T, x, nullptr, nullptr, nullptr, 0., nullptr, 0, nullptr, tr, stream));
-    lap(EPI_STORE, x, y, nullptr, nullptr, nullptr, 0., 0., nullptr, nullptr);
+    HIPCHK(launch_trace(K, PRE_COPY, N, T, x, nullptr, nullptr, nullptr, 0., nullptr, 0, nullptr, P64.tr.p, stream));
+    lap<double>(P64, EPI_STORE, x, y, nullptr, nullptr, nullptr, 0., 0., nullptr, nullptr);
   }
 
   // Stream-ordered copies that return when the data has arrived.
@@ -313,9 +378,9 @@ struct Solver {
     dcol1.upload(col1); dcol2.upload(col2);
     for (int c = 0; c < nc1; ++c)
       for (int rs = 0; rs < K2; ++rs) {
-        probe_set(stream, N, K2, dcol1, c, rs, z);
-        apply_As(z, q);
-        probe_diag(stream, N, K2, dcol1, c, rs, q, diag);
+        probe_set(stream, N, K2, dcol1, c, rs, P64.z);
+        apply_As(P64.z, P64.q);
+        probe_diag(stream, N, K2, dcol1, c, rs, P64.q, P64.diag);
       }
     // Continuous bilinear coarse space on the element corners: Pc interpolates vertex values
     // to every element's solution points; Ac = Pc^T As Pc is assembled from probes of As.
@@ -347,9 +412,9 @@ struct Solver {
     for (int c = 0; c < nc2; ++c) {
       HIPCHK(hipMemsetAsync(dM.p, 0, (size_t)N * 16 * sizeof(double), stream));
       for (int rs = 0; rs < K2; ++rs) {
-        probe_set(stream, N, K2, dcol2, c, rs, z);
-        apply_As(z, q);
-        q1_probe_accum(stream, N, K, T, rs, q, dM);
+        probe_set(stream, N, K2, dcol2, c, rs, P64.z);
+        apply_As(P64.z, P64.q);
+        q1_probe_accum(stream, N, K, T, rs, P64.q, dM);
       }
       d2h(hM.data(), dM.p, hM.size() * sizeof(double));
       for (int f = 0; f < N; ++f) {
@@ -378,40 +443,27 @@ struct Solver {
     }
     AMGHost H;
     build_sa_amg(A1, 400, H);
-    lv.clear();
-    for (size_t l = 0; l < H.levels.size(); ++l) {
-      auto L = std::make_unique<DevLevel>();
-      L->A.upload(H.levels[l].A);
-      if (l + 1 < H.levels.size()) {
-        L->P.upload(H.levels[l].P);
-        L->R.upload(H.levels[l].R);
-        L->M.upload(H.levels[l].Minv);
-      }
-      const int n = H.levels[l].A.n;
-      L->f.alloc(n); L->u.alloc(n); L->t.alloc(n);
-      lv.push_back(std::move(L));
-    }
-    coarse_n = H.levels.back().A.n;
-    coarse_inv.upload(H.coarse_inv);
+    if (mixed) P32.upload_levels(H);
+    else P64.upload_levels(H);
     // Chebyshev interval from lambda_max(D^-1 As) by power iteration
     std::vector<double> hx(D);
     uint64_t st = 88172645463325252ull;
     for (size_t i = 0; i < D; ++i) { st ^= st << 13; st ^= st >> 7; st ^= st << 17; hx[i] = (double)(st % 2000001) / 1e6 - 1.; }
-    h2d(z.p, hx.data(), D * sizeof(double));
+    h2d(P64.z.p, hx.data(), D * sizeof(double));
     std::vector<double> hd(D), hy(D);
-    d2h(hd.data(), diag.p, D * sizeof(double));
+    d2h(hd.data(), P64.diag.p, D * sizeof(double));
     for (size_t i = 0; i < D; ++i)
       if (!(hd[i] > 0.)) throw StatusError(HOC_EINVAL, "Poisson operator has a non-positive diagonal entry");
     double lam = 0.;
     for (int it = 0; it < 30; ++it) {
-      apply_As(z, q);
-      d2h(hy.data(), q.p, D * sizeof(double));
+      apply_As(P64.z, P64.q);
+      d2h(hy.data(), P64.q.p, D * sizeof(double));
       double nrm = 0., ray = 0., xx = 0.;
       for (size_t i = 0; i < D; ++i) { hy[i] /= hd[i]; nrm += hy[i] * hy[i]; ray += hy[i] * hx[i]; xx += hx[i] * hx[i]; }
       lam = ray / xx;
       nrm = std::sqrt(nrm);
       for (size_t i = 0; i < D; ++i) hx[i] = hy[i] / nrm;
-      h2d(z.p, hx.data(), D * sizeof(double));
+      h2d(P64.z.p, hx.data(), D * sizeof(double));
     }
     lam_max = lam;
     const double lmax = 1.1 * lam, lmin = lmax / 30.;
@@ -426,58 +478,59 @@ struct Solver {
       std::vector<double> hx2(D), hAx(D), hAy(D);
       for (size_t i = 0; i < D; ++i) { st ^= st << 13; st ^= st >> 7; st ^= st << 17; hx2[i] = (double)(st % 2000001) / 1e6 - 1.; }
       for (size_t i = 0; i < D; ++i) { st ^= st << 13; st ^= st >> 7; st ^= st << 17; hx[i] = (double)(st % 2000001) / 1e6 - 1.; }
-      h2d(z.p, hx.data(), D * sizeof(double));
-      apply_As(z, q);
-      d2h(hAx.data(), q.p, D * sizeof(double));
-      h2d(z.p, hx2.data(), D * sizeof(double));
-      apply_As(z, q);
-      d2h(hAy.data(), q.p, D * sizeof(double));
+      h2d(P64.z.p, hx.data(), D * sizeof(double));
+      apply_As(P64.z, P64.q);
+      d2h(hAx.data(), P64.q.p, D * sizeof(double));
+      h2d(P64.z.p, hx2.data(), D * sizeof(double));
+      apply_As(P64.z, P64.q);
+      d2h(hAy.data(), P64.q.p, D * sizeof(double));
       double a = 0., b = 0., na = 0., nb = 0.;
       for (size_t i = 0; i < D; ++i) { a += hx2[i] * hAx[i]; b += hx[i] * hAy[i]; na += std::fabs(hx2[i] * hAx[i]); nb += std::fabs(hx[i] * hAy[i]); }
       krylov = std::fabs(a - b) <= 1e-12 * std::max(na, nb) ? HOC_KRYLOV_PCG : HOC_KRYLOV_BICGSTAB;
     }
-    rc.alloc(nv); xc.alloc(nv); ec.alloc((size_t)N * 4);
-    HIPCHK(hipMemsetAsync(z.p, 0, D * sizeof(double), stream));
+    if (mixed) P32.diag.upload_as(hd);
+    HIPCHK(hipMemsetAsync(P64.z.p, 0, D * sizeof(double), stream));
     HIPCHK(hipStreamSynchronize(stream));
   }
 
   // ---------------------------------------------------------------------------------------
-  // V-cycle on the element level and below: u_l = V_l(f_l)
-  void coarse_cycle(size_t l, const double* f, double* u) {
-    DevLevel& L = *lv[l];
-    const int* Fp = F.p;
-    if (l + 1 == lv.size()) { dense(stream, coarse_n, coarse_inv, f, u, Fp); return; }
-    DevLevel& C = *lv[l + 1];
-    scale(stream, L.A.n, L.M, f, u, Fp);
-    csr(stream, 1, L.A.n, L.A.ptr, L.A.col, L.A.val, u, f, L.t, nullptr, nullptr, Fp, L.A.lanes);
-    csr(stream, 0, C.A.n, L.R.ptr, L.R.col, L.R.val, L.t, nullptr, C.f, nullptr, nullptr, Fp, L.R.lanes);
-    coarse_cycle(l + 1, C.f, C.u);
-    csr(stream, 2, L.A.n, L.P.ptr, L.P.col, L.P.val, C.u, nullptr, u, nullptr, nullptr, Fp, L.P.lanes);
-    csr(stream, 3, L.A.n, L.A.ptr, L.A.col, L.A.val, u, f, nullptr, L.M, u, Fp, L.A.lanes);
+  // V-cycle on the Q1 vertex level and below: u_l = V_l(f_l)
+  template <class R>
+  void coarse_cycle(Prec<R>& P, size_t l, const R* f, R* u, const int* Fp) {
+    DevLevel<R>& L = *P.lv[l];
+    if (l + 1 == P.lv.size()) { dense<R>(stream, P.coarse_n, P.coarse_inv, f, u, Fp); return; }
+    DevLevel<R>& C = *P.lv[l + 1];
+    scale<R>(stream, L.A.n, L.M, f, u, Fp);
+    csr<R>(stream, 1, L.A.n, L.A.ptr, L.A.col, L.A.val, u, f, L.t, nullptr, nullptr, Fp, L.A.lanes);
+    csr<R>(stream, 0, C.A.n, L.Rs.ptr, L.Rs.col, L.Rs.val, L.t, nullptr, C.f, nullptr, nullptr, Fp, L.Rs.lanes);
+    coarse_cycle(P, l + 1, C.f.p, C.u.p, Fp);
+    csr<R>(stream, 2, L.A.n, L.P.ptr, L.P.col, L.P.val, C.u, nullptr, u, nullptr, nullptr, Fp, L.P.lanes);
+    csr<R>(stream, 3, L.A.n, L.A.ptr, L.A.col, L.A.val, u, f, nullptr, L.M, u, Fp, L.A.lanes);
   }
 
   // zout = V(rin); if part != nullptr, partials of rin.zout are left in part.
-  void vcycle(const double* rin, double* zout, double* partp) {
-    const int* Fp = F.p;
-    double* rr = const_cast<double*>(rin);
+  template <class R>
+  void vcycle(Prec<R>& P, const R* rin, R* zout, double* partp, const int* Fp) {
+    R* rr = const_cast<R*>(rin);
     // pre-smoothing: Chebyshev(2) from zero
-    HIPCHK(launch_trace(K, PRE_CHEB0, N, T, zout, rr, diag, d_cheb, cheb_theta, nullptr, 0, Fp, tr, stream));
-    lap(EPI_CHEB, zout, nullptr, rr, diag, d_cheb, cheb_c1, cheb_c2, nullptr, Fp, tr, tr2);
+    HIPCHK(launch_trace(K, PRE_CHEB0, N, P.T, zout, rr, P.diag.p, P.dch.p, cheb_theta, nullptr, 0, Fp, P.tr.p, stream));
+    lap(P, EPI_CHEB, zout, (R*)nullptr, rr, P.diag.p, P.dch.p, cheb_c1, cheb_c2, nullptr, Fp, P.tr.p, P.tr2.p);
     // restrict the residual to the corner vertices
-    lap(EPI_RESTRICT, zout, ec, rr, nullptr, nullptr, 0., 0., nullptr, Fp, tr2);
-    q1_gather(stream, nv, d_vptr, d_vinc, ec, rc, Fp);
-    coarse_cycle(0, rc, xc);
+    lap(P, EPI_RESTRICT, zout, P.ec.p, rr, (const R*)nullptr, (R*)nullptr, 0., 0., nullptr, Fp, P.tr2.p);
+    q1_gather<R>(stream, nv, d_vptr, d_vinc, P.ec, P.rc, Fp);
+    coarse_cycle(P, 0, P.rc.p, P.xc.p, Fp);
     // prolongate and post-smooth: Chebyshev(2) from the corrected iterate
-    HIPCHK(launch_trace(K, PRE_PROL, N, T, zout, xc, nullptr, nullptr, 0., nullptr, 0, Fp, tr, stream, d_ev));
-    lap(EPI_CHEBF, zout, nullptr, rr, diag, d_cheb, 0., 1. / cheb_theta, nullptr, Fp, tr, tr2);
-    lap(EPI_CHEB, zout, nullptr, rr, diag, d_cheb, cheb_c1, cheb_c2, partp, Fp, tr2);
+    HIPCHK(launch_trace(K, PRE_PROL, N, P.T, zout, P.xc.p, (const R*)nullptr, (R*)nullptr, 0., nullptr, 0, Fp, P.tr.p,
+                        stream, d_ev.p));
+    lap(P, EPI_CHEBF, zout, (R*)nullptr, rr, P.diag.p, P.dch.p, 0., 1. / cheb_theta, nullptr, Fp, P.tr.p, P.tr2.p);
+    lap(P, EPI_CHEB, zout, (R*)nullptr, rr, P.diag.p, P.dch.p, cheb_c1, cheb_c2, partp, Fp, P.tr2.p);
   }
 
-  // Poll the active flag with a one-iteration lag; returns false when the solve ended.
+  // Poll an active flag with a one-iteration lag; returns false when the solve ended.
   int poll_k = 0;
-  bool poll(bool final_wait) {
+  bool poll(const int* Fp, bool final_wait) {
     const int slot = poll_k & 1;
-    HIPCHK(hipMemcpyAsync(pinned_flags + slot * F_COUNT, F.p, F_COUNT * sizeof(int), hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipMemcpyAsync(pinned_flags + slot * F_COUNT, Fp, F_COUNT * sizeof(int), hipMemcpyDeviceToHost, stream));
     HIPCHK(hipEventRecord(ev[slot], stream));
     ++poll_k;
     if (final_wait) { HIPCHK(hipEventSynchronize(ev[slot])); return pinned_flags[slot * F_COUNT + F_ACTIVE] != 0; }
@@ -487,7 +540,72 @@ struct Solver {
     return pinned_flags[prev * F_COUNT + F_ACTIVE] != 0;
   }
 
+  // Preconditioned Krylov iterations on As x = r in precision R, from the state left by
+  // PH_INIT in (Sp, Fp): r holds the residual of x.  PCG when As is symmetric, else
+  // right-preconditioned BiCGSTAB.  Every kernel is predicated on Fp's active flag.
+  template <class R>
+  void krylov_solve(Prec<R>& P, R* x, R* r, double* Sp, int* Fp, int maxit) {
+    const int lb = lap_blocks(N);
+    const int vg = vec_grid(D);
+    poll_k = 0;
+    if (krylov == HOC_KRYLOV_PCG) {
+      HIPCHK(hipMemsetAsync(P.p.p, 0, D * sizeof(R), stream));
+      for (int it = 0;; ++it) {
+        vcycle(P, r, P.z.p, part.p, Fp);
+        finalize(stream, PH_RZ, part, lb, Sp, Fp, maxit);
+        HIPCHK(launch_trace(K, PRE_PUPD, N, P.T, P.p.p, P.z.p, (const R*)nullptr, (R*)nullptr, 0., Sp, S_BETA, Fp,
+                            P.tr.p, stream));
+        lap(P, EPI_STORE, P.p.p, P.q.p, (const R*)nullptr, (const R*)nullptr, (R*)nullptr, 0., 0., part.p, Fp);
+        finalize(stream, PH_PQ, part, lb, Sp, Fp, maxit);
+        vec<R>(stream, VOP_PCG_UPD, VA, D, x, r, P.p, P.q, nullptr, Sp, Fp, part);
+        finalize(stream, PH_UPD, part, vg, Sp, Fp, maxit);
+        if (!poll(Fp, it > maxit + 2) || it > maxit + 4) break;
+      }
+    } else {
+      vec<R>(stream, VOP_COPY, VA, D, P.rh, nullptr, r, nullptr, nullptr, nullptr, nullptr, nullptr);
+      HIPCHK(hipMemsetAsync(P.p.p, 0, D * sizeof(R), stream));
+      HIPCHK(hipMemsetAsync(P.v.p, 0, D * sizeof(R), stream));
+      const R* none = nullptr;
+      for (int it = 0;; ++it) {
+        vec<R>(stream, VOP_DOT2, VA, D, P.rh, P.rh, r, r, nullptr, Sp, Fp, part);
+        finalize(stream, PH_RHO, part, vg, Sp, Fp, maxit);
+        vec<R>(stream, VOP_BICG_P, VA, D, P.p, nullptr, r, P.v, nullptr, Sp, Fp, nullptr);
+        vcycle(P, P.p.p, P.ph.p, nullptr, Fp);
+        HIPCHK(launch_trace(K, PRE_COPY, N, P.T, P.ph.p, none, none, (R*)nullptr, 0., nullptr, 0, Fp, P.tr.p, stream));
+        lap(P, EPI_STORE, P.ph.p, P.v.p, none, none, (R*)nullptr, 0., 0., nullptr, Fp);
+        vec<R>(stream, VOP_DOT2, VA, D, P.rh, P.rh, P.v, P.v, nullptr, Sp, Fp, part);
+        finalize(stream, PH_ALPHA, part, vg, Sp, Fp, maxit);
+        vec<R>(stream, VOP_BICG_S, VA, D, nullptr, r, nullptr, P.v, nullptr, Sp, Fp, part);
+        finalize(stream, PH_SNORM, part, vg, Sp, Fp, maxit);
+        bicg_half<R>(stream, D, x, P.ph, Sp, Fp);
+        vcycle(P, r, P.sh.p, nullptr, Fp);  // gated by F_ACTIVE only; harmless when converged at the half step
+        HIPCHK(launch_trace(K, PRE_COPY, N, P.T, P.sh.p, none, none, (R*)nullptr, 0., nullptr, 0, Fp, P.tr.p, stream));
+        lap(P, EPI_STORE, P.sh.p, P.t.p, none, none, (R*)nullptr, 0., 0., nullptr, Fp);
+        vec<R>(stream, VOP_DOT2, VA, D, P.t, P.t, r, P.t, nullptr, Sp, Fp, part);
+        finalize(stream, PH_OMEGA, part, vg, Sp, Fp, maxit);
+        vec<R>(stream, VOP_BICG_X, VA, D, x, r, P.ph, P.sh, P.t, Sp, Fp, part);
+        finalize(stream, PH_RNORM, part, vg, Sp, Fp, maxit);
+        if (!poll(Fp, it > maxit + 2) || it > maxit + 4) break;
+      }
+    }
+  }
+
+  // FP64 residual r = bs - As psi with the reference's unscaled norm, and the solve's
+  // convergence state in (S, F).
+  void outer_residual() {
+    const int lb = lap_blocks(N);
+    HIPCHK(launch_trace(K, PRE_COPY, N, T, psi.p, nullptr, nullptr, nullptr, 0., nullptr, 0, nullptr, P64.tr.p, stream));
+    lap<double>(P64, EPI_RESID, psi.p, P64.r.p, bs.p, nullptr, nullptr, 0., 0., part.p, nullptr);
+    finalize(stream, PH_INIT, part, lb, S, F, cfg.maxiter);
+  }
+
   // solve_Poisson (src/HO_2D.cpp:1841-1977) for omega at the current boundary values.
+  //  FP64:  PCG/BiCGSTAB with the FP64 V-cycle on the scaled system, stopping on the
+  //         reference's ||b - A psi|| <= tol ||b||.
+  //  mixed: iterative refinement.  The FP64 residual r = b - A psi is demoted to FP32, the
+  //         correction e solves As e = r with FP32 Krylov + FP32 V-cycle to the relative
+  //         tolerance eta, psi += e in FP64, and the FP64 residual is recomputed; the loop
+  //         ends on the same FP64 test as the plain solve.
   void poisson(const double* omega) {
     const int lb = lap_blocks(N);
     const int maxit = cfg.maxiter;
@@ -495,57 +613,51 @@ struct Solver {
     HIPCHK(hipMemcpyAsync(S.p + S_TOL2, &tol2, sizeof(double), hipMemcpyHostToDevice, stream));
     HIPCHK(launch_rhs(K, N, T, G, omega, b_psi, bs, part, stream));
     finalize(stream, PH_BB, part, lb, S, F, maxit);
-    HIPCHK(launch_trace(K, PRE_COPY, N, T, psi, nullptr, nullptr, nullptr, 0., nullptr, 0, nullptr, tr, stream));
-    lap(EPI_RESID, psi, r, bs, nullptr, nullptr, 0., 0., part, nullptr);
-    finalize(stream, PH_INIT, part, lb, S, F, maxit);
-    zero_if_flag(stream, D, psi, F);
-    const int vg = vec_grid(D);
-    poll_k = 0;
-    if (krylov == HOC_KRYLOV_PCG) {
-      HIPCHK(hipMemsetAsync(p.p, 0, D * sizeof(double), stream));
-      for (int it = 0;; ++it) {
-        vcycle(r, z, part);
-        finalize(stream, PH_RZ, part, lb, S, F, maxit);
-        HIPCHK(launch_trace(K, PRE_PUPD, N, T, p, z, nullptr, nullptr, 0., S.p, S_BETA, F.p, tr, stream));
-        lap(EPI_STORE, p, q, nullptr, nullptr, nullptr, 0., 0., part, F.p);
-        finalize(stream, PH_PQ, part, lb, S, F, maxit);
-        vec(stream, VOP_PCG_UPD, VA, D, psi, r, p, q, nullptr, S, F, part);
-        finalize(stream, PH_UPD, part, vg, S, F, maxit);
-        if (!poll(it > maxit + 2) || it > maxit + 4) break;
-      }
-    } else {
-      vec(stream, VOP_COPY, VA, D, rh, nullptr, r, nullptr, nullptr, nullptr, nullptr, nullptr);
-      HIPCHK(hipMemsetAsync(p.p, 0, D * sizeof(double), stream));
-      HIPCHK(hipMemsetAsync(v.p, 0, D * sizeof(double), stream));
-      for (int it = 0;; ++it) {
-        vec(stream, VOP_DOT2, VA, D, rh, rh, r, r, nullptr, S, F, part);
-        finalize(stream, PH_RHO, part, vg, S, F, maxit);
-        vec(stream, VOP_BICG_P, VA, D, p, nullptr, r, v, nullptr, S, F, nullptr);
-        vcycle(p, ph, nullptr);
-        HIPCHK(launch_trace(K, PRE_COPY, N, T, ph, nullptr, nullptr, nullptr, 0., nullptr, 0, F.p, tr, stream));
-        lap(EPI_STORE, ph, v, nullptr, nullptr, nullptr, 0., 0., nullptr, F.p);
-        vec(stream, VOP_DOT2, VA, D, rh, rh, v, v, nullptr, S, F, part);
-        finalize(stream, PH_ALPHA, part, vg, S, F, maxit);
-        vec(stream, VOP_BICG_S, VA, D, nullptr, r, nullptr, v, nullptr, S, F, part);
-        finalize(stream, PH_SNORM, part, vg, S, F, maxit);
-        bicg_half(stream, D, psi, ph, S, F);
-        vcycle(r, sh, nullptr);  // gated by F_ACTIVE only; harmless when converged at the half step
-        HIPCHK(launch_trace(K, PRE_COPY, N, T, sh, nullptr, nullptr, nullptr, 0., nullptr, 0, F.p, tr, stream));
-        lap(EPI_STORE, sh, t, nullptr, nullptr, nullptr, 0., 0., nullptr, F.p);
-        vec(stream, VOP_DOT2, VA, D, t, t, r, t, nullptr, S, F, part);
-        finalize(stream, PH_OMEGA, part, vg, S, F, maxit);
-        vec(stream, VOP_BICG_X, VA, D, psi, r, ph, sh, t, S, F, part);
-        finalize(stream, PH_RNORM, part, vg, S, F, maxit);
-        if (!poll(it > maxit + 2) || it > maxit + 4) break;
-      }
-    }
-    // statistics (one sync per solve)
+    outer_residual();
+    zero_if_flag<double>(stream, D, psi, F);
     double hs[S_COUNT];
     int hf[F_COUNT];
-    HIPCHK(hipMemcpyAsync(hs, S.p, sizeof(hs), hipMemcpyDeviceToHost, stream));
-    HIPCHK(hipMemcpyAsync(hf, F.p, sizeof(hf), hipMemcpyDeviceToHost, stream));
-    HIPCHK(hipStreamSynchronize(stream));
-    last_iters = hf[F_ITER];
+    auto read_state = [&](const double* Sp, const int* Fp) {
+      HIPCHK(hipMemcpyAsync(hs, Sp, sizeof(hs), hipMemcpyDeviceToHost, stream));
+      HIPCHK(hipMemcpyAsync(hf, Fp, sizeof(hf), hipMemcpyDeviceToHost, stream));
+      HIPCHK(hipStreamSynchronize(stream));
+    };
+    int iters = 0, outer = 0;
+    if (!mixed) {
+      krylov_solve(P64, psi.p, P64.r.p, S.p, F.p, maxit);
+      read_state(S, F);
+      iters = hf[F_ITER];
+    } else {
+      const double eta_floor = cfg.mixed_inner_tol > 0. ? cfg.mixed_inner_tol : 1e-5;
+      constexpr int kMaxOuter = 12;
+      double rr_prev = 0.;
+      for (;;) {
+        read_state(S, F);
+        if (!hf[F_ACTIVE] || !std::isfinite(hs[S_RR]) || iters >= maxit || outer >= kMaxOuter) break;
+        // a pass that did not halve the FP64 residual has reached its rounding floor
+        if (outer > 0 && hs[S_RR] > 0.25 * rr_prev) break;
+        rr_prev = hs[S_RR];
+        // reduction still needed, with a margin for the FP32 residual gap
+        const double need = cfg.tol * std::sqrt(hs[S_BB] / hs[S_RR]);
+        const double eta = std::min(0.5, std::max(eta_floor, 0.3 * need));
+        const double eta2 = eta * eta;
+        HIPCHK(hipMemcpyAsync(S2.p + S_TOL2, &eta2, sizeof(double), hipMemcpyHostToDevice, stream));
+        mixed_demote(stream, VA, D, P64.r, P32.r, P32.x, part, F);
+        finalize(stream, PH_BB, part, vec_grid(D), S2, F2, maxit - iters);
+        finalize(stream, PH_INIT, part, vec_grid(D), S2, F2, maxit - iters);
+        krylov_solve(P32, P32.x.p, P32.r.p, S2.p, F2.p, maxit - iters);
+        mixed_promote_add(stream, D, psi, P32.x, F);
+        outer_residual();
+        ++outer;
+        int hf2[F_COUNT];
+        HIPCHK(hipMemcpyAsync(hf2, F2.p, sizeof(hf2), hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        iters += hf2[F_ITER];
+      }
+    }
+    // statistics
+    last_iters = iters;
+    last_outer = outer;
     if (!std::isfinite(hs[S_BB]) || !std::isfinite(hs[S_RR])) {
       last_rel = std::nan("");
       throw StatusError(HOC_ENAN, "NaN/Inf in the Poisson right-hand side or residual");
@@ -565,8 +677,8 @@ struct Solver {
   void stage(const double* win, double t, const StageOut& so) {
     HIPCHK(launch_update_bcs(E * K, B, ramp(t), stream));
     poisson(win);
-    HIPCHK(launch_trace(K, PRE_COPY, N, T, psi, nullptr, nullptr, nullptr, 0., nullptr, 0, nullptr, tr, stream));
-    HIPCHK(launch_velocity(K, N, T, G, B, psi, tr, win, vel, U, trw, tru, 1. / (cfg.dt * re_inv), stream));
+    HIPCHK(launch_trace(K, PRE_COPY, N, T, psi, nullptr, nullptr, nullptr, 0., nullptr, 0, nullptr, P64.tr.p, stream));
+    HIPCHK(launch_velocity(K, N, T, G, B, psi, P64.tr, win, vel, U, trw, tru, 1. / (cfg.dt * re_inv), stream));
     HIPCHK(launch_stage(K, cfg.advection_type, N, T, G, B, win, U, trw, tru, re_inv, cfg.dt, so, stream));
   }
 
@@ -650,6 +762,7 @@ void hoc_default_config(hoc_config* c) {
   c->krylov = HOC_KRYLOV_AUTO;
   c->device = 0;
   c->fail_on_noconv = 0;
+  c->mixed_inner_tol = 0.;
 }
 
 int hoc_create(hoc_t* out, const hoc_config* cfg) {
@@ -662,6 +775,7 @@ int hoc_create(hoc_t* out, const hoc_config* cfg) {
   if (cfg->problem_type < 1 || cfg->problem_type > 3) return hoc_fail(nullptr, HOC_EINVAL, "problem_type must be 1, 2 or 3");
   if (cfg->precision != HOC_FP64 && cfg->precision != HOC_MIXED) return hoc_fail(nullptr, HOC_EINVAL, "bad precision");
   if (!(cfg->dt > 0.) || cfg->maxiter < 0 || !(cfg->tol >= 0.)) return hoc_fail(nullptr, HOC_EINVAL, "bad dt/tol/maxiter");
+  if (!(cfg->mixed_inner_tol >= 0. && cfg->mixed_inner_tol < 1.)) return hoc_fail(nullptr, HOC_EINVAL, "mixed_inner_tol must be in [0, 1)");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return hoc_fail(nullptr, HOC_EDEVICE, "no HIP device");
   if (cfg->device < 0 || cfg->device >= ndev) return hoc_fail(nullptr, HOC_EDEVICE, "device ordinal out of range");
@@ -821,10 +935,10 @@ int hoc_apply_laplacian(hoc_t h, const double* x, double* y) {
   return guarded(h, [&] {
     NEED_READY(h);
     Solver& s = h->s;
-    s.h2d(s.z.p, x, s.D * sizeof(double));
-    s.apply_As(s.z, s.q);
+    s.h2d(s.P64.z.p, x, s.D * sizeof(double));
+    s.apply_As(s.P64.z, s.P64.q);
     std::vector<double> hy(s.D);
-    s.d2h(hy.data(), s.q.p, s.D * sizeof(double));
+    s.d2h(hy.data(), s.P64.q.p, s.D * sizeof(double));
     // As = -W A  =>  A x = -(As x) / W
     for (size_t v = 0; v < s.D; ++v) {
       const int m = (int)(v % s.K2);
@@ -838,7 +952,9 @@ int hoc_stats_get(hoc_t h, hoc_stats* st) {
     Solver& s = h->s;
     std::memset(st, 0, sizeof(*st));
     st->n_elem = s.N; st->n_dofs = (int64_t)s.D; st->n_bedges = s.E;
-    st->n_levels = (int32_t)s.lv.size() + 1;
+    st->n_levels = (int32_t)(s.mixed ? s.P32.lv.size() : s.P64.lv.size()) + 1;
+    st->last_refinements = s.last_outer;
+    st->precision = s.cfg.precision;
     st->krylov = s.krylov;
     st->last_iters = s.last_iters;
     st->last_rel_residual = s.last_rel;
@@ -869,12 +985,15 @@ int hoc_profile_get(hoc_t h, hoc_kernel_profile* p) {
     NEED_READY(h);
     Solver& s = h->s;
     s.prof_flush();
-    static const char* names[HOC_PROF_VARIANTS] = {"lap_apply", "lap_residual", "lap_cheb", "lap_cheb_first", "lap_restrict"};
+    static const char* names[HOC_PROF_VARIANTS] = {"lap_apply",     "lap_residual",     "lap_cheb",     "lap_cheb_first",
+                                                   "lap_restrict",  "lap_apply_f32",    "lap_residual_f32", "lap_cheb_f32",
+                                                   "lap_cheb_first_f32", "lap_restrict_f32"};
     std::memset(p, 0, sizeof(*p));
     for (int v = 0; v < HOC_PROF_VARIANTS; ++v) {
       p->launches[v] = s.prof.cnt[v];
       p->total_ms[v] = s.prof.ms[v];
-      p->bytes_per_launch[v] = s.lap_bytes(v);
+      const int half = HOC_PROF_VARIANTS / 2;
+      p->bytes_per_launch[v] = v < half ? s.lap_bytes(v, 8.) : s.lap_bytes(v - half, 4.);
       std::strncpy(p->names[v], names[v], sizeof(p->names[v]) - 1);
     }
   });

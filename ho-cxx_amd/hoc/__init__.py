@@ -32,26 +32,26 @@ EXPORTED = [
     "hoc_stage_residual", "hoc_apply_laplacian", "hoc_stats_get", "hoc_profile_enable", "hoc_profile_get",
     "hoc_last_error", "hoc_destroy",
 ]
-PROF_VARIANTS = 5
+PROF_VARIANTS = 10
 
 
 class hoc_config(C.Structure):
     _fields_ = [("K", C.c_int32), ("problem_type", C.c_int32), ("Re", C.c_double), ("dt", C.c_double),
                 ("time_integration_type", C.c_int32), ("advection_type", C.c_int32), ("tol", C.c_double),
                 ("maxiter", C.c_int32), ("precision", C.c_int32), ("krylov", C.c_int32), ("device", C.c_int32),
-                ("fail_on_noconv", C.c_int32)]
+                ("fail_on_noconv", C.c_int32), ("mixed_inner_tol", C.c_double)]
 
 
 class hoc_stats(C.Structure):
     _fields_ = [("n_elem", C.c_int64), ("n_dofs", C.c_int64), ("n_bedges", C.c_int64), ("n_levels", C.c_int32),
                 ("krylov", C.c_int32), ("last_iters", C.c_int32), ("last_rel_residual", C.c_double),
                 ("total_iters", C.c_int64), ("n_solves", C.c_int64), ("setup_seconds", C.c_double),
-                ("fine_op_bytes", C.c_double)]
+                ("fine_op_bytes", C.c_double), ("last_refinements", C.c_int32), ("precision", C.c_int32)]
 
 
 class hoc_kernel_profile(C.Structure):
-    _fields_ = [("launches", C.c_int64 * 5), ("total_ms", C.c_double * 5), ("bytes_per_launch", C.c_double * 5),
-                ("names", (C.c_char * 24) * 5)]
+    _fields_ = [("launches", C.c_int64 * PROF_VARIANTS), ("total_ms", C.c_double * PROF_VARIANTS),
+                ("bytes_per_launch", C.c_double * PROF_VARIANTS), ("names", (C.c_char * 24) * PROF_VARIANTS)]
 
 
 class HocError(RuntimeError):
@@ -114,11 +114,12 @@ class HO2D:
     """One problem on one GPU; mirrors the reference HO_2D call surface."""
 
     def __init__(self, K=3, problem_type=1, Re=100.0, dt=1e-3, time_integration_type=4, advection_type=1,
-                 tol=1e-15, maxiter=150, precision=HOC_FP64, krylov=KRYLOV_AUTO, device=0, fail_on_noconv=False):
+                 tol=1e-15, maxiter=150, precision=HOC_FP64, krylov=KRYLOV_AUTO, device=0, fail_on_noconv=False,
+                 mixed_inner_tol=0.0):
         self.cfg = default_config(K=K, problem_type=problem_type, Re=Re, dt=dt,
                                   time_integration_type=time_integration_type, advection_type=advection_type,
                                   tol=tol, maxiter=maxiter, precision=precision, krylov=krylov, device=device,
-                                  fail_on_noconv=int(fail_on_noconv))
+                                  fail_on_noconv=int(fail_on_noconv), mixed_inner_tol=mixed_inner_tol)
         self.h = C.c_void_p()
         rc = lib().hoc_create(C.byref(self.h), C.byref(self.cfg))
         if rc != HOC_OK:

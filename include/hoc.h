@@ -47,10 +47,13 @@ typedef struct {
   int32_t advection_type;         /* 1 discontinuous mass flux, 2 continuous             */
   double tol;                     /* Krylov relative residual tolerance                  */
   int32_t maxiter;                /* Krylov iteration cap                                */
-  int32_t precision;              /* HOC_FP64 or HOC_MIXED (FP32 V-cycle)               */
+  int32_t precision;              /* HOC_FP64, or HOC_MIXED: FP64 iterative refinement
+                                     around an FP32 Krylov + V-cycle inner solve         */
   int32_t krylov;                 /* HOC_KRYLOV_*; AUTO = PCG when W*A is symmetric     */
   int32_t device;                 /* HIP device ordinal                                  */
   int32_t fail_on_noconv;         /* nonzero: return HOC_ENOCONV when tol is not met     */
+  double mixed_inner_tol;         /* HOC_MIXED: floor of the FP32 inner solve's relative
+                                     tolerance per refinement pass (0 = 1e-5)            */
 } hoc_config;
 
 typedef struct {
@@ -63,6 +66,8 @@ typedef struct {
   int64_t n_solves;
   double setup_seconds;
   double fine_op_bytes;           /* algorithmic bytes of one fine-operator application  */
+  int32_t last_refinements;       /* HOC_MIXED: FP64 refinement passes of the last solve */
+  int32_t precision;              /* HOC_FP64 or HOC_MIXED                               */
 } hoc_stats;
 
 /* Fill cfg with the reference defaults (src/HO_2D.cpp:3698-3713, 3066-3069). */
@@ -119,8 +124,8 @@ int hoc_stats_get(hoc_t h, hoc_stats* s);
 
 /* Live timing of the fine-level Poisson operator kernel (the dominant kernel): with
  * profiling on, every launch is bracketed by HIP events recorded on the handle's own
- * stream.  One entry per fused epilogue variant. */
-#define HOC_PROF_VARIANTS 5
+ * stream.  One entry per fused epilogue variant: entries 0-4 FP64, 5-9 FP32. */
+#define HOC_PROF_VARIANTS 10
 typedef struct {
   int64_t launches[HOC_PROF_VARIANTS];
   double total_ms[HOC_PROF_VARIANTS];
