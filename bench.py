@@ -43,23 +43,63 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(K, Re, tol, n=64, warm=2, steps=6):
+def cpu_threads():
+    """Host threads available to this process (at most 16, the host share of one GPU)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def cpu_baseline(K, Re, tol, n=64, warm=2, steps=4):
+    """The FP64 oracle on a bounded sample of the workload, on 1 thread (the reference's
+    default serial build) and on all host threads (OpenMP linear algebra; the FR residual
+    loops, < 2% of the oracle's step, stay serial)."""
+    import pyoracle
     from hoc import meshes
     from pyoracle import Oracle
     m = meshes.cavity(n, 2)
     dt = cavity_dt(n, K, Re)
-    t0 = time.perf_counter()
-    o = Oracle(m, K, problem_type=1, Re=Re, dt=dt, time_integration_type=4, advection_type=1, tol=tol, maxiter=150)
-    setup = time.perf_counter() - t0
-    o.step(warm)
-    t0 = time.perf_counter()
-    o.step(steps)
-    el = time.perf_counter() - t0
     D = n * n * K * K
-    return {"value": D * steps / el / 1e6, "unit": "Mdof-steps/s", "cores": 1, "kind": "port",
-            "sample": f"oracle (FP64 CPU restatement, 1 thread) on a {n}x{n} cavity, K={K}, Re={Re}, "
-                      f"dt={dt:.3g}, tol={tol:g}: {warm} warm-up + {steps} timed RK4 steps "
-                      f"({el:.1f} s timed, setup {setup:.1f} s)"}
+    res = {}
+    for nt in (1, cpu_threads()):
+        pyoracle.set_threads(nt)
+        t0 = time.perf_counter()
+        o = Oracle(m, K, problem_type=1, Re=Re, dt=dt, time_integration_type=4, advection_type=1, tol=tol, maxiter=150)
+        setup = time.perf_counter() - t0
+        o.step(warm)
+        t0 = time.perf_counter()
+        o.step(steps)
+        el = time.perf_counter() - t0
+        res[nt] = (D * steps / el / 1e6, el, setup)
+    pyoracle.set_threads(1)
+    nt = cpu_threads()
+    v1, el1, _ = res[1]
+    vn, eln, setup = res[nt]
+    return {"value": vn, "unit": "Mdof-steps/s", "cores": nt, "kind": "port",
+            "sample": f"oracle (FP64 CPU restatement of the reference algorithm; OpenMP SpMV/vector/dot in the "
+                      f"AMG-preconditioned Krylov solve) on a {n}x{n} cavity, K={K}, Re={Re}, dt={dt:.3g}, "
+                      f"tol={tol:g}: {warm} warm-up + {steps} timed RK4 steps ({eln:.1f} s timed on {nt} threads, "
+                      f"setup {setup:.1f} s)",
+            "single_thread": {"value": v1, "cores": 1, "timed_s": el1}}
+
+
+def measured_traffic(kernel, n, K):
+    """HBM bytes per launch of `kernel` measured with rocprofv3 FETCH_SIZE + WRITE_SIZE
+    (scaled by a plain copy's readings) on this workload, from the committed summary, or
+    None when the summary does not cover it."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic_cavity512_K5.md")
+    if (n, K) != (512, 5) or not os.path.exists(path):
+        return None
+    for line in open(path):
+        cells = [c.strip().strip("`") for c in line.strip().strip("|").split("|")]
+        if len(cells) >= 5 and cells[0] == kernel:
+            try:
+                return float(cells[4]) * 1e6
+            except ValueError:
+                return None
+    return None
 
 
 def roofline(prof, suffix=""):
@@ -71,7 +111,7 @@ def roofline(prof, suffix=""):
     avg_s = v["total_ms"] / max(1, v["launches"]) / 1e3
     gbs = v["bytes_per_launch"] / avg_s / 1e9
     return {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
-            "traffic": None, "kernel": name, "avg_launch_us": avg_s * 1e6,
+            "traffic": None, "traffic_unit": "bytes per launch", "kernel": name, "avg_launch_us": avg_s * 1e6,
             "bytes_per_launch": v["bytes_per_launch"], "launches": v["launches"],
             "all_variants": {k: {"avg_us": 1e3 * x["total_ms"] / max(1, x["launches"]),
                                  "GBs": x["bytes_per_launch"] / max(1e-12, x["total_ms"] / max(1, x["launches"]) / 1e3) / 1e9,
@@ -125,7 +165,20 @@ def main():
                     help="fp64: plain path only; both (default): plain FP64 line plus the mixed-precision "
                          "run from the same initial state, with its field error against the FP64 run")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 1 GiB device-to-device copy (known bytes), the reference point "
+                         "for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE readings")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        n = 1 << 27
+        a = torch.ones(n, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8}))
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is design-only (DESIGN.md)",
                           "n_gpus": args.gpus}))
@@ -154,6 +207,7 @@ def main():
         "roofline": roofline(prof, "" if primary == hoc.HOC_FP64 else "_f32"),
         "cpu_baseline": None,
     }
+    out["roofline"]["traffic"] = measured_traffic(out["roofline"]["kernel"], n, K)
     if args.precision == "both":
         g2, res2, prof2 = run(hoc, m, n, K, Re, dt, args.tol, hoc.HOC_MIXED, args.warmup, args.steps)
         a = g.solution_areas()
@@ -170,8 +224,9 @@ def main():
                         "field_error_vs_fp64": err, "valid_at_1e-8": worst <= 1e-8,
                         "krylov_iters_per_solve": res2["krylov_iters_per_solve"],
                         "refinements_last_solve": res2["last_refinements"],
-                        "roofline": roofline(prof2, "_f32"),
+                        "roofline": dict(roofline(prof2, "_f32")),
                         "dtype": "f64 outer residual + f32 Krylov/V-cycle"}
+        out["mixed"]["roofline"]["traffic"] = measured_traffic(out["mixed"]["roofline"]["kernel"], n, K)
         log(f"mixed: {res2['value']:.2f} Mdof-steps/s ({res2['value'] / res['value']:.2f}x), field error {err}")
         g2.close()
     if not args.no_cpu_baseline:

@@ -3,6 +3,9 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
+#include <fstream>
+#include <sstream>
 #include <stdexcept>
 #include <unordered_map>
 
@@ -515,6 +518,191 @@ void build_sa_amg(const CSR& A0, int coarse_enough, AMGHost& h) {
     }
   }
   h.coarse_inv = std::move(inv);
+}
+
+// ---------------------------------------------------------------------------------------
+// Gmsh 4.1 ASCII input (src/Mesh.cpp:93-293)
+// ---------------------------------------------------------------------------------------
+namespace {
+struct Lines {
+  std::vector<std::string> v;
+  size_t find(const std::string& key) const {
+    for (size_t i = 0; i < v.size(); ++i)
+      if (v[i].compare(0, key.size(), key) == 0) return i;
+    throw std::runtime_error("gmsh: section " + key + " not found");
+  }
+};
+std::vector<long long> ints(const std::string& s) {
+  std::vector<long long> out;
+  const char* p = s.c_str();
+  char* e = nullptr;
+  for (;;) {
+    while (*p == ' ' || *p == '\t' || *p == '\r') ++p;
+    if (!*p) break;
+    const long long x = std::strtoll(p, &e, 10);
+    if (e == p) break;
+    out.push_back(x);
+    p = e;
+    while (*p && *p != ' ' && *p != '\t' && *p != '\r') ++p;  // skip a fractional part
+  }
+  return out;
+}
+}  // namespace
+
+void read_gmsh41(const std::string& path, int occ, Mesh& m) {
+  std::ifstream in(path);
+  if (!in) throw std::runtime_error("gmsh: cannot open " + path);
+  Lines L;
+  for (std::string ln; std::getline(in, ln);) L.v.push_back(ln);
+  auto at = [&](size_t i) -> const std::string& {
+    if (i >= L.v.size()) throw std::runtime_error("gmsh: unexpected end of file");
+    return L.v[i];
+  };
+  // physical names of dimension 1: boundary names in file order
+  size_t i = L.find("$PhysicalNames") + 1;
+  const long long nphys = ints(at(i++)).at(0);
+  std::vector<long long> phys_tags;
+  m.bnames.clear();
+  for (long long k = 0; k < nphys; ++k) {
+    const std::string& ln = at(i++);
+    const auto v = ints(ln);
+    if (v.size() < 2 || v[0] != 1) continue;
+    const size_t q0 = ln.find('"'), q1 = ln.rfind('"');
+    if (q0 == std::string::npos || q1 <= q0) throw std::runtime_error("gmsh: bad physical name");
+    phys_tags.push_back(v[1]);
+    m.bnames.push_back(ln.substr(q0 + 1, q1 - q0 - 1));
+  }
+  // entities: curve tag -> its first physical tag (0 = none)
+  i = L.find("$Entities") + 1;
+  const auto cnt = ints(at(i++));
+  if (cnt.size() < 4) throw std::runtime_error("gmsh: bad $Entities header");
+  i += (size_t)cnt[0];
+  std::unordered_map<long long, long long> curve_phys;
+  for (long long k = 0; k < cnt[1]; ++k) {
+    // curve: tag minX minY minZ maxX maxY maxZ numPhysical phys... numPoints points...
+    std::istringstream ss(at(i++));
+    long long tag, nph;
+    double bb[6];
+    ss >> tag;
+    for (double& b : bb) ss >> b;
+    ss >> nph;
+    long long ph = 0;
+    if (nph > 0) ss >> ph;
+    curve_phys[tag] = ph;
+  }
+  // nodes (4.1: blocks of tags then coordinates); indices follow file order
+  i = L.find("$Nodes") + 1;
+  const auto nh = ints(at(i++));
+  if (nh.size() < 4) throw std::runtime_error("gmsh: bad $Nodes header");
+  std::unordered_map<long long, int32_t> tag2idx;
+  m.xy.assign(2 * (size_t)nh[1], 0.);
+  int32_t c = 0;
+  for (long long b = 0; b < nh[0]; ++b) {
+    const auto bh = ints(at(i++));
+    if (bh.size() < 4) throw std::runtime_error("gmsh: bad node block");
+    const long long nb = bh[3];
+    for (long long k = 0; k < nb; ++k) tag2idx[ints(at(i + k)).at(0)] = c + (int32_t)k;
+    i += nb;
+    for (long long k = 0; k < nb; ++k) {
+      std::istringstream ss(at(i + k));
+      ss >> m.xy[2 * (size_t)(c + k)] >> m.xy[2 * (size_t)(c + k) + 1];
+    }
+    i += nb;
+    c += (int32_t)nb;
+  }
+  if (c != nh[1]) throw std::runtime_error("gmsh: node count mismatch");
+  m.n_nodes = c;
+  auto idx = [&](long long tag) {
+    auto it = tag2idx.find(tag);
+    if (it == tag2idx.end()) throw std::runtime_error("gmsh: element references an unknown node");
+    return it->second;
+  };
+  // elements: line blocks (boundary edges of tagged curves) and quad blocks
+  i = L.find("$Elements") + 1;
+  const long long nblocks = ints(at(i++)).at(0);
+  int lnod = 0, face_nodes = 0;
+  m.elem.clear(); m.edges.clear(); m.bedge_bdry.clear();
+  for (long long b = 0; b < nblocks; ++b) {
+    const auto bh = ints(at(i++));
+    if (bh.size() < 4) throw std::runtime_error("gmsh: bad element block");
+    const long long dim = bh[0], etag = bh[1], etype = bh[2], nb = bh[3];
+    if (dim == 1) {
+      const int en = etype == 1 ? 2 : etype == 8 ? 3 : etype == 26 ? 4 : 0;
+      if (!en) throw std::runtime_error("gmsh: unsupported line element type " + std::to_string(etype));
+      lnod = en;
+      const long long ph = curve_phys.count(etag) ? curve_phys[etag] : 0;
+      if (ph != 0) {
+        const auto pit = std::find(phys_tags.begin(), phys_tags.end(), ph);
+        if (pit == phys_tags.end()) throw std::runtime_error("gmsh: curve with an unnamed physical tag");
+        const int32_t bid = (int32_t)(pit - phys_tags.begin());
+        for (long long k = 0; k < nb; ++k) {
+          const auto v = ints(at(i + k));
+          if ((int)v.size() < en + 1) throw std::runtime_error("gmsh: short line element");
+          std::vector<int32_t> e(en);
+          for (int q = 0; q < en; ++q) e[q] = idx(v[1 + q]);
+          m.edges.push_back(e);
+          m.bedge_bdry.push_back(bid);
+        }
+      }
+    } else if (dim == 2) {
+      const int fn = etype == 3 ? 4 : etype == 10 ? 9 : etype == 36 ? 16 : 0;
+      if (!fn) throw std::runtime_error("gmsh: unsupported face element type " + std::to_string(etype));
+      if (face_nodes && fn != face_nodes) throw std::runtime_error("gmsh: mixed quad orders");
+      face_nodes = fn;
+      for (long long k = 0; k < nb; ++k) {
+        const auto v = ints(at(i + k));
+        if ((int)v.size() < fn + 1) throw std::runtime_error("gmsh: short face element");
+        for (int q = 0; q < fn; ++q) m.elem.push_back(idx(v[1 + q]));
+      }
+    }
+    i += (size_t)nb;
+  }
+  if (!face_nodes) throw std::runtime_error("gmsh: no quadrilateral elements");
+  const int Lq = face_nodes == 4 ? 2 : face_nodes == 9 ? 3 : 4;
+  if (lnod && lnod != Lq) throw std::runtime_error("gmsh: line and quad orders differ");
+  m.lnod = Lq;
+  m.n_elem = (int)(m.elem.size() / face_nodes);
+  m.n_bedges = (int)m.edges.size();
+  if (occ) {  // reflect about the element diagonal: (i, j) -> (L-1-j, L-1-i)
+    std::vector<int32_t> e(face_nodes);
+    for (int el = 0; el < m.n_elem; ++el) {
+      int32_t* p = m.elem.data() + (size_t)el * face_nodes;
+      for (int jo = 0; jo < Lq; ++jo)
+        for (int io = 0; io < Lq; ++io) e[t2g(Lq, Lq - 1 - jo, Lq - 1 - io)] = p[t2g(Lq, io, jo)];
+      std::copy(e.begin(), e.end(), p);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------------------
+// Legacy-VTK output of solution-point fields
+// ---------------------------------------------------------------------------------------
+void write_vtk_sps(const std::string& path, int K, int n_elem, const double* pts, int n_fields,
+                   const char* const* names, const double* const* fields) {
+  std::ofstream out(path);
+  if (!out) throw std::runtime_error("vtk: cannot open " + path);
+  const size_t D = (size_t)n_elem * K * K;
+  const size_t cells = (size_t)n_elem * (K - 1) * (K - 1);
+  out << "# vtk DataFile Version 3.0\nHO-CXX solution points\nASCII\nDATASET UNSTRUCTURED_GRID\n";
+  out << "POINTS " << D << " double\n";
+  out.precision(17);
+  for (size_t v = 0; v < D; ++v) out << pts[2 * v] << ' ' << pts[2 * v + 1] << " 0\n";
+  out << "CELLS " << cells << ' ' << cells * 5 << '\n';
+  for (int el = 0; el < n_elem; ++el)
+    for (int j = 0; j + 1 < K; ++j)
+      for (int i = 0; i + 1 < K; ++i) {
+        const size_t b = (size_t)el * K * K;
+        out << "4 " << b + j * K + i << ' ' << b + j * K + i + 1 << ' ' << b + (j + 1) * K + i + 1 << ' '
+            << b + (j + 1) * K + i << '\n';
+      }
+  out << "CELL_TYPES " << cells << '\n';
+  for (size_t q = 0; q < cells; ++q) out << "9\n";
+  out << "POINT_DATA " << D << '\n';
+  for (int f = 0; f < n_fields; ++f) {
+    out << "SCALARS " << names[f] << " double 1\nLOOKUP_TABLE default\n";
+    for (size_t v = 0; v < D; ++v) out << fields[f][v] << '\n';
+  }
+  if (!out) throw std::runtime_error("vtk: write failed for " + path);
 }
 
 }  // namespace hoc

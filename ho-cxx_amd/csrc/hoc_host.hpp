@@ -38,6 +38,19 @@ struct Mesh {
 // Throws std::runtime_error on a non-conforming mesh.
 void connect(Mesh& m);
 
+// Gmsh 4.1 ASCII reader (Mesh::read_msh_file, src/Mesh.cpp:93-293): physical names of
+// dimension 1 become the boundaries; only curves with a physical tag contribute boundary
+// edges, in file order; 2-D elements keep file order; occ != 0 reflects every element's
+// nodes into counter-clockwise order (src/Mesh.cpp:281-293).  Fills lnod, n_nodes, xy,
+// n_elem, elem, n_bedges, edges, bedge_bdry, bnames.  Throws std::runtime_error.
+void read_gmsh41(const std::string& path, int occ, Mesh& m);
+
+// Legacy-VTK unstructured grid of the solution points (one quad cell per sub-square of
+// each element's K x K point lattice) with point fields (src/HO_2D.cpp:3369-3692 writes
+// the same fields as raw VTK).  fields: n_fields arrays of n_elem*K*K values.
+void write_vtk_sps(const std::string& path, int K, int n_elem, const double* pts_xy, int n_fields,
+                   const char* const* names, const double* const* fields);
+
 // Greedy element colourings: distance 1 (face neighbours differ) and distance 2
 // (additionally no element has two face neighbours of one colour).
 int color_elements(const Mesh& m, int distance, std::vector<int32_t>& color);

@@ -22,72 +22,12 @@
 #include <vector>
 
 #include "../../include/hoc.h"
+#include "hoc_dev.hpp"
 #include "hoc_fr.hpp"
 #include "hoc_host.hpp"
 #include "hoc_la.hpp"
 
 namespace hoc {
-
-struct HipError : std::runtime_error {
-  explicit HipError(const std::string& s) : std::runtime_error(s) {}
-};
-struct StatusError : std::runtime_error {
-  int code;
-  StatusError(int c, const std::string& s) : std::runtime_error(s), code(c) {}
-};
-
-#define HIPCHK(x)                                                                         \
-  do {                                                                                    \
-    hipError_t e_ = (x);                                                                  \
-    if (e_ != hipSuccess) throw ::hoc::HipError(std::string(#x) + ": " + hipGetErrorString(e_)); \
-  } while (0)
-
-template <class T>
-struct DBuf {
-  T* p = nullptr;
-  size_t n = 0;
-  DBuf() = default;
-  DBuf(const DBuf&) = delete;
-  DBuf& operator=(const DBuf&) = delete;
-  ~DBuf() { if (p) (void)hipFree(p); }
-  void alloc(size_t cnt) {
-    if (p) { (void)hipFree(p); p = nullptr; }
-    n = cnt;
-    if (cnt) {
-      HIPCHK(hipMalloc(&p, cnt * sizeof(T)));
-      HIPCHK(hipMemset(p, 0, cnt * sizeof(T)));
-      HIPCHK(hipDeviceSynchronize());
-    }
-  }
-  void upload(const std::vector<T>& v) {
-    alloc(v.size());
-    if (!v.empty()) HIPCHK(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
-    HIPCHK(hipDeviceSynchronize());
-  }
-  // Upload with conversion (FP64 host data into an FP32 buffer).
-  template <class U>
-  void upload_as(const std::vector<U>& v) {
-    std::vector<T> c(v.begin(), v.end());
-    upload(c);
-  }
-  operator T*() const { return p; }
-};
-
-template <class R>
-struct DevCSR {
-  int n = 0, lanes = 1;
-  DBuf<int32_t> ptr, col;
-  DBuf<R> val;
-  void upload(const CSR& A) {
-    n = A.n; ptr.upload(A.ptr); col.upload(A.col); val.upload_as(A.val);
-    lanes = csr_lanes(A.n, A.val.size());
-  }
-};
-template <class R>
-struct DevLevel {
-  DevCSR<R> A, P, Rs;
-  DBuf<R> M, f, u, t;
-};
 
 // Preconditioned-Krylov state in working precision R: the fine operator's tables and
 // metrics, the Krylov and smoother vectors, face traces and the V-cycle hierarchy.
@@ -127,6 +67,8 @@ struct Prec {
     rc.alloc(H.levels[0].A.n); xc.alloc(H.levels[0].A.n);
   }
 };
+
+constexpr int kMaxCheb = 8;
 
 struct Solver {
   hoc_config cfg{};
@@ -172,7 +114,10 @@ struct Solver {
   int nparts = 0;
   int nv = 0;                             // Q1 coarse space: element-corner vertices
   DBuf<int32_t> d_ev, d_vptr, d_vinc;     // element -> 4 vertices; vertex -> (element*4+corner)
-  double cheb_theta = 1., cheb_c1 = 0., cheb_c2 = 0.;
+  // Chebyshev smoother of degree cheb_m: step 0 scales by 1/theta, step k >= 1 uses
+  // d = cheb_c1[k] d + cheb_c2[k] D^-1 res
+  int cheb_m = 2;
+  double cheb_theta = 1., cheb_c1[kMaxCheb] = {0}, cheb_c2[kMaxCheb] = {0};
   double lam_max = 0.;
   int krylov = HOC_KRYLOV_BICGSTAB;
   // live kernel timing (hoc_profile_*)
@@ -466,12 +411,18 @@ struct Solver {
       h2d(P64.z.p, hx.data(), D * sizeof(double));
     }
     lam_max = lam;
-    const double lmax = 1.1 * lam, lmin = lmax / 30.;
+    const double lmax = 1.1 * lam, lmin = lmax * (cfg.cheb_lower > 0. ? cfg.cheb_lower : 1. / 30.);
     const double theta = 0.5 * (lmax + lmin), delta = 0.5 * (lmax - lmin);
-    const double sigma = theta / delta, rho0 = 1. / sigma, rho1 = 1. / (2. * sigma - rho0);
+    const double sigma = theta / delta;
+    cheb_m = cfg.cheb_degree > 0 ? cfg.cheb_degree : 2;
     cheb_theta = theta;
-    cheb_c1 = rho1 * rho0;
-    cheb_c2 = 2. * rho1 / delta;
+    double rho = 1. / sigma;
+    for (int k = 1; k < cheb_m; ++k) {
+      const double rho_new = 1. / (2. * sigma - rho);
+      cheb_c1[k] = rho_new * rho;
+      cheb_c2[k] = 2. * rho_new / delta;
+      rho = rho_new;
+    }
     // symmetry of As decides the Krylov method
     krylov = cfg.krylov;
     if (krylov == HOC_KRYLOV_AUTO) {
@@ -497,33 +448,38 @@ struct Solver {
   // V-cycle on the Q1 vertex level and below: u_l = V_l(f_l)
   template <class R>
   void coarse_cycle(Prec<R>& P, size_t l, const R* f, R* u, const int* Fp) {
-    DevLevel<R>& L = *P.lv[l];
-    if (l + 1 == P.lv.size()) { dense<R>(stream, P.coarse_n, P.coarse_inv, f, u, Fp); return; }
-    DevLevel<R>& C = *P.lv[l + 1];
-    scale<R>(stream, L.A.n, L.M, f, u, Fp);
-    csr<R>(stream, 1, L.A.n, L.A.ptr, L.A.col, L.A.val, u, f, L.t, nullptr, nullptr, Fp, L.A.lanes);
-    csr<R>(stream, 0, C.A.n, L.Rs.ptr, L.Rs.col, L.Rs.val, L.t, nullptr, C.f, nullptr, nullptr, Fp, L.Rs.lanes);
-    coarse_cycle(P, l + 1, C.f.p, C.u.p, Fp);
-    csr<R>(stream, 2, L.A.n, L.P.ptr, L.P.col, L.P.val, C.u, nullptr, u, nullptr, nullptr, Fp, L.P.lanes);
-    csr<R>(stream, 3, L.A.n, L.A.ptr, L.A.col, L.A.val, u, f, nullptr, L.M, u, Fp, L.A.lanes);
+    csr_vcycle<R>(stream, P.lv, P.coarse_inv, P.coarse_n, l, f, u, Fp);
   }
 
   // zout = V(rin); if part != nullptr, partials of rin.zout are left in part.
   template <class R>
   void vcycle(Prec<R>& P, const R* rin, R* zout, double* partp, const int* Fp) {
     R* rr = const_cast<R*>(rin);
-    // pre-smoothing: Chebyshev(2) from zero
-    HIPCHK(launch_trace(K, PRE_CHEB0, N, P.T, zout, rr, P.diag.p, P.dch.p, cheb_theta, nullptr, 0, Fp, P.tr.p, stream));
-    lap(P, EPI_CHEB, zout, (R*)nullptr, rr, P.diag.p, P.dch.p, cheb_c1, cheb_c2, nullptr, Fp, P.tr.p, P.tr2.p);
+    R* cur = P.tr.p;   // traces of the current iterate
+    R* nxt = P.tr2.p;
+    // pre-smoothing: Chebyshev(m) from zero
+    HIPCHK(launch_trace(K, PRE_CHEB0, N, P.T, zout, rr, P.diag.p, P.dch.p, cheb_theta, nullptr, 0, Fp, cur, stream));
+    for (int k = 1; k < cheb_m; ++k) {
+      lap(P, EPI_CHEB, zout, (R*)nullptr, rr, P.diag.p, P.dch.p, cheb_c1[k], cheb_c2[k], nullptr, Fp, cur, nxt);
+      std::swap(cur, nxt);
+    }
     // restrict the residual to the corner vertices
-    lap(P, EPI_RESTRICT, zout, P.ec.p, rr, (const R*)nullptr, (R*)nullptr, 0., 0., nullptr, Fp, P.tr2.p);
+    lap(P, EPI_RESTRICT, zout, P.ec.p, rr, (const R*)nullptr, (R*)nullptr, 0., 0., nullptr, Fp, cur);
     q1_gather<R>(stream, nv, d_vptr, d_vinc, P.ec, P.rc, Fp);
     coarse_cycle(P, 0, P.rc.p, P.xc.p, Fp);
-    // prolongate and post-smooth: Chebyshev(2) from the corrected iterate
-    HIPCHK(launch_trace(K, PRE_PROL, N, P.T, zout, P.xc.p, (const R*)nullptr, (R*)nullptr, 0., nullptr, 0, Fp, P.tr.p,
+    // prolongate and post-smooth: Chebyshev(m) from the corrected iterate
+    cur = P.tr.p; nxt = P.tr2.p;
+    HIPCHK(launch_trace(K, PRE_PROL, N, P.T, zout, P.xc.p, (const R*)nullptr, (R*)nullptr, 0., nullptr, 0, Fp, cur,
                         stream, d_ev.p));
-    lap(P, EPI_CHEBF, zout, (R*)nullptr, rr, P.diag.p, P.dch.p, 0., 1. / cheb_theta, nullptr, Fp, P.tr.p, P.tr2.p);
-    lap(P, EPI_CHEB, zout, (R*)nullptr, rr, P.diag.p, P.dch.p, cheb_c1, cheb_c2, partp, Fp, P.tr2.p);
+    lap(P, EPI_CHEBF, zout, (R*)nullptr, rr, P.diag.p, P.dch.p, 0., 1. / cheb_theta, cheb_m == 1 ? partp : nullptr, Fp,
+        cur, cheb_m == 1 ? (R*)nullptr : nxt);
+    std::swap(cur, nxt);
+    for (int k = 1; k < cheb_m; ++k) {
+      const bool last = k == cheb_m - 1;
+      lap(P, EPI_CHEB, zout, (R*)nullptr, rr, P.diag.p, P.dch.p, cheb_c1[k], cheb_c2[k], last ? partp : nullptr, Fp,
+          cur, last ? (R*)nullptr : nxt);
+      std::swap(cur, nxt);
+    }
   }
 
   // Poll an active flag with a one-iteration lag; returns false when the solve ended.
@@ -671,6 +627,18 @@ struct Solver {
       throw StatusError(HOC_ENOCONV, "Poisson solve did not converge: rel residual " + std::to_string(last_rel));
   }
 
+  // Validates a mesh given as arrays or read from a file and makes it the handle's mesh.
+  void accept_mesh(Mesh&& m) {
+    for (int32_t v : m.elem) if (v < 0 || v >= m.n_nodes) throw StatusError(HOC_EINVAL, "element node id out of range");
+    for (auto& e : m.edges)
+      for (int32_t v : e) if (v < 0 || v >= m.n_nodes) throw StatusError(HOC_EINVAL, "edge node id out of range");
+    for (int32_t b : m.bedge_bdry)
+      if (b < 0 || b >= (int)m.bnames.size()) throw StatusError(HOC_EINVAL, "boundary id out of range");
+    mesh = std::move(m);
+    mesh_loaded = true;
+    ready = false;
+  }
+
   double ramp(double t) const { return bch.time_dependent ? 2. * std::fabs(std::sin(M_PI / 40. * t)) : 1.; }
 
   // Euler_time_integrate (src/HO_2D.cpp:1806-1839) on input win at time t.
@@ -763,6 +731,8 @@ void hoc_default_config(hoc_config* c) {
   c->device = 0;
   c->fail_on_noconv = 0;
   c->mixed_inner_tol = 0.;
+  c->cheb_degree = 0;
+  c->cheb_lower = 0.;
 }
 
 int hoc_create(hoc_t* out, const hoc_config* cfg) {
@@ -776,6 +746,8 @@ int hoc_create(hoc_t* out, const hoc_config* cfg) {
   if (cfg->precision != HOC_FP64 && cfg->precision != HOC_MIXED) return hoc_fail(nullptr, HOC_EINVAL, "bad precision");
   if (!(cfg->dt > 0.) || cfg->maxiter < 0 || !(cfg->tol >= 0.)) return hoc_fail(nullptr, HOC_EINVAL, "bad dt/tol/maxiter");
   if (!(cfg->mixed_inner_tol >= 0. && cfg->mixed_inner_tol < 1.)) return hoc_fail(nullptr, HOC_EINVAL, "mixed_inner_tol must be in [0, 1)");
+  if (cfg->cheb_degree < 0 || cfg->cheb_degree > hoc::kMaxCheb) return hoc_fail(nullptr, HOC_EINVAL, "cheb_degree must be in 0..8");
+  if (!(cfg->cheb_lower >= 0. && cfg->cheb_lower < 1.)) return hoc_fail(nullptr, HOC_EINVAL, "cheb_lower must be in [0, 1)");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return hoc_fail(nullptr, HOC_EDEVICE, "no HIP device");
   if (cfg->device < 0 || cfg->device >= ndev) return hoc_fail(nullptr, HOC_EDEVICE, "device ordinal out of range");
@@ -800,14 +772,9 @@ int hoc_load_mesh(hoc_t h, int32_t lnod, int32_t n_nodes, const double* xy, int3
     m.lnod = lnod; m.n_nodes = n_nodes; m.n_elem = n_elem; m.n_bedges = n_bedges;
     m.xy.assign(xy, xy + 2 * (size_t)n_nodes);
     m.elem.assign(elem_nodes, elem_nodes + (size_t)n_elem * lnod * lnod);
-    for (int32_t v : m.elem) if (v < 0 || v >= n_nodes) throw hoc::StatusError(HOC_EINVAL, "element node id out of range");
     m.edges.resize(n_bedges);
-    for (int e = 0; e < n_bedges; ++e) {
-      m.edges[e].assign(bedge_nodes + (size_t)e * lnod, bedge_nodes + (size_t)(e + 1) * lnod);
-      for (int32_t v : m.edges[e]) if (v < 0 || v >= n_nodes) throw hoc::StatusError(HOC_EINVAL, "edge node id out of range");
-    }
+    for (int e = 0; e < n_bedges; ++e) m.edges[e].assign(bedge_nodes + (size_t)e * lnod, bedge_nodes + (size_t)(e + 1) * lnod);
     m.bedge_bdry.assign(bedge_bdry, bedge_bdry + n_bedges);
-    for (int32_t b : m.bedge_bdry) if (b < 0 || b >= n_bdry) throw hoc::StatusError(HOC_EINVAL, "boundary id out of range");
     std::string names(bdry_names ? bdry_names : "");
     size_t p0 = 0;
     for (int b = 0; b < n_bdry; ++b) {
@@ -815,9 +782,20 @@ int hoc_load_mesh(hoc_t h, int32_t lnod, int32_t n_nodes, const double* xy, int3
       m.bnames.push_back(names.substr(p0, p1 == std::string::npos ? std::string::npos : p1 - p0));
       p0 = p1 == std::string::npos ? names.size() : p1 + 1;
     }
-    s.mesh = std::move(m);
-    s.mesh_loaded = true;
-    s.ready = false;
+    s.accept_mesh(std::move(m));
+  });
+}
+
+int hoc_load_gmsh(hoc_t h, const char* path, int32_t occ) {
+  return guarded(h, [&] {
+    if (!path) throw hoc::StatusError(HOC_EINVAL, "null path");
+    hoc::Mesh m;
+    try {
+      hoc::read_gmsh41(path, occ, m);
+    } catch (const std::runtime_error& e) {
+      throw hoc::StatusError(HOC_EMESH, e.what());
+    }
+    h->s.accept_mesh(std::move(m));
   });
 }
 
@@ -881,6 +859,27 @@ int hoc_get_solution_areas(hoc_t h, int64_t len, double* a) {
         }
   });
 }
+int hoc_write_vtk(hoc_t h, const char* path) {
+  return guarded(h, [&] {
+    NEED_READY(h);
+    if (!path) throw hoc::StatusError(HOC_EINVAL, "null path");
+    Solver& s = h->s;
+    std::vector<double> pts, w(s.D), p(s.D), uv(2 * s.D), u(s.D), v(s.D);
+    hoc::solution_points(s.mesh, s.tab, pts);
+    s.d2h(w.data(), s.w.p, s.D * sizeof(double));
+    s.d2h(p.data(), s.psi.p, s.D * sizeof(double));
+    s.d2h(uv.data(), s.vel.p, 2 * s.D * sizeof(double));
+    for (size_t i = 0; i < s.D; ++i) { u[i] = uv[2 * i]; v[i] = uv[2 * i + 1]; }
+    const char* names[4] = {"vorticity", "streamfunction", "u", "v"};
+    const double* fields[4] = {w.data(), p.data(), u.data(), v.data()};
+    try {
+      hoc::write_vtk_sps(path, s.K, s.N, pts.data(), 4, names, fields);
+    } catch (const std::runtime_error& e) {
+      throw hoc::StatusError(HOC_EINVAL, e.what());
+    }
+  });
+}
+
 double hoc_get_time(hoc_t h) { return h ? h->s.time : 0.; }
 int hoc_set_time(hoc_t h, double t) { return guarded(h, [&] { h->s.time = t; }); }
 

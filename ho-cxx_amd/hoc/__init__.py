@@ -26,11 +26,14 @@ HOC_FP64, HOC_MIXED = 0, 1
 KRYLOV_AUTO, KRYLOV_BICGSTAB, KRYLOV_PCG = 0, 1, 2
 
 EXPORTED = [
-    "hoc_default_config", "hoc_create", "hoc_load_mesh", "hoc_setup", "hoc_set_vorticity", "hoc_get_vorticity",
+    "hoc_default_config", "hoc_create", "hoc_load_mesh", "hoc_load_gmsh", "hoc_write_vtk", "hoc_setup", "hoc_set_vorticity", "hoc_get_vorticity",
     "hoc_get_streamfunction", "hoc_set_streamfunction", "hoc_get_velocity", "hoc_get_solution_points",
     "hoc_get_solution_areas", "hoc_get_time", "hoc_set_time", "hoc_step", "hoc_solveto", "hoc_poisson",
     "hoc_stage_residual", "hoc_apply_laplacian", "hoc_stats_get", "hoc_profile_enable", "hoc_profile_get",
     "hoc_last_error", "hoc_destroy",
+    "hoc_linsolver_create", "hoc_linsolver_set_tolerances", "hoc_linsolver_build", "hoc_linsolver_solve",
+    "hoc_linsolver_num_iterations", "hoc_linsolver_residual", "hoc_linsolver_levels", "hoc_linsolver_last_error",
+    "hoc_linsolver_destroy",
 ]
 PROF_VARIANTS = 10
 
@@ -39,7 +42,8 @@ class hoc_config(C.Structure):
     _fields_ = [("K", C.c_int32), ("problem_type", C.c_int32), ("Re", C.c_double), ("dt", C.c_double),
                 ("time_integration_type", C.c_int32), ("advection_type", C.c_int32), ("tol", C.c_double),
                 ("maxiter", C.c_int32), ("precision", C.c_int32), ("krylov", C.c_int32), ("device", C.c_int32),
-                ("fail_on_noconv", C.c_int32), ("mixed_inner_tol", C.c_double)]
+                ("fail_on_noconv", C.c_int32), ("mixed_inner_tol", C.c_double), ("cheb_degree", C.c_int32),
+                ("reserved0", C.c_int32), ("cheb_lower", C.c_double)]
 
 
 class hoc_stats(C.Structure):
@@ -78,6 +82,8 @@ def lib():
         L.hoc_load_mesh.argtypes = [C.c_void_p, C.c_int32, C.c_int32, _dp, C.c_int32, _ip, C.c_int32, _ip, _ip,
                                     C.c_int32, C.c_char_p]
         L.hoc_setup.argtypes = [C.c_void_p]
+        L.hoc_load_gmsh.argtypes = [C.c_void_p, C.c_char_p, C.c_int32]
+        L.hoc_write_vtk.argtypes = [C.c_void_p, C.c_char_p]
         for nm in ("hoc_set_vorticity", "hoc_get_vorticity", "hoc_get_streamfunction", "hoc_set_streamfunction",
                    "hoc_get_velocity", "hoc_get_solution_points", "hoc_get_solution_areas"):
             getattr(L, nm).argtypes = [C.c_void_p, C.c_int64, _dp]
@@ -96,6 +102,16 @@ def lib():
         L.hoc_last_error.restype = C.c_char_p
         L.hoc_destroy.argtypes = [C.c_void_p]
         L.hoc_destroy.restype = None
+        L.hoc_linsolver_create.argtypes = [C.POINTER(C.c_void_p), C.c_int32]
+        L.hoc_linsolver_set_tolerances.argtypes = [C.c_void_p, C.c_int32, C.c_double, C.c_double]
+        L.hoc_linsolver_build.argtypes = [C.c_void_p, C.c_int32, _ip, _ip, _dp]
+        L.hoc_linsolver_solve.argtypes = [C.c_void_p, _dp, _dp]
+        for nm, rt in (("hoc_linsolver_num_iterations", C.c_int32), ("hoc_linsolver_residual", C.c_double),
+                       ("hoc_linsolver_levels", C.c_int32), ("hoc_linsolver_last_error", C.c_char_p)):
+            getattr(L, nm).argtypes = [C.c_void_p]
+            getattr(L, nm).restype = rt
+        L.hoc_linsolver_destroy.argtypes = [C.c_void_p]
+        L.hoc_linsolver_destroy.restype = None
         _LIB = L
     return _LIB
 
@@ -115,11 +131,12 @@ class HO2D:
 
     def __init__(self, K=3, problem_type=1, Re=100.0, dt=1e-3, time_integration_type=4, advection_type=1,
                  tol=1e-15, maxiter=150, precision=HOC_FP64, krylov=KRYLOV_AUTO, device=0, fail_on_noconv=False,
-                 mixed_inner_tol=0.0):
+                 mixed_inner_tol=0.0, cheb_degree=0, cheb_lower=0.0):
         self.cfg = default_config(K=K, problem_type=problem_type, Re=Re, dt=dt,
                                   time_integration_type=time_integration_type, advection_type=advection_type,
                                   tol=tol, maxiter=maxiter, precision=precision, krylov=krylov, device=device,
-                                  fail_on_noconv=int(fail_on_noconv), mixed_inner_tol=mixed_inner_tol)
+                                  fail_on_noconv=int(fail_on_noconv), mixed_inner_tol=mixed_inner_tol,
+                                  cheb_degree=cheb_degree, cheb_lower=cheb_lower)
         self.h = C.c_void_p()
         rc = lib().hoc_create(C.byref(self.h), C.byref(self.cfg))
         if rc != HOC_OK:
@@ -150,8 +167,16 @@ class HO2D:
                                       "\n".join(m.bnames).encode()))
         self.D = m.n_elem * self.K * self.K
 
+    def load_gmsh(self, path, occ=0):
+        """Read a Gmsh 4.1 ASCII mesh in the library (= Mesh::read_msh_file)."""
+        self._chk(lib().hoc_load_gmsh(self.h, os.fsencode(path), int(occ)))
+
+    def write_vtk(self, path):
+        self._chk(lib().hoc_write_vtk(self.h, os.fsencode(path)))
+
     def setup(self):
         self._chk(lib().hoc_setup(self.h))
+        self.D = int(self.stats()["n_dofs"])
 
     def _get(self, fn, n):
         out = np.zeros(n)
@@ -223,3 +248,60 @@ class HO2D:
             out[p.names[v].value.decode()] = {"launches": p.launches[v], "total_ms": p.total_ms[v],
                                               "bytes_per_launch": p.bytes_per_launch[v]}
         return out
+
+
+class LinearSolver:
+    """GPU plug-in with the shape of the reference's BaseLinearSolver
+    (old/cxx/linsolvers/linsolver.hpp:42-109): build(num_rows, rowptr, colidx, values),
+    then solve(f, u) with u the initial guess; SA-AMG (spai0) preconditioned BiCGSTAB as the
+    reference's AMGCL configuration (src/HO_2D.cpp:3011-3086)."""
+
+    def __init__(self, device=0, maxiters=200, reltol=0.0, abstol=1e-10):
+        self.h = C.c_void_p()
+        rc = lib().hoc_linsolver_create(C.byref(self.h), device)
+        if rc != HOC_OK:
+            raise HocError(rc, lib().hoc_linsolver_last_error(None).decode())
+        self.set_tolerances(maxiters, reltol, abstol)
+        self.n = 0
+
+    def _chk(self, rc):
+        if rc != HOC_OK:
+            raise HocError(rc, lib().hoc_linsolver_last_error(self.h).decode())
+
+    def set_tolerances(self, maxiters, reltol, abstol):
+        self._chk(lib().hoc_linsolver_set_tolerances(self.h, maxiters, reltol, abstol))
+
+    def build(self, num_rows, rowptr, colidx, values):
+        self._chk(lib().hoc_linsolver_build(self.h, num_rows, np.ascontiguousarray(rowptr, dtype=np.int32),
+                                            np.ascontiguousarray(colidx, dtype=np.int32),
+                                            np.ascontiguousarray(values, dtype=np.float64)))
+        self.n = num_rows
+
+    def solve(self, f, u=None):
+        """Returns the solution; raises HocError(HOC_ENOCONV) when the tolerance was not met."""
+        out = np.zeros(self.n) if u is None else np.array(u, dtype=np.float64, copy=True)
+        self._chk(lib().hoc_linsolver_solve(self.h, np.ascontiguousarray(f, dtype=np.float64), out))
+        return out
+
+    @property
+    def num_iterations(self):
+        return lib().hoc_linsolver_num_iterations(self.h)
+
+    @property
+    def residual(self):
+        return lib().hoc_linsolver_residual(self.h)
+
+    @property
+    def levels(self):
+        return lib().hoc_linsolver_levels(self.h)
+
+    def close(self):
+        if getattr(self, "h", None) and self.h.value:
+            lib().hoc_linsolver_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -54,6 +54,11 @@ typedef struct {
   int32_t fail_on_noconv;         /* nonzero: return HOC_ENOCONV when tol is not met     */
   double mixed_inner_tol;         /* HOC_MIXED: floor of the FP32 inner solve's relative
                                      tolerance per refinement pass (0 = 1e-5)            */
+  int32_t cheb_degree;            /* fine-level Chebyshev smoother degree, pre and post
+                                     (0 = 2; at most 8)                                   */
+  int32_t reserved0;
+  double cheb_lower;              /* smoothing interval [cheb_lower, 1.1] x lambda_max(D^-1 As)
+                                     (0 = 1/30)                                          */
 } hoc_config;
 
 typedef struct {
@@ -90,6 +95,12 @@ int hoc_load_mesh(hoc_t h, int32_t lnod, int32_t n_nodes, const double* xy,
                   int32_t n_bedges, const int32_t* bedge_nodes,
                   const int32_t* bedge_bdry, int32_t n_bdry, const char* bdry_names);
 
+/* = Mesh::read_msh_file (src/Mesh.cpp:93-293): a Gmsh 4.1 ASCII file; occ != 0 for files
+ * written by the OCC kernel (element nodes reflected into CCW order, :281-293; the
+ * OCC_format field of input.dat, src/HO_2D.cpp:244-246).  Boundaries are the physical
+ * curves, named by $PhysicalNames.  Returns HOC_EMESH for unreadable or unsupported files. */
+int hoc_load_gmsh(hoc_t h, const char* path, int32_t occ);
+
 /* = process_mesh_input (src/HO_2D.cpp:3901-3918): connectivity, bases, metrics,
  * IC/BC (setup_IC_BC_SRC), Poisson operator and hierarchy, device upload. */
 int hoc_setup(hoc_t h);
@@ -101,6 +112,10 @@ int hoc_set_streamfunction(hoc_t h, int64_t len, const double* psi);
 int hoc_get_velocity(hoc_t h, int64_t len, double* uv);           /* len = 2*dofs          */
 int hoc_get_solution_points(hoc_t h, int64_t len, double* xy);   /* = getsolnpts_d        */
 int hoc_get_solution_areas(hoc_t h, int64_t len, double* a);     /* = getsolnareas_d      */
+/* Legacy-VTK file of the solution points (one quad cell per sub-square of each element's
+ * K x K point lattice) with vorticity, streamfunction, u and v (the fields that
+ * save_vorticity_vtk / save_smooth_vtk write, src/HO_2D.cpp:3369-3692). */
+int hoc_write_vtk(hoc_t h, const char* path);
 double hoc_get_time(hoc_t h);
 int hoc_set_time(hoc_t h, double t);
 
@@ -137,6 +152,29 @@ int hoc_profile_get(hoc_t h, hoc_kernel_profile* p);
 
 const char* hoc_last_error(hoc_t h);   /* h may be NULL for hoc_create failures */
 void hoc_destroy(hoc_t h);
+
+/* ---- CSR linear-solver plug-in ------------------------------------------------------
+ * Replaces the reference's BaseLinearSolver interface (old/cxx/linsolvers/linsolver.hpp:42-109:
+ * setMaxIterations / setRelativeTolerance / setAbsoluteTolerance, build(num_rows, rowptr,
+ * colidx, values), solve(f, u), getNumIterations, getResidual) for the AMGCL solver the
+ * reference's Poisson step uses (src/HO_2D.cpp:3011-3086): smoothed-aggregation AMG with
+ * spai0 relaxation as the right preconditioner of BiCGSTAB, hierarchy built on the host,
+ * iterations on the GPU.  Stops when ||f - A u|| <= max(reltol ||f||, abstol); defaults
+ * maxiters 200, reltol 0, abstol 1e-10 (the interface's own defaults). */
+typedef struct hoc_linsolver* hoc_ls_t;
+int hoc_linsolver_create(hoc_ls_t* ls, int32_t device);
+int hoc_linsolver_set_tolerances(hoc_ls_t ls, int32_t maxiters, double reltol, double abstol);
+/* CSR with 0-based indices; the matrix is copied. */
+int hoc_linsolver_build(hoc_ls_t ls, int32_t num_rows, const int32_t* rowptr, const int32_t* colidx,
+                        const double* values);
+/* u: initial guess in, solution out (also on HOC_ENOCONV, which reports a solve that ended
+ * at maxiters or on breakdown above the tolerance; the reference reports it as Failure). */
+int hoc_linsolver_solve(hoc_ls_t ls, const double* f, double* u);
+int32_t hoc_linsolver_num_iterations(hoc_ls_t ls);
+double hoc_linsolver_residual(hoc_ls_t ls);      /* ||f - A u|| / ||f|| of the last solve */
+int32_t hoc_linsolver_levels(hoc_ls_t ls);       /* AMG levels, the system matrix included */
+const char* hoc_linsolver_last_error(hoc_ls_t ls);
+void hoc_linsolver_destroy(hoc_ls_t ls);
 
 #ifdef __cplusplus
 }

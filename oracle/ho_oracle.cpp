@@ -37,6 +37,8 @@
 #include <unordered_map>
 #include <vector>
 
+#include <omp.h>
+
 namespace {
 
 enum { SOUTH = 0, EAST = 1, NORTH = 2, WEST = 3 };
@@ -168,6 +170,7 @@ struct CSR {
 };
 
 void spmv(const CSR& A, const double* x, double* y) {
+#pragma omp parallel for schedule(static)
   for (int i = 0; i < A.n; ++i) {
     double s = 0.;
     for (int j = A.ptr[i]; j < A.ptr[i + 1]; ++j) s += A.val[j] * x[A.col[j]];
@@ -476,12 +479,14 @@ struct AMG {
     std::fill(x.begin(), x.end(), 0.);
     relax(L, rhs, x);
     spmv(L.A, x.data(), L.t.data());
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < n; ++i) L.t[i] = rhs[i] - L.t[i];
     AMGLevel& C = levels[l + 1];
     spmv(L.R, L.t.data(), C.f.data());
     std::fill(C.u.begin(), C.u.end(), 0.);
     cycle(l + 1, C.f, C.u);
     spmv(L.P, C.u.data(), L.t.data());
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < n; ++i) x[i] += L.t[i];
     relax(L, rhs, x);  // post-relaxation
   }
@@ -491,18 +496,22 @@ struct AMG {
     const int n = L.A.n;
     std::vector<double> r(n), d(n);
     spmv(L.A, x.data(), r.data());
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < n; ++i) r[i] = rhs[i] - r[i];
     if (relax_mode == 0) {
+#pragma omp parallel for schedule(static)
       for (int i = 0; i < n; ++i) x[i] += L.M[i] * r[i];
       return;
     }
     const double lmax = L.lmax, lmin = lmax / 30.;
     const double theta = 0.5 * (lmax + lmin), delta = 0.5 * (lmax - lmin), sigma = theta / delta;
     double rho = 1. / sigma;
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < n; ++i) { d[i] = L.dinv[i] * r[i] / theta; x[i] += d[i]; }
     for (int k = 1; k < cheb_degree; ++k) {
       spmv(L.A, x.data(), r.data());
       const double rho_new = 1. / (2. * sigma - rho);
+#pragma omp parallel for schedule(static)
       for (int i = 0; i < n; ++i) {
         d[i] = rho_new * rho * d[i] + 2. * rho_new / delta * L.dinv[i] * (rhs[i] - r[i]);
         x[i] += d[i];
@@ -515,7 +524,9 @@ struct AMG {
 
 double dot(const std::vector<double>& a, const std::vector<double>& b) {
   double s = 0.;
-  for (size_t i = 0; i < a.size(); ++i) s += a[i] * b[i];
+  const long n = (long)a.size();
+#pragma omp parallel for schedule(static) reduction(+ : s)
+  for (long i = 0; i < n; ++i) s += a[i] * b[i];
   return s;
 }
 
@@ -526,7 +537,9 @@ double dot(const std::vector<double>& a, const std::vector<double>& b) {
 double wnorm(const std::vector<double>& r, const std::vector<double>* wts) {
   if (!wts) return std::sqrt(dot(r, r));
   double s = 0.;
-  for (size_t i = 0; i < r.size(); ++i) { double u = r[i] / (*wts)[i]; s += u * u; }
+  const long n = (long)r.size();
+#pragma omp parallel for schedule(static) reduction(+ : s)
+  for (long i = 0; i < n; ++i) { double u = r[i] / (*wts)[i]; s += u * u; }
   return std::sqrt(s);
 }
 
@@ -538,6 +551,7 @@ int bicgstab(const CSR& A, AMG& M, const std::vector<double>& b, std::vector<dou
   double eps = tol * norm_b;
   std::vector<double> r(n), rh(n), p(n), v(n), ph(n), sh(n), t(n);
   spmv(A, x.data(), r.data());
+#pragma omp parallel for schedule(static)
   for (int i = 0; i < n; ++i) r[i] = b[i] - r[i];
   rh = r;
   double rho1 = 0., rho2 = 0., alpha = 0., omega = 0.;
@@ -548,6 +562,7 @@ int bicgstab(const CSR& A, AMG& M, const std::vector<double>& b, std::vector<dou
     if (rho1 == 0.) break;
     if (iter) {
       double beta = (rho1 * alpha) / (rho2 * omega);
+#pragma omp parallel for schedule(static)
       for (int i = 0; i < n; ++i) p[i] = r[i] + beta * (p[i] - omega * v[i]);
     } else {
       p = r;
@@ -557,6 +572,7 @@ int bicgstab(const CSR& A, AMG& M, const std::vector<double>& b, std::vector<dou
     double rv = dot(rh, v);
     if (rv == 0.) break;
     alpha = rho1 / rv;
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < n; ++i) r[i] -= alpha * v[i];
     res = wnorm(r, wts);
     if (res > eps) {
@@ -564,11 +580,15 @@ int bicgstab(const CSR& A, AMG& M, const std::vector<double>& b, std::vector<dou
       spmv(A, sh.data(), t.data());
       double tt = dot(t, t);
       omega = tt != 0. ? dot(t, r) / tt : 0.;
-      for (int i = 0; i < n; ++i) x[i] += alpha * ph[i] + omega * sh[i];
-      for (int i = 0; i < n; ++i) r[i] -= omega * t[i];
+#pragma omp parallel for schedule(static)
+      for (int i = 0; i < n; ++i) {
+        x[i] += alpha * ph[i] + omega * sh[i];
+        r[i] -= omega * t[i];
+      }
       res = wnorm(r, wts);
       if (omega == 0.) { ++iter; break; }
     } else {
+#pragma omp parallel for schedule(static)
       for (int i = 0; i < n; ++i) x[i] += alpha * ph[i];
     }
     rho2 = rho1;
@@ -1927,6 +1947,12 @@ ho_oracle* ho_oracle_create(int K, int Lnod, int n_nodes, const double* xy, int 
 
 void ho_oracle_destroy(ho_oracle* h) { delete h; }
 const char* ho_oracle_last_error(ho_oracle* h) { return h ? h->err.c_str() : g_last_error.c_str(); }
+
+int ho_oracle_set_threads(int n) {
+  const int prev = omp_get_max_threads();
+  if (n > 0) omp_set_num_threads(n);
+  return prev;
+}
 int ho_oracle_dofs(ho_oracle* h) { return h->N_el * h->K2(); }
 int ho_oracle_n_bedges(ho_oracle* h) { return h->N_eb; }
 

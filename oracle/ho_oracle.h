@@ -41,6 +41,10 @@ ho_oracle* ho_oracle_create(int K, int Lnod, int n_nodes, const double* xy,
                             int advection_type, double tol, int maxiter);
 void ho_oracle_destroy(ho_oracle*);
 const char* ho_oracle_last_error(ho_oracle*);
+/* Threads of the OpenMP-parallel linear algebra (SpMV, vector updates, dot products of
+ * the AMG-preconditioned Krylov solve); returns the previous count.  Thread count only
+ * changes the summation order of dot products. */
+int ho_oracle_set_threads(int n);
 
 int ho_oracle_dofs(ho_oracle*);           /* N_el*K*K            */
 int ho_oracle_n_bedges(ho_oracle*);

@@ -52,8 +52,16 @@ def lib():
         L.ho_oracle_apply_laplacian_bc.argtypes = [C.c_void_p, _dp, _dp, _dp]
         L.ho_oracle_solve_poisson_bc.argtypes = [C.c_void_p, _dp, _dp]
         L.ho_oracle_velocity_from_psi.argtypes = [C.c_void_p, _dp]
+        L.ho_oracle_set_threads.argtypes = [C.c_int]
+        L.ho_oracle_set_threads.restype = C.c_int
+        L.ho_oracle_set_threads(1)   # deterministic single-thread checker unless asked otherwise
         _LIB = L
     return _LIB
+
+
+def set_threads(n):
+    """Threads of the oracle's OpenMP linear algebra; returns the previous count."""
+    return lib().ho_oracle_set_threads(int(n))
 
 
 class Oracle:
