@@ -505,7 +505,7 @@ __global__ __launch_bounds__(kWG) void k_velocity(int N, Tab T, Geo G, BCDev B, 
                                                    const double* __restrict__ trpsi, const double* __restrict__ omega,
                                                    double* __restrict__ vel, double* __restrict__ U,
                                                    double* __restrict__ trw, double* __restrict__ tru,
-                                                   double inv_dt_re) {
+                                                   double inv_dt_re, InstDev I) {
   constexpr int K2 = K * K;
   __shared__ __align__(16) double lds[kWG * K2 * 2];
   const int e0 = blockIdx.x * kWG;
@@ -547,7 +547,7 @@ __global__ __launch_bounds__(kWG) void k_velocity(int N, Tab T, Geo G, BCDev B, 
           const size_t i = (size_t)e * K + m;
           const double vj = B.vjump[i] + sg * B.pvel[i];
           B.vjump[i] = vj;
-          B.diff[i] = vj * inv_dt_re;
+          B.diff[i] = vj * (I.inv_dt_re ? I.inv_dt_re[G.inst[el]] : inv_dt_re);
         }
       }
     }
@@ -639,7 +639,7 @@ template <int K, int ADV>
 __global__ __launch_bounds__(kWG) void k_stage(int N, Tab T, Geo G, BCDev B, const double* __restrict__ omega,
                                                 const double* __restrict__ U, const double* __restrict__ trw,
                                                 const double* __restrict__ tru, double re_inv, double dt,
-                                                StageOut so) {
+                                                InstDev I, StageOut so) {
   constexpr int K2 = K * K;
   __shared__ __align__(16) double lds[kWG * K2 * 2];
   const int e0 = blockIdx.x * kWG;
@@ -785,8 +785,10 @@ __global__ __launch_bounds__(kWG) void k_stage(int N, Tab T, Geo G, BCDev B, con
         r -= bdf[2][i] * T.grad[j][0] + bdf[3][i] * T.grad[j][1];
         const double J = G.jac[base + v];
         const double adv = r / J;
-        const double diff = dif[v] * (re_inv / J);
-        kk[v] = dt * (adv + diff);
+        const double ri = I.re_inv ? I.re_inv[G.inst[el]] : re_inv;
+        const double dti = I.dt ? I.dt[G.inst[el]] : dt;
+        const double diff = dif[v] * (ri / J);
+        kk[v] = dti * (adv + diff);
       }
   } else {
 #pragma unroll
@@ -843,9 +845,10 @@ __global__ __launch_bounds__(kWG) void k_stage(int N, Tab T, Geo G, BCDev B, con
 // ==========================================================================================
 // Per-stage boundary values at time t (update_BCs, src/HO_2D.cpp:1979-2217).
 // ==========================================================================================
-__global__ void k_update_bcs(int n, BCDev B, double tc) {
+__global__ void k_update_bcs(int n, int K, BCDev B, double tc0, InstDev I) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  const double tc = I.ramp ? I.ramp[I.edge_inst[i / K]] : tc0;
   B.psi[i] = B.psi_r[i] != 0. ? B.psi0[i] * tc : B.psi0[i];
   B.nvel[i] = B.nvel_r[i] != 0. ? B.nvel0[i] * tc : B.nvel0[i];
   B.pvel[i] = B.pvel_r[i] != 0. ? B.pvel0[i] * tc : B.pvel0[i];
@@ -942,27 +945,29 @@ hipError_t launch_rhs(int K, int N, const Tab& T, const Geo& G, const double* om
 
 hipError_t launch_velocity(int K, int N, const Tab& T, const Geo& G, const BCDev& B, const double* psi,
                            const double* trpsi, const double* omega, double* vel, double* U, double* trw,
-                           double* tru, double inv_dt_re, hipStream_t s) {
+                           double* tru, double inv_dt_re, const InstDev& I, hipStream_t s) {
   HOC_K_SWITCH(K, hipLaunchKernelGGL((k_velocity<KK>), dim3(nblk(N)), dim3(kWG), 0, s, N, T, G, B, psi, trpsi, omega,
-                                     vel, U, trw, tru, inv_dt_re));
+                                     vel, U, trw, tru, inv_dt_re, I));
   return hipGetLastError();
 }
 
 hipError_t launch_stage(int K, int adv, int N, const Tab& T, const Geo& G, const BCDev& B, const double* omega,
                         const double* U, const double* trw, const double* tru, double re_inv, double dt,
-                        const StageOut& so, hipStream_t s) {
+                        const InstDev& I, const StageOut& so, hipStream_t s) {
   HOC_K_SWITCH(K, {
     if (adv == 1)
-      hipLaunchKernelGGL((k_stage<KK, 1>), dim3(nblk(N)), dim3(kWG), 0, s, N, T, G, B, omega, U, trw, tru, re_inv, dt, so);
+      hipLaunchKernelGGL((k_stage<KK, 1>), dim3(nblk(N)), dim3(kWG), 0, s, N, T, G, B, omega, U, trw, tru, re_inv, dt, I,
+                         so);
     else
-      hipLaunchKernelGGL((k_stage<KK, 2>), dim3(nblk(N)), dim3(kWG), 0, s, N, T, G, B, omega, U, trw, tru, re_inv, dt, so);
+      hipLaunchKernelGGL((k_stage<KK, 2>), dim3(nblk(N)), dim3(kWG), 0, s, N, T, G, B, omega, U, trw, tru, re_inv, dt, I,
+                         so);
   });
   return hipGetLastError();
 }
 
-hipError_t launch_update_bcs(int n, const BCDev& B, double tc, hipStream_t s) {
+hipError_t launch_update_bcs(int n, int K, const BCDev& B, double tc, const InstDev& I, hipStream_t s) {
   if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_update_bcs, dim3((n + 255) / 256), dim3(256), 0, s, n, B, tc);
+  hipLaunchKernelGGL(k_update_bcs, dim3((n + 255) / 256), dim3(256), 0, s, n, K, B, tc, I);
   return hipGetLastError();
 }
 

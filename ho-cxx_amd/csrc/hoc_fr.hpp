@@ -37,11 +37,12 @@ hipError_t launch_rhs(int K, int N, const Tab& T, const Geo& G, const double* om
                       double* part, hipStream_t s);
 hipError_t launch_velocity(int K, int N, const Tab& T, const Geo& G, const BCDev& B, const double* psi,
                            const double* trpsi, const double* omega, double* vel, double* U, double* trw,
-                           double* tru, double inv_dt_re, hipStream_t s);
+                           double* tru, double inv_dt_re, const InstDev& I, hipStream_t s);
 hipError_t launch_stage(int K, int adv, int N, const Tab& T, const Geo& G, const BCDev& B, const double* omega,
                         const double* U, const double* trw, const double* tru, double re_inv, double dt,
-                        const StageOut& so, hipStream_t s);
-hipError_t launch_update_bcs(int n, const BCDev& B, double tc, hipStream_t s);
+                        const InstDev& I, const StageOut& so, hipStream_t s);
+// n = E*K boundary values; tc = time factor (per instance through I.ramp when set)
+hipError_t launch_update_bcs(int n, int K, const BCDev& B, double tc, const InstDev& I, hipStream_t s);
 int lap_blocks(int N);
 
 }  // namespace hoc

@@ -277,7 +277,7 @@ void solution_points(const Mesh& m, const Tables& t, std::vector<double>& xy) {
       }
 }
 
-void build_bcs(const Mesh& m, const Tables& t, int problem_type, BCs& bc) {
+void build_bcs(const Mesh& m, const Tables& t, int problem_type, BCs& bc, const std::vector<int32_t>& edge_inst) {
   const int K = t.K, E = m.n_bedges, L = t.L;
   bc.sw_psi.assign(E, 0); bc.sw_diff.assign(E, 0); bc.no_slip_edge.assign(E, 0);
   for (auto* v : {&bc.psi, &bc.psi_ramp, &bc.diff, &bc.diff_ramp, &bc.nvel, &bc.nvel_ramp, &bc.pvel, &bc.pvel_ramp})
@@ -292,22 +292,26 @@ void build_bcs(const Mesh& m, const Tables& t, int problem_type, BCs& bc) {
     for (int j = 0; j < L; ++j) y += t.gsb[j][k] * m.xy[2 * (size_t)m.edges[e][j] + 1];
     return y;
   };
-  // inlet extent (src/HO_2D.cpp:730-741)
-  double y_min = 0., y_max = 0.;
+  // inlet extent (src/HO_2D.cpp:730-741), per ensemble instance (edge_inst empty: one)
+  int ni = 1;
+  for (int32_t v : edge_inst) ni = std::max(ni, v + 1);
+  auto inst = [&](int e) { return edge_inst.empty() ? 0 : edge_inst[e]; };
+  std::vector<double> ymin(ni, 0.), ymax(ni, 0.);
   for (int b = 0; b < NB; ++b) {
     if (m.bnames[b] != "inlet" || bedges[b].empty()) continue;
-    const auto& e0 = m.edges[bedges[b][0]];
-    y_min = y_max = m.xy[2 * (size_t)e0[0] + 1];
-    for (int e : bedges[b])
+    std::vector<char> started(ni, 0);
+    for (int e : bedges[b]) {
+      const int i = inst(e);
+      if (!started[i]) { ymin[i] = ymax[i] = m.xy[2 * (size_t)m.edges[e][0] + 1]; started[i] = 1; }
       for (int end : {0, L - 1}) {
         double y = m.xy[2 * (size_t)m.edges[e][end] + 1];
-        y_min = std::min(y_min, y); y_max = std::max(y_max, y);
+        ymin[i] = std::min(ymin[i], y); ymax[i] = std::max(ymax[i], y);
       }
+    }
   }
   auto set = [&](std::vector<double>& v, std::vector<double>& r, int e, int k, double val, bool ramp) {
     v[(size_t)e * K + k] = val; r[(size_t)e * K + k] = ramp ? 1. : 0.;
   };
-  const double h2 = (y_max - y_min) * (y_max - y_min);
   if (problem_type == 1) {  // src/HO_2D.cpp:700-726
     for (int e = 0; e < E; ++e) { bc.sw_psi[e] = 0; bc.sw_diff[e] = 1; bc.no_slip_edge[e] = 1; }
     for (int b = 0; b < NB; ++b)
@@ -318,6 +322,7 @@ void build_bcs(const Mesh& m, const Tables& t, int problem_type, BCs& bc) {
     for (int b = 0; b < NB; ++b) {
       const std::string& nm = m.bnames[b];
       for (int e : bedges[b]) {
+        const double y_min = ymin[inst(e)], y_max = ymax[inst(e)], h2 = (y_max - y_min) * (y_max - y_min);
         if (nm == "inlet") {
           bc.sw_psi[e] = 0; bc.sw_diff[e] = 0;
           for (int k = 0; k < K; ++k) {
@@ -341,6 +346,7 @@ void build_bcs(const Mesh& m, const Tables& t, int problem_type, BCs& bc) {
     for (int b = 0; b < NB; ++b) {
       const std::string& nm = m.bnames[b];
       for (int e : bedges[b]) {
+        const double y_min = ymin[inst(e)], y_max = ymax[inst(e)];
         if (nm == "inlet") {
           bc.sw_psi[e] = 0; bc.sw_diff[e] = 0;
           for (int k = 0; k < K; ++k) {

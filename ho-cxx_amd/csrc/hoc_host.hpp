@@ -94,7 +94,10 @@ struct BCs {
   std::vector<double> edge_sign;               // 2*(ijp%2)-1 of the owner face
   bool time_dependent = false;
 };
-void build_bcs(const Mesh& m, const Tables& t, int problem_type, BCs& bc);
+// edge_inst: ensemble instance of each boundary edge (empty: one instance); inlet extents
+// are taken per instance.
+void build_bcs(const Mesh& m, const Tables& t, int problem_type, BCs& bc,
+               const std::vector<int32_t>& edge_inst = {});
 
 // Scalar CSR and smoothed-aggregation AMG hierarchy (host build, device apply).
 struct CSR {

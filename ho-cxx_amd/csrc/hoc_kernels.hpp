@@ -56,8 +56,18 @@ struct GeoT {
   const int8_t *sw_psi, *sw_diff;          // [E] 0 Dirichlet / 1 Neumann
   const int8_t* no_slip;                   // [E]
   const R* esign;                          // [E] 2*(ijp%2)-1 of the owner face
+  const int32_t* inst;                     // [N] ensemble instance of each element, or null
 };
 using Geo = GeoT<double>;
+
+// Per-instance parameters of an ensemble (null pointers: one instance, scalar arguments).
+struct InstDev {
+  const double* re_inv;      // [n_inst] 1/Re
+  const double* dt;          // [n_inst] time step
+  const double* inv_dt_re;   // [n_inst] 1/(dt Re^-1), wall-vorticity flux scale
+  const double* ramp;        // [n_inst] time factor of the boundary values at the stage time
+  const int32_t* edge_inst;  // [E] instance of each boundary edge
+};
 
 // Time-dependent boundary values [E][K] (edge flux-point order).
 struct BCDev {

@@ -289,6 +289,20 @@ __global__ void k_mixed_promote_add(size_t n, double* __restrict__ x64, const fl
     x64[v] += (double)e32[v];
 }
 
+// ---- ensembles ---------------------------------------------------------------------------------
+__global__ void k_inst_norms(VecArgs A, const int32_t* __restrict__ off, int K2, const double* __restrict__ b,
+                             double* __restrict__ out) {
+  const size_t v0 = (size_t)off[blockIdx.x] * K2, v1 = (size_t)off[blockIdx.x + 1] * K2;
+  double p0 = 0.;
+  for (size_t v = v0 + threadIdx.x; v < v1; v += blockDim.x) {
+    const double u = b[v] / dof_weight(A, v);
+    p0 += u * u;
+  }
+  double p1 = 0.;
+  block_sum2(p0, p1);
+  if (threadIdx.x == 0) out[blockIdx.x] = p0;
+}
+
 // ---- launchers -------------------------------------------------------------------------------
 static int grid_for(size_t n, int bs) {
   size_t g = (n + bs - 1) / bs;
@@ -350,6 +364,9 @@ void mixed_demote(hipStream_t s, const VecArgs& A, size_t n, const double* r64, 
 }
 void mixed_promote_add(hipStream_t s, size_t n, double* x64, const float* e32, const int* F) {
   hipLaunchKernelGGL(k_mixed_promote_add, dim3(vec_grid(n)), dim3(256), 0, s, n, x64, e32, F);
+}
+void inst_norms(hipStream_t s, const VecArgs& A, int n_inst, const int32_t* off, int K2, const double* b, double* out) {
+  hipLaunchKernelGGL(k_inst_norms, dim3(n_inst), dim3(256), 0, s, A, off, K2, b, out);
 }
 void probe_set(hipStream_t s, int N, int K2, const int32_t* color, int c, int rs, double* x) {
   size_t n = (size_t)N * K2;

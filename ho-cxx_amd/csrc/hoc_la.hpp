@@ -49,6 +49,10 @@ void mixed_demote(hipStream_t s, const VecArgs& A, size_t n, const double* r64, 
                   const int* F);
 void mixed_promote_add(hipStream_t s, size_t n, double* x64, const float* e32, const int* F);
 
+// Ensembles: out[i] = ||b_i / W||^2 over instance i's elements [off[i], off[i+1]) (K2 values
+// per element), one workgroup per instance (deterministic order).
+void inst_norms(hipStream_t s, const VecArgs& A, int n_inst, const int32_t* off, int K2, const double* b, double* out);
+
 // Setup-time probing (FP64).
 void probe_set(hipStream_t s, int N, int K2, const int32_t* color, int c, int rs, double* x);
 void probe_diag(hipStream_t s, int N, int K2, const int32_t* color, int c, int rs, const double* y, double* diag);

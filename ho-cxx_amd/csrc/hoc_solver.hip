@@ -130,6 +130,13 @@ struct Solver {
     int64_t cnt[HOC_PROF_VARIANTS] = {0};
   } prof;
   // stats
+  // ensembles: instance i owns elements [inst_off[i], inst_off[i+1]) and has its own Re, dt, clock
+  int n_inst = 1;
+  std::vector<int32_t> inst_off;
+  std::vector<double> inst_re, inst_dt, inst_time;
+  DBuf<int32_t> d_inst, d_einst, d_ioff;
+  DBuf<double> d_ireinv, d_idt, d_iidr, d_iramp, d_inorm;
+  InstDev I{};
   int last_iters = 0, last_outer = 0;
   double last_rel = 0.;
   int64_t total_iters = 0, n_solves = 0;
@@ -151,7 +158,13 @@ struct Solver {
     connect(mesh);
     build_tables(cfg.K, mesh.lnod, tab);
     build_geometry(mesh, tab, geo);
-    build_bcs(mesh, tab, cfg.problem_type, bch);
+    std::vector<int32_t> edge_inst;
+    if (n_inst > 1 && (int)inst_off.size() == n_inst + 1) {
+      edge_inst.resize(mesh.n_bedges);
+      for (int e = 0; e < mesh.n_bedges; ++e)
+        edge_inst[e] = (int32_t)(std::upper_bound(inst_off.begin(), inst_off.end(), mesh.be_elem[e]) - inst_off.begin() - 1);
+    }
+    build_bcs(mesh, tab, cfg.problem_type, bch, edge_inst);
     K = cfg.K; K2 = K * K; N = mesh.n_elem; E = mesh.n_bedges; D = (size_t)N * K2;
     re_inv = cfg.Re > 1.e-3 ? 1. / cfg.Re : -1.;  // src/HO_2D.cpp:255
     // tables
@@ -212,7 +225,8 @@ struct Solver {
     if (affine) { d_gel.upload(gel); d_gface.upload(gface); }
     G = Geo{N, E, d_jac, d_xxi, d_xeta, d_yxi, d_yeta, d_Ga, d_Gb, d_Gc, d_fa, d_fb, d_fA, d_fB, d_fjac, d_fAn,
             affine ? d_gel.p : nullptr, affine ? d_gface.p : nullptr,
-            d_nb, d_nbf, d_swpsi, d_swdiff, d_noslip, d_esign};
+            d_nb, d_nbf, d_swpsi, d_swdiff, d_noslip, d_esign, nullptr};
+    setup_instances();
     const size_t EK = std::max<size_t>((size_t)E * K, 1);
     for (auto* b : {&b_psi, &b_diff, &b_nvel, &b_pvel, &b_vj, &b_vort, &b_adv}) b->alloc(EK);
     auto upl = [&](DBuf<double>& d, const std::vector<double>& h) { if (h.empty()) d.alloc(1); else d.upload(h); };
@@ -238,7 +252,7 @@ struct Solver {
       if (affine) { P.gel.upload_as(gel); P.gface.upload_as(gface); }
       P.G = GeoT<float>{N, E, nullptr, nullptr, nullptr, nullptr, nullptr, P.Ga, P.Gb, P.Gc, P.fa, P.fb, P.fA, P.fB,
                         P.fjac, P.fAn, affine ? P.gel.p : nullptr, affine ? P.gface.p : nullptr,
-                        d_nb, d_nbf, d_swpsi, d_swdiff, d_noslip, nullptr};
+                        d_nb, d_nbf, d_swpsi, d_swdiff, d_noslip, nullptr, nullptr};
       P.prof_off = HOC_PROF_VARIANTS / 2;
       P.alloc_vectors(D, N, K);
     }
@@ -569,6 +583,22 @@ struct Solver {
     HIPCHK(hipMemcpyAsync(S.p + S_TOL2, &tol2, sizeof(double), hipMemcpyHostToDevice, stream));
     HIPCHK(launch_rhs(K, N, T, G, omega, b_psi, bs, part, stream));
     finalize(stream, PH_BB, part, lb, S, F, maxit);
+    if (n_inst > 1) {
+      // ||b - A psi|| <= tol min_i ||b_i|| bounds every instance's own relative residual
+      // (instances with a zero right-hand side are left out of the minimum)
+      inst_norms(stream, VA, n_inst, d_ioff, K2, bs, d_inorm);
+      std::vector<double> bi(n_inst);
+      double bb = 0.;
+      HIPCHK(hipMemcpyAsync(bi.data(), d_inorm.p, n_inst * sizeof(double), hipMemcpyDeviceToHost, stream));
+      HIPCHK(hipMemcpyAsync(&bb, S.p + S_BB, sizeof(double), hipMemcpyDeviceToHost, stream));
+      HIPCHK(hipStreamSynchronize(stream));
+      double mx = 0., mn = HUGE_VAL;
+      for (double v : bi) mx = std::max(mx, v);
+      for (double v : bi) if (v > 1e-30 * mx) mn = std::min(mn, v);
+      const double t2 = (bb > 0. && mn < HUGE_VAL) ? tol2 * mn / bb : tol2;
+      HIPCHK(hipMemcpyAsync(S.p + S_TOL2, &t2, sizeof(double), hipMemcpyHostToDevice, stream));
+      HIPCHK(hipStreamSynchronize(stream));
+    }
     outer_residual();
     zero_if_flag<double>(stream, D, psi, F);
     double hs[S_COUNT];
@@ -637,47 +667,96 @@ struct Solver {
     mesh = std::move(m);
     mesh_loaded = true;
     ready = false;
+    n_inst = 1;
+    inst_off.clear(); inst_re.clear(); inst_dt.clear(); inst_time.clear();
   }
 
   double ramp(double t) const { return bch.time_dependent ? 2. * std::fabs(std::sin(M_PI / 40. * t)) : 1.; }
 
-  // Euler_time_integrate (src/HO_2D.cpp:1806-1839) on input win at time t.
-  void stage(const double* win, double t, const StageOut& so) {
-    HIPCHK(launch_update_bcs(E * K, B, ramp(t), stream));
+  // Boundary values at stage fraction c of the step (each instance at its own clock), or at
+  // the absolute time t_abs for all instances when t_abs is given.
+  void update_bcs(double c, double t_abs = std::nan("")) {
+    if (n_inst > 1 && std::isnan(t_abs)) {
+      std::vector<double> r(n_inst);
+      for (int i = 0; i < n_inst; ++i) r[i] = ramp(inst_time[i] + c * inst_dt[i]);
+      HIPCHK(hipMemcpyAsync(d_iramp.p, r.data(), n_inst * sizeof(double), hipMemcpyHostToDevice, stream));
+      HIPCHK(launch_update_bcs(E * K, K, B, 1., I, stream));
+    } else {
+      InstDev I0 = I;
+      I0.ramp = nullptr;
+      HIPCHK(launch_update_bcs(E * K, K, B, ramp(std::isnan(t_abs) ? time + c * cfg.dt : t_abs), I0, stream));
+    }
+  }
+
+  // Euler_time_integrate (src/HO_2D.cpp:1806-1839) on input win; boundary values already set.
+  void stage_body(const double* win, const StageOut& so) {
     poisson(win);
     HIPCHK(launch_trace(K, PRE_COPY, N, T, psi, nullptr, nullptr, nullptr, 0., nullptr, 0, nullptr, P64.tr.p, stream));
-    HIPCHK(launch_velocity(K, N, T, G, B, psi, P64.tr, win, vel, U, trw, tru, 1. / (cfg.dt * re_inv), stream));
-    HIPCHK(launch_stage(K, cfg.advection_type, N, T, G, B, win, U, trw, tru, re_inv, cfg.dt, so, stream));
+    HIPCHK(launch_velocity(K, N, T, G, B, psi, P64.tr, win, vel, U, trw, tru, 1. / (cfg.dt * re_inv), I, stream));
+    HIPCHK(launch_stage(K, cfg.advection_type, N, T, G, B, win, U, trw, tru, re_inv, cfg.dt, I, so, stream));
+  }
+  void stage(const double* win, double c, const StageOut& so) {
+    update_bcs(c);
+    stage_body(win, so);
   }
 
   // solve_advection_diffusion (src/HO_2D.cpp:1465-1531)
   void rk_step() {
-    const double t0 = time, dt = cfg.dt;
     StageOut so;
     if (cfg.time_integration_type == 1) {
       so.base = w; so.final_mode = 1;
-      stage(w, t0, so);
+      stage(w, 0., so);
     } else if (cfg.time_integration_type == 2) {
       so.k = k1; so.next = vort; so.c = 1.; so.base = w;
-      stage(w, t0, so);
+      stage(w, 0., so);
       StageOut s2; s2.base = w; s2.final_mode = 2; s2.k1 = k1;
-      stage(vort, t0 + dt, s2);
+      stage(vort, 1., s2);
     } else if (cfg.time_integration_type == 4) {
       const double c[4] = {0., 0.5, 0.5, 1.};
       double* ks[3] = {k1, k2, k3};
-      stage_rk4(w, t0 + c[0] * dt, ks[0], 0.5);
-      stage_rk4(vort, t0 + c[1] * dt, ks[1], 0.5);
-      stage_rk4(vort, t0 + c[2] * dt, ks[2], 1.0);
+      stage_rk4(w, c[0], ks[0], 0.5);
+      stage_rk4(vort, c[1], ks[1], 0.5);
+      stage_rk4(vort, c[2], ks[2], 1.0);
       StageOut s4; s4.base = w; s4.final_mode = 4; s4.k1 = k1; s4.k2 = k2; s4.k3 = k3;
-      stage(vort, t0 + c[3] * dt, s4);
+      stage(vort, c[3], s4);
     } else {
       throw StatusError(HOC_EINVAL, "time_integration_type must be 1, 2 or 4");
     }
-    time = t0 + dt;
+    time += cfg.dt;
+    for (int i = 0; i < n_inst && n_inst > 1; ++i) inst_time[i] += inst_dt[i];
   }
-  void stage_rk4(const double* win, double t, double* kout, double cnext) {
+  void stage_rk4(const double* win, double c, double* kout, double cnext) {
     StageOut so; so.k = kout; so.next = vort; so.c = cnext; so.base = w;
-    stage(win, t, so);
+    stage(win, c, so);
+  }
+
+  // Ensemble tables (instance of every element and boundary edge, per-instance Re and dt).
+  void setup_instances() {
+    I = InstDev{};
+    G.inst = nullptr;
+    if (n_inst <= 1) return;
+    if ((int)inst_off.size() != n_inst + 1 || inst_off[0] != 0 || inst_off[n_inst] != N)
+      throw StatusError(HOC_EINVAL, "instance element offsets do not cover the mesh");
+    std::vector<int32_t> ei(N), bi(std::max(E, 1), 0);
+    for (int i = 0; i < n_inst; ++i) {
+      if (inst_off[i + 1] <= inst_off[i]) throw StatusError(HOC_EINVAL, "instance element offsets must increase");
+      for (int e = inst_off[i]; e < inst_off[i + 1]; ++e) ei[e] = i;
+    }
+    // instances must not share faces
+    for (int e = 0; e < N; ++e)
+      for (int f = 0; f < 4; ++f) {
+        const int nb = mesh.nb[(size_t)e * 4 + f];
+        if (nb >= 0 && ei[nb] != ei[e]) throw StatusError(HOC_EINVAL, "ensemble instances share a face");
+      }
+    for (int e = 0; e < E; ++e) bi[e] = ei[mesh.be_elem[e]];
+    std::vector<double> ri(n_inst), idr(n_inst);
+    for (int i = 0; i < n_inst; ++i) { ri[i] = 1. / inst_re[i]; idr[i] = 1. / (inst_dt[i] * ri[i]); }
+    d_inst.upload(ei); d_einst.upload(bi); d_ioff.upload(inst_off);
+    d_ireinv.upload(ri); d_idt.upload(inst_dt); d_iidr.upload(idr);
+    d_iramp.alloc(n_inst); d_inorm.alloc(n_inst);
+    inst_time.assign(n_inst, 0.);
+    G.inst = d_inst;
+    I = InstDev{d_ireinv, d_idt, d_iidr, d_iramp, d_einst};
   }
 };
 
@@ -799,6 +878,25 @@ int hoc_load_gmsh(hoc_t h, const char* path, int32_t occ) {
   });
 }
 
+int hoc_set_instances(hoc_t h, int32_t n_inst, const int32_t* elem_offsets, const double* Re, const double* dt) {
+  return guarded(h, [&] {
+    Solver& s = h->s;
+    if (!s.mesh_loaded) throw hoc::StatusError(HOC_ESTATE, "hoc_set_instances before hoc_load_mesh");
+    if (n_inst < 1 || !elem_offsets || !Re || !dt) throw hoc::StatusError(HOC_EINVAL, "bad instance arrays");
+    if (elem_offsets[0] != 0 || elem_offsets[n_inst] != s.mesh.n_elem)
+      throw hoc::StatusError(HOC_EINVAL, "instance element offsets must run from 0 to n_elem");
+    for (int i = 0; i < n_inst; ++i) {
+      if (elem_offsets[i + 1] <= elem_offsets[i]) throw hoc::StatusError(HOC_EINVAL, "instance element offsets must increase");
+      if (!(Re[i] > 1e-3) || !(dt[i] > 0.)) throw hoc::StatusError(HOC_EINVAL, "instance Re must exceed 1e-3 and dt be positive");
+    }
+    s.n_inst = n_inst;
+    s.inst_off.assign(elem_offsets, elem_offsets + n_inst + 1);
+    s.inst_re.assign(Re, Re + n_inst);
+    s.inst_dt.assign(dt, dt + n_inst);
+    s.ready = false;
+  });
+}
+
 int hoc_setup(hoc_t h) {
   return guarded(h, [&] {
     if (!h->s.mesh_loaded) throw hoc::StatusError(HOC_ESTATE, "hoc_setup before hoc_load_mesh");
@@ -808,6 +906,17 @@ int hoc_setup(hoc_t h) {
 
 #define NEED_READY(h) if (!h->s.ready) throw hoc::StatusError(HOC_ESTATE, "call hoc_setup first")
 #define NEED_LEN(len, want) if ((len) != (int64_t)(want)) throw hoc::StatusError(HOC_EINVAL, "length mismatch")
+
+int hoc_get_instance_times(hoc_t h, int32_t n_inst, double* t) {
+  return guarded(h, [&] {
+    NEED_READY(h);
+    Solver& s = h->s;
+    if (n_inst != s.n_inst || !t) throw hoc::StatusError(HOC_EINVAL, "length mismatch");
+    if (s.n_inst == 1) t[0] = s.time;
+    else std::copy(s.inst_time.begin(), s.inst_time.end(), t);
+  });
+}
+
 
 int hoc_set_vorticity(hoc_t h, int64_t len, const double* w) {
   return guarded(h, [&] {
@@ -899,6 +1008,7 @@ int hoc_solveto(hoc_t h, double outer_dt, int32_t nsub, int32_t integ, double Re
     if (integ != 1 && integ != 2 && integ != 4) throw hoc::StatusError(HOC_EINVAL, "integ must be 1, 2 or 4");
     if (!(Re > 0.)) throw hoc::StatusError(HOC_EINVAL, "Re must be positive");
     Solver& s = h->s;
+    if (s.n_inst > 1) throw hoc::StatusError(HOC_EINVAL, "hoc_solveto sets one dt and Re: not for an ensemble");
     s.cfg.dt = outer_dt / (double)nsub;   // src/HO_2D.cpp:4199
     s.cfg.time_integration_type = integ;  // :4203
     s.re_inv = 1.0 / Re;                  // :4204
@@ -912,7 +1022,7 @@ int hoc_poisson(hoc_t h, const double* omega, double t, double* psi_out) {
     NEED_READY(h);
     Solver& s = h->s;
     s.h2d(s.kscratch.p, omega, s.D * sizeof(double));
-    HIPCHK(hoc::launch_update_bcs(s.E * s.K, s.B, s.ramp(t), s.stream));
+    s.update_bcs(0., t);
     s.poisson(s.kscratch);
     s.d2h(psi_out, s.psi.p, s.D * sizeof(double));
   });
@@ -925,7 +1035,8 @@ int hoc_stage_residual(hoc_t h, const double* omega, double t, double* k) {
     s.h2d(s.vort.p, omega, s.D * sizeof(double));
     hoc::StageOut so;
     so.k = s.kscratch;
-    s.stage(s.vort, t, so);
+    s.update_bcs(0., t);
+    s.stage_body(s.vort, so);
     s.d2h(k, s.kscratch.p, s.D * sizeof(double));
   });
 }

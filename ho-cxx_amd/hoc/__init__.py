@@ -26,7 +26,8 @@ HOC_FP64, HOC_MIXED = 0, 1
 KRYLOV_AUTO, KRYLOV_BICGSTAB, KRYLOV_PCG = 0, 1, 2
 
 EXPORTED = [
-    "hoc_default_config", "hoc_create", "hoc_load_mesh", "hoc_load_gmsh", "hoc_write_vtk", "hoc_setup", "hoc_set_vorticity", "hoc_get_vorticity",
+    "hoc_default_config", "hoc_create", "hoc_load_mesh", "hoc_load_gmsh", "hoc_write_vtk", "hoc_setup",
+    "hoc_set_instances", "hoc_get_instance_times", "hoc_set_vorticity", "hoc_get_vorticity",
     "hoc_get_streamfunction", "hoc_set_streamfunction", "hoc_get_velocity", "hoc_get_solution_points",
     "hoc_get_solution_areas", "hoc_get_time", "hoc_set_time", "hoc_step", "hoc_solveto", "hoc_poisson",
     "hoc_stage_residual", "hoc_apply_laplacian", "hoc_stats_get", "hoc_profile_enable", "hoc_profile_get",
@@ -83,6 +84,8 @@ def lib():
                                     C.c_int32, C.c_char_p]
         L.hoc_setup.argtypes = [C.c_void_p]
         L.hoc_load_gmsh.argtypes = [C.c_void_p, C.c_char_p, C.c_int32]
+        L.hoc_set_instances.argtypes = [C.c_void_p, C.c_int32, _ip, _dp, _dp]
+        L.hoc_get_instance_times.argtypes = [C.c_void_p, C.c_int32, _dp]
         L.hoc_write_vtk.argtypes = [C.c_void_p, C.c_char_p]
         for nm in ("hoc_set_vorticity", "hoc_get_vorticity", "hoc_get_streamfunction", "hoc_set_streamfunction",
                    "hoc_get_velocity", "hoc_get_solution_points", "hoc_get_solution_areas"):
@@ -170,6 +173,19 @@ class HO2D:
     def load_gmsh(self, path, occ=0):
         """Read a Gmsh 4.1 ASCII mesh in the library (= Mesh::read_msh_file)."""
         self._chk(lib().hoc_load_gmsh(self.h, os.fsencode(path), int(occ)))
+
+    def set_instances(self, elem_offsets, Re, dt):
+        """Ensemble of independent meshes (see meshes.concat): per-instance Re and dt."""
+        off = np.ascontiguousarray(elem_offsets, dtype=np.int32)
+        self._chk(lib().hoc_set_instances(self.h, len(off) - 1, off, np.ascontiguousarray(Re, dtype=np.float64),
+                                          np.ascontiguousarray(dt, dtype=np.float64)))
+        self.n_inst = len(off) - 1
+
+    def instance_times(self):
+        n = getattr(self, "n_inst", 1)
+        out = np.zeros(n)
+        self._chk(lib().hoc_get_instance_times(self.h, n, out))
+        return out
 
     def write_vtk(self, path):
         self._chk(lib().hoc_write_vtk(self.h, os.fsencode(path)))

@@ -327,3 +327,30 @@ def read_gmsh41(path: str, occ: int = 0) -> QuadMesh:
                  np.asarray(bid, dtype=np.int32), names)
     m.validate()
     return m
+
+
+def concat(parts):
+    """Union of independent meshes (an ensemble): nodes, elements and boundary edges are
+    renumbered instance by instance; boundaries are merged by name.  Returns the union and
+    the element offsets of the instances (length len(parts) + 1)."""
+    lnod = parts[0].lnod
+    if any(p.lnod != lnod for p in parts):
+        raise ValueError("all instances must have the same geometry order")
+    names = []
+    for p in parts:
+        for nm in p.bnames:
+            if nm not in names:
+                names.append(nm)
+    xy, elem, bed, bid, off = [], [], [], [], [0]
+    n0 = 0
+    for p in parts:
+        xy.append(p.xy)
+        elem.append(p.elem + n0)
+        bed.append(p.bedges + n0)
+        remap = np.array([names.index(nm) for nm in p.bnames], dtype=np.int32)
+        bid.append(remap[p.bedge_bdry])
+        n0 += p.n_nodes
+        off.append(off[-1] + p.n_elem)
+    m = QuadMesh(lnod, np.concatenate(xy), np.concatenate(elem).astype(np.int32),
+                 np.concatenate(bed).astype(np.int32), np.concatenate(bid).astype(np.int32), names)
+    return m, np.array(off, dtype=np.int32)

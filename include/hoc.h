@@ -101,6 +101,17 @@ int hoc_load_mesh(hoc_t h, int32_t lnod, int32_t n_nodes, const double* xy,
  * curves, named by $PhysicalNames.  Returns HOC_EMESH for unreadable or unsupported files. */
 int hoc_load_gmsh(hoc_t h, const char* path, int32_t occ);
 
+/* Ensembles (no counterpart in the reference, which runs one problem per process): the
+ * loaded mesh is the union of n_inst independent meshes, instance i owning the elements
+ * [elem_offsets[i], elem_offsets[i+1]) (instances may not share faces).  Every instance
+ * advances with its own Re and dt (and its own clock for time-dependent boundary values)
+ * in the same kernel launches; each Poisson solve is stopped on
+ * ||b - A psi|| <= tol * min_i ||b_i||, which bounds every instance's relative residual by
+ * tol.  Call after hoc_load_mesh / hoc_load_gmsh and before hoc_setup; hoc_solveto is
+ * rejected for an ensemble (it sets one dt and Re). */
+int hoc_set_instances(hoc_t h, int32_t n_inst, const int32_t* elem_offsets, const double* Re, const double* dt);
+int hoc_get_instance_times(hoc_t h, int32_t n_inst, double* t);
+
 /* = process_mesh_input (src/HO_2D.cpp:3901-3918): connectivity, bases, metrics,
  * IC/BC (setup_IC_BC_SRC), Poisson operator and hierarchy, device upload. */
 int hoc_setup(hoc_t h);

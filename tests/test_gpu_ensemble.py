@@ -1,0 +1,97 @@
+"""Ensembles: independent meshes with their own Re and dt advanced by the same launches
+(one handle), checked instance by instance against separate FP64 oracle runs."""
+import numpy as np
+import pytest
+
+import hoc
+from hoc import meshes
+from pyoracle import Oracle
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def run_ensemble(parts, K, Re, dt, steps, problem_type=1, precision=hoc.HOC_FP64, tol=1e-13):
+    m, off = meshes.concat(parts)
+    g = hoc.HO2D(K=K, problem_type=problem_type, Re=Re[0], dt=dt[0], time_integration_type=4, advection_type=1,
+                 tol=tol, maxiter=800, precision=precision)
+    g.load_mesh(m)
+    g.set_instances(off, Re, dt)
+    g.setup()
+    g.step(steps)
+    return g, off
+
+
+def test_uneven_cavity_ensemble_vs_oracle(gpu_available):
+    parts = [meshes.cavity(4, 2), meshes.cavity(6, 2, seed=3, rotate=True), meshes.cavity(5, 2, seed=5, jitter=0.1)]
+    K = 4
+    Re = np.array([100.0, 400.0, 1600.0])
+    dt = np.array([2e-4, 1e-4, 1.5e-4])
+    steps = 5
+    g, off = run_ensemble(parts, K, Re, dt, steps)
+    w, p = g.get_vorticity(), g.get_streamfunction()
+    assert np.allclose(g.instance_times(), steps * dt, rtol=1e-14, atol=0)
+    for i, part in enumerate(parts):
+        o = Oracle(part, K, problem_type=1, Re=Re[i], dt=dt[i], time_integration_type=4, advection_type=1,
+                   tol=1e-13, maxiter=800)
+        o.step(steps)
+        sl = slice(off[i] * K * K, off[i + 1] * K * K)
+        ew, ep = rel_l2(w[sl], o.vorticity()), rel_l2(p[sl], o.streamfunction())
+        print("instance", i, "omega", ew, "psi", ep)
+        assert ew < 1e-10 and ep < 1e-10
+
+
+def test_ensemble_with_ramped_channels(gpu_available):
+    """Backward-facing steps with time-dependent inflow: each instance ramps on its own clock."""
+    parts = [meshes.backward_facing_step(16, 6, 2, jitter=0.05, seed=1),
+             meshes.backward_facing_step(20, 8, 2, jitter=0.05, seed=2)]
+    K = 3
+    Re = np.array([200.0, 400.0])
+    dt = np.array([5e-3, 2e-3])
+    steps = 3
+    g, off = run_ensemble(parts, K, Re, dt, steps, problem_type=2)
+    w = g.get_vorticity()
+    for i, part in enumerate(parts):
+        o = Oracle(part, K, problem_type=2, Re=Re[i], dt=dt[i], time_integration_type=4, advection_type=1,
+                   tol=1e-13, maxiter=800)
+        o.step(steps)
+        sl = slice(off[i] * K * K, off[i + 1] * K * K)
+        ew = rel_l2(w[sl], o.vorticity())
+        print("channel", i, "omega", ew)
+        assert ew < 1e-10
+
+
+def test_mixed_ensemble_matches_fp64(gpu_available):
+    parts = [meshes.cavity(8, 2), meshes.cavity(12, 2), meshes.cavity(10, 2, seed=1, rotate=True)]
+    Re = np.array([100.0, 800.0, 3200.0])
+    dt = np.array([5e-5, 2e-5, 3e-5])
+    g64, _ = run_ensemble(parts, 5, Re, dt, 3, tol=1e-10)
+    g32, _ = run_ensemble(parts, 5, Re, dt, 3, tol=1e-10, precision=hoc.HOC_MIXED)
+    ew = rel_l2(g32.get_vorticity(), g64.get_vorticity())
+    ep = rel_l2(g32.get_streamfunction(), g64.get_streamfunction())
+    print("mixed ensemble omega", ew, "psi", ep)
+    assert ew < 1e-8 and ep < 1e-8
+
+
+def test_ensemble_errors(gpu_available):
+    m, off = meshes.concat([meshes.cavity(3, 2), meshes.cavity(3, 2)])
+    g = hoc.HO2D(K=3, dt=1e-4)
+    with pytest.raises(hoc.HocError) as e:
+        g.set_instances(off, [100.0, 100.0], [1e-4, 1e-4])      # before a mesh
+    assert e.value.code == hoc.HOC_ESTATE
+    g.load_mesh(m)
+    with pytest.raises(hoc.HocError) as e:
+        g.set_instances(np.array([0, 5, 17]), [100.0, 100.0], [1e-4, 1e-4])
+    assert e.value.code == hoc.HOC_EINVAL
+    one = meshes.cavity(4, 2)                                    # one connected mesh split in two
+    g2 = hoc.HO2D(K=3, dt=1e-4)
+    g2.load_mesh(one)
+    g2.set_instances(np.array([0, 8, 16]), [100.0, 100.0], [1e-4, 1e-4])
+    with pytest.raises(hoc.HocError) as e:
+        g2.setup()
+    assert e.value.code == hoc.HOC_EINVAL
+    g.set_instances(off, [100.0, 200.0], [1e-4, 2e-4])
+    g.setup()
+    with pytest.raises(hoc.HocError) as e:
+        g.solveto(1e-3, 2, 4, 100.0)
+    assert e.value.code == hoc.HOC_EINVAL
