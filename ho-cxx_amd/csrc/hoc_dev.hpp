@@ -85,8 +85,8 @@ void csr_vcycle(hipStream_t s, std::vector<std::unique_ptr<DevLevel<R>>>& lv, co
   DevLevel<R>& L = *lv[l];
   if (l + 1 == lv.size()) { dense<R>(s, coarse_n, coarse_inv, f, u, Fp); return; }
   DevLevel<R>& C = *lv[l + 1];
-  scale<R>(s, L.A.n, L.M, f, u, Fp);
-  csr<R>(s, 1, L.A.n, L.A.ptr, L.A.col, L.A.val, u, f, L.t, nullptr, nullptr, Fp, L.A.lanes);
+  // pre-relaxation from zero (u = M f) fused with its residual t = f - A u
+  csr<R>(s, 4, L.A.n, L.A.ptr, L.A.col, L.A.val, nullptr, f, L.t, L.M, u, Fp, L.A.lanes);
   csr<R>(s, 0, C.A.n, L.Rs.ptr, L.Rs.col, L.Rs.val, L.t, nullptr, C.f, nullptr, nullptr, Fp, L.Rs.lanes);
   csr_vcycle(s, lv, coarse_inv, coarse_n, l + 1, C.f.p, C.u.p, Fp);
   csr<R>(s, 2, L.A.n, L.P.ptr, L.P.col, L.P.val, C.u, nullptr, u, nullptr, nullptr, Fp, L.P.lanes);

@@ -41,6 +41,19 @@ __device__ __forceinline__ void traces(const TabT<R>& T, const R (&u)[K * K], R 
     }
 }
 
+// Face traces are stored face-major: plane f holds [N][2][K] (values, then normal
+// derivatives) of every element's face f, so a wavefront that reads face f' of its
+// neighbours (adjacent elements) reads contiguous memory.
+template <int K, class R>
+__device__ __forceinline__ void store_traces(R* __restrict__ tr, int N, int el, const R (&bv)[4][K], const R (&bg)[4][K]) {
+#pragma unroll
+  for (int f = 0; f < 4; ++f) {
+    R* t = tr + ((size_t)f * N + el) * 2 * K;
+#pragma unroll
+    for (int k = 0; k < K; ++k) { t[k] = bv[f][k]; t[K + k] = bg[f][k]; }
+  }
+}
+
 // Common value and common normal flux on an interior face (src/HO_2D.cpp:2833-2870 applied
 // to both sides, then the average of :2742-2768).
 template <int K, class R>
@@ -130,7 +143,7 @@ __device__ __forceinline__ void boundary_face(const TabT<R>& T, const GeoT<R>& G
 template <int K, bool AFF = false, class R>
 __device__ __forceinline__ void load_nbr(const GeoT<R>& G, const R* __restrict__ tr, int nb, int fn,
                                          R (&vn)[K], R (&gn)[K], R (&fan)[K], R (&fbn)[K]) {
-  const R* t = tr + ((size_t)nb * 4 + fn) * 2 * K;
+  const R* t = tr + ((size_t)fn * G.N + nb) * 2 * K;
   if (AFF) {
     const R a = G.gface[((size_t)nb * 4 + fn) * 2], b = G.gface[((size_t)nb * 4 + fn) * 2 + 1];
 #pragma unroll
@@ -275,11 +288,7 @@ __global__ __launch_bounds__(kWG) void k_trace(int N, TabT<R> T, R* __restrict__
   if (!live) return;
   R bv[4][K], bg[4][K];
   traces<K>(T, u, bv, bg);
-  R* t = tr + (size_t)el * 8 * K;
-#pragma unroll
-  for (int f = 0; f < 4; ++f)
-#pragma unroll
-    for (int k = 0; k < K; ++k) { t[f * 2 * K + k] = bv[f][k]; t[f * 2 * K + K + k] = bg[f][k]; }
+  store_traces<K>(tr, N, el, bv, bg);
 }
 
 // ==========================================================================================
@@ -404,11 +413,7 @@ __global__ __launch_bounds__(kWG) void k_lap(int N, TabT<R> T, GeoT<R> G, R* __r
       if (tr_out && live) {  // traces of the updated iterate for the next operator application
         R bv[4][K], bg[4][K];
         traces<K>(T, u, bv, bg);
-        R* t = tr_out + (size_t)el * 8 * K;
-#pragma unroll
-        for (int f = 0; f < 4; ++f)
-#pragma unroll
-          for (int k = 0; k < K; ++k) { t[f * 2 * K + k] = bv[f][k]; t[f * 2 * K + K + k] = bg[f][k]; }
+        store_traces<K>(tr_out, N, el, bv, bg);
       }
       if (part) {  // r . x (PCG's r.z after the last smoothing step)
         tile_load<K2>(r, lds, e0, N);
@@ -611,13 +616,8 @@ __global__ __launch_bounds__(kWG) void k_velocity(int N, Tab T, Geo G, BCDev B, 
   if (!live) return;
   double bv[4][K], bg[4][K];
   traces<K>(T, w, bv, bg);
-  double* t = trw + (size_t)el * 8 * K;
-#pragma unroll
-  for (int f = 0; f < 4; ++f)
-#pragma unroll
-    for (int k = 0; k < K; ++k) { t[f * 2 * K + k] = bv[f][k]; t[f * 2 * K + K + k] = bg[f][k]; }
-  // contravariant normal-velocity traces (src/HO_2D.cpp:2332, 2489-2503)
-  double* tu = tru + (size_t)el * 4 * K;
+  store_traces<K>(trw, N, el, bv, bg);
+  // contravariant normal-velocity traces (src/HO_2D.cpp:2332, 2489-2503), face-major [4][N][K]
 #pragma unroll
   for (int rc = 0; rc < K; ++rc)
 #pragma unroll
@@ -625,8 +625,8 @@ __global__ __launch_bounds__(kWG) void k_velocity(int N, Tab T, Geo G, BCDev B, 
       double s = 0., s2 = 0.;
 #pragma unroll
       for (int m = 0; m < K; ++m) { s += U1[rc * K + m] * T.bb[m][ib]; s2 += U2[m * K + rc] * T.bb[m][ib]; }
-      tu[ib * K + rc] = s;
-      tu[(2 + ib) * K + rc] = s2;
+      tru[((size_t)ib * N + el) * K + rc] = s;
+      tru[((size_t)(2 + ib) * N + el) * K + rc] = s2;
     }
 }
 
@@ -709,8 +709,8 @@ __global__ __launch_bounds__(kWG) void k_stage(int N, Tab T, Geo G, BCDev B, con
       const int fn = G.nbf[(size_t)el * 4 + f];
       const int rev = rev_flag(f, fn);
       const double sgn = face_sign(f, fn);
-      const double* tw = trw + ((size_t)nb * 4 + fn) * 2 * K;
-      const double* tuu = tru + ((size_t)nb * 4 + fn) * K;
+      const double* tw = trw + ((size_t)fn * N + nb) * 2 * K;
+      const double* tuu = tru + ((size_t)fn * N + nb) * K;
       double wn[K], un[K];
 #pragma unroll
       for (int k = 0; k < K; ++k) { wn[k] = tw[k]; un[k] = tuu[k]; }

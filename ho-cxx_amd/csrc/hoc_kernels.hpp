@@ -3,7 +3,7 @@
 // Mapping: one thread per element, 64 elements (one wavefront) per workgroup.
 // Element-major fields ([el][j][i], K*K values per element) are moved between HBM
 // and registers through an LDS tile so that every global access is coalesced.
-// Neighbour data are read from per-face trace arrays [el][face][...][K].
+// Neighbour data are read from face-major trace arrays [face][el][value|derivative][K].
 // Face numbering ijp: 0 west (xi=-1), 1 east, 2 south (eta=-1), 3 north.
 #pragma once
 

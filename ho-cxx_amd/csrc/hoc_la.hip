@@ -159,7 +159,8 @@ __global__ void k_finalize(int phase, const double* __restrict__ pa, int np, dou
 }
 
 // ---- CSR levels (one thread per row) ------------------------------------------------------
-// mode 0: y = A x;  1: y = f - A x;  2: y += A x;  3: x += M (f - A x) (y unused)
+// mode 0: y = A x;  1: y = f - A x;  2: y += A x;  3: x += M (f - A x) (y unused);
+//      4: xio = M f and y = f - A (M f) (spai0 step from zero fused with its residual; x unused)
 template <class R>
 __global__ void k_csr(int mode, int n, const int32_t* __restrict__ ptr, const int32_t* __restrict__ col,
                       const R* __restrict__ val, const R* __restrict__ x, const R* __restrict__ f, R* __restrict__ y,
@@ -168,11 +169,15 @@ __global__ void k_csr(int mode, int n, const int32_t* __restrict__ ptr, const in
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   R s = 0;
-  for (int j = ptr[i]; j < ptr[i + 1]; ++j) s += val[j] * x[col[j]];
+  if (mode == 4)
+    for (int j = ptr[i]; j < ptr[i + 1]; ++j) s += val[j] * (M[col[j]] * f[col[j]]);
+  else
+    for (int j = ptr[i]; j < ptr[i + 1]; ++j) s += val[j] * x[col[j]];
   if (mode == 0) y[i] = s;
   else if (mode == 1) y[i] = f[i] - s;
   else if (mode == 2) y[i] += s;
-  else xio[i] = xio[i] + M[i] * (f[i] - s);
+  else if (mode == 3) xio[i] = xio[i] + M[i] * (f[i] - s);
+  else { y[i] = f[i] - s; xio[i] = M[i] * f[i]; }
 }
 
 // Same operations with LPR lanes per row (rows of many nonzeros): the lanes of a group
@@ -185,15 +190,20 @@ __global__ void k_csr_vec(int mode, int n, const int32_t* __restrict__ ptr, cons
   const int lane = threadIdx.x % LPR;
   const int i = (blockIdx.x * blockDim.x + threadIdx.x) / LPR;
   R s = 0;
-  if (i < n)
-    for (int j = ptr[i] + lane; j < ptr[i + 1]; j += LPR) s += val[j] * x[col[j]];
+  if (i < n) {
+    if (mode == 4)
+      for (int j = ptr[i] + lane; j < ptr[i + 1]; j += LPR) s += val[j] * (M[col[j]] * f[col[j]]);
+    else
+      for (int j = ptr[i] + lane; j < ptr[i + 1]; j += LPR) s += val[j] * x[col[j]];
+  }
 #pragma unroll
   for (int off = LPR / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, LPR);
   if (i >= n || lane != 0) return;
   if (mode == 0) y[i] = s;
   else if (mode == 1) y[i] = f[i] - s;
   else if (mode == 2) y[i] += s;
-  else xio[i] = xio[i] + M[i] * (f[i] - s);
+  else if (mode == 3) xio[i] = xio[i] + M[i] * (f[i] - s);
+  else { y[i] = f[i] - s; xio[i] = M[i] * f[i]; }
 }
 
 // x = M f (spai0 from a zero guess)

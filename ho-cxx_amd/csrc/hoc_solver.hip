@@ -4,7 +4,7 @@
 //   update_BCs(t)            k_update_bcs
 //   solve_Poisson            k_rhs, then PCG or BiCGSTAB on As = -W A (W = Gauss weights
 //                            w_i w_j, which makes As symmetric on affine meshes) with a
-//                            V-cycle: matrix-free fine level + Chebyshev(2) Jacobi
+//                            V-cycle: matrix-free fine level + Chebyshev(3) Jacobi
 //                            smoothing, continuous bilinear coarse space on the element
 //                            corners (Ac = Pc^T As Pc assembled from probes of As), then
 //                            smoothed aggregation below
@@ -125,8 +125,10 @@ struct Solver {
     bool on = false;
     std::vector<hipEvent_t> e0, e1;
     std::vector<int> var;
+    std::vector<double> nbytes;
     int n = 0;
     double ms[HOC_PROF_VARIANTS] = {0};
+    double bytes[HOC_PROF_VARIANTS] = {0};
     int64_t cnt[HOC_PROF_VARIANTS] = {0};
   } prof;
   // stats
@@ -278,6 +280,7 @@ struct Solver {
     if (rec) {
       HIPCHK(hipEventRecord(prof.e1[prof.n], stream));
       prof.var[prof.n] = P.prof_off + epi;
+      prof.nbytes[prof.n] = lap_bytes(epi, (double)sizeof(R), tout != nullptr);
       if (++prof.n == (int)prof.e0.size()) prof_flush();
     }
   }
@@ -289,13 +292,15 @@ struct Solver {
       HIPCHK(hipEventElapsedTime(&ms, prof.e0[i], prof.e1[i]));
       prof.ms[prof.var[i]] += ms;
       prof.cnt[prof.var[i]] += 1;
+      prof.bytes[prof.var[i]] += prof.nbytes[i];
     }
     prof.n = 0;
   }
   // Algorithmic HBM bytes of one fine-operator launch per epilogue, for values of vb bytes:
-  // the field, the neighbour face traces (value + derivative), metrics, connectivity, and
-  // the epilogue's own vectors, each read or written once.
-  double lap_bytes(int epi, double vb = 8.) const {
+  // the field, the neighbour face traces (value + derivative), metrics, connectivity, the
+  // epilogue's own vectors and (traces_out) the updated iterate's face traces, each read or
+  // written once.
+  double lap_bytes(int epi, double vb = 8., bool traces_out = false) const {
     const double vec = (double)N * K2 * vb;
     double b = vec                                   // x
              + (double)N * 4 * 2 * K * vb            // neighbour traces
@@ -309,6 +314,7 @@ struct Solver {
       case EPI_CHEBF: b += 4 * vec; break;                    // r, diag, d out, x out
       case EPI_RESTRICT: b += vec + (double)N * 4 * vb; break;  // r, corner sums
     }
+    if (traces_out) b += (double)N * 4 * 2 * K * vb;
     return b;
   }
 
@@ -425,10 +431,10 @@ struct Solver {
       h2d(P64.z.p, hx.data(), D * sizeof(double));
     }
     lam_max = lam;
-    const double lmax = 1.1 * lam, lmin = lmax * (cfg.cheb_lower > 0. ? cfg.cheb_lower : 1. / 30.);
+    const double lmax = 1.1 * lam, lmin = lmax * (cfg.cheb_lower > 0. ? cfg.cheb_lower : 1. / 60.);
     const double theta = 0.5 * (lmax + lmin), delta = 0.5 * (lmax - lmin);
     const double sigma = theta / delta;
-    cheb_m = cfg.cheb_degree > 0 ? cfg.cheb_degree : 2;
+    cheb_m = cfg.cheb_degree > 0 ? cfg.cheb_degree : 3;
     cheb_theta = theta;
     double rho = 1. / sigma;
     for (int k = 1; k < cheb_m; ++k) {
@@ -1082,10 +1088,10 @@ int hoc_profile_enable(hoc_t h, int32_t on) {
     s.prof_flush();
     if (on && s.prof.e0.empty()) {
       const int cap = 4096;
-      s.prof.e0.resize(cap); s.prof.e1.resize(cap); s.prof.var.resize(cap);
+      s.prof.e0.resize(cap); s.prof.e1.resize(cap); s.prof.var.resize(cap); s.prof.nbytes.resize(cap);
       for (int i = 0; i < cap; ++i) { HIPCHK(hipEventCreate(&s.prof.e0[i])); HIPCHK(hipEventCreate(&s.prof.e1[i])); }
     }
-    if (on) for (int v = 0; v < HOC_PROF_VARIANTS; ++v) { s.prof.ms[v] = 0.; s.prof.cnt[v] = 0; }
+    if (on) for (int v = 0; v < HOC_PROF_VARIANTS; ++v) { s.prof.ms[v] = 0.; s.prof.cnt[v] = 0; s.prof.bytes[v] = 0.; }
     s.prof.on = on != 0;
   });
 }
@@ -1103,7 +1109,8 @@ int hoc_profile_get(hoc_t h, hoc_kernel_profile* p) {
       p->launches[v] = s.prof.cnt[v];
       p->total_ms[v] = s.prof.ms[v];
       const int half = HOC_PROF_VARIANTS / 2;
-      p->bytes_per_launch[v] = v < half ? s.lap_bytes(v, 8.) : s.lap_bytes(v - half, 4.);
+      p->bytes_per_launch[v] = s.prof.cnt[v] > 0 ? s.prof.bytes[v] / (double)s.prof.cnt[v]
+                                                 : (v < half ? s.lap_bytes(v, 8.) : s.lap_bytes(v - half, 4.));
       std::strncpy(p->names[v], names[v], sizeof(p->names[v]) - 1);
     }
   });

@@ -55,10 +55,10 @@ typedef struct {
   double mixed_inner_tol;         /* HOC_MIXED: floor of the FP32 inner solve's relative
                                      tolerance per refinement pass (0 = 1e-5)            */
   int32_t cheb_degree;            /* fine-level Chebyshev smoother degree, pre and post
-                                     (0 = 2; at most 8)                                   */
+                                     (0 = 3; at most 8)                                   */
   int32_t reserved0;
   double cheb_lower;              /* smoothing interval [cheb_lower, 1.1] x lambda_max(D^-1 As)
-                                     (0 = 1/30)                                          */
+                                     (0 = 1/60)                                          */
 } hoc_config;
 
 typedef struct {
@@ -155,7 +155,7 @@ int hoc_stats_get(hoc_t h, hoc_stats* s);
 typedef struct {
   int64_t launches[HOC_PROF_VARIANTS];
   double total_ms[HOC_PROF_VARIANTS];
-  double bytes_per_launch[HOC_PROF_VARIANTS];  /* algorithmic HBM bytes of one launch */
+  double bytes_per_launch[HOC_PROF_VARIANTS];  /* algorithmic HBM bytes per launch (average) */
   char names[HOC_PROF_VARIANTS][24];
 } hoc_kernel_profile;
 int hoc_profile_enable(hoc_t h, int32_t on);   /* on != 0 also clears the counters */

@@ -60,21 +60,24 @@ def test_fr_poisson_matrix_matches_direct(gpu_available, K):
 
 
 def test_fr_poisson_K5_spai0_stalls_like_the_reference(gpu_available):
-    """AMGCL's default spai0 relaxation does not reach 1e-10 on the K=5 FR matrix within
-    the reference's 150 iterations (lambda_max(D^-1 A) ~ 3.5): the plug-in reproduces it and
-    reports HOC_ENOCONV where the reference continues silently."""
-    m = meshes.cavity(6, 2)
-    o = Oracle(m, 5, problem_type=1, Re=100.0, dt=1e-4, time_integration_type=4, advection_type=1, tol=1e-14)
+    """AMGCL's default spai0 relaxation does not reach 1e-10 on the K=5 FR matrix of a
+    16 x 16 cavity within the reference's 150 iterations (the oracle's restatement stalls
+    on the same system, tests/test_oracle_pins.py): the plug-in reproduces it and reports
+    HOC_ENOCONV where the reference continues silently."""
+    m = meshes.cavity(16, 2)
+    o = Oracle(m, 5, problem_type=1, Re=100.0, dt=1e-5, time_integration_type=4, advection_type=1, tol=1e-10)
     ptr, col, val = o.csr()
     n = len(ptr) - 1
     A = sp.csr_matrix((val, col, ptr), shape=(n, n))
-    f = np.random.default_rng(5).standard_normal(n)
+    xy = o.table("sps_xy").reshape(-1, 2)
+    w = np.sin(3 * xy[:, 0]) * np.cos(2 * xy[:, 1])
+    f = -w * o.table("vol_jac")
     ls = solver_for(A, reltol=1e-10, abstol=0.0, maxiters=150)
     with pytest.raises(hoc.HocError) as e:
         ls.solve(f)
     assert e.value.code == hoc.HOC_ENOCONV
-    print("K=5 spai0: its", ls.num_iterations, "res", ls.residual)
-    assert ls.residual > 1e-10
+    print("K=5 spai0: its", ls.num_iterations, "res", ls.residual, "levels", ls.levels)
+    assert ls.num_iterations == 150 and ls.residual > 1e-6
 
 
 def test_zero_rhs_and_exact_guess(gpu_available):
