@@ -2,10 +2,11 @@
 // (src/HO_2D.hpp:220-290) over the C ABI of include/hoc.h.
 //
 // Existing callers of the reference class switch by type: the mesh/boundary loaders,
-// process_mesh_input, the solution getters/setters, solveto_d and getallvorts_d keep
-// their names, argument meaning and ownership (caller-owned buffers, copied).  Errors
-// that the reference handles with exit()/assert throw hoc_gpu_error instead; methods the
-// reference declares void still return nothing.  Problem types 1-3 are supported.
+// process_mesh_input, the solution getters/setters, the hybrid driver's open-boundary
+// setters, solveto_d and getallvorts_d keep their names, argument meaning and ownership
+// (caller-owned buffers, copied).  Errors that the reference handles with exit()/assert
+// throw hoc_gpu_error instead; methods the reference declares void still return nothing.
+// Problem types 1-3 and 11 (hybrid) are supported.
 #pragma once
 
 #include <cstdint>
@@ -50,22 +51,35 @@ class HO_2D_GPU {
     lnod_ = _iorder + 1;
     xy_.assign(_xy, _xy + _nnodes);
     elem_.assign(_idxelems, _idxelems + _nelems);
-    bedge_.clear(); bbdry_.clear(); names_.clear();
+    bedge_.clear(); bbdry_.clear(); names_.clear(); nval_.clear(); vals_.clear();
     add_boundary("wall", _nwall, _idxwall);
     add_boundary("open", _nopen, _idxopen);
   }
-  void load_inout_arrays_d(const int32_t, const double*, const int32_t _nin, const int32_t* _idxin,
-                           const int32_t, const double*, const int32_t _nout, const int32_t* _idxout) {
-    add_boundary("inlet", _nin, _idxin);
-    add_boundary("outlet", _nout, _idxout);
+  // inlets/outlets with 1 (normal) or 2 (tangential, normal) velocity values per edge
+  // (src/HO_2D.cpp:3882-3896, 3779-3793)
+  void load_inout_arrays_d(const int32_t _ninval, const double* _velin, const int32_t _nin, const int32_t* _idxin,
+                           const int32_t _noutval, const double* _velout, const int32_t _nout,
+                           const int32_t* _idxout) {
+    add_boundary("inlet", _nin, _idxin, _ninval, _velin);
+    add_boundary("outlet", _nout, _idxout, _noutval, _velout);
   }
-  // Named boundaries as read from a Gmsh file (src/Mesh.cpp:121-253).
-  void add_boundary(const std::string& name, int32_t n_idx, const int32_t* idx) {
+  // Named boundaries as read from a Gmsh file (src/Mesh.cpp:121-253), with optional values.
+  void add_boundary(const std::string& name, int32_t n_idx, const int32_t* idx, int32_t n_val = 0,
+                    const double* val = nullptr) {
     if (n_idx <= 0) return;
     const int b = (int)names_.size();
+    const int32_t ne = n_idx / lnod_;
+    const int per = (n_val > 0 && val) ? n_val / ne : 0;
+    if (per < 0 || per > 2 || (per > 0 && per * ne != n_val))
+      throw hoc_gpu_error(HOC_EINVAL, "boundary values must be 1 or 2 per edge");
     names_.push_back(name);
     for (int32_t i = 0; i < n_idx; ++i) bedge_.push_back(idx[i]);
-    for (int32_t e = 0; e < n_idx / lnod_; ++e) bbdry_.push_back(b);
+    for (int32_t e = 0; e < ne; ++e) {
+      bbdry_.push_back(b);
+      nval_.push_back(per);
+      vals_.push_back(per ? val[e * per] : 0.);
+      vals_.push_back(per == 2 ? val[e * per + 1] : 0.);
+    }
   }
   void process_mesh_input() {
     clean_up();
@@ -76,6 +90,11 @@ class HO_2D_GPU {
     check(hoc_load_mesh(h_, lnod_, (int32_t)(xy_.size() / 2), xy_.data(), (int32_t)(elem_.size() / L2), elem_.data(),
                         (int32_t)bbdry_.size(), bedge_.data(), bbdry_.data(), (int32_t)names_.size(), joined.c_str()),
           h_);
+    for (int32_t v : nval_)
+      if (v) {
+        check(hoc_set_boundary_values(h_, (int32_t)nval_.size(), nval_.data(), vals_.data()), h_);
+        break;
+      }
     check(hoc_setup(h_), h_);
   }
 
@@ -94,8 +113,29 @@ class HO_2D_GPU {
   void getstreamfunction_d(const int32_t n, double* p) { check(hoc_get_streamfunction(h_, n, p), h_); }
   void getvelocity_d(const int32_t n, double* uv) { check(hoc_get_velocity(h_, n, uv), h_); }
 
-  // ---- setters / marching ----
-  void setsolnvort_d(const int32_t n, const double* w) { check(hoc_set_vorticity(h_, n, w), h_); }
+  // ---- hybrid driver (src/HO_2D.cpp:3994-4184) ----
+  int32_t getopenptlen() { return (int32_t)(2 * (h_ ? hoc_open_point_count(h_) : 0)); }
+  void getopenpts_d(const int32_t n, double* xy) { check(hoc_get_open_points(h_, n, xy), h_); }
+  void setopenvels_d(const int32_t n, double* uv) { check(hoc_set_open_velocities(h_, n, uv), h_); }
+  void setopenvort_d(const int32_t n, double* w) { check(hoc_set_open_vorticity(h_, n, w), h_); }
+  // Stores the driver's vorticity target; it enters the solution through the weighted blend of
+  // problem_type 11 only, as in the reference.
+  void setsolnvort_d(const int32_t n, double* w) {
+    if (cfg_.problem_type == 11) check(hoc_set_vorticity_target(h_, n, w), h_);
+    else if (n != getsolnptlen() / 2) throw hoc_gpu_error(HOC_EINVAL, "setsolnvort_d: length mismatch");
+  }
+  void setptogweights_d(const int32_t n, double* w) {
+    if (cfg_.problem_type == 11) check(hoc_set_p2g_weights(h_, n, w), h_);
+  }
+  // VTK output of the solution points (save_vorticity_vtk / save_smooth_vtk, :3369-3692)
+  void trigger_write(const int32_t indx, const int32_t subidx = -1) {
+    std::string path = "vort_" + std::to_string(indx) + (subidx > -1 ? "_" + std::to_string(subidx) : "") + ".vtk";
+    check(hoc_write_vtk(h_, path.c_str()), h_);
+  }
+  // The vorticity field itself (initial conditions; no counterpart in the reference).
+  void set_vorticity(const int32_t n, const double* w) { check(hoc_set_vorticity(h_, n, w), h_); }
+
+  // ---- marching ----
   void solveto_d(const double outer_dt, const int32_t nstep, const int32_t integ, const double Re,
                  const int32_t /*region*/ = 0) {
     check(hoc_solveto(h_, outer_dt, nstep, integ, Re), h_);
@@ -141,6 +181,7 @@ class HO_2D_GPU {
   bool hybrid_ = false;
   int32_t lnod_ = 2;
   std::vector<double> xy_;
-  std::vector<int32_t> elem_, bedge_, bbdry_;
+  std::vector<int32_t> elem_, bedge_, bbdry_, nval_;
+  std::vector<double> vals_;
   std::vector<std::string> names_;
 };

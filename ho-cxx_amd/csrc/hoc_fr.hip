@@ -475,7 +475,9 @@ __global__ __launch_bounds__(kWG) void k_rhs(int N, Tab T, Geo G, const double* 
       const size_t q = ((size_t)el * 4 + f) * K;
 #pragma unroll
       for (int k = 0; k < K; ++k) { fa[k] = G.fa[q + k]; fb[k] = G.fb[q + k]; }
-      boundary_face<K>(T, G, el, f, G.sw_psi[e], bcpsi + (size_t)e * K, 1.0, bv[f], bg[f], fa, fb, cv[f], cg[f]);
+      // Neumann values enter with the owner face's outward sign: boundary_source . BC_Poisson
+      // (src/HO_2D.cpp:1660-1714, sign (2t-1) of :1685) equals L(0; sign * BC) on such faces
+      boundary_face<K>(T, G, el, f, G.sw_psi[e], bcpsi + (size_t)e * K, G.esign[e], bv[f], bg[f], fa, fb, cv[f], cg[f]);
     }
     const size_t base = (size_t)el * K2;
     second_order_core<K>(T, z, bv, cv, cg, G.Ga + base, G.Gb + base, G.Gc + base, out);
@@ -856,6 +858,38 @@ __global__ void k_update_bcs(int n, int K, BCDev B, double tc0, InstDev I) {
 }
 
 // ==========================================================================================
+// Hybrid boundary values at fraction fac1 of the outer step (update_BCs for problem_type
+// 11, src/HO_2D.cpp:2104-2191, then the sign loop :2203-2215) and the blend of the
+// vorticity field towards the driver's target (:2179-2186).
+// ==========================================================================================
+__global__ void k_update_bcs_hybrid(int n, int K, BCDev B, HybridDev H, double fac1, double fac2) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int e = i / K;
+  const double sg = H.esign[e];
+  double nv;
+  if (H.kind[e] == 1) {  // open: blended driver values
+    const double pv = -(fac1 * H.vp0[i] + fac2 * H.vp1[i]);
+    B.pvel[i] = pv;
+    B.psi[i] = pv;
+    B.diff[i] = fac1 * H.vw0[i] + fac2 * H.vw1[i];
+    nv = -(fac1 * H.vn0[i] + fac2 * H.vn1[i]);
+  } else {               // set-up values: marched psi, inlet vorticity, zero outlet values
+    B.psi[i] = B.psi0[i];
+    B.diff[i] = B.diff0[i];
+    B.pvel[i] = 0.;
+    nv = H.kind[e] == 2 ? -H.inlet_nvel[e] : (H.kind[e] == 0 ? 0. : B.nvel[i]);
+  }
+  B.nvel[i] = nv * sg;
+  if (H.sw_diff[e] == 1) B.diff[i] *= sg;
+}
+
+__global__ void k_hybrid_blend(size_t n, double* __restrict__ w, HybridDev H, double fac1, double fac2) {
+  for (size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (size_t)gridDim.x * blockDim.x)
+    w[v] = (1.0 - H.wgt[v]) * w[v] + H.wgt[v] * (fac1 * H.w0[v] + fac2 * H.w1[v]);
+}
+
+// ==========================================================================================
 // Host launchers
 // ==========================================================================================
 #define HOC_K_SWITCH(KV, BODY)                                   \
@@ -968,6 +1002,20 @@ hipError_t launch_stage(int K, int adv, int N, const Tab& T, const Geo& G, const
 hipError_t launch_update_bcs(int n, int K, const BCDev& B, double tc, const InstDev& I, hipStream_t s) {
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(k_update_bcs, dim3((n + 255) / 256), dim3(256), 0, s, n, K, B, tc, I);
+  return hipGetLastError();
+}
+
+hipError_t launch_update_bcs_hybrid(int n, int K, const BCDev& B, const HybridDev& H, double fac1, double fac2,
+                                    hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_update_bcs_hybrid, dim3((n + 255) / 256), dim3(256), 0, s, n, K, B, H, fac1, fac2);
+  return hipGetLastError();
+}
+
+hipError_t launch_hybrid_blend(size_t n, double* w, const HybridDev& H, double fac1, double fac2, hipStream_t s) {
+  size_t g = (n + 255) / 256;
+  if (g > 4096) g = 4096;
+  hipLaunchKernelGGL(k_hybrid_blend, dim3((unsigned)g), dim3(256), 0, s, n, w, H, fac1, fac2);
   return hipGetLastError();
 }
 

@@ -43,6 +43,11 @@ hipError_t launch_stage(int K, int adv, int N, const Tab& T, const Geo& G, const
                         const InstDev& I, const StageOut& so, hipStream_t s);
 // n = E*K boundary values; tc = time factor (per instance through I.ramp when set)
 hipError_t launch_update_bcs(int n, int K, const BCDev& B, double tc, const InstDev& I, hipStream_t s);
+// problem_type 11: boundary values at fraction fac1 (fac2 = 1 - fac1) of the outer step,
+// and the blend w = (1 - wgt) w + wgt (fac1 w0 + fac2 w1)
+hipError_t launch_update_bcs_hybrid(int n, int K, const BCDev& B, const HybridDev& H, double fac1, double fac2,
+                                    hipStream_t s);
+hipError_t launch_hybrid_blend(size_t n, double* w, const HybridDev& H, double fac1, double fac2, hipStream_t s);
 int lap_blocks(int N);
 
 }  // namespace hoc

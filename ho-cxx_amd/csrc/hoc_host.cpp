@@ -277,7 +277,84 @@ void solution_points(const Mesh& m, const Tables& t, std::vector<double>& xy) {
       }
 }
 
-void build_bcs(const Mesh& m, const Tables& t, int problem_type, BCs& bc, const std::vector<int32_t>& edge_inst) {
+// Hybrid boundaries (src/HO_2D.cpp:893-1134): switches by boundary name; with inlets, psi is
+// marched around the closed loop of non-open boundary edges (wall edges take the running
+// value, inlet edges a linear profile and a uniform vorticity from the neighbouring inflow
+// differences, outlets advance the running value), which the reference does for Lnod = 2.
+static void build_hybrid_bcs(const Mesh& m, const Tables& t, BCs& bc, const std::vector<std::vector<int>>& bedges,
+                             const std::vector<double>& nvel_in, const std::vector<int8_t>& has_nvel) {
+  const int K = t.K, E = m.n_bedges, NB = (int)m.bnames.size();
+  bc.hybrid_kind.assign(E, 0);
+  bc.inlet_nvel.assign(E, 0.);
+  bool inlets = false;
+  for (int b = 0; b < NB; ++b) {
+    const std::string& nm = m.bnames[b];
+    int kind;
+    if (nm == "wall") kind = 0;
+    else if (nm == "open") kind = 1;
+    else if (nm == "inlet") kind = 2;
+    else if (nm == "outlet") kind = 3;
+    else throw std::runtime_error("hybrid boundaries must be named wall, open, inlet or outlet");
+    inlets = inlets || kind == 2;
+    for (int e : bedges[b]) {
+      bc.hybrid_kind[e] = (uint8_t)kind;
+      bc.sw_psi[e] = (kind == 1 || kind == 3) ? 1 : 0;
+      bc.sw_diff[e] = (kind == 0 || kind == 3) ? 1 : 0;
+      bc.no_slip_edge[e] = kind == 0;
+      if (kind >= 2) {
+        if (has_nvel.empty() || !has_nvel[e]) throw std::runtime_error("hybrid inlet/outlet edge without a normal velocity");
+        bc.inlet_nvel[e] = nvel_in[e];
+      }
+    }
+  }
+  if (!inlets) return;
+  if (m.lnod != 2) throw std::runtime_error("hybrid inlets need lnod = 2");
+  // boundary loop through the non-open edges, start at the first wall/open edge in input order
+  std::unordered_map<int32_t, int> starts;  // first node -> edge
+  for (int e = 0; e < E; ++e)
+    if (bc.hybrid_kind[e] != 1) starts[m.edges[e][0]] = e;
+  int e0 = 0;
+  while (e0 < E && bc.hybrid_kind[e0] >= 2) ++e0;
+  if (e0 == E) throw std::runtime_error("hybrid boundary has no wall or open edge");
+  std::vector<int> loop{e0};
+  for (;;) {
+    auto it = starts.find(m.edges[loop.back()][1]);
+    if (it == starts.end()) throw std::runtime_error("hybrid boundary edges do not close a loop");
+    if (it->second == loop[0]) break;
+    loop.push_back(it->second);
+    if ((int)loop.size() > E) throw std::runtime_error("hybrid boundary edges do not close a loop");
+  }
+  const int M = (int)loop.size();
+  std::vector<double> len(M);
+  for (int q = 0; q < M; ++q) {
+    const auto& nd = m.edges[loop[q]];
+    len[q] = std::hypot(m.xy[2 * (size_t)nd[0]] - m.xy[2 * (size_t)nd[1]],
+                        m.xy[2 * (size_t)nd[0] + 1] - m.xy[2 * (size_t)nd[1] + 1]);
+  }
+  auto vin = [&](int q) { return bc.hybrid_kind[loop[q]] == 2 ? bc.inlet_nvel[loop[q]] : 0.; };
+  double run = 0.;
+  for (int q = 0; q < M; ++q) {
+    const int e = loop[q];
+    const double a = run;
+    if (bc.hybrid_kind[e] == 2) {
+      const double vn = bc.inlet_nvel[e];
+      run -= len[q] * vn;
+      const int qp = q == 0 ? M - 1 : q - 1, qn = q == M - 1 ? 0 : q + 1;
+      const double w = (vin(qn) - vn) / (len[qn] + len[q]) + (vn - vin(qp)) / (len[q] + len[qp]);
+      for (int k = 0; k < K; ++k) {
+        bc.psi[(size_t)e * K + k] = t.gsb[0][k] * a + t.gsb[1][k] * run;
+        bc.diff[(size_t)e * K + k] = w;
+      }
+    } else if (bc.hybrid_kind[e] == 3) {
+      run -= len[q] * bc.inlet_nvel[e];
+    } else if (bc.hybrid_kind[e] == 0) {
+      for (int k = 0; k < K; ++k) bc.psi[(size_t)e * K + k] = run;
+    }
+  }
+}
+
+void build_bcs(const Mesh& m, const Tables& t, int problem_type, BCs& bc, const std::vector<int32_t>& edge_inst,
+               const std::vector<double>& edge_normvel, const std::vector<int8_t>& edge_has_nvel) {
   const int K = t.K, E = m.n_bedges, L = t.L;
   bc.sw_psi.assign(E, 0); bc.sw_diff.assign(E, 0); bc.no_slip_edge.assign(E, 0);
   for (auto* v : {&bc.psi, &bc.psi_ramp, &bc.diff, &bc.diff_ramp, &bc.nvel, &bc.nvel_ramp, &bc.pvel, &bc.pvel_ramp})
@@ -367,8 +444,10 @@ void build_bcs(const Mesh& m, const Tables& t, int problem_type, BCs& bc, const 
         }
       }
     }
+  } else if (problem_type == 11) {  // hybrid, src/HO_2D.cpp:893-1134
+    build_hybrid_bcs(m, t, bc, bedges, edge_normvel, edge_has_nvel);
   } else {
-    throw std::runtime_error("problem_type must be 1 (cavity), 2 (backward-facing step) or 3 (cylinder)");
+    throw std::runtime_error("problem_type must be 1 (cavity), 2 (backward-facing step), 3 (cylinder) or 11 (hybrid)");
   }
 }
 

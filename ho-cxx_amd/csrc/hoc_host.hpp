@@ -93,11 +93,18 @@ struct BCs {
   std::vector<double> pvel, pvel_ramp;         // BC_parl_vel base, ramp
   std::vector<double> edge_sign;               // 2*(ijp%2)-1 of the owner face
   bool time_dependent = false;
+  // problem_type 11 (hybrid): kind of each edge (0 wall, 1 open, 2 inlet, 3 outlet) and
+  // the normal velocity an inlet edge carries
+  std::vector<uint8_t> hybrid_kind;
+  std::vector<double> inlet_nvel;
 };
 // edge_inst: ensemble instance of each boundary edge (empty: one instance); inlet extents
 // are taken per instance.
+// edge_normvel: normal velocity carried by each boundary edge (hybrid inlets/outlets;
+// empty when none was given).
 void build_bcs(const Mesh& m, const Tables& t, int problem_type, BCs& bc,
-               const std::vector<int32_t>& edge_inst = {});
+               const std::vector<int32_t>& edge_inst = {}, const std::vector<double>& edge_normvel = {},
+               const std::vector<int8_t>& edge_has_nvel = {});
 
 // Scalar CSR and smoothed-aggregation AMG hierarchy (host build, device apply).
 struct CSR {

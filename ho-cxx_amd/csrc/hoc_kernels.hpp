@@ -60,6 +60,17 @@ struct GeoT {
 };
 using Geo = GeoT<double>;
 
+// Hybrid operation (problem_type 11): per-edge kind and the open-boundary values at the
+// start and end of the outer step, blended per stage (src/HO_2D.cpp:2104-2191).
+struct HybridDev {
+  const uint8_t* kind;       // [E] 0 wall, 1 open, 2 inlet, 3 outlet
+  const double* inlet_nvel;  // [E]
+  const double* esign;       // [E]
+  const int8_t* sw_diff;     // [E]
+  const double *vn0, *vn1, *vp0, *vp1, *vw0, *vw1;  // [E][K] normal/tangential velocity, vorticity
+  const double *w0, *w1, *wgt;                      // [N][K2] solution vorticity targets and weights
+};
+
 // Per-instance parameters of an ensemble (null pointers: one instance, scalar arguments).
 struct InstDev {
   const double* re_inv;      // [n_inst] 1/Re

@@ -132,6 +132,16 @@ struct Solver {
     int64_t cnt[HOC_PROF_VARIANTS] = {0};
   } prof;
   // stats
+  // hybrid operation (problem_type 11): values carried by boundary edges, outer-step window,
+  // open-boundary edges in driver order, start/end values and blend targets
+  std::vector<double> edge_nvel_in;
+  std::vector<int8_t> edge_has_nvel;
+  double time_start = 0., time_end = 0.;
+  std::vector<int32_t> open_edges;
+  std::vector<double> h_vn1, h_vp1, h_vw1;
+  DBuf<uint8_t> d_hkind;
+  DBuf<double> d_hinlet, d_vn0, d_vn1, d_vp0, d_vp1, d_vw0, d_vw1, d_hw0, d_hw1, d_hwgt;
+  HybridDev H{};
   // ensembles: instance i owns elements [inst_off[i], inst_off[i+1]) and has its own Re, dt, clock
   int n_inst = 1;
   std::vector<int32_t> inst_off;
@@ -166,7 +176,7 @@ struct Solver {
       for (int e = 0; e < mesh.n_bedges; ++e)
         edge_inst[e] = (int32_t)(std::upper_bound(inst_off.begin(), inst_off.end(), mesh.be_elem[e]) - inst_off.begin() - 1);
     }
-    build_bcs(mesh, tab, cfg.problem_type, bch, edge_inst);
+    build_bcs(mesh, tab, cfg.problem_type, bch, edge_inst, edge_nvel_in, edge_has_nvel);
     K = cfg.K; K2 = K * K; N = mesh.n_elem; E = mesh.n_bedges; D = (size_t)N * K2;
     re_inv = cfg.Re > 1.e-3 ? 1. / cfg.Re : -1.;  // src/HO_2D.cpp:255
     // tables
@@ -229,6 +239,7 @@ struct Solver {
             affine ? d_gel.p : nullptr, affine ? d_gface.p : nullptr,
             d_nb, d_nbf, d_swpsi, d_swdiff, d_noslip, d_esign, nullptr};
     setup_instances();
+    setup_hybrid();
     const size_t EK = std::max<size_t>((size_t)E * K, 1);
     for (auto* b : {&b_psi, &b_diff, &b_nvel, &b_pvel, &b_vj, &b_vort, &b_adv}) b->alloc(EK);
     auto upl = [&](DBuf<double>& d, const std::vector<double>& h) { if (h.empty()) d.alloc(1); else d.upload(h); };
@@ -675,6 +686,7 @@ struct Solver {
     ready = false;
     n_inst = 1;
     inst_off.clear(); inst_re.clear(); inst_dt.clear(); inst_time.clear();
+    edge_nvel_in.clear(); edge_has_nvel.clear();
   }
 
   double ramp(double t) const { return bch.time_dependent ? 2. * std::fabs(std::sin(M_PI / 40. * t)) : 1.; }
@@ -682,6 +694,14 @@ struct Solver {
   // Boundary values at stage fraction c of the step (each instance at its own clock), or at
   // the absolute time t_abs for all instances when t_abs is given.
   void update_bcs(double c, double t_abs = std::nan("")) {
+    if (cfg.problem_type == 11) {
+      const double t = std::isnan(t_abs) ? time + c * cfg.dt : t_abs;
+      const double span = time_end - time_start;
+      const double fac1 = span > 0. ? (t - time_start) / span : 0.;
+      HIPCHK(launch_update_bcs_hybrid(E * K, K, B, H, fac1, 1. - fac1, stream));
+      HIPCHK(launch_hybrid_blend(D, w, H, fac1, 1. - fac1, stream));
+      return;
+    }
     if (n_inst > 1 && std::isnan(t_abs)) {
       std::vector<double> r(n_inst);
       for (int i = 0; i < n_inst; ++i) r[i] = ramp(inst_time[i] + c * inst_dt[i]);
@@ -734,6 +754,80 @@ struct Solver {
   void stage_rk4(const double* win, double c, double* kout, double cnext) {
     StageOut so; so.k = kout; so.next = vort; so.c = cnext; so.base = w;
     stage(win, c, so);
+  }
+
+  // Hybrid tables: edge kinds, open-boundary order, zero start/end values and targets.
+  void setup_hybrid() {
+    H = HybridDev{};
+    open_edges.clear();
+    time_start = time_end = 0.;
+    if (cfg.problem_type != 11) return;
+    for (int b = 0; b < (int)mesh.bnames.size(); ++b)
+      if (mesh.bnames[b] == "open")
+        for (int e = 0; e < E; ++e) if (mesh.bedge_bdry[e] == b) open_edges.push_back(e);
+    const size_t EK = std::max<size_t>((size_t)E * K, 1);
+    d_hkind.upload(bch.hybrid_kind);
+    d_hinlet.upload(bch.inlet_nvel);
+    for (auto* b : {&d_vn0, &d_vn1, &d_vp0, &d_vp1, &d_vw0, &d_vw1}) b->alloc(EK);
+    for (auto* b : {&d_hw0, &d_hw1, &d_hwgt}) b->alloc(D);
+    h_vn1.assign(EK, 0.); h_vp1.assign(EK, 0.); h_vw1.assign(EK, 0.);
+    H = HybridDev{d_hkind, d_hinlet, d_esign, d_swdiff, d_vn0, d_vn1, d_vp0, d_vp1, d_vw0, d_vw1, d_hw0, d_hw1, d_hwgt};
+  }
+  void need_hybrid() const {
+    if (cfg.problem_type != 11) throw StatusError(HOC_ESTATE, "hybrid driver calls need problem_type 11");
+  }
+  // getopenpts_d (src/HO_2D.cpp:4007-4046): K points along each open edge, edge node order
+  void open_points(double* xy) const {
+    const int L = mesh.lnod;
+    for (size_t i = 0; i < open_edges.size(); ++i) {
+      const auto& nd = mesh.edges[open_edges[i]];  // tensor order after connect()
+      for (int j = 0; j < K; ++j) {
+        double x = 0., y = 0.;
+        for (int l = 0; l < L; ++l) {
+          x += tab.gsb[l][j] * mesh.xy[2 * (size_t)nd[l]];
+          y += tab.gsb[l][j] * mesh.xy[2 * (size_t)nd[l] + 1];
+        }
+        xy[2 * (i * K + j)] = x; xy[2 * (i * K + j) + 1] = y;
+      }
+    }
+  }
+  // setopenvels_d (src/HO_2D.cpp:4050-4117): start = end, then the new Cartesian velocities
+  // as outward-normal and tangential components with the owner face's metrics
+  void set_open_velocities(const double* uv) {
+    const size_t EK = std::max<size_t>((size_t)E * K, 1);
+    HIPCHK(hipMemcpyAsync(d_vn0.p, d_vn1.p, EK * sizeof(double), hipMemcpyDeviceToDevice, stream));
+    HIPCHK(hipMemcpyAsync(d_vp0.p, d_vp1.p, EK * sizeof(double), hipMemcpyDeviceToDevice, stream));
+    for (size_t i = 0; i < open_edges.size(); ++i) {
+      const int e = open_edges[i];
+      const int el = mesh.be_elem[e], ijp = mesh.be_ijp[e];
+      const double coeff = 2. * (ijp % 2) - 1.;
+      for (int j = 0; j < K; ++j) {
+        const size_t q = ((size_t)el * 4 + ijp) * K + j;
+        const double u = uv[2 * (i * K + j)], v = uv[2 * (i * K + j) + 1];
+        double nx, ny;
+        if (ijp / 2 == 1) { nx = -geo.fyxi[q]; ny = geo.fxxi[q]; }
+        else { nx = geo.fyeta[q]; ny = -geo.fxeta[q]; }
+        nx *= coeff / geo.fAn[q];
+        ny *= coeff / geo.fAn[q];
+        h_vn1[(size_t)e * K + j] = nx * u + ny * v;
+        h_vp1[(size_t)e * K + j] = nx * v - ny * u;
+      }
+    }
+    h2d(d_vn1.p, h_vn1.data(), EK * sizeof(double));
+    h2d(d_vp1.p, h_vp1.data(), EK * sizeof(double));
+  }
+  // setopenvort_d (src/HO_2D.cpp:4120-4141)
+  void set_open_vorticity(const double* wv) {
+    const size_t EK = std::max<size_t>((size_t)E * K, 1);
+    HIPCHK(hipMemcpyAsync(d_vw0.p, d_vw1.p, EK * sizeof(double), hipMemcpyDeviceToDevice, stream));
+    for (size_t i = 0; i < open_edges.size(); ++i)
+      for (int j = 0; j < K; ++j) h_vw1[(size_t)open_edges[i] * K + j] = wv[K * i + j];
+    h2d(d_vw1.p, h_vw1.data(), EK * sizeof(double));
+  }
+  // setsolnvort_d (src/HO_2D.cpp:4144-4166)
+  void set_vorticity_target(const double* wv) {
+    HIPCHK(hipMemcpyAsync(d_hw0.p, d_hw1.p, D * sizeof(double), hipMemcpyDeviceToDevice, stream));
+    h2d(d_hw1.p, wv, D * sizeof(double));
   }
 
   // Ensemble tables (instance of every element and boundary edge, per-instance Re and dt).
@@ -804,7 +898,7 @@ extern "C" {
 void hoc_default_config(hoc_config* c) {
   std::memset(c, 0, sizeof(*c));
   c->K = 3;                      // set_defaults (src/HO_2D.cpp:3699)
-  c->problem_type = 1;
+  c->problem_type = 11;             // :3702 (the hybrid driver's default)
   c->Re = 1.0;
   c->dt = 1.0;
   c->time_integration_type = 2;  // :3703
@@ -827,7 +921,8 @@ int hoc_create(hoc_t* out, const hoc_config* cfg) {
   if (cfg->time_integration_type != 1 && cfg->time_integration_type != 2 && cfg->time_integration_type != 4)
     return hoc_fail(nullptr, HOC_EINVAL, "time_integration_type must be 1, 2 or 4");
   if (cfg->advection_type != 1 && cfg->advection_type != 2) return hoc_fail(nullptr, HOC_EINVAL, "advection_type must be 1 or 2");
-  if (cfg->problem_type < 1 || cfg->problem_type > 3) return hoc_fail(nullptr, HOC_EINVAL, "problem_type must be 1, 2 or 3");
+  if ((cfg->problem_type < 1 || cfg->problem_type > 3) && cfg->problem_type != 11)
+    return hoc_fail(nullptr, HOC_EINVAL, "problem_type must be 1, 2, 3 or 11");
   if (cfg->precision != HOC_FP64 && cfg->precision != HOC_MIXED) return hoc_fail(nullptr, HOC_EINVAL, "bad precision");
   if (!(cfg->dt > 0.) || cfg->maxiter < 0 || !(cfg->tol >= 0.)) return hoc_fail(nullptr, HOC_EINVAL, "bad dt/tol/maxiter");
   if (!(cfg->mixed_inner_tol >= 0. && cfg->mixed_inner_tol < 1.)) return hoc_fail(nullptr, HOC_EINVAL, "mixed_inner_tol must be in [0, 1)");
@@ -912,6 +1007,62 @@ int hoc_setup(hoc_t h) {
 
 #define NEED_READY(h) if (!h->s.ready) throw hoc::StatusError(HOC_ESTATE, "call hoc_setup first")
 #define NEED_LEN(len, want) if ((len) != (int64_t)(want)) throw hoc::StatusError(HOC_EINVAL, "length mismatch")
+
+int hoc_set_boundary_values(hoc_t h, int32_t n_bedges, const int32_t* nval, const double* vals) {
+  return guarded(h, [&] {
+    Solver& s = h->s;
+    if (!s.mesh_loaded) throw hoc::StatusError(HOC_ESTATE, "hoc_set_boundary_values before hoc_load_mesh");
+    if (n_bedges != s.mesh.n_bedges || !nval || !vals) throw hoc::StatusError(HOC_EINVAL, "length mismatch");
+    s.edge_nvel_in.assign(n_bedges, 0.);
+    s.edge_has_nvel.assign(n_bedges, 0);
+    for (int e = 0; e < n_bedges; ++e) {  // arrays_to_boundary (src/HO_2D.cpp:3779-3793, 3806-3814)
+      if (nval[e] < 0 || nval[e] > 2) throw hoc::StatusError(HOC_EINVAL, "values per edge must be 0, 1 or 2");
+      if (nval[e] == 1) s.edge_nvel_in[e] = vals[2 * (size_t)e];
+      if (nval[e] == 2) s.edge_nvel_in[e] = vals[2 * (size_t)e + 1];
+      s.edge_has_nvel[e] = nval[e] > 0;
+    }
+    s.ready = false;
+  });
+}
+
+int64_t hoc_open_point_count(hoc_t h) {
+  return (h && h->s.ready && h->s.cfg.problem_type == 11) ? (int64_t)h->s.open_edges.size() * h->s.K : 0;
+}
+
+int hoc_get_open_points(hoc_t h, int64_t len, double* xy) {
+  return guarded(h, [&] {
+    NEED_READY(h); h->s.need_hybrid(); NEED_LEN(len, 2 * hoc_open_point_count(h));
+    h->s.open_points(xy);
+  });
+}
+
+int hoc_set_open_velocities(hoc_t h, int64_t len, const double* uv) {
+  return guarded(h, [&] {
+    NEED_READY(h); h->s.need_hybrid(); NEED_LEN(len, 2 * hoc_open_point_count(h));
+    h->s.set_open_velocities(uv);
+  });
+}
+
+int hoc_set_open_vorticity(hoc_t h, int64_t len, const double* w) {
+  return guarded(h, [&] {
+    NEED_READY(h); h->s.need_hybrid(); NEED_LEN(len, hoc_open_point_count(h));
+    h->s.set_open_vorticity(w);
+  });
+}
+
+int hoc_set_vorticity_target(hoc_t h, int64_t len, const double* w) {
+  return guarded(h, [&] {
+    NEED_READY(h); h->s.need_hybrid(); NEED_LEN(len, h->s.D);
+    h->s.set_vorticity_target(w);
+  });
+}
+
+int hoc_set_p2g_weights(hoc_t h, int64_t len, const double* w) {
+  return guarded(h, [&] {
+    NEED_READY(h); h->s.need_hybrid(); NEED_LEN(len, h->s.D);
+    h->s.h2d(h->s.d_hwgt.p, w, h->s.D * sizeof(double));
+  });
+}
 
 int hoc_get_instance_times(hoc_t h, int32_t n_inst, double* t) {
   return guarded(h, [&] {
@@ -1018,6 +1169,9 @@ int hoc_solveto(hoc_t h, double outer_dt, int32_t nsub, int32_t integ, double Re
     s.cfg.dt = outer_dt / (double)nsub;   // src/HO_2D.cpp:4199
     s.cfg.time_integration_type = integ;  // :4203
     s.re_inv = 1.0 / Re;                  // :4204
+    s.time_start = s.time_end;            // :4208-4211
+    s.time = s.time_start;
+    s.time_end += outer_dt;
     for (int i = 0; i < nsub; ++i) s.rk_step();
     HIPCHK(hipStreamSynchronize(s.stream));
   });

@@ -27,7 +27,9 @@ KRYLOV_AUTO, KRYLOV_BICGSTAB, KRYLOV_PCG = 0, 1, 2
 
 EXPORTED = [
     "hoc_default_config", "hoc_create", "hoc_load_mesh", "hoc_load_gmsh", "hoc_write_vtk", "hoc_setup",
-    "hoc_set_instances", "hoc_get_instance_times", "hoc_set_vorticity", "hoc_get_vorticity",
+    "hoc_set_instances", "hoc_get_instance_times", "hoc_set_boundary_values", "hoc_open_point_count",
+    "hoc_get_open_points", "hoc_set_open_velocities", "hoc_set_open_vorticity", "hoc_set_vorticity_target",
+    "hoc_set_p2g_weights", "hoc_set_vorticity", "hoc_get_vorticity",
     "hoc_get_streamfunction", "hoc_set_streamfunction", "hoc_get_velocity", "hoc_get_solution_points",
     "hoc_get_solution_areas", "hoc_get_time", "hoc_set_time", "hoc_step", "hoc_solveto", "hoc_poisson",
     "hoc_stage_residual", "hoc_apply_laplacian", "hoc_stats_get", "hoc_profile_enable", "hoc_profile_get",
@@ -85,6 +87,12 @@ def lib():
         L.hoc_setup.argtypes = [C.c_void_p]
         L.hoc_load_gmsh.argtypes = [C.c_void_p, C.c_char_p, C.c_int32]
         L.hoc_set_instances.argtypes = [C.c_void_p, C.c_int32, _ip, _dp, _dp]
+        L.hoc_set_boundary_values.argtypes = [C.c_void_p, C.c_int32, _ip, _dp]
+        L.hoc_open_point_count.argtypes = [C.c_void_p]
+        L.hoc_open_point_count.restype = C.c_int64
+        for nm in ("hoc_get_open_points", "hoc_set_open_velocities", "hoc_set_open_vorticity",
+                   "hoc_set_vorticity_target", "hoc_set_p2g_weights"):
+            getattr(L, nm).argtypes = [C.c_void_p, C.c_int64, _dp]
         L.hoc_get_instance_times.argtypes = [C.c_void_p, C.c_int32, _dp]
         L.hoc_write_vtk.argtypes = [C.c_void_p, C.c_char_p]
         for nm in ("hoc_set_vorticity", "hoc_get_vorticity", "hoc_get_streamfunction", "hoc_set_streamfunction",
@@ -180,6 +188,33 @@ class HO2D:
         self._chk(lib().hoc_set_instances(self.h, len(off) - 1, off, np.ascontiguousarray(Re, dtype=np.float64),
                                           np.ascontiguousarray(dt, dtype=np.float64)))
         self.n_inst = len(off) - 1
+
+    # ---- hybrid driver surface (problem_type 11) ----
+    def set_boundary_values(self, nval, vals):
+        """Per boundary edge: nval 0, 1 (normal velocity in vals[e, 0]) or 2 (tangential, normal)."""
+        nv = np.ascontiguousarray(nval, dtype=np.int32)
+        vv = np.ascontiguousarray(np.reshape(vals, (len(nv), 2)), dtype=np.float64).ravel()
+        self._chk(lib().hoc_set_boundary_values(self.h, len(nv), nv, vv))
+
+    def open_points(self):
+        n = lib().hoc_open_point_count(self.h)
+        return self._get("hoc_get_open_points", 2 * n).reshape(-1, 2)
+
+    def _put(self, fn, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float64).ravel()
+        self._chk(getattr(lib(), fn)(self.h, a.size, a))
+
+    def set_open_velocities(self, uv):
+        self._put("hoc_set_open_velocities", uv)
+
+    def set_open_vorticity(self, w):
+        self._put("hoc_set_open_vorticity", w)
+
+    def set_vorticity_target(self, w):
+        self._put("hoc_set_vorticity_target", w)
+
+    def set_p2g_weights(self, w):
+        self._put("hoc_set_p2g_weights", w)
 
     def instance_times(self):
         n = getattr(self, "n_inst", 1)

@@ -40,7 +40,8 @@ enum { HOC_KRYLOV_AUTO = 0, HOC_KRYLOV_BICGSTAB = 1, HOC_KRYLOV_PCG = 2 };
  * (src/HO_2D.cpp:3066-3069). */
 typedef struct {
   int32_t K;                      /* Knod: solution points per direction (P+1), 2..6    */
-  int32_t problem_type;           /* 1 cavity, 2 backward-facing step, 3 cylinder       */
+  int32_t problem_type;           /* 1 cavity, 2 backward-facing step, 3 cylinder,
+                                     11 hybrid (driver-supplied open-boundary values)    */
   double Re;                      /* Reynolds number (<= 1e-3 disables viscosity, :255) */
   double dt;                      /* time step                                           */
   int32_t time_integration_type;  /* 1 Euler, 2 RK2, 4 RK4                               */
@@ -112,11 +113,17 @@ int hoc_load_gmsh(hoc_t h, const char* path, int32_t occ);
 int hoc_set_instances(hoc_t h, int32_t n_inst, const int32_t* elem_offsets, const double* Re, const double* dt);
 int hoc_get_instance_times(hoc_t h, int32_t n_inst, double* t);
 
+/* Values carried by the boundary edges, as arrays_to_boundary receives them with
+ * load_inout_arrays_d (src/HO_2D.cpp:3720-3820, 3882-3896): nval[e] = 0, 1 (normal velocity
+ * in vals[2e]) or 2 (tangential in vals[2e], normal in vals[2e+1]).  Hybrid inlet and outlet
+ * edges need their normal velocity.  Call before hoc_setup. */
+int hoc_set_boundary_values(hoc_t h, int32_t n_bedges, const int32_t* nval, const double* vals);
+
 /* = process_mesh_input (src/HO_2D.cpp:3901-3918): connectivity, bases, metrics,
  * IC/BC (setup_IC_BC_SRC), Poisson operator and hierarchy, device upload. */
 int hoc_setup(hoc_t h);
 
-int hoc_set_vorticity(hoc_t h, int64_t len, const double* w);     /* = setsolnvort_d target */
+int hoc_set_vorticity(hoc_t h, int64_t len, const double* w);     /* the vorticity field itself */
 int hoc_get_vorticity(hoc_t h, int64_t len, double* w);           /* = getallvorts_d        */
 int hoc_get_streamfunction(hoc_t h, int64_t len, double* psi);
 int hoc_set_streamfunction(hoc_t h, int64_t len, const double* psi);
@@ -130,10 +137,23 @@ int hoc_write_vtk(hoc_t h, const char* path);
 double hoc_get_time(hoc_t h);
 int hoc_set_time(hoc_t h, double t);
 
+/* ---- hybrid driver surface (problem_type 11, src/HO_2D.cpp:3994-4233) ----------------
+ * Boundaries named wall / open / inlet / outlet.  Open-boundary arrays run over the
+ * "open" boundary's edges in the order they were loaded, K points per edge.  Each setter
+ * moves the previous "end" values to "start"; every stage blends start and end with the
+ * fraction of the current solveto window (hoc_solveto opens a window [t_end, t_end + outer_dt]). */
+int64_t hoc_open_point_count(hoc_t h);                                  /* = getopenptlen / 2 */
+int hoc_get_open_points(hoc_t h, int64_t len, double* xy);             /* = getopenpts_d, len = 2*count */
+int hoc_set_open_velocities(hoc_t h, int64_t len, const double* uv);   /* = setopenvels_d, len = 2*count */
+int hoc_set_open_vorticity(hoc_t h, int64_t len, const double* w);     /* = setopenvort_d, len = count */
+int hoc_set_vorticity_target(hoc_t h, int64_t len, const double* w);   /* = setsolnvort_d, len = dofs */
+int hoc_set_p2g_weights(hoc_t h, int64_t len, const double* w);        /* = setptogweights_d, len = dofs */
+
 /* nsteps x solve_advection_diffusion (src/HO_2D.cpp:1465-1531), time += dt each. */
 int hoc_step(hoc_t h, int32_t nsteps);
 
-/* = solveto_d (src/HO_2D.cpp:4194-4233): dt = outer_dt/nsub, integ, Re, then nsub steps. */
+/* = solveto_d (src/HO_2D.cpp:4194-4233): dt = outer_dt/nsub, integ, Re; the window
+ * [t_end, t_end + outer_dt] starts at the previous window's end; then nsub steps. */
 int hoc_solveto(hoc_t h, double outer_dt, int32_t nsub, int32_t integ, double Re);
 
 /* Operator-level entry points for parity tests. */

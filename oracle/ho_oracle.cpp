@@ -638,6 +638,14 @@ struct ho_oracle {
   std::vector<double> BC_parl_vel, BC_normal_vel, BC_Poisson, BC_advection, BC_diffusion, BC_vorticity, velocity_jump;
   std::vector<double> boundary_source;  // [eb][K2][K]
 
+  // hybrid operation (problem_type 11; src/HO_2D.cpp:893-1134, 2104-2191, 3694-4278)
+  std::vector<double> edge_normvel, edge_parlvel;  // [N_eb] values passed with the boundary
+  std::vector<int> edge_nval;                      // 0, 1 (normal) or 2 (tangential, normal)
+  std::vector<double> VelNorm_start, VelNorm_end, VelParl_start, VelParl_end, BVort_start, BVort_end;  // [N_eb][K]
+  std::vector<double> Vort_start, Vort_end, Vort_wgt;  // [D]
+  double time_start = 0., time_end = 0.;
+  int ti = 0;
+
   // Poisson
   CSR A;                  // the reference's matrix (src/HO_2D.cpp:3011-3063)
   CSR As;                 // W A, rows scaled by the Gauss weights w_i w_j (solve_scaled)
@@ -1018,8 +1026,10 @@ struct ho_oracle {
           }
         }
       }
+    } else if (problem_type == 11) {
+      setup_hybrid_BCs();
     } else {
-      throw std::runtime_error("problem_type unsupported (1 cavity, 2 BFS, 3 cylinder)");
+      throw std::runtime_error("problem_type unsupported (1 cavity, 2 BFS, 3 cylinder, 11 hybrid)");
     }
     // src/HO_2D.cpp:1227-1237
     for (int eb = 0; eb < N_eb; ++eb) {
@@ -1282,6 +1292,188 @@ struct ho_oracle {
   }
 
   // --- src/HO_2D.cpp:1979-2217 (problem types 1, 2, 3) ---------------------------------------
+  // --- src/HO_2D.cpp:893-1134: boundary switches by name, then psi on inlet and wall edges
+  // marched along the closed boundary loop (requires Lnod = 2, as the reference asserts) ---
+  void setup_hybrid_BCs() {
+    const int D = N_el * K2();
+    VelNorm_start.assign(N_eb * K, 0.); VelNorm_end.assign(N_eb * K, 0.);
+    VelParl_start.assign(N_eb * K, 0.); VelParl_end.assign(N_eb * K, 0.);
+    BVort_start.assign(N_eb * K, 0.); BVort_end.assign(N_eb * K, 0.);
+    Vort_start.assign(D, 0.); Vort_end.assign(D, 0.); Vort_wgt.assign(D, 0.);
+    if (edge_nval.empty()) { edge_nval.assign(N_eb, 0); edge_normvel.assign(N_eb, 0.); edge_parlvel.assign(N_eb, 0.); }
+    bool have_inlets = false;
+    for (int b = 0; b < N_Gb; ++b) {
+      const std::string& nm = bnames[b];
+      if (nm == "open") {
+        BC_no_slip[b] = 0;
+        for (int e : bedges[b]) {
+          BC_switch_Poisson[e] = NEUMANN; BC_switch_diffusion[e] = DIRICHLET;
+          for (int k = 0; k < K; ++k) BC_Poisson[b2(e, k)] = 0.;
+        }
+      } else if (nm == "wall") {
+        BC_no_slip[b] = 1;
+        for (int e : bedges[b]) {
+          BC_switch_Poisson[e] = DIRICHLET; BC_switch_diffusion[e] = NEUMANN;
+          for (int k = 0; k < K; ++k) { BC_normal_vel[b2(e, k)] = 0.; BC_parl_vel[b2(e, k)] = 0.; }
+        }
+      } else if (nm == "inlet") {
+        have_inlets = true;
+        BC_no_slip[b] = 0;
+        for (int e : bedges[b]) { BC_switch_Poisson[e] = DIRICHLET; BC_switch_diffusion[e] = DIRICHLET; }
+      } else if (nm == "outlet") {
+        BC_no_slip[b] = 0;
+        for (int e : bedges[b]) {
+          BC_switch_Poisson[e] = NEUMANN; BC_switch_diffusion[e] = NEUMANN;
+          for (int k = 0; k < K; ++k) { BC_Poisson[b2(e, k)] = 0.; BC_diffusion[b2(e, k)] = 0.; }
+        }
+      } else {
+        throw std::runtime_error("hybrid boundaries must be named wall, open, inlet or outlet");
+      }
+    }
+    if (!have_inlets) return;
+    if (Lnod != 2) throw std::runtime_error("hybrid inlets need Lnod = 2 (src/HO_2D.cpp:1049)");
+    std::vector<int> bdry_of(N_eb, -1);
+    for (int b = 0; b < N_Gb; ++b) for (int e : bedges[b]) bdry_of[e] = b;
+    auto is = [&](int e, const char* nm) { return bnames[bdry_of[e]] == nm; };
+    std::vector<int> firstnode(N_nodes, 0);
+    for (int b = 0; b < N_Gb; ++b)
+      if (bnames[b] != "open")
+        for (int e : bedges[b]) firstnode[edges[e][0]] = e;
+    size_t istart = 0;
+    while (is((int)istart, "inlet") || is((int)istart, "outlet")) ++istart;
+    std::vector<int> ordered{(int)istart};
+    for (;;) {
+      const int next = firstnode[edges[ordered.back()][1]];
+      if (next == ordered[0]) break;
+      ordered.push_back(next);
+      if (ordered.size() > (size_t)N_eb) throw std::runtime_error("hybrid boundary edges do not form a loop");
+    }
+    std::vector<double> len(ordered.size());
+    for (size_t q = 0; q < ordered.size(); ++q) {
+      const auto& nd = edges[ordered[q]];
+      len[q] = std::hypot(nx[nd[tensor2FEM(0)]] - nx[nd[tensor2FEM(1)]], ny[nd[tensor2FEM(0)]] - ny[nd[tensor2FEM(1)]]);
+    }
+    const size_t M = ordered.size();
+    double psi_end = 0.;
+    for (size_t q = 0; q < M; ++q) {
+      const int e = ordered[q];
+      const double psi_start = psi_end;
+      if (is(e, "inlet")) {
+        if (edge_nval[e] < 1) throw std::runtime_error("inlet edge without a normal velocity");
+        const double vn = edge_normvel[e];
+        psi_end -= len[q] * vn;
+        const int qp = q == 0 ? (int)M - 1 : (int)q - 1, qn = q == M - 1 ? 0 : (int)q + 1;
+        const double vp = is(ordered[qp], "inlet") ? edge_normvel[ordered[qp]] : 0.;
+        const double vnx = is(ordered[qn], "inlet") ? edge_normvel[ordered[qn]] : 0.;
+        const double w = (vnx - vn) / (len[qn] + len[q]) + (vn - vp) / (len[q] + len[qp]);
+        for (int k = 0; k < K; ++k) {
+          BC_Poisson[b2(e, k)] = gsb[0 * K + k] * psi_start + gsb[1 * K + k] * psi_end;
+          BC_diffusion[b2(e, k)] = w;
+          BC_normal_vel[b2(e, k)] = -vn;
+        }
+      } else if (is(e, "outlet")) {
+        if (edge_nval[e] < 1) throw std::runtime_error("outlet edge without a normal velocity");
+        psi_end -= len[q] * edge_normvel[e];
+      } else if (is(e, "wall")) {
+        for (int k = 0; k < K; ++k) BC_Poisson[b2(e, k)] = psi_end;
+      }
+    }
+  }
+
+  // --- src/HO_2D.cpp:2104-2191: blend the hybrid values across the outer step -------------
+  void update_hybrid_BCs(double this_time) {
+    const double fac1 = (this_time - time_start) / (time_end - time_start);
+    const double fac2 = 1.0 - fac1;
+    for (int b = 0; b < N_Gb; ++b) {
+      const std::string& nm = bnames[b];
+      const auto& eb = bedges[b];
+      for (size_t i = 0; i < eb.size(); ++i) {
+        const int e = eb[i];
+        for (int j = 0; j < K; ++j) {
+          if (nm == "open") {
+            BC_normal_vel[b2(e, j)] = -(fac1 * VelNorm_start[i * K + j] + fac2 * VelNorm_end[i * K + j]);
+            BC_parl_vel[b2(e, j)] = -(fac1 * VelParl_start[i * K + j] + fac2 * VelParl_end[i * K + j]);
+            BC_Poisson[b2(e, j)] = BC_parl_vel[b2(e, j)];
+            BC_diffusion[b2(e, j)] = fac1 * BVort_start[i * K + j] + fac2 * BVort_end[i * K + j];
+          } else if (nm == "wall") {
+            BC_normal_vel[b2(e, j)] = 0.; BC_parl_vel[b2(e, j)] = 0.;
+          } else if (nm == "inlet") {
+            BC_normal_vel[b2(e, j)] = -edge_normvel[e];
+          }
+        }
+      }
+    }
+    const int D = N_el * K2();
+    for (int v = 0; v < D; ++v)
+      vorticity[v] = (1.0 - Vort_wgt[v]) * vorticity[v] + Vort_wgt[v] * (fac1 * Vort_start[v] + fac2 * Vort_end[v]);
+  }
+
+  // --- src/HO_2D.cpp:3994-4151: the hybrid driver's setters -------------------------------
+  int open_boundary() const {
+    for (int b = 0; b < N_Gb; ++b) if (bnames[b] == "open") return b;
+    return -1;
+  }
+  void open_points(double* xy) const {  // getopenpts_d (:4007-4046)
+    const int b = open_boundary();
+    if (b < 0) return;
+    for (size_t i = 0; i < bedges[b].size(); ++i) {
+      const auto& nd = edges[bedges[b][i]];
+      for (int j = 0; j < K; ++j) {
+        double x = 0., y = 0.;
+        for (int l = 0; l < Lnod; ++l) {
+          x += gsb[l * K + j] * nx[nd[tensor2FEM(l)]];
+          y += gsb[l * K + j] * ny[nd[tensor2FEM(l)]];
+        }
+        xy[2 * (i * K + j)] = x; xy[2 * (i * K + j) + 1] = y;
+      }
+    }
+  }
+  void set_open_vels(const double* xyvel) {  // setopenvels_d (:4050-4117)
+    VelNorm_start = VelNorm_end; VelParl_start = VelParl_end;
+    const int b = open_boundary();
+    if (b < 0) return;
+    for (size_t i = 0; i < bedges[b].size(); ++i) {
+      const int e = bedges[b][i];
+      const int el = be_elem[e], ijp = f2i[be_side[e]];
+      const int t = ijp % 2, d = ijp / 2;
+      const double coeff = 2. * t - 1.;
+      for (int j = 0; j < K; ++j) {
+        const double u = xyvel[2 * (i * K + j)], v = xyvel[2 * (i * K + j) + 1];
+        double nxv, nyv;
+        if (d == 1) { nxv = -f_dydxi[f3(el, ijp, j)]; nyv = f_dxdxi[f3(el, ijp, j)]; }
+        else { nxv = f_dydeta[f3(el, ijp, j)]; nyv = -f_dxdeta[f3(el, ijp, j)]; }
+        nxv *= coeff / face_An[f3(el, ijp, j)];
+        nyv *= coeff / face_An[f3(el, ijp, j)];
+        VelNorm_end[i * K + j] = nxv * u + nyv * v;
+        VelParl_end[i * K + j] = nxv * v - nyv * u;
+      }
+    }
+  }
+  void set_open_vort(const double* w) {  // setopenvort_d (:4120-4141)
+    const int b = open_boundary();
+    if (b < 0) return;
+    for (size_t i = 0; i < bedges[b].size(); ++i)
+      for (int j = 0; j < K; ++j) { BVort_start[i * K + j] = BVort_end[i * K + j]; BVort_end[i * K + j] = w[K * i + j]; }
+  }
+  void set_soln_vort(const double* w) {  // setsolnvort_d (:4144-4166)
+    Vort_start = Vort_end;
+    Vort_end.assign(w, w + N_el * K2());
+  }
+  void set_p2g_weights(const double* w) { Vort_wgt.assign(w, w + N_el * K2()); }  // setptogweights_d (:4170-4184)
+  void solveto(double outer_dt, int nstep, int integ, double Re) {  // solveto_d (:4191-4233)
+    dt = outer_dt / (double)nstep;
+    time_integration_type = integ;
+    Reyn_inv = 1.0 / Re;
+    time_start = time_end;
+    current_time = time_start;
+    time_end += outer_dt;
+    for (int it = 0; it < nstep; ++it) {
+      solve_advection_diffusion();
+      ++ti;
+      current_time += dt;
+    }
+  }
+
   void update_BCs(double this_time) {
     if (problem_type == 1) {
       // no time dependence (is_oscillating = false at :1985)
@@ -1324,6 +1516,8 @@ struct ho_oracle {
           for (int e : bedges[b]) for (int k = 0; k < K; ++k) BC_Poisson[b2(e, k)] = time_coeff * 0.5 * (y_max - y_min);
         }
       }
+    } else if (problem_type == 11) {
+      update_hybrid_BCs(this_time);
     }
     for (int eb = 0; eb < N_eb; ++eb) {
       const int ijp = f2i[be_side[eb]];
@@ -1898,8 +2092,25 @@ ho_oracle* ho_oracle_create(int K, int Lnod, int n_nodes, const double* xy, int 
                             int n_bedges, const int32_t* bedge_nodes, const int32_t* bedge_bdry, int n_bdry,
                             const char* bdry_names, int problem_type, double Re, double dt,
                             int time_integration_type, int advection_type, double tol, int maxiter) {
+  return ho_oracle_create_ex(K, Lnod, n_nodes, xy, n_elem, elem_nodes, n_bedges, bedge_nodes, bedge_bdry, n_bdry,
+                             bdry_names, problem_type, Re, dt, time_integration_type, advection_type, tol, maxiter,
+                             nullptr, nullptr);
+}
+
+ho_oracle* ho_oracle_create_ex(int K, int Lnod, int n_nodes, const double* xy, int n_elem, const int32_t* elem_nodes,
+                               int n_bedges, const int32_t* bedge_nodes, const int32_t* bedge_bdry, int n_bdry,
+                               const char* bdry_names, int problem_type, double Re, double dt,
+                               int time_integration_type, int advection_type, double tol, int maxiter,
+                               const int32_t* edge_nval, const double* edge_vals) {
   ho_oracle* h = new ho_oracle();
   try {
+    h->edge_nval.assign(n_bedges, 0); h->edge_normvel.assign(n_bedges, 0.); h->edge_parlvel.assign(n_bedges, 0.);
+    if (edge_nval && edge_vals)
+      for (int e = 0; e < n_bedges; ++e) {  // arrays_to_boundary (src/HO_2D.cpp:3779-3793)
+        h->edge_nval[e] = edge_nval[e];
+        if (edge_nval[e] == 1) h->edge_normvel[e] = edge_vals[2 * e];
+        else if (edge_nval[e] == 2) { h->edge_parlvel[e] = edge_vals[2 * e]; h->edge_normvel[e] = edge_vals[2 * e + 1]; }
+      }
     if (K < 1 || K > 8) throw std::runtime_error("K must be in 1..8");
     if (Lnod < 2 || Lnod > 4) throw std::runtime_error("Lnod must be in 2..4");
     h->K = K; h->Lnod = Lnod; h->Lnod_in = Lnod - 1;
@@ -1947,6 +2158,33 @@ ho_oracle* ho_oracle_create(int K, int Lnod, int n_nodes, const double* xy, int 
 
 void ho_oracle_destroy(ho_oracle* h) { delete h; }
 const char* ho_oracle_last_error(ho_oracle* h) { return h ? h->err.c_str() : g_last_error.c_str(); }
+
+int ho_oracle_open_points(ho_oracle* h, double* xy) {
+  const int b = h->open_boundary();
+  const int n = b < 0 ? 0 : (int)h->bedges[b].size() * h->K;
+  if (xy) h->open_points(xy);
+  return n;
+}
+int ho_oracle_set_open_vels(ho_oracle* h, const double* xyvel) {
+  try { h->set_open_vels(xyvel); } catch (const std::exception& e) { return fail(h, e); }
+  return 0;
+}
+int ho_oracle_set_open_vort(ho_oracle* h, const double* w) {
+  try { h->set_open_vort(w); } catch (const std::exception& e) { return fail(h, e); }
+  return 0;
+}
+int ho_oracle_set_soln_vort(ho_oracle* h, const double* w) {
+  try { h->set_soln_vort(w); } catch (const std::exception& e) { return fail(h, e); }
+  return 0;
+}
+int ho_oracle_set_p2g_weights(ho_oracle* h, const double* w) {
+  try { h->set_p2g_weights(w); } catch (const std::exception& e) { return fail(h, e); }
+  return 0;
+}
+int ho_oracle_solveto(ho_oracle* h, double outer_dt, int nstep, int integ, double Re) {
+  try { h->solveto(outer_dt, nstep, integ, Re); } catch (const std::exception& e) { return fail(h, e); }
+  return 0;
+}
 
 int ho_oracle_set_threads(int n) {
   const int prev = omp_get_max_threads();

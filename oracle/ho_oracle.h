@@ -5,7 +5,7 @@
  * smoke() entry of __graft_entry__.py and bench.py's cpu_baseline leg may
  * load it.  The product path (ho-cxx_amd/) never links or calls it.
  *
- * It restates, in plain single-threaded C++, the reference solver's
+ * It restates, in plain C++ (OpenMP only in the Krylov/AMG linear algebra), the reference solver's
  * arithmetic (reference: src/HO_2D.cpp, src/Mesh.cpp, src/MathHelpers.hpp)
  * and the published algorithm of the third-party Poisson solver the
  * reference calls (AMGCL 1.4.0: smoothed aggregation + spai0 + BiCGSTAB,
@@ -39,7 +39,28 @@ ho_oracle* ho_oracle_create(int K, int Lnod, int n_nodes, const double* xy,
                             const char* bdry_names, int problem_type,
                             double Re, double dt, int time_integration_type,
                             int advection_type, double tol, int maxiter);
+/* As ho_oracle_create, with the values a boundary carries (arrays_to_boundary,
+ * src/HO_2D.cpp:3720-3820): edge_nval[e] = 0, 1 (normal velocity in edge_vals[2e]) or 2
+ * (tangential in edge_vals[2e], normal in edge_vals[2e+1]).  Needed by problem_type 11
+ * (hybrid: boundaries named wall, open, inlet, outlet). */
+ho_oracle* ho_oracle_create_ex(int K, int Lnod, int n_nodes, const double* xy,
+                               int n_elem, const int32_t* elem_nodes,
+                               int n_bedges, const int32_t* bedge_nodes,
+                               const int32_t* bedge_bdry, int n_bdry,
+                               const char* bdry_names, int problem_type,
+                               double Re, double dt, int time_integration_type,
+                               int advection_type, double tol, int maxiter,
+                               const int32_t* edge_nval, const double* edge_vals);
 void ho_oracle_destroy(ho_oracle*);
+
+/* Hybrid driver surface (src/HO_2D.cpp:3994-4233), problem_type 11.  Open-boundary arrays
+ * follow the order of the "open" boundary's edges. */
+int ho_oracle_open_points(ho_oracle*, double* xy);          /* returns N_open*K; xy may be NULL */
+int ho_oracle_set_open_vels(ho_oracle*, const double* xyvel);   /* setopenvels_d  */
+int ho_oracle_set_open_vort(ho_oracle*, const double* w);       /* setopenvort_d  */
+int ho_oracle_set_soln_vort(ho_oracle*, const double* w);       /* setsolnvort_d  */
+int ho_oracle_set_p2g_weights(ho_oracle*, const double* w);     /* setptogweights_d */
+int ho_oracle_solveto(ho_oracle*, double outer_dt, int nstep, int integ, double Re);  /* solveto_d */
 const char* ho_oracle_last_error(ho_oracle*);
 /* Threads of the OpenMP-parallel linear algebra (SpMV, vector updates, dot products of
  * the AMG-preconditioned Krylov solve); returns the previous count.  Thread count only

@@ -52,6 +52,13 @@ def lib():
         L.ho_oracle_apply_laplacian_bc.argtypes = [C.c_void_p, _dp, _dp, _dp]
         L.ho_oracle_solve_poisson_bc.argtypes = [C.c_void_p, _dp, _dp]
         L.ho_oracle_velocity_from_psi.argtypes = [C.c_void_p, _dp]
+        L.ho_oracle_create_ex.restype = C.c_void_p
+        L.ho_oracle_create_ex.argtypes = L.ho_oracle_create.argtypes + [_ip, _dp]
+        L.ho_oracle_open_points.argtypes = [C.c_void_p, C.c_void_p]
+        for nm in ("ho_oracle_set_open_vels", "ho_oracle_set_open_vort", "ho_oracle_set_soln_vort",
+                   "ho_oracle_set_p2g_weights"):
+            getattr(L, nm).argtypes = [C.c_void_p, _dp]
+        L.ho_oracle_solveto.argtypes = [C.c_void_p, C.c_double, C.c_int, C.c_int, C.c_double]
         L.ho_oracle_set_threads.argtypes = [C.c_int]
         L.ho_oracle_set_threads.restype = C.c_int
         L.ho_oracle_set_threads(1)   # deterministic single-thread checker unless asked otherwise
@@ -69,16 +76,24 @@ class Oracle:
 
     def __init__(self, mesh, K: int, problem_type: int = 1, Re: float = 100.0, dt: float = 1e-3,
                  time_integration_type: int = 4, advection_type: int = 1, tol: float = 1e-15,
-                 maxiter: int = 150):
+                 maxiter: int = 150, edge_nval=None, edge_vals=None):
+        """edge_nval / edge_vals: values carried by the boundary edges (hybrid inlets and
+        outlets): per edge 0, 1 (normal velocity) or 2 (tangential, normal) values in
+        edge_vals[e, 0:2]."""
         L = lib()
         self.mesh, self.K = mesh, K
         names = "\n".join(mesh.bnames).encode()
-        self.h = L.ho_oracle_create(K, mesh.lnod, mesh.n_nodes, np.ascontiguousarray(mesh.xy, dtype=np.float64).ravel(),
-                                    mesh.n_elem, np.ascontiguousarray(mesh.elem, dtype=np.int32).ravel(),
-                                    mesh.n_bedges, np.ascontiguousarray(mesh.bedges, dtype=np.int32).ravel(),
-                                    np.ascontiguousarray(mesh.bedge_bdry, dtype=np.int32),
-                                    len(mesh.bnames), names, problem_type, Re, dt, time_integration_type,
-                                    advection_type, tol, maxiter)
+        args = (K, mesh.lnod, mesh.n_nodes, np.ascontiguousarray(mesh.xy, dtype=np.float64).ravel(),
+                mesh.n_elem, np.ascontiguousarray(mesh.elem, dtype=np.int32).ravel(),
+                mesh.n_bedges, np.ascontiguousarray(mesh.bedges, dtype=np.int32).ravel(),
+                np.ascontiguousarray(mesh.bedge_bdry, dtype=np.int32),
+                len(mesh.bnames), names, problem_type, Re, dt, time_integration_type, advection_type, tol, maxiter)
+        if edge_nval is None:
+            self.h = L.ho_oracle_create(*args)
+        else:
+            nv = np.ascontiguousarray(edge_nval, dtype=np.int32)
+            vv = np.ascontiguousarray(np.reshape(edge_vals, (mesh.n_bedges, 2)), dtype=np.float64).ravel()
+            self.h = L.ho_oracle_create_ex(*args, nv, vv)
         if not self.h:
             raise RuntimeError("oracle create failed: " + L.ho_oracle_last_error(None).decode())
         self.D = L.ho_oracle_dofs(self.h)
@@ -92,6 +107,28 @@ class Oracle:
     def _chk(self, rc):
         if rc != 0:
             raise RuntimeError(lib().ho_oracle_last_error(self.h).decode())
+
+    # hybrid driver surface (problem_type 11)
+    def open_points(self):
+        n = lib().ho_oracle_open_points(self.h, None)
+        out = np.zeros(2 * n)
+        lib().ho_oracle_open_points(self.h, out.ctypes.data)
+        return out.reshape(-1, 2)
+
+    def set_open_vels(self, xyvel):
+        self._chk(lib().ho_oracle_set_open_vels(self.h, np.ascontiguousarray(xyvel, dtype=np.float64).ravel()))
+
+    def set_open_vort(self, w):
+        self._chk(lib().ho_oracle_set_open_vort(self.h, np.ascontiguousarray(w, dtype=np.float64)))
+
+    def set_soln_vort(self, w):
+        self._chk(lib().ho_oracle_set_soln_vort(self.h, np.ascontiguousarray(w, dtype=np.float64)))
+
+    def set_p2g_weights(self, w):
+        self._chk(lib().ho_oracle_set_p2g_weights(self.h, np.ascontiguousarray(w, dtype=np.float64)))
+
+    def solveto(self, outer_dt, nstep, integ, Re):
+        self._chk(lib().ho_oracle_solveto(self.h, outer_dt, nstep, integ, Re))
 
     def vorticity(self):
         out = np.zeros(self.D); lib().ho_oracle_get_vorticity(self.h, out); return out

@@ -124,16 +124,23 @@ def test_assembled_equals_matrix_free_neumann():
     assert rel_l2(A @ x, o.laplacian_mf(x)) < 1e-13
 
 
-@pytest.mark.parametrize("name,mk,K", LAP_MESHES[:1] + [("bfs", None, 4)])
+@pytest.mark.parametrize("name,mk,K", LAP_MESHES[:1] + [("bfs", None, 4), ("bfs_rotated", None, 3)])
 def test_boundary_source_is_operator_bc_part(name, mk, K):
     """boundary_source . BC (:1631-1698) is the FR operator applied to a zero field with
-    those boundary values: A x + S g = L(x; g)."""
+    those boundary values, Neumann values taken with the owner face's outward sign
+    2(ijp%2)-1 (the sign of :1685): A x + S g = L(x; s g).  The rotated step puts Neumann
+    faces of both signs on the outlet."""
     if mk is None:
-        o = oracle(meshes.backward_facing_step(20, 8, 2, jitter=0.1, seed=5), K, problem_type=2, Re=400.0)
+        o = oracle(meshes.backward_facing_step(20, 8, 2, jitter=0.1, seed=5, rotate=name.endswith("rotated")), K,
+                   problem_type=2, Re=400.0)
     else:
         o = oracle(mk(), K)
     rng = np.random.default_rng(7)
     g = rng.standard_normal((o.n_bedges, K))
+    neu = o.table("BC_switch_Poisson") > 0.5
+    s_out = np.array([2 * ([2, 1, 3, 0][int(sd)] % 2) - 1 for sd in o.table("be_side")])
+    if name.endswith("rotated"):
+        assert set(s_out[neu]) == {-1, 1}
     x = rng.standard_normal(o.D)
     S = o.boundary_source()
     be = o.table("be_elem").astype(int)
@@ -141,7 +148,9 @@ def test_boundary_source_is_operator_bc_part(name, mk, K):
     np.add.at(Sg, be, np.einsum("eik,ek->ei", S, g))
     ptr, col, val = o.csr()
     A = sp.csr_matrix((val, col, ptr), shape=(o.D, o.D))
-    assert rel_l2(A @ x + Sg.ravel(), o.laplacian_bc(x, g)) < 1e-13
+    gs = g.copy()
+    gs[neu] *= s_out[neu][:, None]
+    assert rel_l2(A @ x + Sg.ravel(), o.laplacian_bc(x, gs)) < 1e-13
 
 
 @pytest.mark.parametrize("K", [3, 4, 5])
