@@ -239,6 +239,56 @@ def backward_facing_step(nx: int = 384, ny: int = 64, lnod: int = 2, *, length: 
     return m
 
 
+def cylinder_ogrid(n_r: int = 256, n_theta: int = 512, lnod: int = 3, *, r_in: float = 0.5, r_out: float = 10.0,
+                   seed: int | None = None, rotate: bool = False) -> QuadMesh:
+    """O-grid of n_r (radial) x n_theta (azimuthal) isoparametric quads around a cylinder of
+    radius r_in centred at the origin, out to radius r_out, with geometric radial stretching.
+    Geometry nodes of every order lie on the exact circles and rays.  Element axes: xi along
+    the radius, eta along the angle, which makes every element counter-clockwise.
+
+    Boundaries as problem_type 3 expects (src/HO_2D.cpp:809-891): "object" (the cylinder),
+    and the outer circle split by angle into "outlet" (-45..45 deg), "top" (45..135),
+    "inlet" (135..225) and "bottom" (225..315)."""
+    rng = np.random.default_rng(seed)
+    p = lnod - 1
+    NT, NR = n_theta * p, n_r * p + 1
+    gid = np.arange(NT * NR, dtype=np.int64).reshape(NT, NR)      # [theta][r], theta periodic
+    th = 2.0 * np.pi * np.arange(NT) / NT
+    rr = r_in * (r_out / r_in) ** (np.arange(NR) / (NR - 1))
+    xy = np.zeros((NT * NR, 2))
+    xy[:, 0] = (np.cos(th)[:, None] * rr[None, :]).ravel()
+    xy[:, 1] = (np.sin(th)[:, None] * rr[None, :]).ravel()
+    t2g = tensor_to_gmsh(lnod)
+    cells = [(ci, cj) for cj in range(n_theta) for ci in range(n_r)]
+    elem = np.zeros((len(cells), lnod * lnod), dtype=np.int32)
+    rot = rng.integers(0, 4, size=len(cells)) if rotate else np.zeros(len(cells), dtype=np.int64)
+    sides = {"object": [], "outer": []}
+    for e, (ci, cj) in enumerate(cells):
+        T = np.array([[gid[(cj * p + j) % NT, ci * p + i] for i in range(lnod)] for j in range(lnod)])  # T[j, i]
+        if ci == 0:           # xi = -1 side: the cylinder, CCW order (eta decreasing)
+            sides["object"].append([T[p - a, 0] for a in range(lnod)])
+        if ci == n_r - 1:     # xi = +1 side: the outer circle, CCW order (eta increasing)
+            sides["outer"].append([T[a, p] for a in range(lnod)])
+        T = np.rot90(T, int(rot[e]))
+        for j in range(lnod):
+            for i in range(lnod):
+                elem[e, t2g[j, i]] = T[j, i]
+    names = ["inlet", "outlet", "top", "bottom", "object"]
+    bed, bid = [], []
+    for nodes in sides["outer"]:
+        mid = xy[nodes[0]] + xy[nodes[-1]]
+        ang = np.degrees(np.arctan2(mid[1], mid[0])) % 360.0
+        nm = "outlet" if (ang < 45 or ang >= 315) else "top" if ang < 135 else "inlet" if ang < 225 else "bottom"
+        bed.append(nodes)
+        bid.append(names.index(nm))
+    for nodes in sides["object"]:
+        bed.append(nodes)
+        bid.append(names.index("object"))
+    m = QuadMesh(lnod, xy, elem, _emit_bedges(bed, lnod), np.array(bid, dtype=np.int32), names)
+    m.validate()
+    return m
+
+
 def read_gmsh41(path: str, occ: int = 0) -> QuadMesh:
     """Read a Gmsh 4.1 ASCII mesh as src/Mesh.cpp:93-293 does.
 
