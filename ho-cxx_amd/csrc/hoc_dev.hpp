@@ -5,6 +5,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -60,14 +61,37 @@ struct DBuf {
   operator T*() const { return p; }
 };
 
+// A sparse level matrix on the device: ELL when its rows are many and of nearly equal
+// length (coalesced thread-per-row loads), CSR otherwise.  The matrix is checked (every
+// column index below its column count A.m) before upload.
 template <class R>
 struct DevCSR {
-  int n = 0, lanes = 1;
+  static constexpr int kEllMinRows = 16384, kEllMaxWidth = 32;
+  int n = 0, lanes = 1, ell_w = 0;
   DBuf<int32_t> ptr, col;
   DBuf<R> val;
   void upload(const CSR& A) {
-    n = A.n; ptr.upload(A.ptr); col.upload(A.col); val.upload_as(A.val);
+    try {
+      csr_check(A);
+    } catch (const std::invalid_argument& e) {
+      throw StatusError(HOC_EINVAL, std::string("level matrix: ") + e.what());
+    }
+    n = A.n;
+    std::vector<int32_t> ec;
+    std::vector<double> evl;
+    if (ell_pack(A, kEllMinRows, kEllMaxWidth, ell_w, ec, evl)) {
+      ptr.alloc(0);
+      col.upload(ec);
+      val.upload_as(evl);
+      return;
+    }
+    ptr.upload(A.ptr); col.upload(A.col); val.upload_as(A.val);
     lanes = csr_lanes(A.n, A.val.size());
+  }
+  // See k_csr for the modes; outputs must not alias x (nor f in mode 4).
+  void apply(hipStream_t s, int mode, const R* x, const R* f, R* y, const R* M, R* xio, const int* F) const {
+    if (ell_w) ell<R>(s, mode, n, ell_w, col.p, val.p, x, f, y, M, xio, F);
+    else csr<R>(s, mode, n, ptr.p, col.p, val.p, x, f, y, M, xio, F, lanes);
   }
 };
 template <class R>
@@ -76,21 +100,21 @@ struct DevLevel {
   DBuf<R> M, f, u, t;
 };
 
-// V(1,1) cycle with spai0 relaxation from a zero guess on a CSR hierarchy (level l and
-// below; the last level is solved with its dense inverse): u_l = V_l(f_l).  All launches
-// are predicated on the active flag Fp (may be null).
+// V(1,1) cycle with spai0 relaxation from a zero guess on a sparse hierarchy (level l and
+// below; the last level is solved with its dense inverse): u_l = V_l(f_l).  Every step is
+// out of place, so the cycle is a fixed linear operator.  All launches are predicated on
+// the active flag Fp (may be null).
 template <class R>
 void csr_vcycle(hipStream_t s, std::vector<std::unique_ptr<DevLevel<R>>>& lv, const R* coarse_inv, int coarse_n,
                 size_t l, const R* f, R* u, const int* Fp) {
   DevLevel<R>& L = *lv[l];
   if (l + 1 == lv.size()) { dense<R>(s, coarse_n, coarse_inv, f, u, Fp); return; }
   DevLevel<R>& C = *lv[l + 1];
-  // pre-relaxation from zero (u = M f) fused with its residual t = f - A u
-  csr<R>(s, 4, L.A.n, L.A.ptr, L.A.col, L.A.val, nullptr, f, L.t, L.M, u, Fp, L.A.lanes);
-  csr<R>(s, 0, C.A.n, L.Rs.ptr, L.Rs.col, L.Rs.val, L.t, nullptr, C.f, nullptr, nullptr, Fp, L.Rs.lanes);
+  L.A.apply(s, 4, nullptr, f, L.t, L.M, u, Fp);           // u = M f, t = f - A u
+  L.Rs.apply(s, 0, L.t, nullptr, C.f, nullptr, nullptr, Fp);
   csr_vcycle(s, lv, coarse_inv, coarse_n, l + 1, C.f.p, C.u.p, Fp);
-  csr<R>(s, 2, L.A.n, L.P.ptr, L.P.col, L.P.val, C.u, nullptr, u, nullptr, nullptr, Fp, L.P.lanes);
-  csr<R>(s, 3, L.A.n, L.A.ptr, L.A.col, L.A.val, u, f, nullptr, L.M, u, Fp, L.A.lanes);
+  L.P.apply(s, 2, C.u, u, L.t, nullptr, nullptr, Fp);      // t = u + P u_c
+  L.A.apply(s, 3, L.t, f, u, L.M, nullptr, Fp);            // u = t + M (f - A t)
 }
 
 }  // namespace hoc

@@ -238,7 +238,7 @@ __device__ __forceinline__ void second_order_core(const TabT<R>& T, const R (&u)
 // ==========================================================================================
 // Trace kernel with fused vector pre-operations producing the traced field u:
 //   PRE_COPY   u = x
-//   PRE_CHEB0  u = r / (diag * theta); writes x = d = u          (Chebyshev step from zero)
+//   PRE_CHEB0  u = r * dinv * coef (coef = 1/theta); writes x = d = u   (Chebyshev step from zero)
 //   PRE_PROL   u = x + Pc xc (bilinear from the corner vertices); writes x = u
 //   PRE_PUPD   u = z + beta * p; writes p = u                     (PCG direction)
 //   PRE_ZERO   u = 0                                              (RHS assembly)
@@ -274,7 +274,7 @@ __global__ __launch_bounds__(kWG) void k_trace(int N, TabT<R> T, R* __restrict__
       const int j = m / K, i = m % K;
       v = lds[li] + (q1w(T, 0, j, i) * xc[0] + q1w(T, 1, j, i) * xc[1] + q1w(T, 2, j, i) * xc[2] + q1w(T, 3, j, i) * xc[3]);
     }
-    if (PRE == PRE_CHEB0) v = lds[li] / (lds2[li] * coef);
+    if (PRE == PRE_CHEB0) v = lds[li] * lds2[li] * coef;
     if (PRE == PRE_PUPD) v = lds[li] + beta * lds2[li];
     u[m] = v;
   }
@@ -295,14 +295,20 @@ __global__ __launch_bounds__(kWG) void k_trace(int N, TabT<R> T, R* __restrict__
 // Poisson operator As = -W * L (W = w_i w_j) applied matrix-free, with fused epilogues:
 //   EPI_STORE     y = As x; partial sums of x.y                    (PCG / probing)
 //   EPI_RESID     y = r - As x; partial sums of y.y and (y/W).(y/W) (initial residual)
-//   EPI_CHEB      res = r - As x; d = c1 d + c2 res/diag; x += d; optional partial r.x
-//   EPI_CHEBF     res = r - As x; d = res/(diag theta); x += d     (first step from x != 0)
+//   EPI_CHEB      res = r - As x; e = c1 d + c2 res dinv; x += e; y = e unless y is null;
+//                 optional partial r.x
+//   EPI_CHEBF     res = r - As x; e = c2 res dinv (c2 = 1/theta); x += e; y = e unless null
+//                 (first step from x != 0)
+//   (dinv = 1/diag(As), the Jacobi scaling of the Chebyshev smoother; d and y are distinct
+//   direction buffers, and the last step of a sweep passes y = null)
 //   EPI_RESTRICT  ec[el][a] = sum_ij q1w(a,ij) (r - As x)_ij        (Pc^T residual, per element)
 // ==========================================================================================
 template <int K, int EPI, bool AFF, class R>
-__global__ __launch_bounds__(kWG) void k_lap(int N, TabT<R> T, GeoT<R> G, R* __restrict__ x, const R* __restrict__ tr,
+// Affine FP32 instances with K <= 5 are held to 128 VGPRs (4 waves per SIMD; their LDS tile
+// allows more), which they fit without spilling; the others keep the compiler's allocation.
+__global__ __launch_bounds__(kWG, (AFF && sizeof(R) == 4 && K <= 5) ? 4 : 1) void k_lap(int N, TabT<R> T, GeoT<R> G, R* __restrict__ x, const R* __restrict__ tr,
                                               R* __restrict__ y, const R* __restrict__ r,
-                                              const R* __restrict__ diag, R* __restrict__ d,
+                                              const R* __restrict__ dinv, R* __restrict__ d,
                                               R c1, R c2, double* __restrict__ part,
                                               const int* __restrict__ active, R* __restrict__ tr_out) {
   if (active && !*active) return;
@@ -388,10 +394,10 @@ __global__ __launch_bounds__(kWG) void k_lap(int N, TabT<R> T, GeoT<R> G, R* __r
         for (int c = 0; c < 4; ++c) y[(size_t)el * 4 + c] = s[c];
     } else {  // Chebyshev updates
       __syncthreads();
-      tile_load<K2>(diag, lds, e0, N);
+      tile_load<K2>(dinv, lds, e0, N);
       R dd[K2];
 #pragma unroll
-      for (int m = 0; m < K2; ++m) dd[m] = res[m] / lds[threadIdx.x * K2 + m];
+      for (int m = 0; m < K2; ++m) dd[m] = res[m] * lds[threadIdx.x * K2 + m];
       if (EPI == EPI_CHEB) {
         __syncthreads();
         tile_load<K2>(d, lds, e0, N);
@@ -401,10 +407,12 @@ __global__ __launch_bounds__(kWG) void k_lap(int N, TabT<R> T, GeoT<R> G, R* __r
 #pragma unroll
         for (int m = 0; m < K2; ++m) dd[m] *= c2;
       }
-      __syncthreads();
+      if (y) {  // uniform: the direction for the next step of the sweep
+        __syncthreads();
 #pragma unroll
-      for (int m = 0; m < K2; ++m) lds[threadIdx.x * K2 + m] = dd[m];
-      tile_store<K2>(d, lds, e0, N);
+        for (int m = 0; m < K2; ++m) lds[threadIdx.x * K2 + m] = dd[m];
+        tile_store<K2>(y, lds, e0, N);
+      }
 #pragma unroll
       for (int m = 0; m < K2; ++m) u[m] += dd[m];
 #pragma unroll

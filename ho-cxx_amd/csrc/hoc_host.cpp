@@ -494,6 +494,35 @@ CSR matmul(const CSR& A, const CSR& B) {
 }
 }  // namespace
 
+void csr_check(const CSR& A) {
+  if (A.n < 0 || A.m < 0 || (int64_t)A.ptr.size() != (int64_t)A.n + 1 || A.ptr[0] != 0 ||
+      (size_t)A.ptr[A.n] != A.col.size() || A.col.size() != A.val.size())
+    throw std::invalid_argument("inconsistent CSR matrix");
+  for (int i = 0; i < A.n; ++i)
+    if (A.ptr[i + 1] < A.ptr[i]) throw std::invalid_argument("CSR row pointers decrease");
+  for (int32_t c : A.col)
+    if (c < 0 || c >= A.m) throw std::invalid_argument("CSR column index out of range");
+}
+
+bool ell_pack(const CSR& A, int min_rows, int max_width, int& w, std::vector<int32_t>& col, std::vector<double>& val) {
+  w = 0;
+  for (int i = 0; i < A.n; ++i) w = std::max(w, A.ptr[i + 1] - A.ptr[i]);
+  const double avg = A.n ? (double)A.val.size() / A.n : 0.;
+  if (A.m <= 0 || A.n < min_rows || w > max_width || w > 1.25 * avg + 1.) { w = 0; return false; }
+  col.assign((size_t)w * A.n, 0);
+  val.assign((size_t)w * A.n, 0.);
+  for (int i = 0; i < A.n; ++i) {
+    const int pad = A.ptr[i + 1] > A.ptr[i] ? A.col[A.ptr[i]] : 0;
+    for (int k = 0; k < w; ++k) {
+      const int j = A.ptr[i] + k;
+      const bool in = j < A.ptr[i + 1];
+      col[(size_t)k * A.n + i] = in ? A.col[j] : pad;
+      val[(size_t)k * A.n + i] = in ? A.val[j] : 0.;
+    }
+  }
+  return true;
+}
+
 void build_sa_amg(const CSR& A0, int coarse_enough, AMGHost& h) {
   h.levels.clear();
   CSR A = A0;

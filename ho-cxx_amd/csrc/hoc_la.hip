@@ -22,11 +22,18 @@ __device__ __forceinline__ void block_sum2(double& a, double& b) {
   }
 }
 
-// Weight w_j w_i of flat dof index v (element-major, K*K per element).
-__device__ __forceinline__ double dof_weight(const VecArgs& A, size_t v) {
-  const int m = (int)(v % (size_t)(A.K * A.K));
-  return A.w[m / A.K] * A.w[m % A.K];
-}
+// Position within its element (j K + i) of a grid-stride loop's flat dof index v
+// (element-major, K*K per element), advanced without a 64-bit division per step; the
+// inverse weight 1/(w_j w_i) of that position is A.winv[m].
+struct DofSlot {
+  int m, step, K2;
+  __device__ DofSlot(const VecArgs& A, size_t v0, size_t stride) {
+    K2 = A.K * A.K;
+    m = (int)(v0 % (size_t)K2);
+    step = (int)(stride % (size_t)K2);
+  }
+  __device__ void next() { m += step; if (m >= K2) m -= K2; }
+};
 
 // Elementwise Krylov updates with fused partial norms.  Scalars come from S.
 template <class R>
@@ -36,14 +43,16 @@ __global__ void k_vec(int op, VecArgs A, size_t n, R* __restrict__ a, R* __restr
   if (F && !F[F_ACTIVE]) return;
   if (op >= VOP_BICG_X && F && F[F_HALF]) return;
   double p0 = 0., p1 = 0.;
-  for (size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (size_t)gridDim.x * blockDim.x) {
+  const size_t v0 = (size_t)blockIdx.x * blockDim.x + threadIdx.x, stride = (size_t)gridDim.x * blockDim.x;
+  DofSlot ds(A, v0, stride);
+  for (size_t v = v0; v < n; v += stride, ds.next()) {
     switch (op) {
       case VOP_PCG_UPD: {  // x += alpha p; r -= alpha q      a=x b=r c=p d=q
         const R al = (R)S[S_ALPHA];
         a[v] += al * c[v];
         const R r = b[v] - al * d[v];
         b[v] = r;
-        const double ru = r / dof_weight(A, v);
+        const double ru = r * A.winv[ds.m];
         p0 += (double)r * r; p1 += ru * ru;
       } break;
       case VOP_BICG_P: {   // p = r + beta (p - omega v)      a=p c=r d=v
@@ -52,14 +61,14 @@ __global__ void k_vec(int op, VecArgs A, size_t n, R* __restrict__ a, R* __restr
       case VOP_BICG_S: {   // s = r - alpha v (in r)          b=r d=v
         const R s = b[v] - (R)S[S_ALPHA] * d[v];
         b[v] = s;
-        const double su = s / dof_weight(A, v);
+        const double su = s * A.winv[ds.m];
         p0 += (double)s * s; p1 += su * su;
       } break;
       case VOP_BICG_X: {   // x += alpha ph + omega sh; r = s - omega t   a=x b=r(s) c=ph d=sh e=t
         a[v] += (R)S[S_ALPHA] * c[v] + (R)S[S_OMEGA] * d[v];
         const R r = b[v] - (R)S[S_OMEGA] * e[v];
         b[v] = r;
-        const double ru = r / dof_weight(A, v);
+        const double ru = r * A.winv[ds.m];
         p0 += (double)r * r; p1 += ru * ru;
       } break;
       case VOP_DOT2: {     // partials of a.c and b.d (read only)
@@ -158,9 +167,23 @@ __global__ void k_finalize(int phase, const double* __restrict__ pa, int np, dou
   }
 }
 
-// ---- CSR levels (one thread per row) ------------------------------------------------------
-// mode 0: y = A x;  1: y = f - A x;  2: y += A x;  3: x += M (f - A x) (y unused);
-//      4: xio = M f and y = f - A (M f) (spai0 step from zero fused with its residual; x unused)
+// ---- Sparse levels -------------------------------------------------------------------------
+// mode 0: y = A x;  1: y = f - A x;  2: y = f + A x;  3: y = x + M (f - A x) (spai0 step);
+//      4: xio = M f and y = f - A (M f) (spai0 step from zero fused with its residual; x unused).
+// Every mode writes only row i of its outputs, and no output may alias an input read at
+// other rows (x, or f in mode 4): the relaxation is a true Jacobi-type step, independent of
+// the order in which rows complete.
+template <class R>
+__device__ __forceinline__ void sparse_out(int mode, int i, R s, const R* __restrict__ x, const R* __restrict__ f,
+                                           R* __restrict__ y, const R* __restrict__ M, R* __restrict__ xio) {
+  if (mode == 0) y[i] = s;
+  else if (mode == 1) y[i] = f[i] - s;
+  else if (mode == 2) y[i] = f[i] + s;
+  else if (mode == 3) y[i] = x[i] + M[i] * (f[i] - s);
+  else { y[i] = f[i] - s; xio[i] = M[i] * f[i]; }
+}
+
+// CSR, one thread per row (short rows).
 template <class R>
 __global__ void k_csr(int mode, int n, const int32_t* __restrict__ ptr, const int32_t* __restrict__ col,
                       const R* __restrict__ val, const R* __restrict__ x, const R* __restrict__ f, R* __restrict__ y,
@@ -173,15 +196,11 @@ __global__ void k_csr(int mode, int n, const int32_t* __restrict__ ptr, const in
     for (int j = ptr[i]; j < ptr[i + 1]; ++j) s += val[j] * (M[col[j]] * f[col[j]]);
   else
     for (int j = ptr[i]; j < ptr[i + 1]; ++j) s += val[j] * x[col[j]];
-  if (mode == 0) y[i] = s;
-  else if (mode == 1) y[i] = f[i] - s;
-  else if (mode == 2) y[i] += s;
-  else if (mode == 3) xio[i] = xio[i] + M[i] * (f[i] - s);
-  else { y[i] = f[i] - s; xio[i] = M[i] * f[i]; }
+  sparse_out(mode, i, s, x, f, y, M, xio);
 }
 
-// Same operations with LPR lanes per row (rows of many nonzeros): the lanes of a group
-// stride over the row together, so a wavefront's loads are contiguous.
+// CSR with LPR lanes per row (rows of many nonzeros and uneven length): the lanes of a
+// group stride over the row together, so a wavefront's loads are contiguous.
 template <int LPR, class R>
 __global__ void k_csr_vec(int mode, int n, const int32_t* __restrict__ ptr, const int32_t* __restrict__ col,
                           const R* __restrict__ val, const R* __restrict__ x, const R* __restrict__ f,
@@ -199,11 +218,35 @@ __global__ void k_csr_vec(int mode, int n, const int32_t* __restrict__ ptr, cons
 #pragma unroll
   for (int off = LPR / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, LPR);
   if (i >= n || lane != 0) return;
-  if (mode == 0) y[i] = s;
-  else if (mode == 1) y[i] = f[i] - s;
-  else if (mode == 2) y[i] += s;
-  else if (mode == 3) xio[i] = xio[i] + M[i] * (f[i] - s);
-  else { y[i] = f[i] - s; xio[i] = M[i] * f[i]; }
+  sparse_out(mode, i, s, x, f, y, M, xio);
+}
+
+// ELL (rows of nearly equal length, e.g. the Q1 vertex operator): w slots per row stored
+// slot-major, col[k n + i], so consecutive threads (rows) read consecutive words.  Padding
+// slots hold a column of the same row and value 0.
+template <class R>
+__global__ void k_ell(int mode, int n, int w, const int32_t* __restrict__ col, const R* __restrict__ val,
+                      const R* __restrict__ x, const R* __restrict__ f, R* __restrict__ y, const R* __restrict__ M,
+                      R* __restrict__ xio, const int* __restrict__ F) {
+  if (F && !F[F_ACTIVE]) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  R s = 0;
+  if (mode == 4) {
+#pragma unroll 4
+    for (int k = 0; k < w; ++k) {
+      const size_t q = (size_t)k * n + i;
+      const int c = col[q];
+      s += val[q] * (M[c] * f[c]);
+    }
+  } else {
+#pragma unroll 4
+    for (int k = 0; k < w; ++k) {
+      const size_t q = (size_t)k * n + i;
+      s += val[q] * x[col[q]];
+    }
+  }
+  sparse_out(mode, i, s, x, f, y, M, xio);
 }
 
 // x = M f (spai0 from a zero guess)
@@ -280,11 +323,13 @@ __global__ void k_mixed_demote(VecArgs A, size_t n, const double* __restrict__ r
                                float* __restrict__ e32, double* __restrict__ part, const int* __restrict__ F) {
   if (F && !F[F_ACTIVE]) return;
   double p0 = 0.;
-  for (size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (size_t)gridDim.x * blockDim.x) {
+  const size_t v0 = (size_t)blockIdx.x * blockDim.x + threadIdx.x, stride = (size_t)gridDim.x * blockDim.x;
+  DofSlot ds(A, v0, stride);
+  for (size_t v = v0; v < n; v += stride, ds.next()) {
     const double r = r64[v];
     r32[v] = (float)r;
     e32[v] = 0.f;
-    const double ru = r / dof_weight(A, v);
+    const double ru = r * A.winv[ds.m];
     p0 += ru * ru;
   }
   double p1 = p0;
@@ -304,8 +349,9 @@ __global__ void k_inst_norms(VecArgs A, const int32_t* __restrict__ off, int K2,
                              double* __restrict__ out) {
   const size_t v0 = (size_t)off[blockIdx.x] * K2, v1 = (size_t)off[blockIdx.x + 1] * K2;
   double p0 = 0.;
-  for (size_t v = v0 + threadIdx.x; v < v1; v += blockDim.x) {
-    const double u = b[v] / dof_weight(A, v);
+  DofSlot ds(A, v0 + threadIdx.x, blockDim.x);
+  for (size_t v = v0 + threadIdx.x; v < v1; v += blockDim.x, ds.next()) {
+    const double u = b[v] * A.winv[ds.m];
     p0 += u * u;
   }
   double p1 = 0.;
@@ -352,6 +398,11 @@ void csr(hipStream_t s, int mode, int n, const int32_t* ptr, const int32_t* col,
     default: hipLaunchKernelGGL((k_csr_vec<16, R>), grid, dim3(256), 0, s, mode, n, ptr, col, val, x, f, y, M, xio, F); break;
   }
 }
+template <class R>
+void ell(hipStream_t s, int mode, int n, int w, const int32_t* col, const R* val, const R* x, const R* f, R* y,
+         const R* M, R* xio, const int* F) {
+  hipLaunchKernelGGL(k_ell<R>, dim3((n + 255) / 256), dim3(256), 0, s, mode, n, w, col, val, x, f, y, M, xio, F);
+}
 int csr_lanes(int n, size_t nnz) {
   const double avg = n ? (double)nnz / n : 0.;
   return avg >= 24 ? 16 : avg >= 12 ? 8 : avg >= 6 ? 4 : avg >= 3 ? 2 : 1;
@@ -396,6 +447,8 @@ void q1_probe_accum(hipStream_t s, int N, int K, const Tab& T, int rs, const dou
   template void zero_if_flag<R>(hipStream_t, size_t, R*, const int*);                                                 \
   template void csr<R>(hipStream_t, int, int, const int32_t*, const int32_t*, const R*, const R*, const R*, R*,        \
                        const R*, R*, const int*, int);                                                                 \
+  template void ell<R>(hipStream_t, int, int, int, const int32_t*, const R*, const R*, const R*, R*, const R*, R*,   \
+                       const int*);                                                                            \
   template void scale<R>(hipStream_t, int, const R*, const R*, R*, const int*);                                       \
   template void dense<R>(hipStream_t, int, const R*, const R*, R*, const int*);                                       \
   template void q1_gather<R>(hipStream_t, int, const int32_t*, const int32_t*, const R*, R*, const int*);

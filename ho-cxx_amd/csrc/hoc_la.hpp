@@ -18,9 +18,11 @@ enum { F_ACTIVE = 0, F_ITER, F_ZERO, F_HALF, F_COUNT = 8 };
 enum { PH_BB = 0, PH_INIT, PH_RZ, PH_PQ, PH_UPD, PH_RHO, PH_ALPHA, PH_SNORM, PH_OMEGA, PH_RNORM };
 enum { VOP_PCG_UPD = 0, VOP_BICG_P, VOP_BICG_S, VOP_DOT2, VOP_COPY, VOP_BICG_X };
 
+// K: points per direction of an element (1 for a plain vector); winv: device table of the
+// K*K inverse quadrature weights 1/(w_j w_i), indexed j K + i.
 struct VecArgs {
   int K;
-  double w[8];
+  const double* winv;
 };
 
 int vec_grid(size_t n);
@@ -36,6 +38,9 @@ template <class R>
 void csr(hipStream_t s, int mode, int n, const int32_t* ptr, const int32_t* col, const R* val, const R* x, const R* f,
          R* y, const R* M, R* xio, const int* F, int lanes_per_row);
 int csr_lanes(int n, size_t nnz);  // lanes per row suited to a matrix's average row length
+template <class R>
+void ell(hipStream_t s, int mode, int n, int w, const int32_t* col, const R* val, const R* x, const R* f, R* y,
+         const R* M, R* xio, const int* F);
 template <class R>
 void scale(hipStream_t s, int n, const R* M, const R* f, R* x, const int* F);
 template <class R>

@@ -33,7 +33,7 @@ struct LinSolver {
   int n = 0;
   bool built = false;
   std::vector<std::unique_ptr<DevLevel<double>>> lv;
-  DBuf<double> coarse_inv;
+  DBuf<double> coarse_inv, d_winv;
   int coarse_n = 0;
   DBuf<double> x, f, r, rh, p, v, ph, sh, t, part, S;
   DBuf<int> F;
@@ -87,8 +87,9 @@ struct LinSolver {
     S.alloc(S_COUNT); F.alloc(F_COUNT);
     if (!pinned) HIPCHK(hipHostMalloc((void**)&pinned, 2 * F_COUNT * sizeof(int), hipHostMallocDefault));
     for (auto& e : ev) if (!e) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-    std::memset(&VA, 0, sizeof(VA));
-    VA.K = 1; VA.w[0] = 1.;  // unit weights: plain Euclidean norms
+    d_winv.upload(std::vector<double>{1.});  // unit weights: plain Euclidean norms
+    VA.K = 1;
+    VA.winv = d_winv.p;
     built = true;
   }
 
@@ -109,8 +110,7 @@ struct LinSolver {
     csr_vcycle<double>(stream, lv, coarse_inv, coarse_n, 0, in, out, F.p);
   }
   void spmv(const double* in, double* out) {
-    DevCSR<double>& A = lv[0]->A;
-    csr<double>(stream, 0, n, A.ptr, A.col, A.val, in, nullptr, out, nullptr, nullptr, F.p, A.lanes);
+    lv[0]->A.apply(stream, 0, in, nullptr, out, nullptr, nullptr, F.p);
   }
 
   // Returns true when the stopping test was met.
@@ -131,8 +131,7 @@ struct LinSolver {
     const double tol2 = bb > 0. ? std::max(reltol * reltol, abstol * abstol / bb) : 0.;
     HIPCHK(hipMemcpyAsync(S.p + S_TOL2, &tol2, sizeof(double), hipMemcpyHostToDevice, stream));
     // r = f - A x (the spmv's gating flag is not set yet: run it unpredicated)
-    DevCSR<double>& A = lv[0]->A;
-    csr<double>(stream, 1, n, A.ptr, A.col, A.val, x, f, r, nullptr, nullptr, nullptr, A.lanes);
+    lv[0]->A.apply(stream, 1, x, f, r, nullptr, nullptr, nullptr);
     vec<double>(stream, VOP_DOT2, VA, n, r, r, r, r, nullptr, nullptr, nullptr, part);
     finalize(stream, PH_INIT, part, vg, S, F, maxiters);
     zero_if_flag<double>(stream, n, x, F);

@@ -38,13 +38,14 @@ struct Prec {
   TabT<R> T{};
   GeoT<R> G{};
   DBuf<R> Ga, Gb, Gc, fa, fb, fA, fB, fjac, fAn, gel, gface;  // FP32 copies (unused for FP64)
-  DBuf<R> r, z, p, q, rh, v, ph, sh, t, x, dch, diag, tr, tr2, ec, rc, xc;
+  // dinv: 1 / diag(As); dch, dch2: Chebyshev directions (ping-pong)
+  DBuf<R> r, z, p, q, rh, v, ph, sh, t, x, dch, dch2, dinv, tr, tr2, ec, rc, xc;
   std::vector<std::unique_ptr<DevLevel<R>>> lv;  // Q1 vertex level and the SA levels below
   DBuf<R> coarse_inv;
   int coarse_n = 0;
   int prof_off = 0;  // offset of this precision's entries in the live kernel profile
   void alloc_vectors(size_t D, int N, int K) {
-    for (auto* f : {&r, &z, &p, &q, &rh, &v, &ph, &sh, &t, &x, &dch, &diag}) f->alloc(D);
+    for (auto* f : {&r, &z, &p, &q, &rh, &v, &ph, &sh, &t, &x, &dch, &dch2, &dinv}) f->alloc(D);
     tr.alloc((size_t)N * 8 * K); tr2.alloc((size_t)N * 8 * K);
     ec.alloc((size_t)N * 4);
   }
@@ -87,6 +88,7 @@ struct Solver {
   double time = 0., re_inv = 0.;
   Tab T{};
   VecArgs VA{};
+  DBuf<double> d_winv;  // 1/(w_j w_i), the inverse quadrature weights VA.winv points at
   bool mixed = false;
 
   // device: geometry and connectivity
@@ -191,8 +193,14 @@ struct Solver {
       T.x[m] = tab.sps[m];
     }
     T.gp[0] = tab.gprime[0]; T.gp[1] = tab.gprime[1];
-    VA.K = K;
-    for (int m = 0; m < 8; ++m) VA.w[m] = m < K ? tab.wgt[m] : 0.;
+    {
+      std::vector<double> wi((size_t)K * K);
+      for (int j = 0; j < K; ++j)
+        for (int i = 0; i < K; ++i) wi[(size_t)j * K + i] = 1. / (tab.wgt[j] * tab.wgt[i]);
+      d_winv.upload(wi);
+      VA.K = K;
+      VA.winv = d_winv.p;
+    }
     for (size_t v = 0; v < D; ++v)
       if (!(geo.jac[v] > 0.)) throw StatusError(HOC_EMESH, "non-positive Jacobian at a solution point (element orientation must be CCW)");
     // geometry upload
@@ -291,7 +299,7 @@ struct Solver {
     if (rec) {
       HIPCHK(hipEventRecord(prof.e1[prof.n], stream));
       prof.var[prof.n] = P.prof_off + epi;
-      prof.nbytes[prof.n] = lap_bytes(epi, (double)sizeof(R), tout != nullptr);
+      prof.nbytes[prof.n] = lap_bytes(epi, (double)sizeof(R), tout != nullptr, y != nullptr);
       if (++prof.n == (int)prof.e0.size()) prof_flush();
     }
   }
@@ -310,8 +318,8 @@ struct Solver {
   // Algorithmic HBM bytes of one fine-operator launch per epilogue, for values of vb bytes:
   // the field, the neighbour face traces (value + derivative), metrics, connectivity, the
   // epilogue's own vectors and (traces_out) the updated iterate's face traces, each read or
-  // written once.
-  double lap_bytes(int epi, double vb = 8., bool traces_out = false) const {
+  // written once; dir_out: a Chebyshev step stores its direction.
+  double lap_bytes(int epi, double vb = 8., bool traces_out = false, bool dir_out = true) const {
     const double vec = (double)N * K2 * vb;
     double b = vec                                   // x
              + (double)N * 4 * 2 * K * vb            // neighbour traces
@@ -321,8 +329,8 @@ struct Solver {
     switch (epi) {
       case EPI_STORE: b += vec; break;                        // y
       case EPI_RESID: b += 2 * vec; break;                    // r, y
-      case EPI_CHEB: b += 5 * vec; break;                     // r, diag, d (rw), x out
-      case EPI_CHEBF: b += 4 * vec; break;                    // r, diag, d out, x out
+      case EPI_CHEB: b += (dir_out ? 5 : 4) * vec; break;     // r, dinv, d, x out (+ direction out)
+      case EPI_CHEBF: b += (dir_out ? 4 : 3) * vec; break;    // r, dinv, x out (+ direction out)
       case EPI_RESTRICT: b += vec + (double)N * 4 * vb; break;  // r, corner sums
     }
     if (traces_out) b += (double)N * 4 * 2 * K * vb;
@@ -356,7 +364,7 @@ struct Solver {
       for (int rs = 0; rs < K2; ++rs) {
         probe_set(stream, N, K2, dcol1, c, rs, P64.z);
         apply_As(P64.z, P64.q);
-        probe_diag(stream, N, K2, dcol1, c, rs, P64.q, P64.diag);
+        probe_diag(stream, N, K2, dcol1, c, rs, P64.q, P64.dinv);
       }
     // Continuous bilinear coarse space on the element corners: Pc interpolates vertex values
     // to every element's solution points; Ac = Pc^T As Pc is assembled from probes of As.
@@ -417,8 +425,10 @@ struct Solver {
       for (auto& pr : rows[r0]) { A1.col.push_back(pr.first); A1.val.push_back(pr.second); }
       A1.ptr[r0 + 1] = (int)A1.col.size();
     }
+    // Aggregation stops at <= 768 unknowns, solved by a dense inverse: one launch reading
+    // at most 4.7 MB replaces a level's four sparse launches (each latency-bound at ~5 us).
     AMGHost H;
-    build_sa_amg(A1, 400, H);
+    build_sa_amg(A1, 768, H);
     if (mixed) P32.upload_levels(H);
     else P64.upload_levels(H);
     // Chebyshev interval from lambda_max(D^-1 As) by power iteration
@@ -427,7 +437,7 @@ struct Solver {
     for (size_t i = 0; i < D; ++i) { st ^= st << 13; st ^= st >> 7; st ^= st << 17; hx[i] = (double)(st % 2000001) / 1e6 - 1.; }
     h2d(P64.z.p, hx.data(), D * sizeof(double));
     std::vector<double> hd(D), hy(D);
-    d2h(hd.data(), P64.diag.p, D * sizeof(double));
+    d2h(hd.data(), P64.dinv.p, D * sizeof(double));  // the diagonal, inverted below
     for (size_t i = 0; i < D; ++i)
       if (!(hd[i] > 0.)) throw StatusError(HOC_EINVAL, "Poisson operator has a non-positive diagonal entry");
     double lam = 0.;
@@ -470,7 +480,9 @@ struct Solver {
       for (size_t i = 0; i < D; ++i) { a += hx2[i] * hAx[i]; b += hx[i] * hAy[i]; na += std::fabs(hx2[i] * hAx[i]); nb += std::fabs(hx[i] * hAy[i]); }
       krylov = std::fabs(a - b) <= 1e-12 * std::max(na, nb) ? HOC_KRYLOV_PCG : HOC_KRYLOV_BICGSTAB;
     }
-    if (mixed) P32.diag.upload_as(hd);
+    for (size_t i = 0; i < D; ++i) hd[i] = 1. / hd[i];
+    h2d(P64.dinv.p, hd.data(), D * sizeof(double));
+    if (mixed) P32.dinv.upload_as(hd);
     HIPCHK(hipMemsetAsync(P64.z.p, 0, D * sizeof(double), stream));
     HIPCHK(hipStreamSynchronize(stream));
   }
@@ -489,10 +501,15 @@ struct Solver {
     R* cur = P.tr.p;   // traces of the current iterate
     R* nxt = P.tr2.p;
     // pre-smoothing: Chebyshev(m) from zero
-    HIPCHK(launch_trace(K, PRE_CHEB0, N, P.T, zout, rr, P.diag.p, P.dch.p, cheb_theta, nullptr, 0, Fp, cur, stream));
+    // directions ping-pong between dch and dch2; the last step of a sweep stores none
+    R* din = P.dch.p;
+    R* dout = P.dch2.p;
+    HIPCHK(launch_trace(K, PRE_CHEB0, N, P.T, zout, rr, P.dinv.p, din, 1. / cheb_theta, nullptr, 0, Fp, cur, stream));
     for (int k = 1; k < cheb_m; ++k) {
-      lap(P, EPI_CHEB, zout, (R*)nullptr, rr, P.diag.p, P.dch.p, cheb_c1[k], cheb_c2[k], nullptr, Fp, cur, nxt);
+      lap(P, EPI_CHEB, zout, k + 1 < cheb_m ? dout : (R*)nullptr, rr, P.dinv.p, din, cheb_c1[k], cheb_c2[k], nullptr,
+          Fp, cur, nxt);
       std::swap(cur, nxt);
+      std::swap(din, dout);
     }
     // restrict the residual to the corner vertices
     lap(P, EPI_RESTRICT, zout, P.ec.p, rr, (const R*)nullptr, (R*)nullptr, 0., 0., nullptr, Fp, cur);
@@ -502,14 +519,17 @@ struct Solver {
     cur = P.tr.p; nxt = P.tr2.p;
     HIPCHK(launch_trace(K, PRE_PROL, N, P.T, zout, P.xc.p, (const R*)nullptr, (R*)nullptr, 0., nullptr, 0, Fp, cur,
                         stream, d_ev.p));
-    lap(P, EPI_CHEBF, zout, (R*)nullptr, rr, P.diag.p, P.dch.p, 0., 1. / cheb_theta, cheb_m == 1 ? partp : nullptr, Fp,
-        cur, cheb_m == 1 ? (R*)nullptr : nxt);
+    din = P.dch.p;
+    dout = P.dch2.p;
+    lap(P, EPI_CHEBF, zout, cheb_m > 1 ? din : (R*)nullptr, rr, P.dinv.p, (R*)nullptr, 0., 1. / cheb_theta,
+        cheb_m == 1 ? partp : nullptr, Fp, cur, cheb_m == 1 ? (R*)nullptr : nxt);
     std::swap(cur, nxt);
     for (int k = 1; k < cheb_m; ++k) {
       const bool last = k == cheb_m - 1;
-      lap(P, EPI_CHEB, zout, (R*)nullptr, rr, P.diag.p, P.dch.p, cheb_c1[k], cheb_c2[k], last ? partp : nullptr, Fp,
-          cur, last ? (R*)nullptr : nxt);
+      lap(P, EPI_CHEB, zout, last ? (R*)nullptr : dout, rr, P.dinv.p, din, cheb_c1[k], cheb_c2[k],
+          last ? partp : nullptr, Fp, cur, last ? (R*)nullptr : nxt);
       std::swap(cur, nxt);
+      std::swap(din, dout);
     }
   }
 
