@@ -67,7 +67,7 @@ struct DBuf {
 template <class R>
 struct DevCSR {
   static constexpr int kEllMinRows = 16384, kEllMaxWidth = 32;
-  int n = 0, lanes = 1, ell_w = 0;
+  int n = 0, lanes = 1, ell_w = 0, ell_lanes = 1;
   DBuf<int32_t> ptr, col;
   DBuf<R> val;
   void upload(const CSR& A) {
@@ -79,7 +79,7 @@ struct DevCSR {
     n = A.n;
     std::vector<int32_t> ec;
     std::vector<double> evl;
-    if (ell_pack(A, kEllMinRows, kEllMaxWidth, ell_w, ec, evl)) {
+    if (ell_pack(A, kEllMinRows, kEllMaxWidth, ell_w, ell_lanes, ec, evl)) {
       ptr.alloc(0);
       col.upload(ec);
       val.upload_as(evl);
@@ -90,7 +90,7 @@ struct DevCSR {
   }
   // See k_csr for the modes; outputs must not alias x (nor f in mode 4).
   void apply(hipStream_t s, int mode, const R* x, const R* f, R* y, const R* M, R* xio, const int* F) const {
-    if (ell_w) ell<R>(s, mode, n, ell_w, col.p, val.p, x, f, y, M, xio, F);
+    if (ell_w) ell<R>(s, mode, n, ell_w, ell_lanes, col.p, val.p, x, f, y, M, xio, F);
     else csr<R>(s, mode, n, ptr.p, col.p, val.p, x, f, y, M, xio, F, lanes);
   }
 };

@@ -504,11 +504,15 @@ void csr_check(const CSR& A) {
     if (c < 0 || c >= A.m) throw std::invalid_argument("CSR column index out of range");
 }
 
-bool ell_pack(const CSR& A, int min_rows, int max_width, int& w, std::vector<int32_t>& col, std::vector<double>& val) {
+bool ell_pack(const CSR& A, int min_rows, int max_width, int& w, int& lanes, std::vector<int32_t>& col,
+              std::vector<double>& val) {
   w = 0;
+  lanes = 1;
   for (int i = 0; i < A.n; ++i) w = std::max(w, A.ptr[i + 1] - A.ptr[i]);
   const double avg = A.n ? (double)A.val.size() / A.n : 0.;
   if (A.m <= 0 || A.n < min_rows || w > max_width || w > 1.25 * avg + 1.) { w = 0; return false; }
+  lanes = w >= 16 ? 4 : 1;
+  w = (w + lanes - 1) / lanes * lanes;
   col.assign((size_t)w * A.n, 0);
   val.assign((size_t)w * A.n, 0.);
   for (int i = 0; i < A.n; ++i) {
@@ -516,8 +520,9 @@ bool ell_pack(const CSR& A, int min_rows, int max_width, int& w, std::vector<int
     for (int k = 0; k < w; ++k) {
       const int j = A.ptr[i] + k;
       const bool in = j < A.ptr[i + 1];
-      col[(size_t)k * A.n + i] = in ? A.col[j] : pad;
-      val[(size_t)k * A.n + i] = in ? A.val[j] : 0.;
+      const size_t q = ell_index(A.n, lanes, i, k);
+      col[q] = in ? A.col[j] : pad;
+      val[q] = in ? A.val[j] : 0.;
     }
   }
   return true;

@@ -6,6 +6,7 @@
 // operator probing, and an SA-AMG hierarchy for the element-level Poisson matrix.
 #pragma once
 
+#include <cstddef>
 #include <cstdint>
 #include <string>
 #include <vector>
@@ -125,10 +126,17 @@ void build_sa_amg(const CSR& A, int coarse_enough, AMGHost& h);
 // Throws std::invalid_argument unless A is a well-formed n x m CSR matrix (row pointers from
 // 0 to nnz, non-decreasing; every column index in [0, m)).
 void csr_check(const CSR& A);
-// ELL packing for the device: w slots per row, slot-major (col[k n + i], val[k n + i]).
-// Padding slots repeat a column of the same row (column 0 for an empty row) with value 0, so
-// every stored index is below A.m.  Returns false, packing nothing, when the rows are fewer
-// than min_rows, longer than max_width, or too uneven (max > 1.25 avg + 1) for ELL to pay.
-bool ell_pack(const CSR& A, int min_rows, int max_width, int& w, std::vector<int32_t>& col, std::vector<double>& val);
+// ELL packing for the device: w slots per row (a multiple of lanes), slot k of row i at
+// ell_index(n, lanes, i, k): groups of `lanes` consecutive slots are stored row-major within
+// a slot-group plane, so the `lanes` threads of a row, and consecutive rows, read consecutive
+// words.  lanes = 4 for rows of 16 or more slots, else 1.  Padding slots repeat a column of the same
+// row (column 0 for an empty row) with value 0, so every stored index is below A.m.  Returns
+// false, packing nothing, when the rows are fewer than min_rows, longer than max_width, or too
+// uneven (max > 1.25 avg + 1) for ELL to pay.
+inline size_t ell_index(int n, int lanes, int i, int k) {
+  return ((size_t)(k / lanes) * n + i) * lanes + k % lanes;
+}
+bool ell_pack(const CSR& A, int min_rows, int max_width, int& w, int& lanes, std::vector<int32_t>& col,
+              std::vector<double>& val);
 
 }  // namespace hoc

@@ -100,18 +100,29 @@ __global__ void k_zero_if_flag(size_t n, R* __restrict__ x, const int* __restric
   for (size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (size_t)gridDim.x * blockDim.x) x[v] = 0;
 }
 
-// Scalar logic of the Krylov solvers.  pa: np partials of q0 then np of q1.
+// Scalar logic of the Krylov solvers.  pa: np partials of q0 then np of q1.  With mirror
+// set (pinned host memory), thread 0 finally publishes seq * 2 + (active flag) there with a
+// system-scope store, whether or not the phase ran, so the host can follow the iterations
+// without a copy (the host reads only this word, so the store needs no ordering).
+__device__ void finalize_phase(int phase, double a, double b, double* __restrict__ S, int* __restrict__ F,
+                               int maxiter);
 __global__ void k_finalize(int phase, const double* __restrict__ pa, int np, double* __restrict__ S,
-                           int* __restrict__ F, int maxiter) {
-  if (phase != PH_INIT && phase != PH_BB && !F[F_ACTIVE]) return;
-  if (phase >= PH_OMEGA && F[F_HALF]) {  // BiCGSTAB converged at the half step
-    if (phase == PH_RNORM && threadIdx.x == 0) F[F_ACTIVE] = 0;
-    return;
+                           int* __restrict__ F, int maxiter, int* __restrict__ mirror, int seq) {
+  const bool live = phase == PH_INIT || phase == PH_BB || F[F_ACTIVE];
+  const bool half = live && phase >= PH_OMEGA && F[F_HALF];  // BiCGSTAB converged at the half step
+  if (live && !half) {  // uniform over the block
+    double a = 0., b = 0.;
+    for (int i = threadIdx.x; i < np; i += blockDim.x) { a += pa[i]; b += pa[np + i]; }
+    block_sum2(a, b);
+    if (threadIdx.x == 0) finalize_phase(phase, a, b, S, F, maxiter);
+  } else if (half && phase == PH_RNORM && threadIdx.x == 0) {
+    F[F_ACTIVE] = 0;
   }
-  double a = 0., b = 0.;
-  for (int i = threadIdx.x; i < np; i += blockDim.x) { a += pa[i]; b += pa[np + i]; }
-  block_sum2(a, b);
-  if (threadIdx.x != 0) return;
+  if (mirror && threadIdx.x == 0)
+    __hip_atomic_store(mirror, seq * 2 + (F[F_ACTIVE] ? 1 : 0), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+__device__ void finalize_phase(int phase, double a, double b, double* __restrict__ S, int* __restrict__ F,
+                               int maxiter) {
   const double tol2bb = S[S_TOL2] * S[S_BB];
   switch (phase) {
     case PH_BB: S[S_BB] = a; break;
@@ -221,31 +232,38 @@ __global__ void k_csr_vec(int mode, int n, const int32_t* __restrict__ ptr, cons
   sparse_out(mode, i, s, x, f, y, M, xio);
 }
 
-// ELL (rows of nearly equal length, e.g. the Q1 vertex operator): w slots per row stored
-// slot-major, col[k n + i], so consecutive threads (rows) read consecutive words.  Padding
-// slots hold a column of the same row and value 0.
-template <class R>
+// ELL (rows of nearly equal length, e.g. the Q1 vertex operator) with L lanes per row: slot k
+// of row i is at ((k / L) n + i) L + k % L, so the L threads of a row, and consecutive rows,
+// read consecutive words, and each thread walks w / L slots.  Padding slots hold a column of
+// the same row and value 0.
+template <int L, class R>
 __global__ void k_ell(int mode, int n, int w, const int32_t* __restrict__ col, const R* __restrict__ val,
                       const R* __restrict__ x, const R* __restrict__ f, R* __restrict__ y, const R* __restrict__ M,
                       R* __restrict__ xio, const int* __restrict__ F) {
   if (F && !F[F_ACTIVE]) return;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = t / L, lane = t % L;
   R s = 0;
-  if (mode == 4) {
+  if (i < n) {
+    const int groups = w / L;
+    if (mode == 4) {
 #pragma unroll 4
-    for (int k = 0; k < w; ++k) {
-      const size_t q = (size_t)k * n + i;
-      const int c = col[q];
-      s += val[q] * (M[c] * f[c]);
-    }
-  } else {
+      for (int g = 0; g < groups; ++g) {
+        const size_t q = ((size_t)g * n + i) * L + lane;
+        const int c = col[q];
+        s += val[q] * (M[c] * f[c]);
+      }
+    } else {
 #pragma unroll 4
-    for (int k = 0; k < w; ++k) {
-      const size_t q = (size_t)k * n + i;
-      s += val[q] * x[col[q]];
+      for (int g = 0; g < groups; ++g) {
+        const size_t q = ((size_t)g * n + i) * L + lane;
+        s += val[q] * x[col[q]];
+      }
     }
   }
+#pragma unroll
+  for (int off = L / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, L);
+  if (i >= n || lane != 0) return;
   sparse_out(mode, i, s, x, f, y, M, xio);
 }
 
@@ -382,8 +400,9 @@ template <class R>
 void zero_if_flag(hipStream_t s, size_t n, R* x, const int* F) {
   hipLaunchKernelGGL(k_zero_if_flag<R>, dim3(vec_grid(n)), dim3(256), 0, s, n, x, F);
 }
-void finalize(hipStream_t s, int phase, const double* part, int np, double* S, int* F, int maxiter) {
-  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, phase, part, np, S, F, maxiter);
+void finalize(hipStream_t s, int phase, const double* part, int np, double* S, int* F, int maxiter, int* mirror,
+              int seq) {
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, phase, part, np, S, F, maxiter, mirror, seq);
 }
 template <class R>
 void csr(hipStream_t s, int mode, int n, const int32_t* ptr, const int32_t* col, const R* val, const R* x, const R* f,
@@ -399,9 +418,13 @@ void csr(hipStream_t s, int mode, int n, const int32_t* ptr, const int32_t* col,
   }
 }
 template <class R>
-void ell(hipStream_t s, int mode, int n, int w, const int32_t* col, const R* val, const R* x, const R* f, R* y,
-         const R* M, R* xio, const int* F) {
-  hipLaunchKernelGGL(k_ell<R>, dim3((n + 255) / 256), dim3(256), 0, s, mode, n, w, col, val, x, f, y, M, xio, F);
+void ell(hipStream_t s, int mode, int n, int w, int lanes, const int32_t* col, const R* val, const R* x, const R* f,
+         R* y, const R* M, R* xio, const int* F) {
+  if (lanes == 4)
+    hipLaunchKernelGGL((k_ell<4, R>), dim3((unsigned)(((size_t)n * 4 + 255) / 256)), dim3(256), 0, s, mode, n, w, col,
+                       val, x, f, y, M, xio, F);
+  else
+    hipLaunchKernelGGL((k_ell<1, R>), dim3((n + 255) / 256), dim3(256), 0, s, mode, n, w, col, val, x, f, y, M, xio, F);
 }
 int csr_lanes(int n, size_t nnz) {
   const double avg = n ? (double)nnz / n : 0.;
@@ -447,8 +470,8 @@ void q1_probe_accum(hipStream_t s, int N, int K, const Tab& T, int rs, const dou
   template void zero_if_flag<R>(hipStream_t, size_t, R*, const int*);                                                 \
   template void csr<R>(hipStream_t, int, int, const int32_t*, const int32_t*, const R*, const R*, const R*, R*,        \
                        const R*, R*, const int*, int);                                                                 \
-  template void ell<R>(hipStream_t, int, int, int, const int32_t*, const R*, const R*, const R*, R*, const R*, R*,   \
-                       const int*);                                                                            \
+  template void ell<R>(hipStream_t, int, int, int, int, const int32_t*, const R*, const R*, const R*, R*, const R*,  \
+                       R*, const int*);                                                                        \
   template void scale<R>(hipStream_t, int, const R*, const R*, R*, const int*);                                       \
   template void dense<R>(hipStream_t, int, const R*, const R*, R*, const int*);                                       \
   template void q1_gather<R>(hipStream_t, int, const int32_t*, const int32_t*, const R*, R*, const int*);

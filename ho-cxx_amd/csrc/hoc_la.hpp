@@ -33,14 +33,16 @@ template <class R>
 void bicg_half(hipStream_t s, size_t n, R* x, const R* ph, const double* S, int* F);
 template <class R>
 void zero_if_flag(hipStream_t s, size_t n, R* x, const int* F);
-void finalize(hipStream_t s, int phase, const double* part, int np, double* S, int* F, int maxiter);
+// mirror (optional): pinned host int that receives seq * 2 + (active flag) when the kernel ends.
+void finalize(hipStream_t s, int phase, const double* part, int np, double* S, int* F, int maxiter,
+              int* mirror = nullptr, int seq = 0);
 template <class R>
 void csr(hipStream_t s, int mode, int n, const int32_t* ptr, const int32_t* col, const R* val, const R* x, const R* f,
          R* y, const R* M, R* xio, const int* F, int lanes_per_row);
 int csr_lanes(int n, size_t nnz);  // lanes per row suited to a matrix's average row length
 template <class R>
-void ell(hipStream_t s, int mode, int n, int w, const int32_t* col, const R* val, const R* x, const R* f, R* y,
-         const R* M, R* xio, const int* F);
+void ell(hipStream_t s, int mode, int n, int w, int lanes, const int32_t* col, const R* val, const R* x, const R* f,
+         R* y, const R* M, R* xio, const int* F);
 template <class R>
 void scale(hipStream_t s, int n, const R* M, const R* f, R* x, const int* F);
 template <class R>

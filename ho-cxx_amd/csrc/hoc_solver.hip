@@ -111,8 +111,11 @@ struct Solver {
   Prec<float> P32;
   DBuf<double> part, S, S2;
   DBuf<int> F, F2;
-  int* pinned_flags = nullptr;  // 2 x F_COUNT
-  hipEvent_t ev[2] = {nullptr, nullptr};
+  // Krylov progress published by the end-of-iteration finalize kernel (seq * 2 + active), in
+  // pinned coherent host memory; seq counts iterations over the handle's lifetime.
+  int* mirror = nullptr;
+  int* mirror_dev = nullptr;
+  int seq_next = 0;
   int nparts = 0;
   int nv = 0;                             // Q1 coarse space: element-corner vertices
   DBuf<int32_t> d_ev, d_vptr, d_vinc;     // element -> 4 vertices; vertex -> (element*4+corner)
@@ -159,8 +162,7 @@ struct Solver {
   ~Solver() {
     for (auto e : prof.e0) (void)hipEventDestroy(e);
     for (auto e : prof.e1) (void)hipEventDestroy(e);
-    if (pinned_flags) (void)hipHostFree(pinned_flags);
-    for (auto& e : ev) if (e) (void)hipEventDestroy(e);
+    if (mirror) (void)hipHostFree(mirror);
     if (stream) (void)hipStreamDestroy(stream);
   }
 
@@ -280,8 +282,11 @@ struct Solver {
     nparts = std::max(lap_blocks(N), vec_grid(D));
     part.alloc(2 * (size_t)nparts);
     S.alloc(S_COUNT); F.alloc(F_COUNT); S2.alloc(S_COUNT); F2.alloc(F_COUNT);
-    if (!pinned_flags) HIPCHK(hipHostMalloc((void**)&pinned_flags, 2 * F_COUNT * sizeof(int), hipHostMallocDefault));
-    for (auto& e : ev) if (!e) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    if (!mirror) {
+      HIPCHK(hipHostMalloc((void**)&mirror, 64, hipHostMallocCoherent | hipHostMallocMapped));
+      __atomic_store_n(mirror, -2, __ATOMIC_RELEASE);
+      HIPCHK(hipHostGetDevicePointer((void**)&mirror_dev, mirror, 0));
+    }
     build_hierarchy();
     time = 0.;
     ready = true;
@@ -533,18 +538,27 @@ struct Solver {
     }
   }
 
-  // Poll an active flag with a one-iteration lag; returns false when the solve ended.
-  int poll_k = 0;
-  bool poll(const int* Fp, bool final_wait) {
-    const int slot = poll_k & 1;
-    HIPCHK(hipMemcpyAsync(pinned_flags + slot * F_COUNT, Fp, F_COUNT * sizeof(int), hipMemcpyDeviceToHost, stream));
-    HIPCHK(hipEventRecord(ev[slot], stream));
-    ++poll_k;
-    if (final_wait) { HIPCHK(hipEventSynchronize(ev[slot])); return pinned_flags[slot * F_COUNT + F_ACTIVE] != 0; }
-    if (poll_k < 2) return true;
-    const int prev = (poll_k - 2) & 1;
-    HIPCHK(hipEventSynchronize(ev[prev]));
-    return pinned_flags[prev * F_COUNT + F_ACTIVE] != 0;
+  // Wait, with a one-iteration lag (none when final_wait), until the finalize kernel of the
+  // iteration numbered seq has published its status; returns false when the solve ended.
+  // seq_first: the first iteration of this solve (no wait before it has run).  The wait spins
+  // on the pinned word and checks the stream every 4096 reads, so an idle stream without the
+  // expected status (or a device error) ends it with an error instead of a hang.
+  bool poll(int seq, int seq_first, bool final_wait) {
+    const int target = final_wait ? seq : seq - 1;
+    if (target < seq_first) return true;
+    for (unsigned spin = 1;; ++spin) {
+      const int v = __atomic_load_n(mirror, __ATOMIC_ACQUIRE);
+      if ((v >> 1) >= target) return (v & 1) != 0;
+      if ((spin & 4095) == 0) {
+        const hipError_t q = hipStreamQuery(stream);
+        if (q == hipSuccess) {
+          const int v2 = __atomic_load_n(mirror, __ATOMIC_ACQUIRE);
+          if ((v2 >> 1) >= target) return (v2 & 1) != 0;
+          throw HipError("Krylov status of an iteration was not published");
+        }
+        if (q != hipErrorNotReady) HIPCHK(q);
+      }
+    }
   }
 
   // Preconditioned Krylov iterations on As x = r in precision R, from the state left by
@@ -554,7 +568,7 @@ struct Solver {
   void krylov_solve(Prec<R>& P, R* x, R* r, double* Sp, int* Fp, int maxit) {
     const int lb = lap_blocks(N);
     const int vg = vec_grid(D);
-    poll_k = 0;
+    const int seq0 = seq_next;
     if (krylov == HOC_KRYLOV_PCG) {
       HIPCHK(hipMemsetAsync(P.p.p, 0, D * sizeof(R), stream));
       for (int it = 0;; ++it) {
@@ -565,8 +579,9 @@ struct Solver {
         lap(P, EPI_STORE, P.p.p, P.q.p, (const R*)nullptr, (const R*)nullptr, (R*)nullptr, 0., 0., part.p, Fp);
         finalize(stream, PH_PQ, part, lb, Sp, Fp, maxit);
         vec<R>(stream, VOP_PCG_UPD, VA, D, x, r, P.p, P.q, nullptr, Sp, Fp, part);
-        finalize(stream, PH_UPD, part, vg, Sp, Fp, maxit);
-        if (!poll(Fp, it > maxit + 2) || it > maxit + 4) break;
+        const int seq = seq_next++;
+        finalize(stream, PH_UPD, part, vg, Sp, Fp, maxit, mirror_dev, seq);
+        if (!poll(seq, seq0, it > maxit + 2) || it > maxit + 4) break;
       }
     } else {
       vec<R>(stream, VOP_COPY, VA, D, P.rh, nullptr, r, nullptr, nullptr, nullptr, nullptr, nullptr);
@@ -591,8 +606,9 @@ struct Solver {
         vec<R>(stream, VOP_DOT2, VA, D, P.t, P.t, r, P.t, nullptr, Sp, Fp, part);
         finalize(stream, PH_OMEGA, part, vg, Sp, Fp, maxit);
         vec<R>(stream, VOP_BICG_X, VA, D, x, r, P.ph, P.sh, P.t, Sp, Fp, part);
-        finalize(stream, PH_RNORM, part, vg, Sp, Fp, maxit);
-        if (!poll(Fp, it > maxit + 2) || it > maxit + 4) break;
+        const int seq = seq_next++;
+        finalize(stream, PH_RNORM, part, vg, Sp, Fp, maxit, mirror_dev, seq);
+        if (!poll(seq, seq0, it > maxit + 2) || it > maxit + 4) break;
       }
     }
   }

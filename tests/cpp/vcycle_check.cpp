@@ -1,7 +1,8 @@
 // vcycle_check.cpp - CPU check of the sparse-level data the device V-cycle consumes.
 //
-// Builds the smoothed-aggregation hierarchy (hoc_host.cpp) of the Q1 (bilinear FEM)
-// stiffness matrix on a g x g vertex grid, then for every level matrix A, P, R:
+// Builds the smoothed-aggregation hierarchy (hoc_host.cpp) of the square of the Q1 (bilinear
+// FEM) stiffness matrix on a g x g vertex grid (a 25-point stencil, the width of the FR
+// solver's Q1 vertex operator), then for every level matrix A, P, R:
 //   * csr_check passes and ell_pack stores only column indices below the column count;
 //   * the ELL product equals the CSR product;
 // and replays the device V-cycle's five sparse steps per level (modes 4, 0, 2, 3 and the
@@ -9,6 +10,7 @@
 // vectors sized like the device buffers, checking that no step writes a vector it reads at
 // other rows, and that the resulting preconditioner is linear and symmetric.
 // Prints "OK" and exits 0 on success.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <random>
@@ -40,9 +42,30 @@ static CSR q1_stiffness(int g) {
   return A;
 }
 
+static CSR square(const CSR& A) {  // A A for a symmetric A with sorted rows
+  CSR C;
+  C.n = C.m = A.n;
+  C.ptr.push_back(0);
+  std::vector<double> acc(A.n, 0.);
+  std::vector<int> mark(A.n, -1), cols;
+  for (int i = 0; i < A.n; ++i) {
+    cols.clear();
+    for (int j = A.ptr[i]; j < A.ptr[i + 1]; ++j)
+      for (int l = A.ptr[A.col[j]]; l < A.ptr[A.col[j] + 1]; ++l) {
+        const int c = A.col[l];
+        if (mark[c] != i) { mark[c] = i; acc[c] = 0.; cols.push_back(c); }
+        acc[c] += A.val[j] * A.val[l];
+      }
+    std::sort(cols.begin(), cols.end());
+    for (int c : cols) { C.col.push_back(c); C.val.push_back(acc[c]); }
+    C.ptr.push_back((int)C.col.size());
+  }
+  return C;
+}
+
 struct Dev {  // a level matrix as the device holds it
   CSR A;
-  int w = 0;
+  int w = 0, lanes = 1;
   std::vector<int32_t> ec;
   Vec ev;
 };
@@ -51,9 +74,18 @@ static Dev make_dev(const CSR& A) {
   csr_check(A);
   Dev d;
   d.A = A;
-  if (ell_pack(A, 16384, 32, d.w, d.ec, d.ev))
+  if (ell_pack(A, 16384, 32, d.w, d.lanes, d.ec, d.ev)) {
     for (int32_t c : d.ec)
       if (c < 0 || c >= A.m) throw std::runtime_error("ELL column index out of range");
+    if (d.w % d.lanes || d.ec.size() != (size_t)d.w * A.n) throw std::runtime_error("ELL shape");
+    std::vector<char> seen(d.ec.size(), 0);  // the index map is a bijection onto the storage
+    for (int i = 0; i < A.n; ++i)
+      for (int k = 0; k < d.w; ++k) {
+        const size_t q = ell_index(A.n, d.lanes, i, k);
+        if (q >= seen.size() || seen[q]) throw std::runtime_error("ELL index map");
+        seen[q] = 1;
+      }
+  }
   return d;
 }
 
@@ -67,7 +99,7 @@ static void apply(const Dev& d, int mode, const Vec* x, const Vec* f, Vec* y, co
     double s = 0.;
     const int len = d.w ? d.w : d.A.ptr[i + 1] - d.A.ptr[i];
     for (int k = 0; k < len; ++k) {
-      const size_t q = d.w ? (size_t)k * n + i : (size_t)d.A.ptr[i] + k;
+      const size_t q = d.w ? ell_index(n, d.lanes, i, k) : (size_t)d.A.ptr[i] + k;
       const int c = d.w ? d.ec.at(q) : d.A.col.at(q);
       const double v = d.w ? d.ev.at(q) : d.A.val.at(q);
       s += v * (mode == 4 ? M->at(c) * f->at(c) : x->at(c));
@@ -107,7 +139,7 @@ static void vcycle(std::vector<Level>& L, const Vec& cinv, size_t l, const Vec& 
 int main() {
   try {
     const int g = 160;  // 25 600 rows: the finest A and P take the ELL form
-    const CSR A0 = q1_stiffness(g);
+    const CSR A0 = square(q1_stiffness(g));
     AMGHost H;
     build_sa_amg(A0, 768, H);
     std::vector<Level> L(H.levels.size());
@@ -126,10 +158,10 @@ int main() {
       }
       const size_t n = H.levels[l].A.n;
       L[l].f.assign(n, 0.); L[l].u.assign(n, 0.); L[l].t.assign(n, 0.);
-      std::printf("level %zu: n %d nnz %zu ell_w A %d P %d R %d\n", l, H.levels[l].A.n, H.levels[l].A.val.size(),
-                  L[l].A.w, L[l].P.w, L[l].R.w);
+      std::printf("level %zu: n %d nnz %zu ell w/lanes A %d/%d P %d/%d R %d/%d\n", l, H.levels[l].A.n,
+                  H.levels[l].A.val.size(), L[l].A.w, L[l].A.lanes, L[l].P.w, L[l].P.lanes, L[l].R.w, L[l].R.lanes);
     }
-    if (n_ell == 0) throw std::runtime_error("no level took the ELL form");
+    if (n_ell == 0 || L[0].A.lanes != 4) throw std::runtime_error("the finest operator did not take the 4-lane ELL form");
     // ELL product == CSR product on every ELL matrix
     std::mt19937_64 rng(7);
     std::uniform_real_distribution<double> U(-1., 1.);
