@@ -241,7 +241,6 @@ __device__ __forceinline__ void second_order_core(const TabT<R>& T, const R (&u)
 //   PRE_CHEB0  u = r * dinv * coef (coef = 1/theta); writes x = d = u   (Chebyshev step from zero)
 //   PRE_PROL   u = x + Pc xc (bilinear from the corner vertices); writes x = u
 //   PRE_PUPD   u = z + beta * p; writes p = u                     (PCG direction)
-//   PRE_ZERO   u = 0                                              (RHS assembly)
 // ==========================================================================================
 template <int K, int PRE, class R>
 __global__ __launch_bounds__(kWG) void k_trace(int N, TabT<R> T, R* __restrict__ x, const R* __restrict__ a,
@@ -250,33 +249,36 @@ __global__ __launch_bounds__(kWG) void k_trace(int N, TabT<R> T, R* __restrict__
                                                 R* __restrict__ tr, const int32_t* __restrict__ ev) {
   if (active && !*active) return;
   constexpr int K2 = K * K;
-  __shared__ __align__(16) R lds[kWG * K2];
-  __shared__ __align__(16) R lds2[kWG * K2];
+  __shared__ __align__(16) R lds[kWG * K2];  // one tile: two inputs are staged in turn
   const int e0 = blockIdx.x * kWG;
   const int el = e0 + threadIdx.x;
   const bool live = el < N;
   R u[K2];
-  if (PRE == PRE_COPY || PRE == PRE_PROL) tile_load<K2>(x, lds, e0, N);
-  if (PRE == PRE_CHEB0) { tile_load<K2>(a, lds, e0, N); tile_load<K2>(b, lds2, e0, N); }
-  if (PRE == PRE_PUPD) { tile_load<K2>(a, lds, e0, N); tile_load<K2>(x, lds2, e0, N); }
   R beta = 0;
   if (PRE == PRE_PUPD) beta = (R)scal[scal_idx];
   R xc[4] = {0, 0, 0, 0};
   if (PRE == PRE_PROL && live)
 #pragma unroll
     for (int c = 0; c < 4; ++c) xc[c] = a[ev[(size_t)el * 4 + c]];
+  tile_load<K2>(PRE == PRE_COPY || PRE == PRE_PROL ? (const R*)x : a, lds, e0, N);
 #pragma unroll
   for (int m = 0; m < K2; ++m) {
     const int li = threadIdx.x * K2 + m;
-    R v = 0;
-    if (PRE == PRE_COPY) v = lds[li];
+    R v = lds[li];
     if (PRE == PRE_PROL) {
       const int j = m / K, i = m % K;
-      v = lds[li] + (q1w(T, 0, j, i) * xc[0] + q1w(T, 1, j, i) * xc[1] + q1w(T, 2, j, i) * xc[2] + q1w(T, 3, j, i) * xc[3]);
+      v += q1w(T, 0, j, i) * xc[0] + q1w(T, 1, j, i) * xc[1] + q1w(T, 2, j, i) * xc[2] + q1w(T, 3, j, i) * xc[3];
     }
-    if (PRE == PRE_CHEB0) v = lds[li] * lds2[li] * coef;
-    if (PRE == PRE_PUPD) v = lds[li] + beta * lds2[li];
     u[m] = v;
+  }
+  if (PRE == PRE_CHEB0 || PRE == PRE_PUPD) {  // second input
+    __syncthreads();
+    tile_load<K2>(PRE == PRE_CHEB0 ? b : (const R*)x, lds, e0, N);
+#pragma unroll
+    for (int m = 0; m < K2; ++m) {
+      const R w = lds[threadIdx.x * K2 + m];
+      u[m] = PRE == PRE_CHEB0 ? u[m] * w * coef : u[m] + beta * w;
+    }
   }
   if (PRE == PRE_CHEB0 || PRE == PRE_PROL || PRE == PRE_PUPD) {
     __syncthreads();
@@ -513,9 +515,9 @@ __global__ __launch_bounds__(kWG) void k_rhs(int N, Tab T, Geo G, const double* 
 // Velocity from the stream function (src/HO_2D.cpp:3174-3366) with the no-slip wall
 // vorticity flux (update_diffusion_BC, src/HO_2D.cpp:2647-2664), fused with the traces the
 // stage kernel needs from neighbours: omega value/derivative traces and the contravariant
-// normal-velocity trace.
+// normal-velocity trace.  AFF: metrics constant per element (G.gel, G.gface, G.gjm).
 // ==========================================================================================
-template <int K>
+template <int K, bool AFF>
 __global__ __launch_bounds__(kWG) void k_velocity(int N, Tab T, Geo G, BCDev B, const double* __restrict__ psi,
                                                    const double* __restrict__ trpsi, const double* __restrict__ omega,
                                                    double* __restrict__ vel, double* __restrict__ U,
@@ -534,7 +536,7 @@ __global__ __launch_bounds__(kWG) void k_velocity(int N, Tab T, Geo G, BCDev B, 
   if (live) {
     double bv[4][K], bg[4][K], cv[4][K], cg[4][K];
     traces<K>(T, u, bv, bg);
-    faces2<K>(T, G, el, trpsi, G.sw_psi, B.psi, true, bv, bg, cv, cg);
+    faces2<K, AFF>(T, G, el, trpsi, G.sw_psi, B.psi, true, bv, bg, cv, cg);
     // boundary side effects
 #pragma unroll
     for (int f = 0; f < 4; ++f) {
@@ -567,6 +569,12 @@ __global__ __launch_bounds__(kWG) void k_velocity(int N, Tab T, Geo G, BCDev B, 
       }
     }
     const size_t base = (size_t)el * K2;
+    double ea = 0., eb = 0., ec = 0., exx = 0., exe = 0., eyx = 0., eye = 0., ej = 0.;
+    if (AFF) {
+      ea = G.gel[(size_t)el * 3]; eb = G.gel[(size_t)el * 3 + 1]; ec = G.gel[(size_t)el * 3 + 2];
+      exx = G.gjm[(size_t)el * 5]; exe = G.gjm[(size_t)el * 5 + 1]; eyx = G.gjm[(size_t)el * 5 + 2];
+      eye = G.gjm[(size_t)el * 5 + 3]; ej = G.gjm[(size_t)el * 5 + 4];
+    }
     double ft[K2], gt[K2];
 #pragma unroll
     for (int ky = 0; ky < K; ++ky)
@@ -578,8 +586,9 @@ __global__ __launch_bounds__(kWG) void k_velocity(int N, Tab T, Geo G, BCDev B, 
         dx += (cv[0][ky] - bv[0][ky]) * T.grad[kx][0] + (cv[1][ky] - bv[1][ky]) * T.grad[kx][1];
         de += (cv[2][kx] - bv[2][kx]) * T.grad[ky][0] + (cv[3][kx] - bv[3][kx]) * T.grad[ky][1];
         const int v = ky * K + kx;
-        ft[v] = dx * G.Ga[base + v] - de * G.Gb[base + v];
-        gt[v] = de * G.Gc[base + v] - dx * G.Gb[base + v];
+        const double ga = AFF ? ea : G.Ga[base + v], gb = AFF ? eb : G.Gb[base + v], gc = AFF ? ec : G.Gc[base + v];
+        ft[v] = dx * ga - de * gb;
+        gt[v] = de * gc - dx * gb;
       }
     double fB[2][K], gB[2][K];
 #pragma unroll
@@ -598,8 +607,9 @@ __global__ __launch_bounds__(kWG) void k_velocity(int N, Tab T, Geo G, BCDev B, 
         const int v = ky * K + kx;
         const double tx = ft[v] + fB[0][ky] * T.rad[kx][0] + fB[1][ky] * T.rad[kx][1];
         const double ty = gt[v] + gB[0][kx] * T.rad[ky][0] + gB[1][kx] * T.rad[ky][1];
-        const double xxi = G.xxi[base + v], xet = G.xeta[base + v], yxi = G.yxi[base + v], yet = G.yeta[base + v];
-        const double J = G.jac[base + v];
+        const double xxi = AFF ? exx : G.xxi[base + v], xet = AFF ? exe : G.xeta[base + v];
+        const double yxi = AFF ? eyx : G.yxi[base + v], yet = AFF ? eye : G.yeta[base + v];
+        const double J = AFF ? ej : G.jac[base + v];
         const double dpdx = (tx * xxi + ty * xet) / J;
         const double dpdy = (tx * yxi + ty * yet) / J;
         vx[v] = dpdy;    // u =  dpsi/dy
@@ -643,9 +653,10 @@ __global__ __launch_bounds__(kWG) void k_velocity(int N, Tab T, Geo G, BCDev B, 
 // ==========================================================================================
 // Stage kernel: diffusion (src/HO_2D.cpp:2666-2831, BC_vorticity at :2726-2727),
 // update_advection_BC (:2219-2243), advection type 1 (:2245-2437) or 2 (:2439-2645),
-// k = dt (adv + diff) (:1833-1836) and the RK update (:1465-1531).
+// k = dt (adv + diff) (:1833-1836) and the RK update (:1465-1531).  AFF: metrics constant per
+// element (G.gel, G.gface, G.gjm).
 // ==========================================================================================
-template <int K, int ADV>
+template <int K, int ADV, bool AFF>
 __global__ __launch_bounds__(kWG) void k_stage(int N, Tab T, Geo G, BCDev B, const double* __restrict__ omega,
                                                 const double* __restrict__ U, const double* __restrict__ trw,
                                                 const double* __restrict__ tru, double re_inv, double dt,
@@ -669,9 +680,19 @@ __global__ __launch_bounds__(kWG) void k_stage(int N, Tab T, Geo G, BCDev B, con
     // ---- diffusion
     double bv[4][K], bg[4][K], cv[4][K], cg[4][K];
     traces<K>(T, w, bv, bg);
-    faces2<K>(T, G, el, trw, G.sw_diff, B.diff, false, bv, bg, cv, cg);
+    faces2<K, AFF>(T, G, el, trw, G.sw_diff, B.diff, false, bv, bg, cv, cg);
     double dif[K2];
-    second_order_core<K>(T, w, bv, cv, cg, G.Ga + base, G.Gb + base, G.Gc + base, dif);
+    double ej = 0.;
+    if (AFF) {
+      double ga[K2], gb[K2], gc[K2];
+      const double a = G.gel[(size_t)el * 3], b = G.gel[(size_t)el * 3 + 1], c = G.gel[(size_t)el * 3 + 2];
+#pragma unroll
+      for (int m = 0; m < K2; ++m) { ga[m] = a; gb[m] = b; gc[m] = c; }
+      second_order_core<K>(T, w, bv, cv, cg, ga, gb, gc, dif);
+      ej = G.gjm[(size_t)el * 5 + 4];
+    } else {
+      second_order_core<K>(T, w, bv, cv, cg, G.Ga + base, G.Gb + base, G.Gc + base, dif);
+    }
     // ---- boundary vorticity and advective boundary flux
     double bca[4][K];
 #pragma unroll
@@ -793,7 +814,7 @@ __global__ __launch_bounds__(kWG) void k_stage(int N, Tab T, Geo G, BCDev B, con
         for (int d = 0; d < K; ++d) r -= f2[d * K + i] * T.D[d][j];
         r -= bdf[0][j] * T.grad[i][0] + bdf[1][j] * T.grad[i][1];
         r -= bdf[2][i] * T.grad[j][0] + bdf[3][i] * T.grad[j][1];
-        const double J = G.jac[base + v];
+        const double J = AFF ? ej : G.jac[base + v];
         const double adv = r / J;
         const double ri = I.re_inv ? I.re_inv[G.inst[el]] : re_inv;
         const double dti = I.dt ? I.dt[G.inst[el]] : dt;
@@ -988,22 +1009,30 @@ hipError_t launch_rhs(int K, int N, const Tab& T, const Geo& G, const double* om
 hipError_t launch_velocity(int K, int N, const Tab& T, const Geo& G, const BCDev& B, const double* psi,
                            const double* trpsi, const double* omega, double* vel, double* U, double* trw,
                            double* tru, double inv_dt_re, const InstDev& I, hipStream_t s) {
-  HOC_K_SWITCH(K, hipLaunchKernelGGL((k_velocity<KK>), dim3(nblk(N)), dim3(kWG), 0, s, N, T, G, B, psi, trpsi, omega,
-                                     vel, U, trw, tru, inv_dt_re, I));
+  const bool aff = G.gel && G.gface && G.gjm;
+  HOC_K_SWITCH(K, {
+    if (aff)
+      hipLaunchKernelGGL((k_velocity<KK, true>), dim3(nblk(N)), dim3(kWG), 0, s, N, T, G, B, psi, trpsi, omega, vel, U,
+                         trw, tru, inv_dt_re, I);
+    else
+      hipLaunchKernelGGL((k_velocity<KK, false>), dim3(nblk(N)), dim3(kWG), 0, s, N, T, G, B, psi, trpsi, omega, vel, U,
+                         trw, tru, inv_dt_re, I);
+  });
   return hipGetLastError();
 }
 
 hipError_t launch_stage(int K, int adv, int N, const Tab& T, const Geo& G, const BCDev& B, const double* omega,
                         const double* U, const double* trw, const double* tru, double re_inv, double dt,
                         const InstDev& I, const StageOut& so, hipStream_t s) {
+  const bool aff = G.gel && G.gface && G.gjm;
+#define LS(A, F)                                                                                                  \
+  hipLaunchKernelGGL((k_stage<KK, A, F>), dim3(nblk(N)), dim3(kWG), 0, s, N, T, G, B, omega, U, trw, tru, re_inv, dt, \
+                     I, so)
   HOC_K_SWITCH(K, {
-    if (adv == 1)
-      hipLaunchKernelGGL((k_stage<KK, 1>), dim3(nblk(N)), dim3(kWG), 0, s, N, T, G, B, omega, U, trw, tru, re_inv, dt, I,
-                         so);
-    else
-      hipLaunchKernelGGL((k_stage<KK, 2>), dim3(nblk(N)), dim3(kWG), 0, s, N, T, G, B, omega, U, trw, tru, re_inv, dt, I,
-                         so);
+    if (adv == 1) { if (aff) LS(1, true); else LS(1, false); }
+    else { if (aff) LS(2, true); else LS(2, false); }
   });
+#undef LS
   return hipGetLastError();
 }
 

@@ -51,6 +51,7 @@ struct GeoT {
   const R *fA, *fB, *fjac, *fAn;           // [N][4][K]  raw face metrics
   const R* gel;                            // [N][3]     Ga, Gb, Gc of affine elements
   const R* gface;                          // [N][4][2]  fa, fb of affine elements
+  const R* gjm;                            // [N][5]     xxi, xeta, yxi, yeta, jac of affine elements
   const int32_t* nb;                       // [N][4] neighbour or -1-edge
   const int8_t* nbf;                       // [N][4] neighbour's face
   const int8_t *sw_psi, *sw_diff;          // [E] 0 Dirichlet / 1 Neumann

@@ -95,7 +95,7 @@ struct Solver {
   DBuf<double> d_jac, d_xxi, d_xeta, d_yxi, d_yeta, d_Ga, d_Gb, d_Gc, d_fa, d_fb, d_fA, d_fB, d_fjac, d_fAn;
   DBuf<int32_t> d_nb;
   DBuf<int8_t> d_nbf, d_swpsi, d_swdiff, d_noslip;
-  DBuf<double> d_esign, d_gel, d_gface;
+  DBuf<double> d_esign, d_gel, d_gface, d_gjm;
   Geo G{};
   // boundary values
   DBuf<double> b_psi, b_diff, b_nvel, b_pvel, b_vj, b_vort, b_adv;
@@ -228,14 +228,21 @@ struct Solver {
     // (tolerances relative to each element's metric scale: rounding leaves ~1e-16 noise
     // in components that vanish on rectangles)
     bool affine = true;
-    std::vector<double> gel((size_t)N * 3), gface((size_t)N * 8);
+    std::vector<double> gel((size_t)N * 3), gface((size_t)N * 8), gjm((size_t)N * 5);
     for (int e = 0; e < N && affine; ++e) {
       const size_t b0 = (size_t)e * K2;
       const double sc = std::max(std::fabs(Ga[b0]), std::fabs(Gc[b0]));
       gel[(size_t)e * 3] = Ga[b0]; gel[(size_t)e * 3 + 1] = Gb[b0]; gel[(size_t)e * 3 + 2] = Gc[b0];
-      for (int m = 1; m < K2 && affine; ++m)
+      const double* jm[5] = {geo.xxi.data(), geo.xeta.data(), geo.yxi.data(), geo.yeta.data(), geo.jac.data()};
+      const double sj = std::max(std::max(std::fabs(jm[0][b0]), std::fabs(jm[1][b0])),
+                                 std::max(std::fabs(jm[2][b0]), std::fabs(jm[3][b0])));
+      for (int c = 0; c < 5; ++c) gjm[(size_t)e * 5 + c] = jm[c][b0];
+      for (int m = 1; m < K2 && affine; ++m) {
         affine = std::fabs(Ga[b0 + m] - Ga[b0]) <= 1e-12 * sc && std::fabs(Gb[b0 + m] - Gb[b0]) <= 1e-12 * sc &&
                  std::fabs(Gc[b0 + m] - Gc[b0]) <= 1e-12 * sc;
+        for (int c = 0; c < 4 && affine; ++c) affine = std::fabs(jm[c][b0 + m] - jm[c][b0]) <= 1e-12 * sj;
+        affine = affine && std::fabs(jm[4][b0 + m] - jm[4][b0]) <= 1e-12 * std::fabs(jm[4][b0]);
+      }
       for (int f = 0; f < 4 && affine; ++f) {
         const size_t q0 = ((size_t)e * 4 + f) * K;
         const double fs = std::max(std::fabs(fa[q0]), 1e-300);
@@ -244,9 +251,9 @@ struct Solver {
           affine = std::fabs(fa[q0 + k] - fa[q0]) <= 1e-12 * fs && std::fabs(fb[q0 + k] - fb[q0]) <= 1e-12 * fs;
       }
     }
-    if (affine) { d_gel.upload(gel); d_gface.upload(gface); }
+    if (affine) { d_gel.upload(gel); d_gface.upload(gface); d_gjm.upload(gjm); }
     G = Geo{N, E, d_jac, d_xxi, d_xeta, d_yxi, d_yeta, d_Ga, d_Gb, d_Gc, d_fa, d_fb, d_fA, d_fB, d_fjac, d_fAn,
-            affine ? d_gel.p : nullptr, affine ? d_gface.p : nullptr,
+            affine ? d_gel.p : nullptr, affine ? d_gface.p : nullptr, affine ? d_gjm.p : nullptr,
             d_nb, d_nbf, d_swpsi, d_swdiff, d_noslip, d_esign, nullptr};
     setup_instances();
     setup_hybrid();
@@ -274,7 +281,7 @@ struct Solver {
       P.fjac.upload_as(geo.fjac); P.fAn.upload_as(geo.fAn);
       if (affine) { P.gel.upload_as(gel); P.gface.upload_as(gface); }
       P.G = GeoT<float>{N, E, nullptr, nullptr, nullptr, nullptr, nullptr, P.Ga, P.Gb, P.Gc, P.fa, P.fb, P.fA, P.fB,
-                        P.fjac, P.fAn, affine ? P.gel.p : nullptr, affine ? P.gface.p : nullptr,
+                        P.fjac, P.fAn, affine ? P.gel.p : nullptr, affine ? P.gface.p : nullptr, nullptr,
                         d_nb, d_nbf, d_swpsi, d_swdiff, d_noslip, nullptr, nullptr};
       P.prof_off = HOC_PROF_VARIANTS / 2;
       P.alloc_vectors(D, N, K);
