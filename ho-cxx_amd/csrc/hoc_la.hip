@@ -114,7 +114,7 @@ __global__ void k_finalize(int phase, const double* __restrict__ pa, int np, dou
     double a = 0., b = 0.;
     for (int i = threadIdx.x; i < np; i += blockDim.x) { a += pa[i]; b += pa[np + i]; }
     block_sum2(a, b);
-    if (threadIdx.x == 0) finalize_phase(phase, a, b, S, F, maxiter);
+    if (threadIdx.x == 0) finalize_phase(phase, a, b, S, F, maxiter >= 0 ? maxiter : F[F_MAXIT]);
   } else if (half && phase == PH_RNORM && threadIdx.x == 0) {
     F[F_ACTIVE] = 0;
   }
@@ -176,6 +176,32 @@ __device__ void finalize_phase(int phase, double a, double b, double* __restrict
       F[F_ACTIVE] = (b > tol2bb && F[F_ITER] < maxiter && S[S_OMEGA] != 0. && isfinite(b)) ? 1 : 0;
     } break;
   }
+}
+
+__global__ void k_refine_control(double* __restrict__ S, int* __restrict__ F, double* __restrict__ S2,
+                                 int* __restrict__ F2, double tol, double eta_floor, int maxit, int max_outer,
+                                 int* __restrict__ mirror, int seq) {
+  if (threadIdx.x != 0) return;
+  const int outer = F[F_OUTER];
+  if (outer > 0) F[F_TOTAL] += F2[F_ITER];  // iterations of the previous pass
+  const double rr = S[S_RR], bb = S[S_BB];
+  const bool go = F[F_ACTIVE] && isfinite(rr) && F[F_TOTAL] < maxit && outer < max_outer &&
+                  !(outer > 0 && rr > 0.25 * S[S_RRPREV]);
+  if (go) {
+    S[S_RRPREV] = rr;
+    // reduction still needed, with a margin for the FP32 residual gap
+    const double need = tol * sqrt(bb / rr);
+    const double eta = fmin(0.5, fmax(eta_floor, 0.3 * need));
+    S2[S_TOL2] = eta * eta;
+    F2[F_MAXIT] = maxit - F[F_TOTAL];
+    finalize_phase(PH_BB, rr, 0., S2, F2, F2[F_MAXIT]);
+    finalize_phase(PH_INIT, 0., rr, S2, F2, F2[F_MAXIT]);
+    F[F_OUTER] = outer + 1;
+  } else {
+    F2[F_ACTIVE] = 0;
+    F2[F_ITER] = 0;
+  }
+  __hip_atomic_store(mirror, seq * 2 + (go ? 1 : 0), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 // ---- Sparse levels -------------------------------------------------------------------------
@@ -403,6 +429,11 @@ void zero_if_flag(hipStream_t s, size_t n, R* x, const int* F) {
 void finalize(hipStream_t s, int phase, const double* part, int np, double* S, int* F, int maxiter, int* mirror,
               int seq) {
   hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, phase, part, np, S, F, maxiter, mirror, seq);
+}
+void refine_control(hipStream_t s, double* S, int* F, double* S2, int* F2, double tol, double eta_floor, int maxit,
+                    int max_outer, int* mirror, int seq) {
+  hipLaunchKernelGGL(k_refine_control, dim3(1), dim3(64), 0, s, S, F, S2, F2, tol, eta_floor, maxit, max_outer,
+                     mirror, seq);
 }
 template <class R>
 void csr(hipStream_t s, int mode, int n, const int32_t* ptr, const int32_t* col, const R* val, const R* x, const R* f,

@@ -13,8 +13,10 @@
 namespace hoc {
 
 // Device scalar slots (double) and flags (int) of the Krylov solvers.
-enum { S_BB = 0, S_RR, S_RZ, S_PQ, S_ALPHA, S_BETA, S_OMEGA, S_RHO, S_TOL2, S_COUNT = 16 };
-enum { F_ACTIVE = 0, F_ITER, F_ZERO, F_HALF, F_COUNT = 8 };
+// S_RRPREV, F_MAXIT, F_OUTER, F_TOTAL: state of the mixed-precision refinement loop
+// (F_MAXIT: iteration budget read by finalize kernels launched with maxiter < 0).
+enum { S_BB = 0, S_RR, S_RZ, S_PQ, S_ALPHA, S_BETA, S_OMEGA, S_RHO, S_TOL2, S_RRPREV, S_COUNT = 16 };
+enum { F_ACTIVE = 0, F_ITER, F_ZERO, F_HALF, F_MAXIT, F_OUTER, F_TOTAL, F_COUNT = 8 };
 enum { PH_BB = 0, PH_INIT, PH_RZ, PH_PQ, PH_UPD, PH_RHO, PH_ALPHA, PH_SNORM, PH_OMEGA, PH_RNORM };
 enum { VOP_PCG_UPD = 0, VOP_BICG_P, VOP_BICG_S, VOP_DOT2, VOP_COPY, VOP_BICG_X };
 
@@ -34,8 +36,17 @@ void bicg_half(hipStream_t s, size_t n, R* x, const R* ph, const double* S, int*
 template <class R>
 void zero_if_flag(hipStream_t s, size_t n, R* x, const int* F);
 // mirror (optional): pinned host int that receives seq * 2 + (active flag) when the kernel ends.
+// maxiter < 0: the budget is read from F[F_MAXIT].
 void finalize(hipStream_t s, int phase, const double* part, int np, double* S, int* F, int maxiter,
               int* mirror = nullptr, int seq = 0);
+// Control step of the mixed-precision refinement loop (one thread), run after each FP64
+// residual (S, F of the outer solve).  The loop goes on while the residual test fails, the
+// residual is finite, the previous pass at least halved it, and the iteration and pass
+// budgets last; then it sets the inner solve's tolerance (eta^2 in S2), budget and initial
+// state (its right-hand side norm is the outer residual norm) and counts the pass; otherwise
+// it leaves the inner solve inactive.  Publishes seq * 2 + (go) to mirror.
+void refine_control(hipStream_t s, double* S, int* F, double* S2, int* F2, double tol, double eta_floor, int maxit,
+                    int max_outer, int* mirror, int seq);
 template <class R>
 void csr(hipStream_t s, int mode, int n, const int32_t* ptr, const int32_t* col, const R* val, const R* x, const R* f,
          R* y, const R* M, R* xio, const int* F, int lanes_per_row);

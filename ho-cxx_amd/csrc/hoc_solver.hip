@@ -571,8 +571,10 @@ struct Solver {
   // Preconditioned Krylov iterations on As x = r in precision R, from the state left by
   // PH_INIT in (Sp, Fp): r holds the residual of x.  PCG when As is symmetric, else
   // right-preconditioned BiCGSTAB.  Every kernel is predicated on Fp's active flag.
+  // maxit bounds the host loop; maxit_dev is the device budget (< 0: Fp[F_MAXIT]).
   template <class R>
-  void krylov_solve(Prec<R>& P, R* x, R* r, double* Sp, int* Fp, int maxit) {
+  void krylov_solve(Prec<R>& P, R* x, R* r, double* Sp, int* Fp, int maxit_host, int maxit_dev) {
+    const int maxit = maxit_dev;
     const int lb = lap_blocks(N);
     const int vg = vec_grid(D);
     const int seq0 = seq_next;
@@ -588,7 +590,7 @@ struct Solver {
         vec<R>(stream, VOP_PCG_UPD, VA, D, x, r, P.p, P.q, nullptr, Sp, Fp, part);
         const int seq = seq_next++;
         finalize(stream, PH_UPD, part, vg, Sp, Fp, maxit, mirror_dev, seq);
-        if (!poll(seq, seq0, it > maxit + 2) || it > maxit + 4) break;
+        if (!poll(seq, seq0, it > maxit_host + 2) || it > maxit_host + 4) break;
       }
     } else {
       vec<R>(stream, VOP_COPY, VA, D, P.rh, nullptr, r, nullptr, nullptr, nullptr, nullptr, nullptr);
@@ -615,7 +617,7 @@ struct Solver {
         vec<R>(stream, VOP_BICG_X, VA, D, x, r, P.ph, P.sh, P.t, Sp, Fp, part);
         const int seq = seq_next++;
         finalize(stream, PH_RNORM, part, vg, Sp, Fp, maxit, mirror_dev, seq);
-        if (!poll(seq, seq0, it > maxit + 2) || it > maxit + 4) break;
+        if (!poll(seq, seq0, it > maxit_host + 2) || it > maxit_host + 4) break;
       }
     }
   }
@@ -670,36 +672,27 @@ struct Solver {
     };
     int iters = 0, outer = 0;
     if (!mixed) {
-      krylov_solve(P64, psi.p, P64.r.p, S.p, F.p, maxit);
+      krylov_solve(P64, psi.p, P64.r.p, S.p, F.p, maxit, maxit);
       read_state(S, F);
       iters = hf[F_ITER];
     } else {
+      // The pass decisions are taken on the device (refine_control); the host waits only for
+      // the control step's published verdict, while the demotion behind it keeps the GPU busy.
       const double eta_floor = cfg.mixed_inner_tol > 0. ? cfg.mixed_inner_tol : 1e-5;
       constexpr int kMaxOuter = 12;
-      double rr_prev = 0.;
+      HIPCHK(hipMemsetAsync(F.p + F_OUTER, 0, 2 * sizeof(int), stream));  // F_OUTER, F_TOTAL
       for (;;) {
-        read_state(S, F);
-        if (!hf[F_ACTIVE] || !std::isfinite(hs[S_RR]) || iters >= maxit || outer >= kMaxOuter) break;
-        // a pass that did not halve the FP64 residual has reached its rounding floor
-        if (outer > 0 && hs[S_RR] > 0.25 * rr_prev) break;
-        rr_prev = hs[S_RR];
-        // reduction still needed, with a margin for the FP32 residual gap
-        const double need = cfg.tol * std::sqrt(hs[S_BB] / hs[S_RR]);
-        const double eta = std::min(0.5, std::max(eta_floor, 0.3 * need));
-        const double eta2 = eta * eta;
-        HIPCHK(hipMemcpyAsync(S2.p + S_TOL2, &eta2, sizeof(double), hipMemcpyHostToDevice, stream));
+        const int seq_c = seq_next++;
+        refine_control(stream, S, F, S2, F2, cfg.tol, eta_floor, maxit, kMaxOuter, mirror_dev, seq_c);
         mixed_demote(stream, VA, D, P64.r, P32.r, P32.x, part, F);
-        finalize(stream, PH_BB, part, vec_grid(D), S2, F2, maxit - iters);
-        finalize(stream, PH_INIT, part, vec_grid(D), S2, F2, maxit - iters);
-        krylov_solve(P32, P32.x.p, P32.r.p, S2.p, F2.p, maxit - iters);
+        if (!poll(seq_c, seq_c, true)) break;
+        krylov_solve(P32, P32.x.p, P32.r.p, S2.p, F2.p, maxit, -1);
         mixed_promote_add(stream, D, psi, P32.x, F);
         outer_residual();
-        ++outer;
-        int hf2[F_COUNT];
-        HIPCHK(hipMemcpyAsync(hf2, F2.p, sizeof(hf2), hipMemcpyDeviceToHost, stream));
-        HIPCHK(hipStreamSynchronize(stream));
-        iters += hf2[F_ITER];
       }
+      read_state(S, F);
+      iters = hf[F_TOTAL];
+      outer = hf[F_OUTER];
     }
     // statistics
     last_iters = iters;
