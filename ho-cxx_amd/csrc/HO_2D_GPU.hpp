@@ -9,7 +9,9 @@
 // Problem types 1-3 and 11 (hybrid) are supported.
 #pragma once
 
+#include <cmath>
 #include <cstdint>
+#include <fstream>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -83,6 +85,7 @@ class HO_2D_GPU {
   }
   void process_mesh_input() {
     clean_up();
+    n_samples_ = 0;
     check(hoc_create(&h_, &cfg_), nullptr);
     std::string joined;
     for (size_t i = 0; i < names_.size(); ++i) joined += (i ? "\n" : "") + names_[i];
@@ -127,10 +130,35 @@ class HO_2D_GPU {
   void setptogweights_d(const int32_t n, double* w) {
     if (cfg_.problem_type == 11) check(hoc_set_p2g_weights(h_, n, w), h_);
   }
-  // VTK output of the solution points (save_vorticity_vtk / save_smooth_vtk, :3369-3692)
+  // trigger_write (src/HO_2D.cpp:4262-4271): the solution-point file and the node-averaged file
+  // (save_vorticity_vtk / save_smooth_vtk), named as the reference names them, from the state of
+  // the last stage; with sample points set, their values as well (src/HO_2D.cpp:3593-3612).
   void trigger_write(const int32_t indx, const int32_t subidx = -1) {
-    std::string path = "vort_" + std::to_string(indx) + (subidx > -1 ? "_" + std::to_string(subidx) : "") + ".vtk";
-    check(hoc_write_vtk(h_, path.c_str()), h_);
+    const std::string mid = std::to_string(cfg_.problem_type) + "_K" + std::to_string(cfg_.K) +
+                            (subidx > -1 ? "_" + pad(subidx, 2) : "") + "_" + pad(indx, 5);
+    check(hoc_write_vtk(h_, ("vorticity" + mid + ".vtk").c_str()), h_);
+    check(hoc_write_smooth_vtk(h_, 0, ("problem" + mid + ".vtk").c_str()), h_);
+    if (n_samples_ > 0) {
+      const std::string sp = "sampling_points" + std::to_string(cfg_.problem_type) + "_K" + std::to_string(cfg_.K) +
+                             "_L" + std::to_string(lnod_) + "_" + pad(indx, 5) + ".dat";
+      check(hoc_write_samples(h_, 0, sp.c_str()), h_);
+    }
+  }
+  // read_process_sample_points (src/HO_2D.cpp:319-474) on a file of "x y" pairs; a last pair
+  // within 1e-6 of the previous one is dropped, as the reference does.  Call after
+  // process_mesh_input.
+  void read_process_sample_points(const std::string& path) {
+    std::ifstream in(path);
+    if (!in) throw hoc_gpu_error(HOC_EINVAL, "cannot open sample points file " + path);
+    std::vector<double> xy;
+    double a, b;
+    while (in >> a >> b) { xy.push_back(a); xy.push_back(b); }
+    const size_t n = xy.size() / 2;
+    if (n >= 2 && std::hypot(xy[2 * n - 2] - xy[2 * n - 4], xy[2 * n - 1] - xy[2 * n - 3]) < 1e-6) {
+      xy.resize(xy.size() - 2);
+    }
+    n_samples_ = (int32_t)(xy.size() / 2);
+    check(hoc_set_sample_points(h_, n_samples_, xy.data()), h_);
   }
   // The vorticity field itself (initial conditions; no counterpart in the reference).
   void set_vorticity(const int32_t n, const double* w) { check(hoc_set_vorticity(h_, n, w), h_); }
@@ -153,6 +181,10 @@ class HO_2D_GPU {
   hoc_t handle() const { return h_; }
 
  private:
+  static std::string pad(int32_t v, int width) {  // std::setw(width) << std::setfill('0') << v
+    std::string t = std::to_string(v);
+    return t.size() < (size_t)width ? std::string(width - t.size(), '0') + t : t;
+  }
   int64_t dofs() const { return (int64_t)(elem_.size() / (lnod_ * lnod_)) * cfg_.K * cfg_.K; }
   static void check(int rc, hoc_t h) {
     if (rc != HOC_OK) throw hoc_gpu_error(rc, hoc_last_error(h));
@@ -180,6 +212,7 @@ class HO_2D_GPU {
   hoc_t h_ = nullptr;
   bool hybrid_ = false;
   int32_t lnod_ = 2;
+  int32_t n_samples_ = 0;
   std::vector<double> xy_;
   std::vector<int32_t> elem_, bedge_, bbdry_, nval_;
   std::vector<double> vals_;

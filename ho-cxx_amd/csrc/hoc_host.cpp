@@ -796,6 +796,148 @@ void read_gmsh41(const std::string& path, int occ, Mesh& m) {
 // ---------------------------------------------------------------------------------------
 // Legacy-VTK output of solution-point fields
 // ---------------------------------------------------------------------------------------
+int gmsh_slot(int lnod, int i, int j) { return t2g(lnod, i, j); }
+
+void locate_sample_points(const Mesh& m, const Tables& t, int n, const double* xy, std::vector<int32_t>& elem,
+                          std::vector<double>& local_xy, std::vector<double>& shape) {
+  const int L = m.lnod, L2 = L * L, N = m.n_elem;
+  constexpr double kSlope = 0.3819660112501051;  // fixed ray slope in [-1, 1]
+  elem.assign(n, -1);
+  local_xy.assign(2 * (size_t)n, 0.);
+  shape.assign((size_t)n * L2, 0.);
+  auto X = [&](int nd) { return m.xy[2 * (size_t)nd]; };
+  auto Y = [&](int nd) { return m.xy[2 * (size_t)nd + 1]; };
+  // the nodes of side s in order: corner s, the side's interior nodes, corner s+1 (Gmsh order)
+  auto side_node = [&](int el, int s, int q) {
+    const int32_t* en = &m.elem[(size_t)el * L2];
+    if (q == 0) return en[s];
+    if (q == L - 1) return en[(s + 1) % 4];
+    return en[4 + s * (L - 2) + (q - 1)];
+  };
+  std::vector<double> mnx(N), mxx(N), mny(N), mxy(N);
+  for (int el = 0; el < N; ++el) {  // bounding boxes of the element sides (src/HO_2D.cpp:342-354)
+    const int n0 = m.elem[(size_t)el * L2];
+    mnx[el] = mxx[el] = X(n0);
+    mny[el] = mxy[el] = Y(n0);
+    for (int s = 0; s < 4; ++s)
+      for (int q = 0; q < L; ++q) {
+        const int nd = side_node(el, s, q);
+        mnx[el] = std::min(mnx[el], X(nd)); mxx[el] = std::max(mxx[el], X(nd));
+        mny[el] = std::min(mny[el], Y(nd)); mxy[el] = std::max(mxy[el], Y(nd));
+      }
+  }
+  // ray from p with slope kSlope against the segment a-b (are_intersecting, src/MathHelpers.hpp:99-114)
+  auto hit = [&](double px, double py, int a, int b) {
+    const double dx = X(b) - X(a), dy = Y(b) - Y(a);
+    const double sp = (kSlope * (X(a) - px) + py - Y(a)) / (dy - kSlope * dx);
+    const double tp = X(a) - px + sp * dx;
+    return (tp > -1.e-8 && sp > -1.e-8 && sp < 1. + 1.e-8) ? 1 : 0;
+  };
+  std::vector<double> bx(L), by(L), gx(L), gy(L);
+  for (int p = 0; p < n; ++p) {
+    const double px = xy[2 * p], py = xy[2 * p + 1];
+    int found = -1;
+    for (int el = 0; el < N && found < 0; ++el) {  // src/HO_2D.cpp:357-378
+      if (px < mnx[el] || px > mxx[el] || py < mny[el] || py > mxy[el]) continue;
+      int hits = 0;
+      for (int s = 0; s < 4; ++s)
+        for (int q = 0; q + 1 < L; ++q) hits += hit(px, py, side_node(el, s, q), side_node(el, s, q + 1));
+      if (hits & 1) found = el;
+    }
+    if (found < 0) {  // nearest mesh node (src/HO_2D.cpp:380-394)
+      const int n0 = m.elem[0];
+      double dist = std::sqrt((px - X(n0)) * (px - X(n0)) + (py - Y(n0)) * (py - Y(n0))) + 1.;
+      for (int el = 0; el < N; ++el)
+        for (int i = 0; i < L2; ++i) {
+          const int nd = m.elem[(size_t)el * L2 + i];
+          const double d = std::sqrt((px - X(nd)) * (px - X(nd)) + (py - Y(nd)) * (py - Y(nd)));
+          if (dist > d) { dist = d; found = el; }
+        }
+    }
+    elem[p] = found;
+    // local coordinates (src/HO_2D.cpp:405-472)
+    double xsi = 0.08, eta = 0.08, dxi = 0., deta = 0.;
+    int it = 0;
+    do {
+      for (int k = 0; k < L; ++k) {
+        gx[k] = gy[k] = 0.;
+        bx[k] = by[k] = 1.;
+        double den = 1.;
+        for (int j = 0; j < L; ++j) {
+          if (j == k) continue;
+          den *= t.gps[k] - t.gps[j];
+          bx[k] *= xsi - t.gps[j];
+          by[k] *= eta - t.gps[j];
+          double nxi = 1., net = 1.;
+          for (int i = 0; i < L; ++i) {
+            if (i == k || i == j) continue;
+            nxi *= xsi - t.gps[i];
+            net *= eta - t.gps[i];
+          }
+          gx[k] += nxi;
+          gy[k] += net;
+        }
+        gx[k] /= den; bx[k] /= den; gy[k] /= den; by[k] /= den;
+      }
+      double f = px, g = py, fx = 0., fe = 0., gxi = 0., ge = 0.;
+      for (int j = 0; j < L; ++j)
+        for (int i = 0; i < L; ++i) {
+          const int nd = m.elem[(size_t)found * L2 + t2g(L, i, j)];
+          f -= bx[i] * by[j] * X(nd);
+          g -= bx[i] * by[j] * Y(nd);
+          fx -= by[j] * gx[i] * X(nd);
+          gxi -= by[j] * gx[i] * Y(nd);
+          fe -= gy[j] * bx[i] * X(nd);
+          ge -= gy[j] * bx[i] * Y(nd);
+        }
+      dxi = -1. / (fx * ge - fe * gxi) * (f * ge - g * fe);
+      deta = -1. / (fx * ge - fe * gxi) * (g * fx - f * gxi);
+      xsi += dxi;
+      eta += deta;
+    } while ((std::fabs(dxi) > 0.001 || std::fabs(deta) > 0.001) && ++it < 100);
+    if (!std::isfinite(xsi) || !std::isfinite(eta))
+      throw std::runtime_error("sample point: local coordinates did not converge");
+    local_xy[2 * p] = xsi;
+    local_xy[2 * p + 1] = eta;
+    for (int j = 0; j < L; ++j)
+      for (int i = 0; i < L; ++i) shape[(size_t)p * L2 + t2g(L, i, j)] = bx[i] * by[j];
+  }
+}
+
+void write_vtk_nodes(const std::string& path, const Mesh& m, const double* nodes) {
+  std::ofstream out(path);
+  if (!out) throw std::runtime_error("vtk: cannot open " + path);
+  const int L2 = m.lnod * m.lnod;
+  out << "# vtk DataFile Version 3.0\n2D Unstructured Grid of Quads\nASCII\n\nDATASET UNSTRUCTURED_GRID\n";
+  out << "POINTS " << m.n_nodes << " double\n";
+  for (int nd = 0; nd < m.n_nodes; ++nd) out << m.xy[2 * (size_t)nd] << ' ' << m.xy[2 * (size_t)nd + 1] << " 0.0\n";
+  out << "CELLS " << m.n_elem << ' ' << (size_t)m.n_elem * (L2 + 1) << '\n';
+  for (int el = 0; el < m.n_elem; ++el) {
+    out << L2;
+    for (int l = 0; l < L2; ++l) out << ' ' << m.elem[(size_t)el * L2 + l];
+    out << '\n';
+  }
+  out << "CELL_TYPES " << m.n_elem << '\n';
+  for (int el = 0; el < m.n_elem; ++el) out << "70\n";
+  out << "POINT_DATA " << m.n_nodes << '\n';
+  out << "SCALARS vorticity double 1 \nLOOKUP_TABLE default \n";
+  for (int nd = 0; nd < m.n_nodes; ++nd) out << nodes[4 * (size_t)nd] << '\n';
+  out << "SCALARS psi double 1 \nLOOKUP_TABLE default \n";
+  for (int nd = 0; nd < m.n_nodes; ++nd) out << nodes[4 * (size_t)nd + 1] << '\n';
+  out << "VECTORS velocity double \n";
+  for (int nd = 0; nd < m.n_nodes; ++nd) out << nodes[4 * (size_t)nd + 2] << ' ' << nodes[4 * (size_t)nd + 3] << " 0.0\n";
+  if (!out) throw std::runtime_error("vtk: write failed for " + path);
+}
+
+void write_samples(const std::string& path, int n, const double* xy, const double* vals) {
+  std::ofstream out(path);
+  if (!out) throw std::runtime_error("samples: cannot open " + path);
+  for (int p = 0; p < n; ++p)
+    out << xy[2 * p] << ' ' << xy[2 * p + 1] << ' ' << vals[3 * p] << ' ' << vals[3 * p + 1] << ' ' << vals[3 * p + 2]
+        << '\n';
+  if (!out) throw std::runtime_error("samples: write failed for " + path);
+}
+
 void write_vtk_sps(const std::string& path, int K, int n_elem, const double* pts, int n_fields,
                    const char* const* names, const double* const* fields) {
   std::ofstream out(path);

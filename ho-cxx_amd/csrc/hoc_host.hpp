@@ -71,6 +71,27 @@ struct Tables {
 };
 void build_tables(int K, int lnod, Tables& t);
 
+// Sample points (read_process_sample_points, src/HO_2D.cpp:319-474) for n points xy: the
+// containing element (bounding box, then a ray-crossing test of the element's sides; the nearest
+// mesh node's element when none contains the point), its local coordinates by Newton's method
+// from (0.08, 0.08) with the reference's stopping test (both steps <= 1e-3; at most 100
+// iterations here), and the geometry shape functions, in Gmsh node order, of the last iterate
+// before the final update, as the reference keeps them.  The reference draws the ray's slope
+// with std::rand(); here it is fixed, so the result does not depend on the run.
+// elem: n, local_xy: 2n, shape: n * lnod^2.
+void locate_sample_points(const Mesh& m, const Tables& t, int n, const double* xy, std::vector<int32_t>& elem,
+                          std::vector<double>& local_xy, std::vector<double>& shape);
+// Gmsh node slot of tensor position (i along xi, j along eta) for lnod = 2..4
+// (src/Mesh.cpp:34-91).
+int gmsh_slot(int lnod, int i, int j);
+// Node-averaged output files in the reference's formats: the unstructured grid of the mesh
+// nodes with vorticity, psi and velocity (save_smooth_vtk, src/HO_2D.cpp:3534-3589) from
+// nodes[n_nodes][4] = vorticity, psi, u, v; and the sample-point lines "x y vorticity u v"
+// (src/HO_2D.cpp:3593-3612) from vals[n][3].
+void write_vtk_nodes(const std::string& path, const Mesh& m, const double* nodes);
+void write_samples(const std::string& path, int n, const double* xy, const double* vals);
+
+
 struct Geometry {
   // volume, [el][j][i]
   std::vector<double> xxi, xeta, yxi, yeta, jac;

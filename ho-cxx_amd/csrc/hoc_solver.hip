@@ -26,6 +26,7 @@
 #include "hoc_fr.hpp"
 #include "hoc_host.hpp"
 #include "hoc_la.hpp"
+#include "hoc_out.hpp"
 
 namespace hoc {
 
@@ -296,6 +297,8 @@ struct Solver {
     }
     build_hierarchy();
     time = 0.;
+    smooth_ready = false;
+    n_sp = 0;
     ready = true;
     HIPCHK(hipStreamSynchronize(stream));
     setup_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -708,6 +711,90 @@ struct Solver {
     if (!std::isfinite(last_rel)) throw StatusError(HOC_ENAN, "NaN/Inf in the Poisson solve");
     if (cfg.fail_on_noconv && last_rel > cfg.tol)
       throw StatusError(HOC_ENOCONV, "Poisson solve did not converge: rel residual " + std::to_string(last_rel));
+  }
+
+  // ---- node-averaged output and sample points ---------------------------------------------
+  DBuf<int32_t> d_elnodes, d_nptr, d_ninc, d_sel;
+  DBuf<double> d_bmet, d_elemout, d_nodeout, d_sshape, d_sval;
+  bool smooth_ready = false;
+  int n_sp = 0;
+  std::vector<double> sp_xy, sp_loc;
+  std::vector<int32_t> sp_elem;
+
+  // Element node lists, node incidences in element order and the boundary faces' metrics,
+  // uploaded on the first output call.
+  void smooth_setup() {
+    if (smooth_ready) return;
+    const int L2 = mesh.lnod * mesh.lnod;
+    d_elnodes.upload(mesh.elem);
+    std::vector<int32_t> ptr((size_t)mesh.n_nodes + 1, 0), inc((size_t)N * L2);
+    for (int32_t nd : mesh.elem) ptr[(size_t)nd + 1]++;
+    for (int i = 0; i < mesh.n_nodes; ++i) ptr[i + 1] += ptr[i];
+    std::vector<int32_t> pos(ptr.begin(), ptr.end() - 1);
+    for (int el = 0; el < N; ++el)
+      for (int l = 0; l < L2; ++l) inc[pos[mesh.elem[(size_t)el * L2 + l]]++] = el * L2 + l;
+    d_nptr.upload(ptr);
+    d_ninc.upload(inc);
+    std::vector<double> bm((size_t)std::max(E, 1) * K * 4, 0.);
+    for (int e = 0; e < E; ++e)
+      for (int k = 0; k < K; ++k) {
+        const size_t q = ((size_t)mesh.be_elem[e] * 4 + mesh.be_ijp[e]) * K + k;
+        double* o = &bm[((size_t)e * K + k) * 4];
+        o[0] = geo.fxxi[q]; o[1] = geo.fxeta[q]; o[2] = geo.fyxi[q]; o[3] = geo.fyeta[q];
+      }
+    d_bmet.upload(bm);
+    d_elemout.alloc((size_t)N * L2 * 4);
+    d_nodeout.alloc((size_t)mesh.n_nodes * 4);
+    smooth_ready = true;
+  }
+  // What the reference's main loop runs before each dump (src/HO_2D.cpp:1411-1414): boundary
+  // values at the current time, the stream function and the velocity of the current vorticity.
+  void refresh_fields() {
+    update_bcs(0.);
+    poisson(w);
+    HIPCHK(launch_trace(K, PRE_COPY, N, T, psi, nullptr, nullptr, nullptr, 0., nullptr, 0, nullptr, P64.tr.p, stream));
+    HIPCHK(launch_velocity(K, N, T, G, B, psi, P64.tr, w, vel, U, trw, tru, 1. / (cfg.dt * re_inv), I, stream));
+  }
+  // Node-averaged vorticity, psi and velocity into d_nodeout (save_smooth_vtk,
+  // src/HO_2D.cpp:3369-3531).
+  void node_fields(bool refresh) {
+    smooth_setup();
+    if (refresh) refresh_fields();
+    SmoothArgs A{};
+    A.N = N; A.K = K; A.L = mesh.lnod;
+    for (int k = 0; k < K; ++k)  // sps_gps_basis (src/HO_2D.cpp:571-583)
+      for (int l = 0; l < A.L; ++l) {
+        double num = 1., den = 1.;
+        for (int j = 0; j < K; ++j) {
+          if (j == k) continue;
+          den *= tab.sps[k] - tab.sps[j];
+          num *= tab.gps[l] - tab.sps[j];
+        }
+        A.sgb[k][l] = num / den;
+      }
+    for (int j = 0; j < A.L; ++j)
+      for (int i = 0; i < A.L; ++i) A.slot[j * A.L + i] = gmsh_slot(A.L, i, j);
+    A.w = w; A.psi = psi; A.vel = vel; A.nb = d_nb;
+    A.bvort = b_vort; A.bnvel = b_nvel; A.bpvel = b_pvel; A.bmet = d_bmet; A.elemout = d_elemout;
+    HIPCHK(launch_smooth_elem(A, stream));
+    HIPCHK(launch_smooth_nodes(mesh.n_nodes, d_nptr, d_ninc, d_elemout, d_nodeout, stream));
+  }
+  void set_sample_points(int n, const double* xy) {
+    std::vector<double> shape;
+    try {
+      locate_sample_points(mesh, tab, n, xy, sp_elem, sp_loc, shape);
+    } catch (const std::runtime_error& e) {
+      throw StatusError(HOC_EINVAL, e.what());
+    }
+    sp_xy.assign(xy, xy + 2 * (size_t)n);
+    n_sp = n;
+    if (n > 0) { d_sel.upload(sp_elem); d_sshape.upload(shape); d_sval.alloc(3 * (size_t)n); }
+  }
+  void sample_values(bool refresh, double* out) {
+    node_fields(refresh);
+    HIPCHK(launch_sample(n_sp, mesh.lnod * mesh.lnod, d_sel, d_sshape, d_elnodes, d_nodeout, d_sval, stream));
+    if (n_sp > 0) d2h(out, d_sval.p, 3 * (size_t)n_sp * sizeof(double));
+    else HIPCHK(hipStreamSynchronize(stream));
   }
 
   // Validates a mesh given as arrays or read from a file and makes it the handle's mesh.
@@ -1161,6 +1248,76 @@ int hoc_get_solution_areas(hoc_t h, int64_t len, double* a) {
         }
   });
 }
+int hoc_get_node_fields(hoc_t h, int32_t refresh, int32_t n_nodes, double* out) {
+  return guarded(h, [&] {
+    NEED_READY(h);
+    Solver& s = h->s;
+    if (n_nodes != s.mesh.n_nodes || !out) throw hoc::StatusError(HOC_EINVAL, "n_nodes must be the mesh's node count");
+    s.node_fields(refresh != 0);
+    s.d2h(out, s.d_nodeout.p, (size_t)n_nodes * 4 * sizeof(double));
+  });
+}
+
+int hoc_write_smooth_vtk(hoc_t h, int32_t refresh, const char* path) {
+  return guarded(h, [&] {
+    NEED_READY(h);
+    if (!path) throw hoc::StatusError(HOC_EINVAL, "null path");
+    Solver& s = h->s;
+    s.node_fields(refresh != 0);
+    std::vector<double> nodes((size_t)s.mesh.n_nodes * 4);
+    s.d2h(nodes.data(), s.d_nodeout.p, nodes.size() * sizeof(double));
+    try {
+      hoc::write_vtk_nodes(path, s.mesh, nodes.data());
+    } catch (const std::runtime_error& e) {
+      throw hoc::StatusError(HOC_EINVAL, e.what());
+    }
+  });
+}
+
+int hoc_set_sample_points(hoc_t h, int32_t n, const double* xy) {
+  return guarded(h, [&] {
+    NEED_READY(h);
+    if (n < 0 || (n > 0 && !xy)) throw hoc::StatusError(HOC_EINVAL, "bad sample point array");
+    for (int32_t i = 0; i < 2 * n; ++i)
+      if (!std::isfinite(xy[i])) throw hoc::StatusError(HOC_EINVAL, "non-finite sample point coordinate");
+    h->s.set_sample_points(n, xy);
+  });
+}
+
+int hoc_get_sample_location(hoc_t h, int32_t n, int32_t* elem, double* local_xy) {
+  return guarded(h, [&] {
+    NEED_READY(h);
+    Solver& s = h->s;
+    if (n != s.n_sp) throw hoc::StatusError(HOC_EINVAL, "n must be the number of sample points set");
+    if (elem) std::copy(s.sp_elem.begin(), s.sp_elem.end(), elem);
+    if (local_xy) std::copy(s.sp_loc.begin(), s.sp_loc.end(), local_xy);
+  });
+}
+
+int hoc_get_sample_values(hoc_t h, int32_t refresh, int32_t n, double* out) {
+  return guarded(h, [&] {
+    NEED_READY(h);
+    Solver& s = h->s;
+    if (n != s.n_sp || (n > 0 && !out)) throw hoc::StatusError(HOC_EINVAL, "n must be the number of sample points set");
+    s.sample_values(refresh != 0, out);
+  });
+}
+
+int hoc_write_samples(hoc_t h, int32_t refresh, const char* path) {
+  return guarded(h, [&] {
+    NEED_READY(h);
+    if (!path) throw hoc::StatusError(HOC_EINVAL, "null path");
+    Solver& s = h->s;
+    std::vector<double> vals(3 * (size_t)s.n_sp);
+    s.sample_values(refresh != 0, vals.data());
+    try {
+      hoc::write_samples(path, s.n_sp, s.sp_xy.data(), vals.data());
+    } catch (const std::runtime_error& e) {
+      throw hoc::StatusError(HOC_EINVAL, e.what());
+    }
+  });
+}
+
 int hoc_write_vtk(hoc_t h, const char* path) {
   return guarded(h, [&] {
     NEED_READY(h);
@@ -1261,6 +1418,7 @@ int hoc_stats_get(hoc_t h, hoc_stats* st) {
     st->n_levels = (int32_t)(s.mixed ? s.P32.lv.size() : s.P64.lv.size()) + 1;
     st->last_refinements = s.last_outer;
     st->precision = s.cfg.precision;
+    st->n_nodes = s.mesh.n_nodes;
     st->krylov = s.krylov;
     st->last_iters = s.last_iters;
     st->last_rel_residual = s.last_rel;

@@ -27,6 +27,8 @@ KRYLOV_AUTO, KRYLOV_BICGSTAB, KRYLOV_PCG = 0, 1, 2
 
 EXPORTED = [
     "hoc_default_config", "hoc_create", "hoc_load_mesh", "hoc_load_gmsh", "hoc_write_vtk", "hoc_setup",
+    "hoc_get_node_fields", "hoc_write_smooth_vtk", "hoc_set_sample_points", "hoc_get_sample_location",
+    "hoc_get_sample_values", "hoc_write_samples",
     "hoc_set_instances", "hoc_get_instance_times", "hoc_set_boundary_values", "hoc_open_point_count",
     "hoc_get_open_points", "hoc_set_open_velocities", "hoc_set_open_vorticity", "hoc_set_vorticity_target",
     "hoc_set_p2g_weights", "hoc_set_vorticity", "hoc_get_vorticity",
@@ -53,7 +55,8 @@ class hoc_stats(C.Structure):
     _fields_ = [("n_elem", C.c_int64), ("n_dofs", C.c_int64), ("n_bedges", C.c_int64), ("n_levels", C.c_int32),
                 ("krylov", C.c_int32), ("last_iters", C.c_int32), ("last_rel_residual", C.c_double),
                 ("total_iters", C.c_int64), ("n_solves", C.c_int64), ("setup_seconds", C.c_double),
-                ("fine_op_bytes", C.c_double), ("last_refinements", C.c_int32), ("precision", C.c_int32)]
+                ("fine_op_bytes", C.c_double), ("last_refinements", C.c_int32), ("precision", C.c_int32),
+                ("n_nodes", C.c_int64)]
 
 
 class hoc_kernel_profile(C.Structure):
@@ -95,6 +98,12 @@ def lib():
             getattr(L, nm).argtypes = [C.c_void_p, C.c_int64, _dp]
         L.hoc_get_instance_times.argtypes = [C.c_void_p, C.c_int32, _dp]
         L.hoc_write_vtk.argtypes = [C.c_void_p, C.c_char_p]
+        L.hoc_get_node_fields.argtypes = [C.c_void_p, C.c_int32, C.c_int32, _dp]
+        L.hoc_write_smooth_vtk.argtypes = [C.c_void_p, C.c_int32, C.c_char_p]
+        L.hoc_set_sample_points.argtypes = [C.c_void_p, C.c_int32, C.c_void_p]
+        L.hoc_get_sample_location.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]
+        L.hoc_get_sample_values.argtypes = [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p]
+        L.hoc_write_samples.argtypes = [C.c_void_p, C.c_int32, C.c_char_p]
         for nm in ("hoc_set_vorticity", "hoc_get_vorticity", "hoc_get_streamfunction", "hoc_set_streamfunction",
                    "hoc_get_velocity", "hoc_get_solution_points", "hoc_get_solution_areas"):
             getattr(L, nm).argtypes = [C.c_void_p, C.c_int64, _dp]
@@ -224,6 +233,52 @@ class HO2D:
 
     def write_vtk(self, path):
         self._chk(lib().hoc_write_vtk(self.h, os.fsencode(path)))
+
+    # ---- node-averaged output and sample points (save_smooth_vtk, read_process_sample_points)
+    def node_fields(self, refresh=False, n_nodes=None):
+        """[n_nodes, 4] = vorticity, psi, u, v averaged at the mesh nodes.  refresh: recompute psi
+        and the velocity for the current vorticity first (as before a dump in the reference's
+        main loop)."""
+        n = int(self.stats()["n_nodes"] if n_nodes is None else n_nodes)
+        out = np.zeros(4 * n)
+        self._chk(lib().hoc_get_node_fields(self.h, int(bool(refresh)), n, out))
+        return out.reshape(n, 4)
+
+    def write_smooth_vtk(self, path, refresh=False):
+        self._chk(lib().hoc_write_smooth_vtk(self.h, int(bool(refresh)), os.fsencode(path)))
+
+    def set_sample_points(self, xy):
+        xy = np.ascontiguousarray(xy, dtype=np.float64).reshape(-1, 2)
+        self._nsp = len(xy)
+        self._chk(lib().hoc_set_sample_points(self.h, self._nsp, xy.ctypes.data if self._nsp else None))
+
+    def load_sample_points(self, path):
+        """Reads "x y" pairs as read_process_sample_points does (src/HO_2D.cpp:323-336): a last pair
+        within 1e-6 of the previous one is dropped."""
+        v = np.array(open(path).read().split(), dtype=np.float64)
+        xy = v[: len(v) // 2 * 2].reshape(-1, 2)
+        if len(xy) >= 2 and np.hypot(*(xy[-1] - xy[-2])) < 1e-6:
+            xy = xy[:-1]
+        self.set_sample_points(xy)
+        return xy
+
+    def sample_location(self):
+        n = getattr(self, "_nsp", 0)
+        el = np.zeros(n, dtype=np.int32)
+        loc = np.zeros(2 * n)
+        self._chk(lib().hoc_get_sample_location(self.h, n, el.ctypes.data if n else None,
+                                                loc.ctypes.data if n else None))
+        return el, loc.reshape(-1, 2)
+
+    def sample_values(self, refresh=False):
+        """[n, 3] = vorticity, u, v at the sample points."""
+        n = getattr(self, "_nsp", 0)
+        out = np.zeros(3 * n)
+        self._chk(lib().hoc_get_sample_values(self.h, int(bool(refresh)), n, out.ctypes.data if n else None))
+        return out.reshape(-1, 3)
+
+    def write_samples(self, path, refresh=False):
+        self._chk(lib().hoc_write_samples(self.h, int(bool(refresh)), os.fsencode(path)))
 
     def setup(self):
         self._chk(lib().hoc_setup(self.h))

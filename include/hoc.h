@@ -74,6 +74,7 @@ typedef struct {
   double fine_op_bytes;           /* algorithmic bytes of one fine-operator application  */
   int32_t last_refinements;       /* HOC_MIXED: FP64 refinement passes of the last solve */
   int32_t precision;              /* HOC_FP64 or HOC_MIXED                               */
+  int64_t n_nodes;                /* mesh nodes (the node-averaged output's length)      */
 } hoc_stats;
 
 /* Fill cfg with the reference defaults (src/HO_2D.cpp:3698-3713, 3066-3069). */
@@ -134,6 +135,32 @@ int hoc_get_solution_areas(hoc_t h, int64_t len, double* a);     /* = getsolnare
  * K x K point lattice) with vorticity, streamfunction, u and v (the fields that
  * save_vorticity_vtk / save_smooth_vtk write, src/HO_2D.cpp:3369-3692). */
 int hoc_write_vtk(hoc_t h, const char* path);
+/* Node-averaged fields (save_smooth_vtk, src/HO_2D.cpp:3369-3531): vorticity, psi and velocity
+ * extrapolated from each element's solution points to its geometry nodes, with the boundary
+ * vorticity and the boundary velocity (normal and tangential values turned Cartesian) on boundary
+ * faces, averaged over the elements sharing each mesh node.  out: [n_nodes][4] = vorticity,
+ * psi, u, v (n_nodes = the mesh's node count).  refresh != 0 first sets the boundary values at
+ * the current time and recomputes psi and the velocity for the current vorticity, as the
+ * reference's main loop does before each dump (src/HO_2D.cpp:1411-1414); refresh == 0 uses the
+ * state left by the last stage, as trigger_write does (src/HO_2D.cpp:4262-4271). */
+int hoc_get_node_fields(hoc_t h, int32_t refresh, int32_t n_nodes, double* out);
+/* The same fields as a legacy-VTK file of the mesh nodes in the reference's layout
+ * (src/HO_2D.cpp:3534-3589: POINTS, CELLS of lnod^2 Gmsh-ordered nodes, cell type 70,
+ * SCALARS vorticity, SCALARS psi, VECTORS velocity). */
+int hoc_write_smooth_vtk(hoc_t h, int32_t refresh, const char* path);
+/* Sample points (read_process_sample_points, src/HO_2D.cpp:319-474, for points passed as an
+ * array instead of a file): each point's containing element (element bounding box, then a
+ * ray-crossing test of its sides; the nearest mesh node's element when no element contains it)
+ * and its local coordinates by Newton's method, with the reference's stopping test.  The
+ * reference draws the ray's slope at random; here it is fixed.  n = 0 clears the points. */
+int hoc_set_sample_points(hoc_t h, int32_t n, const double* xy);
+/* Containing element (n) and local coordinates (2n, xi then eta) of each sample point. */
+int hoc_get_sample_location(hoc_t h, int32_t n, int32_t* elem, double* local_xy);
+/* Vorticity, u and v at the sample points, interpolated with the geometry shape functions from
+ * the node-averaged fields (src/HO_2D.cpp:3593-3612).  out: [n][3]; refresh as above. */
+int hoc_get_sample_values(hoc_t h, int32_t refresh, int32_t n, double* out);
+/* The same values as a text file of lines "x y vorticity u v" (src/HO_2D.cpp:3593-3612). */
+int hoc_write_samples(hoc_t h, int32_t refresh, const char* path);
 double hoc_get_time(hoc_t h);
 int hoc_set_time(hoc_t h, double t);
 

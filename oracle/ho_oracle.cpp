@@ -2078,6 +2078,205 @@ struct ho_oracle {
       for (int i = 0; i < D; ++i) vorticity[i] += (k1[i] + 2. * k2[i] + 2. * k3[i] + k4[i]) / 6.;
     } else throw std::runtime_error("time_integration_type must be 1, 2 or 4");
   }
+
+  // --- src/HO_2D.cpp:3369-3531 (save_smooth_vtk without the file) --------------------------
+  // Node-averaged fields: out[node][4] = vorticity, stream function, u, v.
+  void smooth_fields(double* out) const {
+    const int L = Lnod, L2 = L * L;
+    const size_t NL = (size_t)N_el * L2;
+    std::vector<double> om(NL, 0.), ps(NL, 0.), ux(NL, 0.), uy(NL, 0.);
+    for (int el = 0; el < N_el; ++el)  // :3413-3428, sps extrapolated to the geometry nodes
+      for (int Ly = 0; Ly < L; ++Ly)
+        for (int Lx = 0; Lx < L; ++Lx) {
+          const size_t g = (size_t)el * L2 + tensor2FEM(Lx, Ly);
+          for (int ky = 0; ky < K; ++ky)
+            for (int kx = 0; kx < K; ++kx) {
+              const double c = sgb[ky * L + Ly] * sgb[kx * L + Lx];
+              const int v = v3(el, ky, kx);
+              om[g] += c * vorticity[v];
+              ps[g] += c * stream_function[v];
+              ux[g] += c * vel[2 * v];
+              uy[g] += c * vel[2 * v + 1];
+            }
+        }
+    // :3431-3478, Cartesian boundary velocity from the normal and tangential values
+    std::vector<double> cx((size_t)N_eb * K, 0.), cy((size_t)N_eb * K, 0.);
+    for (int eb = 0; eb < N_eb; ++eb) {
+      const int el = be_elem[eb], side = be_side[eb], ijp = f2i[side], dir = ijp / 2;
+      const double parl_sign = -1. + 2. * (side / 2);
+      for (int k = 0; k < K; ++k) {
+        const int f = f3(el, ijp, k);
+        const double dxx = f_dxdxi[f], dxe = f_dxdeta[f], dyx = f_dydxi[f], dye = f_dydeta[f];
+        const double g1 = std::sqrt(dxx * dxx + dyx * dyx), g2 = std::sqrt(dxe * dxe + dye * dye);
+        const int idx = b2(eb, (side / 2) * (K - 2 * k - 1) + k);
+        if (dir == 0) {
+          const double un = BC_normal_vel[idx], ut = parl_sign * BC_parl_vel[idx];
+          cx[idx] = (1. / g2) * (un * dye + ut * dxe);
+          cy[idx] = (1. / g2) * (-un * dxe + ut * dye);
+        } else {
+          const double un = BC_normal_vel[idx], ut = parl_sign * BC_parl_vel[idx];
+          cx[idx] = (1. / g1) * (-un * dyx + ut * dxx);
+          cy[idx] = (1. / g1) * (un * dxx + ut * dyx);
+        }
+      }
+    }
+    for (int eb = 0; eb < N_eb; ++eb) {  // :3485-3506, boundary faces take the boundary values
+      const int el = be_elem[eb], side = be_side[eb], ijp = f2i[side], d = ijp / 2, t = ijp % 2;
+      for (int l = 0; l < L; ++l) {
+        const int su2pv = l * ((1 - d) * L + d) + t * (L - 1) * (d * (L - 1) + 1);
+        const size_t g = (size_t)el * L2 + tensor2FEM(su2pv % L, su2pv / L);
+        om[g] = 0.; ux[g] = 0.; uy[g] = 0.;
+        for (int m = 0; m < K; ++m) {
+          const double c = sgb[m * L + l];
+          const int nidx = b2(eb, (side / 2) * (K - 2 * m - 1) + m);
+          om[g] += c * BC_vorticity[nidx];
+          ux[g] += c * cx[nidx];
+          uy[g] += c * cy[nidx];
+        }
+      }
+    }
+    std::vector<double> sw(N_nodes, 0.), sp(N_nodes, 0.), sx(N_nodes, 0.), sy(N_nodes, 0.);
+    std::vector<int> rep(N_nodes, 0);
+    for (int el = 0; el < N_el; ++el)  // :3511-3528, averages over the elements sharing a node
+      for (int l = 0; l < L2; ++l) {
+        const int nd = elem_nodes[(size_t)el * L2 + l];
+        const size_t g = (size_t)el * L2 + l;
+        sx[nd] += 1. * ux[g]; sy[nd] += 1. * uy[g];
+        sw[nd] += om[g]; sp[nd] += ps[g];
+        rep[nd]++;
+      }
+    for (int nd = 0; nd < N_nodes; ++nd) {
+      out[4 * nd] = sw[nd] / rep[nd];
+      out[4 * nd + 1] = sp[nd] / rep[nd];
+      const double r = 1. / rep[nd];
+      out[4 * nd + 2] = sx[nd] * r;
+      out[4 * nd + 3] = sy[nd] * r;
+    }
+  }
+
+  // --- src/HO_2D.cpp:319-474 (read_process_sample_points on given points) ------------------
+  // The reference casts its point-in-element ray with a slope drawn by std::rand() (seeded by
+  // the clock); this restatement uses the fixed slope below.  The Newton iteration keeps the
+  // reference's stopping test and its use of the shape functions of the last iterate before
+  // the final update (:405-472); it is capped at 100 iterations (the reference has no cap).
+  static constexpr double kSampleSlope = 0.3819660112501051;
+  std::vector<int> sp_elem;
+  std::vector<double> sp_xy, sp_loc, sp_shape;  // sp_shape: [sp][L2] in Gmsh node order
+  static int ray_hits(double px, double py, double slope, double x1, double y1, double x2, double y2) {
+    // are_intersecting (src/MathHelpers.hpp:99-114)
+    const double dx = x2 - x1, dy = y2 - y1;
+    const double s = (slope * (x1 - px) + py - y1) / (dy - slope * dx);
+    const double t = x1 - px + s * dx;
+    return (t > -1.e-8 && s > -1.e-8 && s < 1. + 1.e-8) ? 1 : 0;
+  }
+  void set_sample_points(int n, const double* xy) {
+    const int L = Lnod, L2 = L * L;
+    sp_xy.assign(xy, xy + 2 * n);
+    sp_elem.assign(n, -1);
+    sp_loc.assign(2 * (size_t)n, 0.);
+    sp_shape.assign((size_t)n * L2, 0.);
+    std::vector<double> mnx(N_el), mxx(N_el), mny(N_el), mxy(N_el);
+    for (int el = 0; el < N_el; ++el) {  // :342-354, bounding boxes of the element edges' nodes
+      const int n0 = elem_nodes[(size_t)el * L2];
+      mnx[el] = mxx[el] = nx[n0];
+      mny[el] = mxy[el] = ny[n0];
+      for (int s = 0; s < 4; ++s)
+        for (int nd : edges[elem_edges[el * 4 + s]]) {
+          mnx[el] = std::min(mnx[el], nx[nd]); mxx[el] = std::max(mxx[el], nx[nd]);
+          mny[el] = std::min(mny[el], ny[nd]); mxy[el] = std::max(mxy[el], ny[nd]);
+        }
+    }
+    for (int sp = 0; sp < n; ++sp) {  // :357-395
+      const double px = xy[2 * sp], py = xy[2 * sp + 1];
+      bool found = false;
+      for (int el = 0; el < N_el && !found; ++el) {
+        if (px < mnx[el] || px > mxx[el] || py < mny[el] || py > mxy[el]) continue;
+        int hits = 0;
+        for (int s = 0; s < 4; ++s) {
+          const std::vector<int>& en = edges[elem_edges[el * 4 + s]];
+          for (int seg = 0; seg < Lnod_in; ++seg) {
+            const int a = en[tensor2FEM(seg)], b = en[tensor2FEM(seg + 1)];
+            hits += ray_hits(px, py, kSampleSlope, nx[a], ny[a], nx[b], ny[b]);
+          }
+        }
+        if (hits & 1) { sp_elem[sp] = el; found = true; }
+      }
+      if (!found) {
+        const int n0 = elem_nodes[0];
+        double dist = std::sqrt((px - nx[n0]) * (px - nx[n0]) + (py - ny[n0]) * (py - ny[n0])) + 1.;
+        for (int el = 0; el < N_el; ++el)
+          for (int i = 0; i < L2; ++i) {
+            const int nd = elem_nodes[(size_t)el * L2 + i];
+            const double d = std::sqrt((px - nx[nd]) * (px - nx[nd]) + (py - ny[nd]) * (py - ny[nd]));
+            if (dist > d) { dist = d; sp_elem[sp] = el; }
+          }
+      }
+      // :405-472, local coordinates by Newton's method
+      const int el = sp_elem[sp];
+      std::vector<double> bx(L), by(L), gx(L), gy(L);
+      double xsi = 0.08, eta = 0.08, dxi = 0., det_ = 0.;
+      int it = 0;
+      do {
+        for (int k = 0; k < L; ++k) {
+          gx[k] = gy[k] = 0.;
+          bx[k] = by[k] = 1.;
+          double den = 1.;
+          for (int j = 0; j < L; ++j) {
+            if (j == k) continue;
+            den *= gps[k] - gps[j];
+            bx[k] *= xsi - gps[j];
+            by[k] *= eta - gps[j];
+            double nxi = 1., net = 1.;
+            for (int i = 0; i < L; ++i) {
+              if (i == k || i == j) continue;
+              nxi *= xsi - gps[i];
+              net *= eta - gps[i];
+            }
+            gx[k] += nxi;
+            gy[k] += net;
+          }
+          gx[k] /= den; bx[k] /= den; gy[k] /= den; by[k] /= den;
+        }
+        double f = px, g = py, fx = 0., fe = 0., gxx = 0., ge = 0.;
+        for (int j = 0; j < L; ++j)
+          for (int i = 0; i < L; ++i) {
+            const int nd = elem_nodes[(size_t)el * L2 + tensor2FEM(i, j)];
+            f -= bx[i] * by[j] * nx[nd];
+            g -= bx[i] * by[j] * ny[nd];
+            fx -= by[j] * gx[i] * nx[nd];
+            gxx -= by[j] * gx[i] * ny[nd];
+            fe -= gy[j] * bx[i] * nx[nd];
+            ge -= gy[j] * bx[i] * ny[nd];
+          }
+        dxi = -1. / (fx * ge - fe * gxx) * (f * ge - g * fe);
+        det_ = -1. / (fx * ge - fe * gxx) * (g * fx - f * gxx);
+        xsi += dxi;
+        eta += det_;
+      } while ((std::fabs(dxi) > 0.001 || std::fabs(det_) > 0.001) && ++it < 100);
+      sp_loc[2 * sp] = xsi;
+      sp_loc[2 * sp + 1] = eta;
+      for (int j = 0; j < L; ++j)
+        for (int i = 0; i < L; ++i) sp_shape[(size_t)sp * L2 + tensor2FEM(i, j)] = bx[i] * by[j];
+    }
+  }
+  // src/HO_2D.cpp:3593-3612: out[sp][3] = vorticity, u, v from the node-averaged fields.
+  void sample_values(double* out) const {
+    const int L2 = Lnod * Lnod, n = (int)sp_elem.size();
+    std::vector<double> nodes((size_t)N_nodes * 4);
+    smooth_fields(nodes.data());
+    for (int sp = 0; sp < n; ++sp) {
+      double w = 0., u = 0., v = 0.;
+      const int el = sp_elem[sp];
+      for (int l = 0; l < L2; ++l) {
+        const int nd = elem_nodes[(size_t)el * L2 + l];
+        const double c = sp_shape[(size_t)sp * L2 + l];
+        w += c * nodes[4 * nd];
+        u += c * nodes[4 * nd + 2];
+        v += c * nodes[4 * nd + 3];
+      }
+      out[3 * sp] = w; out[3 * sp + 1] = u; out[3 * sp + 2] = v;
+    }
+  }
 };
 
 static int fail(ho_oracle* h, const std::exception& e) {
@@ -2328,6 +2527,33 @@ int ho_oracle_set_bc(ho_oracle* h, const char* name, const double* vals) {
   else if (n == "BC_parl_vel") v = &h->BC_parl_vel;
   else return -1;
   std::copy(vals, vals + v->size(), v->begin());
+  return 0;
+}
+
+int ho_oracle_n_nodes(ho_oracle* h) { return h->N_nodes; }
+int ho_oracle_smooth_fields(ho_oracle* h, double* out) {
+  try { h->smooth_fields(out); } catch (const std::exception& e) { return fail(h, e); }
+  return 0;
+}
+int ho_oracle_set_sample_points(ho_oracle* h, int n, const double* xy) {
+  try { h->set_sample_points(n, xy); } catch (const std::exception& e) { return fail(h, e); }
+  return 0;
+}
+int ho_oracle_sample_location(ho_oracle* h, int32_t* elem, double* loc) {
+  std::copy(h->sp_elem.begin(), h->sp_elem.end(), elem);
+  std::copy(h->sp_loc.begin(), h->sp_loc.end(), loc);
+  return 0;
+}
+int ho_oracle_sample_values(ho_oracle* h, double* out) {
+  try { h->sample_values(out); } catch (const std::exception& e) { return fail(h, e); }
+  return 0;
+}
+int ho_oracle_refresh(ho_oracle* h) {  // src/HO_2D.cpp:1411-1414, before each dump
+  try {
+    h->update_BCs(h->current_time);
+    h->solve_Poisson(h->vorticity.data());
+    h->calc_velocity();
+  } catch (const std::exception& e) { return fail(h, e); }
   return 0;
 }
 

@@ -114,6 +114,20 @@ int ho_oracle_apply_laplacian_bc(ho_oracle*, const double* x, const double* bcva
 int ho_oracle_solve_poisson_bc(ho_oracle*, const double* w, double* psi);
 int ho_oracle_velocity_from_psi(ho_oracle*, double* uv);
 
+/* Node-averaged output (save_smooth_vtk, src/HO_2D.cpp:3369-3531): out[n_nodes][4] = vorticity,
+ * stream function, u, v at the mesh nodes. */
+int ho_oracle_n_nodes(ho_oracle*);
+int ho_oracle_smooth_fields(ho_oracle*, double* out);
+/* Sample points (read_process_sample_points, src/HO_2D.cpp:319-474, with a fixed ray slope):
+ * containing element, local coordinates, and vorticity/u/v interpolated from the node-averaged
+ * fields (src/HO_2D.cpp:3593-3612), out[n][3]. */
+int ho_oracle_set_sample_points(ho_oracle*, int n, const double* xy);
+int ho_oracle_sample_location(ho_oracle*, int32_t* elem, double* local_xy);
+int ho_oracle_sample_values(ho_oracle*, double* out);
+/* update_BCs(t), solve_Poisson(vorticity), velocity: what the reference's main loop runs
+ * before each dump (src/HO_2D.cpp:1411-1414). */
+int ho_oracle_refresh(ho_oracle*);
+
 #ifdef __cplusplus
 }
 #endif

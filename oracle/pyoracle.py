@@ -59,6 +59,13 @@ def lib():
                    "ho_oracle_set_p2g_weights"):
             getattr(L, nm).argtypes = [C.c_void_p, _dp]
         L.ho_oracle_solveto.argtypes = [C.c_void_p, C.c_double, C.c_int, C.c_int, C.c_double]
+        L.ho_oracle_n_nodes.argtypes = [C.c_void_p]
+        L.ho_oracle_n_nodes.restype = C.c_int
+        L.ho_oracle_smooth_fields.argtypes = [C.c_void_p, _dp]
+        L.ho_oracle_set_sample_points.argtypes = [C.c_void_p, C.c_int, _dp]
+        L.ho_oracle_sample_location.argtypes = [C.c_void_p, _ip, _dp]
+        L.ho_oracle_sample_values.argtypes = [C.c_void_p, _dp]
+        L.ho_oracle_refresh.argtypes = [C.c_void_p]
         L.ho_oracle_set_threads.argtypes = [C.c_int]
         L.ho_oracle_set_threads.restype = C.c_int
         L.ho_oracle_set_threads(1)   # deterministic single-thread checker unless asked otherwise
@@ -216,3 +223,31 @@ class Oracle:
         out = np.zeros(2 * self.D)
         self._chk(lib().ho_oracle_velocity_from_psi(self.h, out))
         return out.reshape(-1, 2)
+
+    def refresh(self):
+        """Boundary values, stream function and velocity for the current vorticity (what the
+        reference's main loop runs before each dump)."""
+        self._chk(lib().ho_oracle_refresh(self.h))
+
+    def smooth_fields(self):
+        """Node-averaged [n_nodes, 4] = vorticity, stream function, u, v."""
+        n = lib().ho_oracle_n_nodes(self.h)
+        out = np.zeros(4 * n)
+        self._chk(lib().ho_oracle_smooth_fields(self.h, out))
+        return out.reshape(n, 4)
+
+    def set_sample_points(self, xy):
+        xy = np.ascontiguousarray(xy, dtype=np.float64).reshape(-1, 2)
+        self._nsp = len(xy)
+        self._chk(lib().ho_oracle_set_sample_points(self.h, self._nsp, xy.ravel()))
+
+    def sample_location(self):
+        el = np.zeros(self._nsp, dtype=np.int32)
+        loc = np.zeros(2 * self._nsp)
+        self._chk(lib().ho_oracle_sample_location(self.h, el, loc))
+        return el, loc.reshape(-1, 2)
+
+    def sample_values(self):
+        out = np.zeros(3 * self._nsp)
+        self._chk(lib().ho_oracle_sample_values(self.h, out))
+        return out.reshape(-1, 3)
