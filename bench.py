@@ -52,10 +52,10 @@ def cpu_threads():
     return max(1, min(16, n))
 
 
-def cpu_baseline(K, Re, tol, n=64, warm=2, steps=4):
-    """The FP64 oracle on a bounded sample of the workload, on 1 thread (the reference's
-    default serial build) and on all host threads (OpenMP linear algebra; the FR residual
-    loops, < 2% of the oracle's step, stay serial)."""
+def cpu_baseline(K, Re, tol, n=64, warm=2, steps_1=6, steps_n=16):
+    """The FP64 oracle on a bounded sample of the workload (about 20 s of CPU work), on 1 thread
+    (the reference's default serial build; steps_1 timed steps) and on all host threads (OpenMP
+    linear algebra; the FR residual loops, < 2% of the oracle's step, stay serial; steps_n)."""
     import pyoracle
     from hoc import meshes
     from pyoracle import Oracle
@@ -64,6 +64,7 @@ def cpu_baseline(K, Re, tol, n=64, warm=2, steps=4):
     D = n * n * K * K
     res = {}
     for nt in (1, cpu_threads()):
+        steps = steps_1 if nt == 1 else steps_n
         pyoracle.set_threads(nt)
         t0 = time.perf_counter()
         o = Oracle(m, K, problem_type=1, Re=Re, dt=dt, time_integration_type=4, advection_type=1, tol=tol, maxiter=150)
@@ -72,17 +73,17 @@ def cpu_baseline(K, Re, tol, n=64, warm=2, steps=4):
         t0 = time.perf_counter()
         o.step(steps)
         el = time.perf_counter() - t0
-        res[nt] = (D * steps / el / 1e6, el, setup)
+        res[nt] = (D * steps / el / 1e6, el, setup, steps)
     pyoracle.set_threads(1)
     nt = cpu_threads()
-    v1, el1, _ = res[1]
-    vn, eln, setup = res[nt]
+    v1, el1, _, s1 = res[1]
+    vn, eln, setup, sn = res[nt]
     return {"value": vn, "unit": "Mdof-steps/s", "cores": nt, "kind": "port",
             "sample": f"oracle (FP64 CPU restatement of the reference algorithm; OpenMP SpMV/vector/dot in the "
                       f"AMG-preconditioned Krylov solve) on a {n}x{n} cavity, K={K}, Re={Re}, dt={dt:.3g}, "
-                      f"tol={tol:g}: {warm} warm-up + {steps} timed RK4 steps ({eln:.1f} s timed on {nt} threads, "
+                      f"tol={tol:g}: {warm} warm-up + {sn} timed RK4 steps ({eln:.1f} s timed on {nt} threads, "
                       f"setup {setup:.1f} s)",
-            "single_thread": {"value": v1, "cores": 1, "timed_s": el1}}
+            "single_thread": {"value": v1, "cores": 1, "timed_s": el1, "timed_steps": s1}}
 
 
 def measured_traffic(kernel, n, K):
