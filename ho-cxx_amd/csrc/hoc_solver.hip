@@ -129,6 +129,8 @@ struct Solver {
   // live kernel timing (hoc_profile_*)
   struct Prof {
     bool on = false;
+    int every = 1;      // record every every-th launch
+    int64_t seen = 0;   // launches since profiling was turned on
     std::vector<hipEvent_t> e0, e1;
     std::vector<int> var;
     std::vector<double> nbytes;
@@ -308,7 +310,7 @@ struct Solver {
   template <class R>
   void lap(Prec<R>& P, int epi, R* x, R* y, const R* rr, const R* dg, R* d, double c1, double c2, double* partp,
            const int* active, const R* tin = nullptr, R* tout = nullptr) {
-    const bool rec = prof.on;
+    const bool rec = prof.on && (prof.seen++ % prof.every) == 0;
     if (rec) HIPCHK(hipEventRecord(prof.e0[prof.n], stream));
     HIPCHK(launch_lap(K, epi, N, P.T, P.G, x, tin ? tin : P.tr.p, y, rr, dg, d, c1, c2, partp, active, stream, tout));
     if (rec) {
@@ -1441,6 +1443,8 @@ int hoc_profile_enable(hoc_t h, int32_t on) {
     }
     if (on) for (int v = 0; v < HOC_PROF_VARIANTS; ++v) { s.prof.ms[v] = 0.; s.prof.cnt[v] = 0; s.prof.bytes[v] = 0.; }
     s.prof.on = on != 0;
+    s.prof.every = on > 1 ? on : 1;
+    s.prof.seen = 0;
   });
 }
 

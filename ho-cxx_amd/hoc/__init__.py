@@ -342,9 +342,10 @@ class HO2D:
         self._chk(lib().hoc_stats_get(self.h, C.byref(s)))
         return {f: getattr(s, f) for f, _ in s._fields_}
 
-    def profile(self, on=True):
-        """Turn live per-launch timing of the fine-level operator kernel on/off."""
-        self._chk(lib().hoc_profile_enable(self.h, int(on)))
+    def profile(self, on=True, every=1):
+        """Turn live per-launch timing of the fine-level operator kernel on/off; every > 1 times
+        every every-th launch only."""
+        self._chk(lib().hoc_profile_enable(self.h, max(1, int(every)) if on else 0))
 
     def profile_read(self):
         p = hoc_kernel_profile()

@@ -196,8 +196,10 @@ int hoc_apply_laplacian(hoc_t h, const double* x, double* y);
 int hoc_stats_get(hoc_t h, hoc_stats* s);
 
 /* Live timing of the fine-level Poisson operator kernel (the dominant kernel): with
- * profiling on, every launch is bracketed by HIP events recorded on the handle's own
- * stream.  One entry per fused epilogue variant: entries 0-4 FP64, 5-9 FP32. */
+ * profiling on, launches are bracketed by HIP events recorded on the handle's own stream
+ * (every launch for on == 1, every on-th launch for on > 1: each event pair costs a little
+ * device time, so sampling keeps a timed run's throughput intact).  One entry per fused
+ * epilogue variant: entries 0-4 FP64, 5-9 FP32. */
 #define HOC_PROF_VARIANTS 10
 typedef struct {
   int64_t launches[HOC_PROF_VARIANTS];
@@ -205,7 +207,8 @@ typedef struct {
   double bytes_per_launch[HOC_PROF_VARIANTS];  /* algorithmic HBM bytes per launch (average) */
   char names[HOC_PROF_VARIANTS][24];
 } hoc_kernel_profile;
-int hoc_profile_enable(hoc_t h, int32_t on);   /* on != 0 also clears the counters */
+int hoc_profile_enable(hoc_t h, int32_t on);   /* on: 0 off, 1 every launch, n > 1 every n-th;
+                                                  on != 0 also clears the counters */
 int hoc_profile_get(hoc_t h, hoc_kernel_profile* p);
 
 const char* hoc_last_error(hoc_t h);   /* h may be NULL for hoc_create failures */
