@@ -95,3 +95,42 @@ def test_ensemble_errors(gpu_available):
     with pytest.raises(hoc.HocError) as e:
         g.solveto(1e-3, 2, 4, 100.0)
     assert e.value.code == hoc.HOC_EINVAL
+
+
+def test_64_instance_ensemble_full_size(gpu_available):
+    """The batched-ensemble workload at its size: 64 cavity instances with seeded sizes between
+    16^2 and 256^2 elements (uneven, 1.5 M elements in all), K=5, Re log-uniform in [100, 3200],
+    lid speed 1, per-instance dt (half the RK4 diffusion limit, capped by an advective CFL of
+    0.05 per element width).  A few RK4 steps in FP64 and in mixed precision: every instance's
+    mixed fields within 1e-8 of its FP64 fields, and the two smallest instances against
+    separate FP64 oracle runs."""
+    import os
+    import sys
+    sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    from bench import cavity_dt
+    rng = np.random.default_rng(2024)
+    ns = rng.integers(16, 257, size=64)
+    Re = np.exp(rng.uniform(np.log(100.0), np.log(3200.0), size=64))
+    K = 5
+    dt = np.array([min(cavity_dt(int(n), K, r), 0.05 / n) for n, r in zip(ns, Re)])
+    parts = [meshes.cavity(int(n), 2) for n in ns]
+    steps = 3
+    g64, off = run_ensemble(parts, K, Re, dt, steps, tol=1e-10)
+    g32, _ = run_ensemble(parts, K, Re, dt, steps, tol=1e-10, precision=hoc.HOC_MIXED)
+    assert off[-1] == int(np.sum(ns * ns))
+    w64, p64 = g64.get_vorticity(), g64.get_streamfunction()
+    w32, p32 = g32.get_vorticity(), g32.get_streamfunction()
+    worst = 0.0
+    for i in range(64):
+        sl = slice(off[i] * K * K, off[i + 1] * K * K)
+        worst = max(worst, rel_l2(w32[sl], w64[sl]), rel_l2(p32[sl], p64[sl]))
+    print("64 instances,", off[-1], "elements: worst per-instance mixed-vs-FP64 error", worst)
+    assert worst < 1e-8
+    for i in np.argsort(ns)[:2]:
+        o = Oracle(parts[i], K, problem_type=1, Re=Re[i], dt=dt[i], time_integration_type=4, advection_type=1,
+                   tol=1e-12, maxiter=800)
+        o.step(steps)
+        sl = slice(off[i] * K * K, off[i + 1] * K * K)
+        ew = rel_l2(w64[sl], o.vorticity())
+        print("instance", i, "n", ns[i], "omega vs oracle", ew)
+        assert ew < 1e-8
