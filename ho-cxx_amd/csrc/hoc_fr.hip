@@ -655,8 +655,15 @@ __global__ __launch_bounds__(kWG) void k_velocity(int N, Tab T, Geo G, BCDev B, 
 // update_advection_BC (:2219-2243), advection type 1 (:2245-2437) or 2 (:2439-2645),
 // k = dt (adv + diff) (:1833-1836) and the RK update (:1465-1531).  AFF: metrics constant per
 // element (G.gel, G.gface, G.gjm).
+// MODE 0: the stage.  MODE 1 (advection type 1): the stage, recording the upwind switch of
+// every interior flux point in so.sig: sign(dw) sign(fo - s fn) where |dw| > 1e-4, else 0, so
+// that the upwind term 1/2 s_f |a| dw equals 1/2 s_f sig (fo - s fn).  MODE 2 (advection type
+// 1): the stage map with the switches frozen at so.sig, which is bilinear in (U, omega) and
+// linear in the boundary values: diffusion of omega/trw with the boundary values in B, advection
+// of so.omega_adv/so.trw_adv by U/tru; boundary arrays are not written.  Called with (U, domega)
+// or (dU, omega) and homogeneous boundary values it gives the stage's tangent.
 // ==========================================================================================
-template <int K, int ADV, bool AFF>
+template <int K, int ADV, bool AFF, int MODE = 0>
 __global__ __launch_bounds__(kWG) void k_stage(int N, Tab T, Geo G, BCDev B, const double* __restrict__ omega,
                                                 const double* __restrict__ U, const double* __restrict__ trw,
                                                 const double* __restrict__ tru, double re_inv, double dt,
@@ -674,6 +681,17 @@ __global__ __launch_bounds__(kWG) void k_stage(int N, Tab T, Geo G, BCDev B, con
   tile_load<2 * K2>(U, lds, e0, N);
 #pragma unroll
   for (int m = 0; m < K2; ++m) { U1[m] = lds[threadIdx.x * 2 * K2 + m]; U2[m] = lds[threadIdx.x * 2 * K2 + K2 + m]; }
+  double wa[MODE == 2 ? K2 : 1];  // MODE 2: the advected vorticity
+  if constexpr (MODE == 2) {
+    __syncthreads();
+    tile_load<K2>(so.omega_adv, lds, e0, N);
+#pragma unroll
+    for (int m = 0; m < K2; ++m) wa[m] = lds[threadIdx.x * K2 + m];
+  }
+  auto wadv = [&](int m) -> double {
+    if constexpr (MODE == 2) return wa[m]; else return w[m];
+  };
+  const double* trwa = MODE == 2 ? so.trw_adv : trw;
   double kk[K2];
   if (live) {
     const size_t base = (size_t)el * K2;
@@ -681,6 +699,14 @@ __global__ __launch_bounds__(kWG) void k_stage(int N, Tab T, Geo G, BCDev B, con
     double bv[4][K], bg[4][K], cv[4][K], cg[4][K];
     traces<K>(T, w, bv, bg);
     faces2<K, AFF>(T, G, el, trw, G.sw_diff, B.diff, false, bv, bg, cv, cg);
+    double bva[MODE == 2 ? 4 : 1][K];
+    if constexpr (MODE == 2) {
+      double bga[4][K];
+      traces<K>(T, wa, bva, bga);
+    }
+    auto bvadv = [&](int f, int k) -> double {
+      if constexpr (MODE == 2) return bva[f][k]; else return bv[f][k];
+    };
     double dif[K2];
     double ej = 0.;
     if (AFF) {
@@ -705,9 +731,11 @@ __global__ __launch_bounds__(kWG) void k_stage(int N, Tab T, Geo G, BCDev B, con
       for (int k = 0; k < K; ++k) {
         const int fp = edge_fp<K>(f, k);
         const double vort = cv[f][k];
-        B.vort[(size_t)e * K + fp] = vort;
         const double a = vort * B.nvel[(size_t)e * K + fp] * G.fAn[q + k];
-        B.adv[(size_t)e * K + fp] = a;
+        if constexpr (MODE != 2) {
+          B.vort[(size_t)e * K + fp] = vort;
+          B.adv[(size_t)e * K + fp] = a;
+        }
         bca[f][k] = a;
       }
     }
@@ -740,7 +768,7 @@ __global__ __launch_bounds__(kWG) void k_stage(int N, Tab T, Geo G, BCDev B, con
       const int fn = G.nbf[(size_t)el * 4 + f];
       const int rev = rev_flag(f, fn);
       const double sgn = face_sign(f, fn);
-      const double* tw = trw + ((size_t)fn * N + nb) * 2 * K;
+      const double* tw = trwa + ((size_t)fn * N + nb) * 2 * K;
       const double* tuu = tru + ((size_t)fn * N + nb) * K;
       double wn[K], un[K];
 #pragma unroll
@@ -752,13 +780,24 @@ __global__ __launch_bounds__(kWG) void k_stage(int N, Tab T, Geo G, BCDev B, con
           const double wnk = rev ? wn[K - 1 - k] : wn[k];
           const double unk = rev ? un[K - 1 - k] : un[k];
           (void)kn;
-          const double fo = bu[f][k] * bv[f][k];
+          const double fo = bu[f][k] * bvadv(f, k);
           const double fnn = unk * wnk;
           double up = 0.5 * (fo + sgn * fnn);
-          const double dw = bv[f][k] - wnk;
-          if (fabs(dw) > 1.e-4) {
-            const double a = (fo - sgn * fnn) / dw;
-            up += 0.5 * ((f & 1) ? 1. : -1.) * fabs(a) * dw;
+          const double dw = bvadv(f, k) - wnk;
+          if constexpr (MODE == 2) {
+            const double sg = (double)so.sig[((size_t)el * 4 + f) * K + k];
+            up += 0.5 * ((f & 1) ? 1. : -1.) * sg * (fo - sgn * fnn);
+          } else {
+            if (fabs(dw) > 1.e-4) {
+              const double a = (fo - sgn * fnn) / dw;
+              up += 0.5 * ((f & 1) ? 1. : -1.) * fabs(a) * dw;
+            }
+            if constexpr (MODE == 1) {
+              const double df = fo - sgn * fnn;
+              const int sd = dw > 0. ? 1 : -1;
+              const int sf = df > 0. ? 1 : (df < 0. ? -1 : 0);
+              so.sig[((size_t)el * 4 + f) * K + k] = (int8_t)(fabs(dw) > 1.e-4 ? sd * sf : 0);
+            }
           }
           upw[f][k] = up;
         }
@@ -770,8 +809,8 @@ __global__ __launch_bounds__(kWG) void k_stage(int N, Tab T, Geo G, BCDev B, con
           const double unk = rev ? un[K - 1 - k] : un[k];
           const double c = 0.5 * (bu[f][k] + sgn * unk);
           cvf[f][k] = c;
-          if (c > 0.) upw[f][k] = c * (t * bv[f][k] + (1. - t) * wnk);
-          else upw[f][k] = c * ((1. - t) * bv[f][k] + t * wnk);
+          if (c > 0.) upw[f][k] = c * (t * bvadv(f, k) + (1. - t) * wnk);
+          else upw[f][k] = c * ((1. - t) * bvadv(f, k) + t * wnk);
         }
       }
     }
@@ -790,7 +829,7 @@ __global__ __launch_bounds__(kWG) void k_stage(int N, Tab T, Geo G, BCDev B, con
     }
     double f1[K2], f2[K2];  // discontinuous fluxes U^1 w, U^2 w at the sps
 #pragma unroll
-    for (int m = 0; m < K2; ++m) { f1[m] = U1[m] * w[m]; f2[m] = U2[m] * w[m]; }
+    for (int m = 0; m < K2; ++m) { f1[m] = U1[m] * wadv(m); f2[m] = U2[m] * wadv(m); }
     double bdf[4][K];
 #pragma unroll
     for (int ib = 0; ib < 2; ++ib)
@@ -1023,14 +1062,18 @@ hipError_t launch_velocity(int K, int N, const Tab& T, const Geo& G, const BCDev
 
 hipError_t launch_stage(int K, int adv, int N, const Tab& T, const Geo& G, const BCDev& B, const double* omega,
                         const double* U, const double* trw, const double* tru, double re_inv, double dt,
-                        const InstDev& I, const StageOut& so, hipStream_t s) {
+                        const InstDev& I, const StageOut& so, hipStream_t s, int mode) {
   const bool aff = G.gel && G.gface && G.gjm;
-#define LS(A, F)                                                                                                  \
-  hipLaunchKernelGGL((k_stage<KK, A, F>), dim3(nblk(N)), dim3(kWG), 0, s, N, T, G, B, omega, U, trw, tru, re_inv, dt, \
-                     I, so)
+  if (mode != 0 && (adv != 1 || !so.sig || (mode == 2 && (!so.omega_adv || !so.trw_adv))))
+    return hipErrorInvalidValue;
+#define LS(A, F, M)                                                                                           \
+  hipLaunchKernelGGL((k_stage<KK, A, F, M>), dim3(nblk(N)), dim3(kWG), 0, s, N, T, G, B, omega, U, trw, tru, re_inv, \
+                     dt, I, so)
   HOC_K_SWITCH(K, {
-    if (adv == 1) { if (aff) LS(1, true); else LS(1, false); }
-    else { if (aff) LS(2, true); else LS(2, false); }
+    if (mode == 1) { if (aff) LS(1, true, 1); else LS(1, false, 1); }
+    else if (mode == 2) { if (aff) LS(1, true, 2); else LS(1, false, 2); }
+    else if (adv == 1) { if (aff) LS(1, true, 0); else LS(1, false, 0); }
+    else { if (aff) LS(2, true, 0); else LS(2, false, 0); }
   });
 #undef LS
   return hipGetLastError();

@@ -16,6 +16,10 @@ struct StageOut {
   double* base = nullptr;    // omega_n (read for next; updated for final_mode)
   int final_mode = 0;        // 0 none, 1 Euler, 2 RK2 (needs k1), 4 RK4 (needs k1..k3)
   const double *k1 = nullptr, *k2 = nullptr, *k3 = nullptr;
+  // linearised stage (launch_stage mode 1 records, mode 2 reads the upwind switches)
+  int8_t* sig = nullptr;               // [N][4][K] upwind switch of each interior flux point
+  const double* omega_adv = nullptr;   // mode 2: advected vorticity and its traces
+  const double* trw_adv = nullptr;
 };
 
 // Poisson-operator kernels in FP64 or (mixed-precision preconditioner) FP32.
@@ -40,7 +44,7 @@ hipError_t launch_velocity(int K, int N, const Tab& T, const Geo& G, const BCDev
                            double* tru, double inv_dt_re, const InstDev& I, hipStream_t s);
 hipError_t launch_stage(int K, int adv, int N, const Tab& T, const Geo& G, const BCDev& B, const double* omega,
                         const double* U, const double* trw, const double* tru, double re_inv, double dt,
-                        const InstDev& I, const StageOut& so, hipStream_t s);
+                        const InstDev& I, const StageOut& so, hipStream_t s, int mode = 0);
 // n = E*K boundary values; tc = time factor (per instance through I.ramp when set)
 hipError_t launch_update_bcs(int n, int K, const BCDev& B, double tc, const InstDev& I, hipStream_t s);
 // problem_type 11: boundary values at fraction fac1 (fac2 = 1 - fac1) of the outer step,

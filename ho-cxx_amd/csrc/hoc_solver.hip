@@ -22,6 +22,7 @@
 #include <vector>
 
 #include "../../include/hoc.h"
+#include "hoc_adj.hpp"
 #include "hoc_dev.hpp"
 #include "hoc_fr.hpp"
 #include "hoc_host.hpp"
@@ -643,12 +644,13 @@ struct Solver {
   //         correction e solves As e = r with FP32 Krylov + FP32 V-cycle to the relative
   //         tolerance eta, psi += e in FP64, and the FP64 residual is recomputed; the loop
   //         ends on the same FP64 test as the plain solve.
+  const double* rhs_bc = nullptr;  // boundary stream-function values of the next solve (null: b_psi)
   void poisson(const double* omega) {
     const int lb = lap_blocks(N);
     const int maxit = cfg.maxiter;
     const double tol2 = cfg.tol * cfg.tol;
     HIPCHK(hipMemcpyAsync(S.p + S_TOL2, &tol2, sizeof(double), hipMemcpyHostToDevice, stream));
-    HIPCHK(launch_rhs(K, N, T, G, omega, b_psi, bs, part, stream));
+    HIPCHK(launch_rhs(K, N, T, G, omega, rhs_bc ? rhs_bc : b_psi.p, bs, part, stream));
     finalize(stream, PH_BB, part, lb, S, F, maxit);
     if (n_inst > 1) {
       // ||b - A psi|| <= tol min_i ||b_i|| bounds every instance's own relative residual
@@ -953,6 +955,344 @@ struct Solver {
   void set_vorticity_target(const double* wv) {
     HIPCHK(hipMemcpyAsync(d_hw0.p, d_hw1.p, D * sizeof(double), hipMemcpyDeviceToDevice, stream));
     h2d(d_hw1.p, wv, D * sizeof(double));
+  }
+
+  // ---- discrete adjoint of the RK4 trajectory -------------------------------------------
+  // One stage maps omega_s to k_s = dt R(omega_s), with psi_s = A^-1 b(omega_s) and
+  // b = -omega J - (boundary source).  Its transpose is applied as
+  //   omega_bar  = (dk/domega)^T lam                          (1-ring: omega and its face traces)
+  //   psi_bar    = (dk/dpsi)^T lam                            (2-ring: psi -> U, wall flux -> k)
+  //   omega_bar += J W z,   As z = psi_bar                    (A^-T = -W As^-1, As symmetric)
+  //   pvel_bar  += (dk/dpvel)^T lam                           (the wall velocity's flux)
+  // each product by coloured probes of the frozen-switch stage (launch_stage mode 2).
+  struct Adj {
+    bool init = false;
+    int nc1 = 0, nc2 = 0;
+    DBuf<int32_t> col1, col2, r1p, r1i, r2p, r2i, own_el;
+    DBuf<int8_t> own_f;
+    DBuf<double> ws[4], Us[4], trws[4], trus[4];  // recorded stages of one step
+    DBuf<int8_t> sig[4];
+    DBuf<double> seed, seed_tr, y, p, dU, dvel, dtrw, dtru, zero_d, zero_tr, zero_e;
+    DBuf<double> lin_diff, lin_vj, lin_pvel, lin_nvel, lin_vort, lin_adv;
+    DBuf<double> lam, xbar, psibar, z, fake, wbar, kbar[4], pvbar;
+    DBuf<double> ckbuf, segbuf, save_w, save_psi, save_vel;
+    int nsteps = 0, every = 0;
+    double t0 = 0.;
+    bool have_forward = false;
+    BCDev Bz{}, Bl{};
+  } adj;
+
+  void adjoint_check() const {
+    if (!ready) throw StatusError(HOC_ESTATE, "call hoc_setup first");
+    if (mixed) throw StatusError(HOC_EINVAL, "the adjoint needs precision HOC_FP64");
+    if (n_inst > 1) throw StatusError(HOC_EINVAL, "the adjoint runs on one instance");
+    if (cfg.problem_type != 1 || bch.time_dependent)
+      throw StatusError(HOC_EINVAL, "the adjoint needs problem_type 1 (fixed walls, time-independent boundary values)");
+    if (cfg.advection_type != 1) throw StatusError(HOC_EINVAL, "the adjoint needs advection_type 1");
+    if (cfg.time_integration_type != 4) throw StatusError(HOC_EINVAL, "the adjoint needs time_integration_type 4");
+    if (krylov != HOC_KRYLOV_PCG)
+      throw StatusError(HOC_EINVAL, "the adjoint needs a symmetric Poisson operator (affine elements)");
+    for (int e = 0; e < E; ++e)
+      if (bch.sw_psi[e] != 0) throw StatusError(HOC_EINVAL, "the adjoint needs Dirichlet stream-function walls");
+  }
+
+  // Greedy colouring in which elements within face distance dist of each other differ.
+  int color_within(int dist, std::vector<int32_t>& color) const {
+    color.assign(N, -1);
+    std::vector<int> mark(N, -1), used;
+    std::vector<int> front, next, seen;
+    int nc = 0;
+    for (int e = 0; e < N; ++e) {
+      seen.clear();
+      front.assign(1, e);
+      mark[e] = e;
+      for (int d = 0; d < dist; ++d) {
+        next.clear();
+        for (int u : front)
+          for (int f = 0; f < 4; ++f) {
+            const int v = mesh.nb[(size_t)u * 4 + f];
+            if (v >= 0 && mark[v] != e) { mark[v] = e; next.push_back(v); seen.push_back(v); }
+          }
+        front.swap(next);
+      }
+      for (int v : seen)
+        if (color[v] >= 0) used[color[v]] = e;
+      int c = 0;
+      while (c < (int)used.size() && used[c] == e) ++c;
+      if (c == (int)used.size()) used.push_back(-1);
+      color[e] = c;
+      nc = std::max(nc, c + 1);
+    }
+    return nc;
+  }
+  // Elements within face distance dist of each element (itself included), as CSR.
+  void rings(int dist, std::vector<int32_t>& ptr, std::vector<int32_t>& idx) const {
+    ptr.assign(1, 0);
+    idx.clear();
+    std::vector<int> mark(N, -1), front, next;
+    for (int e = 0; e < N; ++e) {
+      mark[e] = e;
+      idx.push_back(e);
+      front.assign(1, e);
+      for (int d = 0; d < dist; ++d) {
+        next.clear();
+        for (int u : front)
+          for (int f = 0; f < 4; ++f) {
+            const int v = mesh.nb[(size_t)u * 4 + f];
+            if (v >= 0 && mark[v] != e) { mark[v] = e; next.push_back(v); idx.push_back(v); }
+          }
+        front.swap(next);
+      }
+      ptr.push_back((int32_t)idx.size());
+    }
+  }
+
+  void adjoint_setup() {
+    adjoint_check();
+    if (adj.init) return;
+    std::vector<int32_t> c1, c2, p1, i1, p2, i2;
+    adj.nc1 = color_within(2, c1);
+    adj.nc2 = color_within(4, c2);
+    rings(1, p1, i1);
+    rings(2, p2, i2);
+    adj.col1.upload(c1); adj.col2.upload(c2);
+    adj.r1p.upload(p1); adj.r1i.upload(i1); adj.r2p.upload(p2); adj.r2i.upload(i2);
+    std::vector<int32_t> oe(std::max(E, 1), 0);
+    std::vector<int8_t> of(std::max(E, 1), -1);
+    for (int e = 0; e < E; ++e) { oe[e] = mesh.be_elem[e]; of[e] = mesh.be_ijp[e]; }
+    adj.own_el.upload(oe); adj.own_f.upload(of);
+    const size_t TR = (size_t)4 * N * 2 * K, TU = (size_t)4 * N * K, EK = std::max<size_t>((size_t)E * K, 1);
+    for (int s4 = 0; s4 < 4; ++s4) {
+      adj.ws[s4].alloc(D); adj.Us[s4].alloc(2 * D); adj.trws[s4].alloc(TR); adj.trus[s4].alloc(TU);
+      adj.sig[s4].alloc((size_t)4 * N * K); adj.kbar[s4].alloc(D);
+    }
+    for (DBuf<double>* b : {&adj.seed, &adj.y, &adj.zero_d, &adj.lam, &adj.xbar, &adj.psibar, &adj.z, &adj.fake,
+                            &adj.wbar, &adj.save_w, &adj.save_psi})
+      b->alloc(D);
+    adj.dU.alloc(2 * D); adj.dvel.alloc(2 * D); adj.save_vel.alloc(2 * D);
+    adj.seed_tr.alloc(TR); adj.dtrw.alloc(TR); adj.zero_tr.alloc(TR); adj.dtru.alloc(TU);
+    adj.p.alloc(std::max(N, 1));
+    for (DBuf<double>* b : {&adj.zero_e, &adj.lin_diff, &adj.lin_vj, &adj.lin_pvel, &adj.lin_vort, &adj.lin_adv,
+                            &adj.pvbar})
+      b->alloc(EK);
+    adj.lin_nvel.alloc(EK);
+    HIPCHK(hipMemcpy(adj.lin_nvel.p, b_nvel.p, EK * sizeof(double), hipMemcpyDeviceToDevice));
+    // homogeneous boundary values: zero psi (null), zero wall velocity and flux; the normal
+    // velocity of the walls (a constant) multiplies the boundary vorticity's variation
+    adj.Bl = B;
+    adj.Bl.psi = nullptr;
+    adj.Bl.diff = adj.lin_diff; adj.Bl.vjump = adj.lin_vj; adj.Bl.pvel = adj.lin_pvel;
+    adj.Bl.nvel = adj.lin_nvel; adj.Bl.vort = adj.lin_vort; adj.Bl.adv = adj.lin_adv;
+    adj.Bz = adj.Bl;
+    adj.Bz.diff = adj.zero_e;
+    adj.init = true;
+  }
+
+  void dcopy(double* dst, const double* src, size_t n) {
+    HIPCHK(hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToDevice, stream));
+  }
+
+  // One RK4 step from w = omega_n recording every stage's input, velocity, traces and upwind
+  // switches (w itself is not advanced).
+  void adj_record_step() {
+    const double c[4] = {0., 0.5, 0.5, 1.};
+    const double cn[3] = {0.5, 0.5, 1.};
+    dcopy(adj.ws[0], w, D);
+    for (int st = 0; st < 4; ++st) {
+      update_bcs(c[st]);
+      poisson(adj.ws[st]);
+      HIPCHK(launch_trace(K, PRE_COPY, N, T, psi, nullptr, nullptr, nullptr, 0., nullptr, 0, nullptr, P64.tr.p, stream));
+      HIPCHK(launch_velocity(K, N, T, G, B, psi, P64.tr, adj.ws[st], vel, adj.Us[st], adj.trws[st], adj.trus[st],
+                             1. / (cfg.dt * re_inv), I, stream));
+      StageOut so;
+      so.k = kscratch;
+      so.sig = adj.sig[st];
+      if (st < 3) { so.next = adj.ws[st + 1]; so.c = cn[st]; so.base = w; }
+      HIPCHK(launch_stage(K, 1, N, T, G, B, adj.ws[st], adj.Us[st], adj.trws[st], adj.trus[st], re_inv, cfg.dt, I, so,
+                          stream, 1));
+    }
+  }
+
+  // xbar = (dk_s/domega_s)^T lam for recorded stage st, adding (dk_s/dpvel)^T lam to adj.pvbar.
+  void adj_stage(int st, const double* lam, double* xbar) {
+    const double idr = 1. / (cfg.dt * re_inv);
+    StageOut so;
+    so.k = adj.y;
+    so.sig = adj.sig[st];
+    // (1) omega and its face traces, with psi (so U and the wall flux) fixed
+    so.omega_adv = adj.seed;
+    so.trw_adv = adj.seed_tr;
+    for (int c = 0; c < adj.nc1; ++c)
+      for (int m = 0; m < K2; ++m) {
+        HIPCHK(launch_adj_seed(N, K2, adj.col1, c, m, adj.seed, stream));
+        HIPCHK(launch_trace(K, PRE_COPY, N, T, adj.seed.p, nullptr, nullptr, nullptr, 0., nullptr, 0, nullptr,
+                            adj.seed_tr.p, stream));
+        HIPCHK(launch_stage(K, 1, N, T, G, adj.Bz, adj.seed, adj.Us[st], adj.seed_tr, adj.trus[st], re_inv, cfg.dt, I,
+                            so, stream, 2));
+        HIPCHK(launch_adj_dot(N, K2, lam, adj.y, adj.p, stream));
+        HIPCHK(launch_adj_gather(N, K2, adj.col1, c, m, adj.r1p, adj.r1i, adj.p, xbar, stream));
+      }
+    // (2) psi: velocity and wall flux of the seed with homogeneous boundary values, then the
+    //     stage advecting the recorded vorticity with that velocity
+    so.omega_adv = adj.ws[st];
+    so.trw_adv = adj.trws[st];
+    auto psi_probe = [&](const double* ps, const double* pstr) {
+      HIPCHK(launch_velocity(K, N, T, G, adj.Bl, ps, pstr, adj.zero_d, adj.dvel, adj.dU, adj.dtrw, adj.dtru, idr, I,
+                             stream));
+      HIPCHK(launch_stage(K, 1, N, T, G, adj.Bl, adj.zero_d, adj.dU, adj.zero_tr, adj.dtru, re_inv, cfg.dt, I, so,
+                          stream, 2));
+      HIPCHK(launch_adj_dot(N, K2, lam, adj.y, adj.p, stream));
+    };
+    for (int c = 0; c < adj.nc2; ++c)
+      for (int m = 0; m < K2; ++m) {
+        HIPCHK(launch_adj_seed(N, K2, adj.col2, c, m, adj.seed, stream));
+        HIPCHK(launch_trace(K, PRE_COPY, N, T, adj.seed.p, nullptr, nullptr, nullptr, 0., nullptr, 0, nullptr,
+                            adj.seed_tr.p, stream));
+        psi_probe(adj.seed, adj.seed_tr);
+        HIPCHK(launch_adj_gather(N, K2, adj.col2, c, m, adj.r2p, adj.r2i, adj.p, adj.psibar, stream));
+      }
+    // (3) wall velocity: a seed on the edges owned through face f at edge flux point j reaches
+    //     only the owner elements' wall flux
+    for (int f = 0; f < 4 && E > 0; ++f)
+      for (int j = 0; j < K; ++j) {
+        HIPCHK(launch_adj_edge_seed(E, K, adj.own_f, f, j, adj.lin_pvel, stream));
+        psi_probe(adj.zero_d, adj.zero_tr);
+        HIPCHK(launch_adj_edge_gather(E, K, adj.own_el, adj.own_f, f, j, adj.p, adj.pvbar, stream));
+      }
+    if (E > 0) HIPCHK(hipMemsetAsync(adj.lin_pvel.p, 0, (size_t)E * K * sizeof(double), stream));
+    // (4) the Poisson solve: As z = psi_bar (a right-hand side -W b = psi_bar comes from the
+    //     vorticity psi_bar / (W J) with zero boundary values), then omega_bar += W J z
+    HIPCHK(launch_adj_wj(N, K, tab.wgt, d_jac, adj.psibar, adj.fake, 0, stream));
+    rhs_bc = adj.zero_e;
+    std::swap(psi.p, adj.z.p);
+    try {
+      poisson(adj.fake);
+    } catch (...) {
+      std::swap(psi.p, adj.z.p);
+      rhs_bc = nullptr;
+      throw;
+    }
+    std::swap(psi.p, adj.z.p);
+    rhs_bc = nullptr;
+    HIPCHK(launch_adj_wj(N, K, tab.wgt, d_jac, adj.z, xbar, 1, stream));
+  }
+
+  // Forward run of nsteps RK4 steps keeping the vorticity every `every` steps.
+  void adjoint_forward(int nsteps, int every) {
+    adjoint_setup();
+    if (nsteps < 0 || every < 1) throw StatusError(HOC_EINVAL, "nsteps must be >= 0 and every >= 1");
+    adj.have_forward = false;
+    const int nck = std::max(1, (nsteps + every - 1) / every);
+    adj.ckbuf.alloc((size_t)nck * D);
+    adj.segbuf.alloc((size_t)std::min(every, std::max(nsteps, 1)) * D);
+    adj.t0 = time;
+    for (int n = 0; n < nsteps; ++n) {
+      if (n % every == 0) dcopy(adj.ckbuf.p + (size_t)(n / every) * D, w, D);
+      rk_step();
+    }
+    HIPCHK(hipStreamSynchronize(stream));
+    adj.nsteps = nsteps;
+    adj.every = every;
+    adj.have_forward = true;
+  }
+
+  // Reverse sweep over the recorded trajectory: segments of `every` steps are recomputed from
+  // their checkpoint (keeping each step's vorticity), then each step is re-run recording its
+  // stages and transposed stage by stage:
+  //   k4_bar = w_bar/6, k3_bar = k2_bar = w_bar/3, k1_bar = w_bar/6,
+  //   for s = 4..1: x = (dk_s/domega_s)^T k_s_bar;  w_bar += x;  k_{s-1}_bar += c_s x
+  // with omega_s = omega_n + c_s k_{s-1}, c = (1/2, 1/2, 1).  The handle's state (vorticity,
+  // stream function, velocity, time) is restored afterwards.
+  void adjoint_backward(const double* wbar_h, double* w0bar_h, double* pvbar_h) {
+    if (!adj.have_forward) throw StatusError(HOC_ESTATE, "call hoc_adjoint_forward first");
+    adjoint_check();
+    const double time_end_ = time;
+    dcopy(adj.save_w, w, D);
+    dcopy(adj.save_psi, psi, D);
+    dcopy(adj.save_vel, vel, 2 * D);
+    h2d(adj.wbar.p, wbar_h, D * sizeof(double));
+    const size_t EK = std::max<size_t>((size_t)E * K, 1);
+    HIPCHK(hipMemsetAsync(adj.pvbar.p, 0, EK * sizeof(double), stream));
+    const double cn[3] = {0.5, 0.5, 1.};
+    const int every = adj.every, nsteps = adj.nsteps;
+    const int nseg = (nsteps + every - 1) / every;
+    auto restore = [&] {
+      dcopy(w, adj.save_w, D);
+      dcopy(psi, adj.save_psi, D);
+      dcopy(vel, adj.save_vel, 2 * D);
+      time = time_end_;
+    };
+    try {
+      for (int sg = nseg - 1; sg >= 0; --sg) {
+        const int n0 = sg * every, n1 = std::min(n0 + every, nsteps);
+        dcopy(w, adj.ckbuf.p + (size_t)sg * D, D);
+        time = adj.t0 + n0 * cfg.dt;
+        for (int n = n0; n < n1; ++n) {
+          dcopy(adj.segbuf.p + (size_t)(n - n0) * D, w, D);
+          if (n + 1 < n1) rk_step();
+        }
+        for (int n = n1 - 1; n >= n0; --n) {
+          dcopy(w, adj.segbuf.p + (size_t)(n - n0) * D, D);
+          time = adj.t0 + n * cfg.dt;
+          adj_record_step();
+          HIPCHK(launch_adj_axpby(D, 1. / 6., adj.wbar, 0., adj.kbar[3], stream));
+          HIPCHK(launch_adj_axpby(D, 1. / 3., adj.wbar, 0., adj.kbar[2], stream));
+          HIPCHK(launch_adj_axpby(D, 1. / 3., adj.wbar, 0., adj.kbar[1], stream));
+          HIPCHK(launch_adj_axpby(D, 1. / 6., adj.wbar, 0., adj.kbar[0], stream));
+          for (int st = 3; st >= 0; --st) {
+            adj_stage(st, adj.kbar[st], adj.xbar);
+            HIPCHK(launch_adj_axpby(D, 1., adj.xbar, 1., adj.wbar, stream));
+            if (st > 0) HIPCHK(launch_adj_axpby(D, cn[st - 1], adj.xbar, 1., adj.kbar[st - 1], stream));
+          }
+        }
+      }
+      d2h(w0bar_h, adj.wbar.p, D * sizeof(double));
+      if (pvbar_h && E > 0) d2h(pvbar_h, adj.pvbar.p, (size_t)E * K * sizeof(double));
+    } catch (...) {
+      restore();
+      throw;
+    }
+    restore();
+    HIPCHK(hipStreamSynchronize(stream));
+  }
+
+  // (dk/domega)^T lam and (dk/dpvel)^T lam of one stage at time t (hoc_stage_residual's map).
+  void stage_vjp(const double* omega_h, double t, const double* lam_h, double* out_h, double* pvbar_h) {
+    adjoint_setup();
+    const size_t EK = std::max<size_t>((size_t)E * K, 1);
+    h2d(adj.ws[0].p, omega_h, D * sizeof(double));
+    h2d(adj.lam.p, lam_h, D * sizeof(double));
+    update_bcs(0., t);
+    poisson(adj.ws[0]);
+    HIPCHK(launch_trace(K, PRE_COPY, N, T, psi, nullptr, nullptr, nullptr, 0., nullptr, 0, nullptr, P64.tr.p, stream));
+    HIPCHK(launch_velocity(K, N, T, G, B, psi, P64.tr, adj.ws[0], vel, adj.Us[0], adj.trws[0], adj.trus[0],
+                           1. / (cfg.dt * re_inv), I, stream));
+    StageOut so;
+    so.k = kscratch;
+    so.sig = adj.sig[0];
+    HIPCHK(launch_stage(K, 1, N, T, G, B, adj.ws[0], adj.Us[0], adj.trws[0], adj.trus[0], re_inv, cfg.dt, I, so,
+                        stream, 1));
+    HIPCHK(hipMemsetAsync(adj.pvbar.p, 0, EK * sizeof(double), stream));
+    adj_stage(0, adj.lam, adj.xbar);
+    d2h(out_h, adj.xbar.p, D * sizeof(double));
+    if (pvbar_h && E > 0) d2h(pvbar_h, adj.pvbar.p, (size_t)E * K * sizeof(double));
+  }
+
+  // Boundary flux points [E][K] (edge flux-point order of the boundary arrays), x and y.
+  void boundary_points(double* xy) const {
+    const int L = mesh.lnod;
+    for (int e = 0; e < E; ++e) {
+      const auto& nd = mesh.edges[e];
+      for (int j = 0; j < K; ++j) {
+        double x = 0., y = 0.;
+        for (int l = 0; l < L; ++l) {
+          x += tab.gsb[l][j] * mesh.xy[2 * (size_t)nd[l]];
+          y += tab.gsb[l][j] * mesh.xy[2 * (size_t)nd[l] + 1];
+        }
+        xy[2 * ((size_t)e * K + j)] = x;
+        xy[2 * ((size_t)e * K + j) + 1] = y;
+      }
+    }
   }
 
   // Ensemble tables (instance of every element and boundary edge, per-instance Re and dt).
@@ -1393,6 +1733,61 @@ int hoc_stage_residual(hoc_t h, const double* omega, double t, double* k) {
     s.update_bcs(0., t);
     s.stage_body(s.vort, so);
     s.d2h(k, s.kscratch.p, s.D * sizeof(double));
+  });
+}
+
+int64_t hoc_boundary_point_count(hoc_t h) { return h && h->s.mesh_loaded ? (int64_t)h->s.E * h->s.K : 0; }
+
+int hoc_get_boundary_points(hoc_t h, int64_t len, double* xy) {
+  return guarded(h, [&] {
+    NEED_READY(h);
+    NEED_LEN(len, 2 * (int64_t)h->s.E * h->s.K);
+    h->s.boundary_points(xy);
+  });
+}
+
+int hoc_get_wall_velocity(hoc_t h, int64_t len, double* v) {
+  return guarded(h, [&] {
+    NEED_READY(h);
+    NEED_LEN(len, (int64_t)h->s.E * h->s.K);
+    if (len) h->s.d2h(v, h->s.b_pvel0.p, len * sizeof(double));
+  });
+}
+
+int hoc_set_wall_velocity(hoc_t h, int64_t len, const double* v) {
+  return guarded(h, [&] {
+    NEED_READY(h);
+    NEED_LEN(len, (int64_t)h->s.E * h->s.K);
+    for (int64_t i = 0; i < len; ++i)
+      if (!std::isfinite(v[i])) throw hoc::StatusError(HOC_EINVAL, "wall velocity must be finite");
+    if (len) h->s.h2d(h->s.b_pvel0.p, v, len * sizeof(double));
+  });
+}
+
+int hoc_adjoint_forward(hoc_t h, int32_t nsteps, int32_t checkpoint_every) {
+  return guarded(h, [&] {
+    NEED_READY(h);
+    h->s.adjoint_forward(nsteps, checkpoint_every);
+  });
+}
+
+int hoc_adjoint_backward(hoc_t h, int64_t len, const double* wbar_final, double* wbar_initial, int64_t len_wall,
+                         double* wall_velocity_bar) {
+  return guarded(h, [&] {
+    NEED_READY(h);
+    NEED_LEN(len, h->s.D);
+    if (wall_velocity_bar) NEED_LEN(len_wall, (int64_t)h->s.E * h->s.K);
+    if (!wbar_final || !wbar_initial) throw hoc::StatusError(HOC_EINVAL, "null buffer");
+    h->s.adjoint_backward(wbar_final, wbar_initial, wall_velocity_bar);
+  });
+}
+
+int hoc_stage_residual_vjp(hoc_t h, const double* omega, double t, const double* lam, double* omega_bar,
+                           double* wall_velocity_bar) {
+  return guarded(h, [&] {
+    NEED_READY(h);
+    if (!omega || !lam || !omega_bar) throw hoc::StatusError(HOC_EINVAL, "null buffer");
+    h->s.stage_vjp(omega, t, lam, omega_bar, wall_velocity_bar);
   });
 }
 

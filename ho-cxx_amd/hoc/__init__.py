@@ -34,7 +34,9 @@ EXPORTED = [
     "hoc_set_p2g_weights", "hoc_set_vorticity", "hoc_get_vorticity",
     "hoc_get_streamfunction", "hoc_set_streamfunction", "hoc_get_velocity", "hoc_get_solution_points",
     "hoc_get_solution_areas", "hoc_get_time", "hoc_set_time", "hoc_step", "hoc_solveto", "hoc_poisson",
-    "hoc_stage_residual", "hoc_apply_laplacian", "hoc_stats_get", "hoc_profile_enable", "hoc_profile_get",
+    "hoc_stage_residual", "hoc_apply_laplacian", "hoc_boundary_point_count", "hoc_get_boundary_points",
+    "hoc_get_wall_velocity", "hoc_set_wall_velocity", "hoc_adjoint_forward", "hoc_adjoint_backward",
+    "hoc_stage_residual_vjp", "hoc_stats_get", "hoc_profile_enable", "hoc_profile_get",
     "hoc_last_error", "hoc_destroy",
     "hoc_linsolver_create", "hoc_linsolver_set_tolerances", "hoc_linsolver_build", "hoc_linsolver_solve",
     "hoc_linsolver_num_iterations", "hoc_linsolver_residual", "hoc_linsolver_levels", "hoc_linsolver_last_error",
@@ -115,6 +117,13 @@ def lib():
         L.hoc_poisson.argtypes = [C.c_void_p, _dp, C.c_double, _dp]
         L.hoc_stage_residual.argtypes = [C.c_void_p, _dp, C.c_double, _dp]
         L.hoc_apply_laplacian.argtypes = [C.c_void_p, _dp, _dp]
+        L.hoc_boundary_point_count.argtypes = [C.c_void_p]
+        L.hoc_boundary_point_count.restype = C.c_int64
+        for nm in ("hoc_get_boundary_points", "hoc_get_wall_velocity", "hoc_set_wall_velocity"):
+            getattr(L, nm).argtypes = [C.c_void_p, C.c_int64, C.c_void_p]
+        L.hoc_adjoint_forward.argtypes = [C.c_void_p, C.c_int32, C.c_int32]
+        L.hoc_adjoint_backward.argtypes = [C.c_void_p, C.c_int64, _dp, _dp, C.c_int64, C.c_void_p]
+        L.hoc_stage_residual_vjp.argtypes = [C.c_void_p, _dp, C.c_double, _dp, _dp, C.c_void_p]
         L.hoc_stats_get.argtypes = [C.c_void_p, C.POINTER(hoc_stats)]
         L.hoc_profile_enable.argtypes = [C.c_void_p, C.c_int32]
         L.hoc_profile_get.argtypes = [C.c_void_p, C.POINTER(hoc_kernel_profile)]
@@ -331,6 +340,49 @@ class HO2D:
         out = np.zeros(self.D)
         self._chk(lib().hoc_stage_residual(self.h, np.ascontiguousarray(w, dtype=np.float64), t, out))
         return out
+
+    # ---- wall velocity and the discrete adjoint ------------------------------------------------
+    def boundary_points(self):
+        """[E*K, 2] boundary flux points in the edge flux-point order of the boundary arrays."""
+        n = int(lib().hoc_boundary_point_count(self.h))
+        out = np.zeros(2 * n)
+        self._chk(lib().hoc_get_boundary_points(self.h, 2 * n, out.ctypes.data if n else None))
+        return out.reshape(-1, 2)
+
+    def wall_velocity(self):
+        """BC_parl_vel at the boundary flux points [E*K] (-1 along the lid of the cavity)."""
+        n = int(lib().hoc_boundary_point_count(self.h))
+        out = np.zeros(n)
+        self._chk(lib().hoc_get_wall_velocity(self.h, n, out.ctypes.data if n else None))
+        return out
+
+    def set_wall_velocity(self, v):
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        self._chk(lib().hoc_set_wall_velocity(self.h, v.size, v.ctypes.data if v.size else None))
+
+    def adjoint_forward(self, nsteps, checkpoint_every):
+        """Advance nsteps RK4 steps keeping checkpoints for adjoint_backward."""
+        self._chk(lib().hoc_adjoint_forward(self.h, int(nsteps), int(checkpoint_every)))
+
+    def adjoint_backward(self, wbar_final):
+        """Gradient of an objective w.r.t. the vorticity and the wall velocity at the start of the
+        last adjoint_forward run, from its gradient w.r.t. the final vorticity."""
+        wb = np.ascontiguousarray(wbar_final, dtype=np.float64)
+        out = np.zeros(self.D)
+        n = int(lib().hoc_boundary_point_count(self.h))
+        pv = np.zeros(n)
+        self._chk(lib().hoc_adjoint_backward(self.h, wb.size, wb, out, n, pv.ctypes.data if n else None))
+        return out, pv
+
+    def stage_vjp(self, w, lam, t=0.0):
+        """(dk/domega)^T lam and (dk/dwall_velocity)^T lam of stage(w, t)."""
+        out = np.zeros(self.D)
+        n = int(lib().hoc_boundary_point_count(self.h))
+        pv = np.zeros(n)
+        self._chk(lib().hoc_stage_residual_vjp(self.h, np.ascontiguousarray(w, dtype=np.float64), t,
+                                               np.ascontiguousarray(lam, dtype=np.float64), out,
+                                               pv.ctypes.data if n else None))
+        return out, pv
 
     def apply_laplacian(self, x):
         out = np.zeros(self.D)

@@ -193,6 +193,36 @@ int hoc_stage_residual(hoc_t h, const double* omega, double t, double* k);
  * applied matrix-free. */
 int hoc_apply_laplacian(hoc_t h, const double* x, double* y);
 
+/* Boundary flux points and the tangential wall velocity.  Boundary arrays are [E][K] in the
+ * edge flux-point order of the BC_* arrays (src/HO_2D.cpp:2229).  The wall velocity is
+ * BC_parl_vel as setup_IC_BC_SRC sets it (src/HO_2D.cpp:717-725: -1 along the lid of the
+ * cavity); setting it gives the walls any tangential velocity profile, e.g. a non-uniform lid.
+ * hoc_boundary_point_count returns E*K. */
+int64_t hoc_boundary_point_count(hoc_t h);
+int hoc_get_boundary_points(hoc_t h, int64_t len, double* xy);          /* len = 2 E K */
+int hoc_get_wall_velocity(hoc_t h, int64_t len, double* v);             /* len = E K   */
+int hoc_set_wall_velocity(hoc_t h, int64_t len, const double* v);
+
+/* Discrete adjoint of the RK4 trajectory (no counterpart in the reference; for gradient-based
+ * callers).  Supported: FP64, one instance, problem_type 1 walls with Dirichlet stream
+ * function, advection_type 1, time_integration_type 4, affine elements (symmetric Poisson
+ * operator); otherwise HOC_EINVAL.
+ * hoc_adjoint_forward advances nsteps RK4 steps like hoc_step, keeping the vorticity every
+ * checkpoint_every steps on the device.  hoc_adjoint_backward takes the gradient of an objective
+ * with respect to the final vorticity and returns its gradient with respect to the vorticity at
+ * the start of that run and (optional, len_wall = E K) to the wall velocity BC_parl_vel.  It
+ * recomputes segments of checkpoint_every steps from their checkpoint; the upwind switches of
+ * the advective flux (|dw| > 1e-4) are taken as fixed at their recomputed state.  The handle's
+ * vorticity, stream function, velocity and time are left as hoc_adjoint_forward left them.
+ * Poisson solves of the sweep count in hoc_stats. */
+int hoc_adjoint_forward(hoc_t h, int32_t nsteps, int32_t checkpoint_every);
+int hoc_adjoint_backward(hoc_t h, int64_t len, const double* wbar_final, double* wbar_initial, int64_t len_wall,
+                         double* wall_velocity_bar);
+/* Transposed tangent of hoc_stage_residual's map omega -> k at time t: omega_bar =
+ * (dk/domega)^T lam and (optional) wall_velocity_bar = (dk/dBC_parl_vel)^T lam. */
+int hoc_stage_residual_vjp(hoc_t h, const double* omega, double t, const double* lam, double* omega_bar,
+                           double* wall_velocity_bar);
+
 int hoc_stats_get(hoc_t h, hoc_stats* s);
 
 /* Live timing of the fine-level Poisson operator kernel (the dominant kernel): with
