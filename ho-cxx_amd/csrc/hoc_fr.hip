@@ -517,13 +517,18 @@ __global__ __launch_bounds__(kWG) void k_rhs(int N, Tab T, Geo G, const double* 
 // stage kernel needs from neighbours: omega value/derivative traces and the contravariant
 // normal-velocity trace.  AFF: metrics constant per element (G.gel, G.gface, G.gjm).
 // ==========================================================================================
-template <int K, bool AFF>
+template <int K, bool AFF, bool BATCH = false>
 __global__ __launch_bounds__(kWG) void k_velocity(int N, Tab T, Geo G, BCDev B, const double* __restrict__ psi,
                                                    const double* __restrict__ trpsi, const double* __restrict__ omega,
                                                    double* __restrict__ vel, double* __restrict__ U,
                                                    double* __restrict__ trw, double* __restrict__ tru,
-                                                   double inv_dt_re, InstDev I) {
+                                                   double inv_dt_re, InstDev I, VelBatch vb) {
   constexpr int K2 = K * K;
+  if constexpr (BATCH) {  // probe blockIdx.y of a batch
+    const size_t b = blockIdx.y;
+    psi += b * vb.psi; trpsi += b * vb.trpsi; omega += b * vb.omega; vel += b * vb.vel; U += b * vb.U;
+    trw += b * vb.trw; tru += b * vb.tru; B.vjump += b * vb.vjump; B.diff += b * vb.diff; B.pvel += b * vb.pvel;
+  }
   __shared__ __align__(16) double lds[kWG * K2 * 2];
   const int e0 = blockIdx.x * kWG;
   const int el = e0 + threadIdx.x;
@@ -669,6 +674,11 @@ __global__ __launch_bounds__(kWG) void k_stage(int N, Tab T, Geo G, BCDev B, con
                                                 const double* __restrict__ tru, double re_inv, double dt,
                                                 InstDev I, StageOut so) {
   constexpr int K2 = K * K;
+  if constexpr (MODE == 2) {  // probe blockIdx.y of a batch
+    const size_t b = blockIdx.y;
+    omega += b * so.bs_w; U += b * so.bs_U; trw += b * so.bs_trw; tru += b * so.bs_tru;
+    so.omega_adv += b * so.bs_wa; so.trw_adv += b * so.bs_trwa; so.k += b * so.bs_k; B.diff += b * so.bs_diff;
+  }
   __shared__ __align__(16) double lds[kWG * K2 * 2];
   const int e0 = blockIdx.x * kWG;
   const int el = e0 + threadIdx.x;
@@ -1049,13 +1059,31 @@ hipError_t launch_velocity(int K, int N, const Tab& T, const Geo& G, const BCDev
                            const double* trpsi, const double* omega, double* vel, double* U, double* trw,
                            double* tru, double inv_dt_re, const InstDev& I, hipStream_t s) {
   const bool aff = G.gel && G.gface && G.gjm;
+  const VelBatch vb{};
   HOC_K_SWITCH(K, {
     if (aff)
       hipLaunchKernelGGL((k_velocity<KK, true>), dim3(nblk(N)), dim3(kWG), 0, s, N, T, G, B, psi, trpsi, omega, vel, U,
-                         trw, tru, inv_dt_re, I);
+                         trw, tru, inv_dt_re, I, vb);
     else
       hipLaunchKernelGGL((k_velocity<KK, false>), dim3(nblk(N)), dim3(kWG), 0, s, N, T, G, B, psi, trpsi, omega, vel, U,
-                         trw, tru, inv_dt_re, I);
+                         trw, tru, inv_dt_re, I, vb);
+  });
+  return hipGetLastError();
+}
+
+hipError_t launch_velocity_batch(int K, int N, const Tab& T, const Geo& G, const BCDev& B, const double* psi,
+                                 const double* trpsi, const double* omega, double* vel, double* U, double* trw,
+                                 double* tru, double inv_dt_re, const InstDev& I, int nbatch, const VelBatch& vb,
+                                 hipStream_t s) {
+  if (nbatch < 1 || nbatch > 65535) return hipErrorInvalidValue;
+  const bool aff = G.gel && G.gface && G.gjm;
+  HOC_K_SWITCH(K, {
+    if (aff)
+      hipLaunchKernelGGL((k_velocity<KK, true, true>), dim3(nblk(N), nbatch), dim3(kWG), 0, s, N, T, G, B, psi, trpsi,
+                         omega, vel, U, trw, tru, inv_dt_re, I, vb);
+    else
+      hipLaunchKernelGGL((k_velocity<KK, false, true>), dim3(nblk(N), nbatch), dim3(kWG), 0, s, N, T, G, B, psi, trpsi,
+                         omega, vel, U, trw, tru, inv_dt_re, I, vb);
   });
   return hipGetLastError();
 }
@@ -1066,9 +1094,10 @@ hipError_t launch_stage(int K, int adv, int N, const Tab& T, const Geo& G, const
   const bool aff = G.gel && G.gface && G.gjm;
   if (mode != 0 && (adv != 1 || !so.sig || (mode == 2 && (!so.omega_adv || !so.trw_adv))))
     return hipErrorInvalidValue;
+  if (mode == 2 && (so.nbatch < 1 || so.nbatch > 65535)) return hipErrorInvalidValue;
 #define LS(A, F, M)                                                                                           \
-  hipLaunchKernelGGL((k_stage<KK, A, F, M>), dim3(nblk(N)), dim3(kWG), 0, s, N, T, G, B, omega, U, trw, tru, re_inv, \
-                     dt, I, so)
+  hipLaunchKernelGGL((k_stage<KK, A, F, M>), dim3(nblk(N), M == 2 ? so.nbatch : 1), dim3(kWG), 0, s, N, T, G, B, \
+                     omega, U, trw, tru, re_inv, dt, I, so)
   HOC_K_SWITCH(K, {
     if (mode == 1) { if (aff) LS(1, true, 1); else LS(1, false, 1); }
     else if (mode == 2) { if (aff) LS(1, true, 2); else LS(1, false, 2); }

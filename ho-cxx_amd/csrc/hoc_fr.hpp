@@ -20,6 +20,15 @@ struct StageOut {
   int8_t* sig = nullptr;               // [N][4][K] upwind switch of each interior flux point
   const double* omega_adv = nullptr;   // mode 2: advected vorticity and its traces
   const double* trw_adv = nullptr;
+  // mode 2 batches: nbatch independent products in one launch, probe b reading/writing its
+  // arrays at b * (stride in values); stride 0 shares an array between the probes
+  int nbatch = 1;
+  size_t bs_w = 0, bs_trw = 0, bs_U = 0, bs_tru = 0, bs_wa = 0, bs_trwa = 0, bs_k = 0, bs_diff = 0;
+};
+
+// Batch strides (in values) of the batched velocity launch.
+struct VelBatch {
+  size_t psi = 0, trpsi = 0, omega = 0, vel = 0, U = 0, trw = 0, tru = 0, vjump = 0, diff = 0, pvel = 0;
 };
 
 // Poisson-operator kernels in FP64 or (mixed-precision preconditioner) FP32.
@@ -42,6 +51,11 @@ hipError_t launch_rhs(int K, int N, const Tab& T, const Geo& G, const double* om
 hipError_t launch_velocity(int K, int N, const Tab& T, const Geo& G, const BCDev& B, const double* psi,
                            const double* trpsi, const double* omega, double* vel, double* U, double* trw,
                            double* tru, double inv_dt_re, const InstDev& I, hipStream_t s);
+// nbatch velocity evaluations in one launch (probe b offset by b * the strides of vb)
+hipError_t launch_velocity_batch(int K, int N, const Tab& T, const Geo& G, const BCDev& B, const double* psi,
+                                 const double* trpsi, const double* omega, double* vel, double* U, double* trw,
+                                 double* tru, double inv_dt_re, const InstDev& I, int nbatch, const VelBatch& vb,
+                                 hipStream_t s);
 hipError_t launch_stage(int K, int adv, int N, const Tab& T, const Geo& G, const BCDev& B, const double* omega,
                         const double* U, const double* trw, const double* tru, double re_inv, double dt,
                         const InstDev& I, const StageOut& so, hipStream_t s, int mode = 0);

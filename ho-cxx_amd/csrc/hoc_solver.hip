@@ -972,8 +972,9 @@ struct Solver {
     DBuf<int8_t> own_f;
     DBuf<double> ws[4], Us[4], trws[4], trus[4];  // recorded stages of one step
     DBuf<int8_t> sig[4];
-    DBuf<double> seed, seed_tr, y, p, dU, dvel, dtrw, dtru, zero_d, zero_tr, zero_e;
-    DBuf<double> lin_diff, lin_vj, lin_pvel, lin_nvel, lin_vort, lin_adv;
+    // probe batches (nb = max(K^2, 4K) probes): seeds and their traces, products, velocities
+    DBuf<double> seed, seed_tr, y, p, dU, dvel, dtrw, dtru, lin_diff, lin_vj, pvel_seed;
+    DBuf<double> zero_d, zero_tr, zero_e, lin_nvel, lin_vort, lin_adv;
     DBuf<double> lam, xbar, psibar, z, fake, wbar, kbar[4], pvbar;
     DBuf<double> ckbuf, segbuf, save_w, save_psi, save_vel;
     int nsteps = 0, every = 0;
@@ -1062,26 +1063,28 @@ struct Solver {
     for (int e = 0; e < E; ++e) { oe[e] = mesh.be_elem[e]; of[e] = mesh.be_ijp[e]; }
     adj.own_el.upload(oe); adj.own_f.upload(of);
     const size_t TR = (size_t)4 * N * 2 * K, TU = (size_t)4 * N * K, EK = std::max<size_t>((size_t)E * K, 1);
+    const size_t NB = std::max(K2, 4 * K);
     for (int s4 = 0; s4 < 4; ++s4) {
       adj.ws[s4].alloc(D); adj.Us[s4].alloc(2 * D); adj.trws[s4].alloc(TR); adj.trus[s4].alloc(TU);
       adj.sig[s4].alloc((size_t)4 * N * K); adj.kbar[s4].alloc(D);
     }
-    for (DBuf<double>* b : {&adj.seed, &adj.y, &adj.zero_d, &adj.lam, &adj.xbar, &adj.psibar, &adj.z, &adj.fake,
-                            &adj.wbar, &adj.save_w, &adj.save_psi})
+    for (DBuf<double>* b : {&adj.zero_d, &adj.lam, &adj.xbar, &adj.psibar, &adj.z, &adj.fake, &adj.wbar, &adj.save_w,
+                            &adj.save_psi})
       b->alloc(D);
-    adj.dU.alloc(2 * D); adj.dvel.alloc(2 * D); adj.save_vel.alloc(2 * D);
-    adj.seed_tr.alloc(TR); adj.dtrw.alloc(TR); adj.zero_tr.alloc(TR); adj.dtru.alloc(TU);
-    adj.p.alloc(std::max(N, 1));
-    for (DBuf<double>* b : {&adj.zero_e, &adj.lin_diff, &adj.lin_vj, &adj.lin_pvel, &adj.lin_vort, &adj.lin_adv,
-                            &adj.pvbar})
-      b->alloc(EK);
+    adj.save_vel.alloc(2 * D);
+    adj.zero_tr.alloc(TR);
+    adj.seed.alloc(NB * D); adj.y.alloc(NB * D); adj.seed_tr.alloc(NB * TR); adj.dtrw.alloc(NB * TR);
+    adj.dU.alloc(NB * 2 * D); adj.dvel.alloc(NB * 2 * D); adj.dtru.alloc(NB * TU);
+    adj.p.alloc(NB * std::max(N, 1));
+    adj.lin_diff.alloc(NB * EK); adj.lin_vj.alloc(NB * EK); adj.pvel_seed.alloc((size_t)4 * K * EK);
+    for (DBuf<double>* b : {&adj.zero_e, &adj.lin_vort, &adj.lin_adv, &adj.pvbar}) b->alloc(EK);
     adj.lin_nvel.alloc(EK);
     HIPCHK(hipMemcpy(adj.lin_nvel.p, b_nvel.p, EK * sizeof(double), hipMemcpyDeviceToDevice));
-    // homogeneous boundary values: zero psi (null), zero wall velocity and flux; the normal
-    // velocity of the walls (a constant) multiplies the boundary vorticity's variation
+    // homogeneous boundary values: zero psi (null), zero wall velocity and flux; the walls' normal
+    // velocity (a constant) multiplies the boundary vorticity's variation
     adj.Bl = B;
     adj.Bl.psi = nullptr;
-    adj.Bl.diff = adj.lin_diff; adj.Bl.vjump = adj.lin_vj; adj.Bl.pvel = adj.lin_pvel;
+    adj.Bl.diff = adj.lin_diff; adj.Bl.vjump = adj.lin_vj; adj.Bl.pvel = adj.zero_e;
     adj.Bl.nvel = adj.lin_nvel; adj.Bl.vort = adj.lin_vort; adj.Bl.adv = adj.lin_adv;
     adj.Bz = adj.Bl;
     adj.Bz.diff = adj.zero_e;
@@ -1114,52 +1117,56 @@ struct Solver {
   }
 
   // xbar = (dk_s/domega_s)^T lam for recorded stage st, adding (dk_s/dpvel)^T lam to adj.pvbar.
+  // The K^2 probes of a colour (and the 4K wall-velocity probes) run as one batched launch each.
   void adj_stage(int st, const double* lam, double* xbar) {
     const double idr = 1. / (cfg.dt * re_inv);
+    const size_t TR = (size_t)4 * N * 2 * K, TU = (size_t)4 * N * K, EK = std::max<size_t>((size_t)E * K, 1);
     StageOut so;
     so.k = adj.y;
+    so.bs_k = D;
     so.sig = adj.sig[st];
     // (1) omega and its face traces, with psi (so U and the wall flux) fixed
-    so.omega_adv = adj.seed;
-    so.trw_adv = adj.seed_tr;
-    for (int c = 0; c < adj.nc1; ++c)
-      for (int m = 0; m < K2; ++m) {
-        HIPCHK(launch_adj_seed(N, K2, adj.col1, c, m, adj.seed, stream));
-        HIPCHK(launch_trace(K, PRE_COPY, N, T, adj.seed.p, nullptr, nullptr, nullptr, 0., nullptr, 0, nullptr,
-                            adj.seed_tr.p, stream));
-        HIPCHK(launch_stage(K, 1, N, T, G, adj.Bz, adj.seed, adj.Us[st], adj.seed_tr, adj.trus[st], re_inv, cfg.dt, I,
-                            so, stream, 2));
-        HIPCHK(launch_adj_dot(N, K2, lam, adj.y, adj.p, stream));
-        HIPCHK(launch_adj_gather(N, K2, adj.col1, c, m, adj.r1p, adj.r1i, adj.p, xbar, stream));
-      }
-    // (2) psi: velocity and wall flux of the seed with homogeneous boundary values, then the
-    //     stage advecting the recorded vorticity with that velocity
-    so.omega_adv = adj.ws[st];
-    so.trw_adv = adj.trws[st];
-    auto psi_probe = [&](const double* ps, const double* pstr) {
-      HIPCHK(launch_velocity(K, N, T, G, adj.Bl, ps, pstr, adj.zero_d, adj.dvel, adj.dU, adj.dtrw, adj.dtru, idr, I,
-                             stream));
+    so.nbatch = K2;
+    so.omega_adv = adj.seed; so.bs_wa = D;
+    so.trw_adv = adj.seed_tr; so.bs_trwa = TR;
+    so.bs_w = D; so.bs_trw = TR;
+    for (int c = 0; c < adj.nc1; ++c) {
+      HIPCHK(launch_adj_seed_batch(N, K, adj.col1, c, T, adj.seed, adj.seed_tr, stream));
+      HIPCHK(launch_stage(K, 1, N, T, G, adj.Bz, adj.seed, adj.Us[st], adj.seed_tr, adj.trus[st], re_inv, cfg.dt, I,
+                          so, stream, 2));
+      HIPCHK(launch_adj_dot_batch(N, K2, K2, lam, adj.y, adj.p, stream));
+      HIPCHK(launch_adj_gather_batch(N, K2, adj.col1, c, adj.r1p, adj.r1i, adj.p, xbar, stream));
+    }
+    // (2) psi: velocity and wall flux of the seeds with homogeneous boundary values, then the
+    //     stage advecting the recorded vorticity with those velocities
+    so.omega_adv = adj.ws[st]; so.bs_wa = 0;
+    so.trw_adv = adj.trws[st]; so.bs_trwa = 0;
+    so.bs_w = 0; so.bs_trw = 0; so.bs_U = 2 * D; so.bs_tru = TU; so.bs_diff = EK;
+    VelBatch vb;
+    vb.psi = D; vb.trpsi = TR; vb.vel = 2 * D; vb.U = 2 * D; vb.trw = TR; vb.tru = TU; vb.vjump = EK; vb.diff = EK;
+    BCDev Bv = adj.Bl;
+    auto psi_probes = [&](int nb, const double* ps, const double* pstr) {
+      so.nbatch = nb;
+      HIPCHK(launch_velocity_batch(K, N, T, G, Bv, ps, pstr, adj.zero_d, adj.dvel, adj.dU, adj.dtrw, adj.dtru, idr, I,
+                                   nb, vb, stream));
       HIPCHK(launch_stage(K, 1, N, T, G, adj.Bl, adj.zero_d, adj.dU, adj.zero_tr, adj.dtru, re_inv, cfg.dt, I, so,
                           stream, 2));
-      HIPCHK(launch_adj_dot(N, K2, lam, adj.y, adj.p, stream));
+      HIPCHK(launch_adj_dot_batch(N, K2, nb, lam, adj.y, adj.p, stream));
     };
-    for (int c = 0; c < adj.nc2; ++c)
-      for (int m = 0; m < K2; ++m) {
-        HIPCHK(launch_adj_seed(N, K2, adj.col2, c, m, adj.seed, stream));
-        HIPCHK(launch_trace(K, PRE_COPY, N, T, adj.seed.p, nullptr, nullptr, nullptr, 0., nullptr, 0, nullptr,
-                            adj.seed_tr.p, stream));
-        psi_probe(adj.seed, adj.seed_tr);
-        HIPCHK(launch_adj_gather(N, K2, adj.col2, c, m, adj.r2p, adj.r2i, adj.p, adj.psibar, stream));
-      }
-    // (3) wall velocity: a seed on the edges owned through face f at edge flux point j reaches
-    //     only the owner elements' wall flux
-    for (int f = 0; f < 4 && E > 0; ++f)
-      for (int j = 0; j < K; ++j) {
-        HIPCHK(launch_adj_edge_seed(E, K, adj.own_f, f, j, adj.lin_pvel, stream));
-        psi_probe(adj.zero_d, adj.zero_tr);
-        HIPCHK(launch_adj_edge_gather(E, K, adj.own_el, adj.own_f, f, j, adj.p, adj.pvbar, stream));
-      }
-    if (E > 0) HIPCHK(hipMemsetAsync(adj.lin_pvel.p, 0, (size_t)E * K * sizeof(double), stream));
+    for (int c = 0; c < adj.nc2; ++c) {
+      HIPCHK(launch_adj_seed_batch(N, K, adj.col2, c, T, adj.seed, adj.seed_tr, stream));
+      psi_probes(K2, adj.seed, adj.seed_tr);
+      HIPCHK(launch_adj_gather_batch(N, K2, adj.col2, c, adj.r2p, adj.r2i, adj.p, adj.psibar, stream));
+    }
+    // (3) wall velocity: probe f K + j seeds edge flux point j of the edges owned through face f,
+    //     which reaches only the owner elements' wall flux
+    if (E > 0) {
+      HIPCHK(launch_adj_edge_seed_batch(E, K, adj.own_f, adj.pvel_seed, stream));
+      vb.psi = 0; vb.trpsi = 0; vb.pvel = EK;
+      Bv.pvel = adj.pvel_seed;
+      psi_probes(4 * K, adj.zero_d, adj.zero_tr);
+      HIPCHK(launch_adj_edge_gather_batch(N, E, K, adj.own_el, adj.own_f, adj.p, adj.pvbar, stream));
+    }
     // (4) the Poisson solve: As z = psi_bar (a right-hand side -W b = psi_bar comes from the
     //     vorticity psi_bar / (W J) with zero boundary values), then omega_bar += W J z
     HIPCHK(launch_adj_wj(N, K, tab.wgt, d_jac, adj.psibar, adj.fake, 0, stream));

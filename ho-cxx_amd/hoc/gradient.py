@@ -139,7 +139,7 @@ def main(argv=None):
     J.backward()
     t3 = time.perf_counter()
     gw, ga = wt.grad.numpy(), at.grad.numpy()
-    log(f"J = {float(J):.12g}; forward {t2 - t1:.1f} s, adjoint {t3 - t2:.1f} s")
+    log(f"J = {float(J.detach()):.12g}; forward {t2 - t1:.1f} s, adjoint {t3 - t2:.1f} s")
     rows = []
     for i in range(args.dirs):
         r = np.random.default_rng(50 + i)
@@ -153,7 +153,7 @@ def main(argv=None):
     out = {"config": {"workload": "lid-driven cavity adjoint", "n": args.n, "K": args.K, "Re": args.Re,
                       "steps": args.steps, "checkpoint_every": args.every, "dt": prob.dt, "lid_modes": args.modes,
                       "tol": args.tol, "eps": args.eps, "dofs": prob.g.D},
-           "J": float(J), "forward_s": t2 - t1, "adjoint_s": t3 - t2,
+           "J": float(J.detach()), "forward_s": t2 - t1, "adjoint_s": t3 - t2,
            "max_rel_err": max(r["rel_err"] for r in rows) if rows else None,
            "median_rel_err": float(np.median([r["rel_err"] for r in rows])) if rows else None,
            "directions": rows}
