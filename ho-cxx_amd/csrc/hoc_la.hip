@@ -422,6 +422,17 @@ template <class R>
 void bicg_half(hipStream_t s, size_t n, R* x, const R* ph, const double* S, int* F) {
   hipLaunchKernelGGL(k_bicg_half<R>, dim3(vec_grid(n)), dim3(256), 0, s, n, x, ph, S, F);
 }
+__global__ void k_psi_predict(size_t n, double f, int extrap, double* __restrict__ psi, double* __restrict__ prev) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    const double a = psi[i];
+    if (extrap) psi[i] = a + f * (a - prev[i]);
+    prev[i] = a;
+  }
+}
+void psi_predict(hipStream_t s, size_t n, double f, int extrap, double* psi, double* prev) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_psi_predict, dim3(vec_grid(n)), dim3(256), 0, s, n, f, extrap, psi, prev);
+}
 template <class R>
 void zero_if_flag(hipStream_t s, size_t n, R* x, const int* F) {
   hipLaunchKernelGGL(k_zero_if_flag<R>, dim3(vec_grid(n)), dim3(256), 0, s, n, x, F);

@@ -67,6 +67,11 @@ void mixed_demote(hipStream_t s, const VecArgs& A, size_t n, const double* r64, 
                   const int* F);
 void mixed_promote_add(hipStream_t s, size_t n, double* x64, const float* e32, const int* F);
 
+// Initial guess of a stage solve from the solutions at the last two distinct stage times:
+// extrap = 0: prev = psi (history shift, psi kept);  extrap = 1: psi' = psi + f (psi - prev)
+// and prev = psi (extrapolation and history shift in one pass).
+void psi_predict(hipStream_t s, size_t n, double f, int extrap, double* psi, double* prev);
+
 // Ensembles: out[i] = ||b_i / W||^2 over instance i's elements [off[i], off[i+1]) (K2 values
 // per element), one workgroup per instance (deterministic order).
 void inst_norms(hipStream_t s, const VecArgs& A, int n_inst, const int32_t* off, int K2, const double* b, double* out);

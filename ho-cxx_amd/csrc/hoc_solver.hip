@@ -269,6 +269,8 @@ struct Solver {
     B = BCDev{b_psi, b_diff, b_nvel, b_pvel, b_vj, b_vort, b_adv, b_psi0, b_psi_r, b_diff0, b_diff_r, b_nvel0, b_nvel_r, b_pvel0, b_pvel_r};
     // fields
     for (auto* f : {&w, &psi, &k1, &k2, &k3, &k4, &vort, &kscratch, &bs}) f->alloc(D);
+    psi_prev.alloc(guess_mode() == 2 ? D : 0);
+    n_hist = 0;
     vel.alloc(2 * D); U.alloc(2 * D);
     trw.alloc((size_t)N * 8 * K); tru.alloc((size_t)N * 4 * K);
     mixed = cfg.precision == HOC_MIXED;
@@ -645,7 +647,37 @@ struct Solver {
   //         tolerance eta, psi += e in FP64, and the FP64 residual is recomputed; the loop
   //         ends on the same FP64 test as the plain solve.
   const double* rhs_bc = nullptr;  // boundary stream-function values of the next solve (null: b_psi)
+
+  // Initial guess of the stage solves.  Mode 1: the previous solution, as the reference
+  // (src/HO_2D.cpp:1949-1953).  Mode 2: psi is affine in omega and the RK stage inputs advance
+  // smoothly in the stage time tau = steps + c, so the guess at tau is the linear extrapolation
+  // psi_a + (tau - tau_a) / (tau_a - tau_b) (psi_a - psi_b) from the solutions at the last two
+  // distinct stage times (the previous solution where tau = tau_a).  Only the iteration count
+  // changes: every solve still stops on the same residual test.
+  DBuf<double> psi_prev;        // psi_b
+  double tau_a = 0., tau_b = 0.;
+  int n_hist = 0;               // solutions in the history (0: psi is not a stage solution)
+  int64_t step_count = 0;
+  double stage_tau = std::nan("");  // stage time of the solve in progress (NaN: not a stage solve)
+  int guess_mode() const { return cfg.psi_guess == 0 ? 2 : cfg.psi_guess; }
+  void guess_before_solve() {
+    if (guess_mode() != 2 || std::isnan(stage_tau)) return;
+    const double tau = stage_tau;
+    if (n_hist == 0 || tau == tau_a) return;  // no history, or the same stage time: the previous solution
+    const bool ok = n_hist == 2 && tau > tau_a && tau - tau_a <= 1. && tau_a > tau_b && tau_a - tau_b <= 1.;
+    psi_predict(stream, D, ok ? (tau - tau_a) / (tau_a - tau_b) : 0., ok ? 1 : 0, psi, psi_prev);
+  }
+  void guess_after_solve() {
+    if (guess_mode() != 2 || std::isnan(stage_tau)) { n_hist = 0; return; }
+    if (n_hist >= 1 && stage_tau == tau_a) return;
+    if (n_hist == 0) { tau_a = stage_tau; n_hist = 1; return; }
+    tau_b = tau_a;
+    tau_a = stage_tau;
+    n_hist = 2;
+  }
+
   void poisson(const double* omega) {
+    guess_before_solve();
     const int lb = lap_blocks(N);
     const int maxit = cfg.maxiter;
     const double tol2 = cfg.tol * cfg.tol;
@@ -712,6 +744,7 @@ struct Solver {
     if (hf[F_ZERO]) last_rel = 0.;
     total_iters += last_iters;
     ++n_solves;
+    guess_after_solve();
     if (!std::isfinite(last_rel)) throw StatusError(HOC_ENAN, "NaN/Inf in the Poisson solve");
     if (cfg.fail_on_noconv && last_rel > cfg.tol)
       throw StatusError(HOC_ENOCONV, "Poisson solve did not converge: rel residual " + std::to_string(last_rel));
@@ -850,7 +883,15 @@ struct Solver {
   }
   void stage(const double* win, double c, const StageOut& so) {
     update_bcs(c);
-    stage_body(win, so);
+    stage_tau = (double)step_count + c;
+    try {
+      stage_body(win, so);
+    } catch (...) {
+      stage_tau = std::nan("");
+      n_hist = 0;
+      throw;
+    }
+    stage_tau = std::nan("");
   }
 
   // solve_advection_diffusion (src/HO_2D.cpp:1465-1531)
@@ -876,6 +917,7 @@ struct Solver {
       throw StatusError(HOC_EINVAL, "time_integration_type must be 1, 2 or 4");
     }
     time += cfg.dt;
+    ++step_count;
     for (int i = 0; i < n_inst && n_inst > 1; ++i) inst_time[i] += inst_dt[i];
   }
   void stage_rk4(const double* win, double c, double* kout, double cnext) {
@@ -1234,6 +1276,7 @@ struct Solver {
         const int n0 = sg * every, n1 = std::min(n0 + every, nsteps);
         dcopy(w, adj.ckbuf.p + (size_t)sg * D, D);
         time = adj.t0 + n0 * cfg.dt;
+        n_hist = 0;
         for (int n = n0; n < n1; ++n) {
           dcopy(adj.segbuf.p + (size_t)(n - n0) * D, w, D);
           if (n + 1 < n1) rk_step();
@@ -1400,6 +1443,7 @@ int hoc_create(hoc_t* out, const hoc_config* cfg) {
   if (!(cfg->mixed_inner_tol >= 0. && cfg->mixed_inner_tol < 1.)) return hoc_fail(nullptr, HOC_EINVAL, "mixed_inner_tol must be in [0, 1)");
   if (cfg->cheb_degree < 0 || cfg->cheb_degree > hoc::kMaxCheb) return hoc_fail(nullptr, HOC_EINVAL, "cheb_degree must be in 0..8");
   if (!(cfg->cheb_lower >= 0. && cfg->cheb_lower < 1.)) return hoc_fail(nullptr, HOC_EINVAL, "cheb_lower must be in [0, 1)");
+  if (cfg->psi_guess < 0 || cfg->psi_guess > 2) return hoc_fail(nullptr, HOC_EINVAL, "psi_guess must be 0, 1 or 2");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return hoc_fail(nullptr, HOC_EDEVICE, "no HIP device");
   if (cfg->device < 0 || cfg->device >= ndev) return hoc_fail(nullptr, HOC_EDEVICE, "device ordinal out of range");
@@ -1551,6 +1595,7 @@ int hoc_set_vorticity(hoc_t h, int64_t len, const double* w) {
   return guarded(h, [&] {
     NEED_READY(h); NEED_LEN(len, h->s.D);
     h->s.h2d(h->s.w.p, w, h->s.D * sizeof(double));
+    h->s.n_hist = 0;
   });
 }
 int hoc_get_vorticity(hoc_t h, int64_t len, double* w) {
@@ -1569,6 +1614,7 @@ int hoc_set_streamfunction(hoc_t h, int64_t len, const double* p) {
   return guarded(h, [&] {
     NEED_READY(h); NEED_LEN(len, h->s.D);
     h->s.h2d(h->s.psi.p, p, h->s.D * sizeof(double));
+    h->s.n_hist = 0;
   });
 }
 int hoc_get_velocity(hoc_t h, int64_t len, double* uv) {

@@ -50,7 +50,7 @@ class hoc_config(C.Structure):
                 ("time_integration_type", C.c_int32), ("advection_type", C.c_int32), ("tol", C.c_double),
                 ("maxiter", C.c_int32), ("precision", C.c_int32), ("krylov", C.c_int32), ("device", C.c_int32),
                 ("fail_on_noconv", C.c_int32), ("mixed_inner_tol", C.c_double), ("cheb_degree", C.c_int32),
-                ("reserved0", C.c_int32), ("cheb_lower", C.c_double)]
+                ("psi_guess", C.c_int32), ("cheb_lower", C.c_double)]
 
 
 class hoc_stats(C.Structure):
@@ -160,12 +160,12 @@ class HO2D:
 
     def __init__(self, K=3, problem_type=1, Re=100.0, dt=1e-3, time_integration_type=4, advection_type=1,
                  tol=1e-15, maxiter=150, precision=HOC_FP64, krylov=KRYLOV_AUTO, device=0, fail_on_noconv=False,
-                 mixed_inner_tol=0.0, cheb_degree=0, cheb_lower=0.0):
+                 mixed_inner_tol=0.0, cheb_degree=0, cheb_lower=0.0, psi_guess=0):
         self.cfg = default_config(K=K, problem_type=problem_type, Re=Re, dt=dt,
                                   time_integration_type=time_integration_type, advection_type=advection_type,
                                   tol=tol, maxiter=maxiter, precision=precision, krylov=krylov, device=device,
                                   fail_on_noconv=int(fail_on_noconv), mixed_inner_tol=mixed_inner_tol,
-                                  cheb_degree=cheb_degree, cheb_lower=cheb_lower)
+                                  cheb_degree=cheb_degree, cheb_lower=cheb_lower, psi_guess=psi_guess)
         self.h = C.c_void_p()
         rc = lib().hoc_create(C.byref(self.h), C.byref(self.cfg))
         if rc != HOC_OK:

@@ -57,7 +57,11 @@ typedef struct {
                                      tolerance per refinement pass (0 = 1e-5)            */
   int32_t cheb_degree;            /* fine-level Chebyshev smoother degree, pre and post
                                      (0 = 3; at most 8)                                   */
-  int32_t reserved0;
+  int32_t psi_guess;              /* initial guess of the stage Poisson solves: 1 = the
+                                     previous solution (the reference's warm start),
+                                     2 = linear extrapolation in the stage time from the
+                                     solutions at the last two distinct stage times
+                                     (0 = 2)                                              */
   double cheb_lower;              /* smoothing interval [cheb_lower, 1.1] x lambda_max(D^-1 As)
                                      (0 = 1/60)                                          */
 } hoc_config;
