@@ -51,15 +51,97 @@ __global__ void k_adj_dot(int N, int K2, const double* __restrict__ lam, const d
 
 __global__ void k_adj_gather(int N, int K2, const int32_t* __restrict__ color, int c,
                              const int32_t* __restrict__ rptr, const int32_t* __restrict__ ridx,
-                             const double* __restrict__ p, double* __restrict__ out) {
+                             const double* __restrict__ p, double* __restrict__ out, int ostride, int ooff) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= N || color[e] != c) return;
   for (int m = 0; m < K2; ++m) {
     const double* pm = p + (size_t)m * N;
     double s = 0.;
     for (int j = rptr[e]; j < rptr[e + 1]; ++j) s += pm[ridx[j]];
-    out[(size_t)e * K2 + m] = s;
+    out[(size_t)e * ostride + ooff + m] = s;
   }
+}
+
+// U seeds and their normal traces (k_velocity's: faces 0/1 from U1 along xi at eta point j,
+// faces 2/3 from U2 along eta at xi point i)
+__global__ void k_adj_seed_u(int N, int K, const int32_t* __restrict__ color, int c, int comp, Tab T,
+                             double* __restrict__ U, double* __restrict__ tru) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  const int b = blockIdx.y;
+  if (e >= N) return;
+  const int K2 = K * K, j0 = b / K, i0 = b % K;
+  const bool on = color[e] == c;
+  double* u = U + ((size_t)b * N + e) * 2 * K2;
+  for (int o = 0; o < 2 * K2; ++o) u[o] = (on && o == comp * K2 + b) ? 1. : 0.;
+  double* t = tru + (size_t)b * 4 * N * K;
+  for (int f = 0; f < 4; ++f) {
+    double* q = t + ((size_t)f * N + e) * K;
+    const bool face_on = on && ((f < 2) == (comp == 0));
+    const int pos = f < 2 ? j0 : i0, idx = f < 2 ? i0 : j0, ib = f & 1;
+    for (int k = 0; k < K; ++k) q[k] = (face_on && k == pos) ? T.bb[idx][ib] : 0.;
+  }
+}
+
+__global__ void k_adj_assemble_vel(int N, int K, const int32_t* __restrict__ color, int c,
+                                   const int32_t* __restrict__ nb, const double* __restrict__ dU,
+                                   double* __restrict__ Mv) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  const int m = blockIdx.y;
+  if (e >= N) return;
+  const int K2 = K * K, O = 2 * K2;
+  for (int sl = 0; sl < 5; ++sl) {
+    const int i = sl == 0 ? e : nb[(size_t)e * 4 + sl - 1];
+    if (i < 0 || color[i] != c) continue;
+    const double* src = dU + ((size_t)m * N + e) * O;
+    double* dst = Mv + (((size_t)e * 5 + sl) * K2 + m) * O;
+    for (int o = 0; o < O; ++o) dst[o] = src[o];
+  }
+}
+
+__global__ void k_adj_assemble_diff(int N, int E, int K, const int32_t* __restrict__ color, int c,
+                                    const int32_t* __restrict__ own_el, const double* __restrict__ dd,
+                                    double* __restrict__ Md) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  const int m = blockIdx.y;
+  if (g >= E || color[own_el[g]] != c) return;
+  const int K2 = K * K;
+  for (int j = 0; j < K; ++j) Md[((size_t)g * K + j) * K2 + m] = dd[(size_t)m * E * K + (size_t)g * K + j];
+}
+
+// one thread per (element, dof)
+__global__ void k_adj_vel_transpose(int N, int K, const int32_t* __restrict__ nb, const int8_t* __restrict__ nbf,
+                                    const double* __restrict__ Mv, const double* __restrict__ Md,
+                                    const double* __restrict__ Ubar, const double* __restrict__ diffbar,
+                                    double* __restrict__ psibar) {
+  const int K2 = K * K, O = 2 * K2;
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t)N * K2) return;
+  const int i = (int)(t / K2), m = (int)(t - (size_t)i * K2);
+  double s = 0.;
+  for (int sl = 0; sl < 5; ++sl) {
+    int e = i, es = 0;
+    if (sl > 0) {
+      e = nb[(size_t)i * 4 + sl - 1];
+      if (e < 0) {  // boundary edge of i: its wall flux
+        const int g = -1 - e;
+        for (int j = 0; j < K; ++j) s += Md[((size_t)g * K + j) * K2 + m] * diffbar[(size_t)g * K + j];
+        continue;
+      }
+      es = 1 + nbf[(size_t)i * 4 + sl - 1];
+    }
+    const double* a = Mv + (((size_t)e * 5 + es) * K2 + m) * O;
+    const double* u = Ubar + (size_t)e * O;
+    for (int o = 0; o < O; ++o) s += a[o] * u[o];
+  }
+  psibar[t] = s;
+}
+
+__global__ void k_adj_wall(int E, int K, const int8_t* __restrict__ no_slip, const double* __restrict__ esign,
+                           double idr, const double* __restrict__ diffbar, double* __restrict__ pvbar) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= E * K) return;
+  const int g = i / K;
+  if (no_slip[g]) pvbar[i] += esign[g] * idr * diffbar[i];
 }
 
 __global__ void k_adj_edge_seed(int E, int K, const int8_t* __restrict__ own_f, double* __restrict__ x) {
@@ -119,9 +201,47 @@ hipError_t launch_adj_dot_batch(int N, int K2, int nb, const double* lam, const 
 }
 
 hipError_t launch_adj_gather_batch(int N, int K2, const int32_t* color, int c, const int32_t* rptr,
-                                   const int32_t* ridx, const double* p, double* out, hipStream_t s) {
+                                   const int32_t* ridx, const double* p, double* out, int ostride, int ooff,
+                                   hipStream_t s) {
   if (N == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_adj_gather, dim3(grid(N)), dim3(kTB), 0, s, N, K2, color, c, rptr, ridx, p, out);
+  hipLaunchKernelGGL(k_adj_gather, dim3(grid(N)), dim3(kTB), 0, s, N, K2, color, c, rptr, ridx, p, out, ostride,
+                     ooff);
+  return hipGetLastError();
+}
+
+hipError_t launch_adj_seed_u_batch(int N, int K, const int32_t* color, int c, int comp, const Tab& T, double* U,
+                                   double* tru, hipStream_t s) {
+  if (N == 0) return hipSuccess;
+  if (K > kMaxKd || comp < 0 || comp > 1) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_adj_seed_u, dim3(grid(N), K * K), dim3(kTB), 0, s, N, K, color, c, comp, T, U, tru);
+  return hipGetLastError();
+}
+
+hipError_t launch_adj_assemble_vel(int N, int E, int K, const int32_t* color, int c, const int32_t* nb,
+                                   const int32_t* own_el, const double* dU, const double* dd, double* Mv, double* Md,
+                                   hipStream_t s) {
+  if (N == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_adj_assemble_vel, dim3(grid(N), K * K), dim3(kTB), 0, s, N, K, color, c, nb, dU, Mv);
+  if (E > 0)
+    hipLaunchKernelGGL(k_adj_assemble_diff, dim3(grid(E), K * K), dim3(kTB), 0, s, N, E, K, color, c, own_el, dd, Md);
+  return hipGetLastError();
+}
+
+hipError_t launch_adj_vel_transpose(int N, int K, const int32_t* nb, const int8_t* nbf, const double* Mv,
+                                    const double* Md, const double* Ubar, const double* diffbar, double* psibar,
+                                    hipStream_t s) {
+  const size_t D = (size_t)N * K * K;
+  if (D == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_adj_vel_transpose, dim3(grid(D)), dim3(kTB), 0, s, N, K, nb, nbf, Mv, Md, Ubar, diffbar,
+                     psibar);
+  return hipGetLastError();
+}
+
+hipError_t launch_adj_wall(int E, int K, const int8_t* no_slip, const double* esign, double idr,
+                           const double* diffbar, double* pvbar, hipStream_t s) {
+  if (E == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_adj_wall, dim3(grid((size_t)E * K)), dim3(kTB), 0, s, E, K, no_slip, esign, idr, diffbar,
+                     pvbar);
   return hipGetLastError();
 }
 

@@ -1002,15 +1002,17 @@ struct Solver {
   // ---- discrete adjoint of the RK4 trajectory -------------------------------------------
   // One stage maps omega_s to k_s = dt R(omega_s), with psi_s = A^-1 b(omega_s) and
   // b = -omega J - (boundary source).  Its transpose is applied as
-  //   omega_bar  = (dk/domega)^T lam                          (1-ring: omega and its face traces)
-  //   psi_bar    = (dk/dpsi)^T lam                            (2-ring: psi -> U, wall flux -> k)
-  //   omega_bar += J W z,   As z = psi_bar                    (A^-T = -W As^-1, As symmetric)
-  //   pvel_bar  += (dk/dpvel)^T lam                           (the wall velocity's flux)
-  // each product by coloured probes of the frozen-switch stage (launch_stage mode 2).
+  //   omega_bar  = (dk/domega)^T lam                 (probes: omega and its face traces, 1-ring)
+  //   U_bar      = (dk/dU)^T lam,  diff_bar = (dk/ddiff)^T lam   (probes: velocity, wall flux)
+  //   pvel_bar  += sign idr diff_bar                 (no-slip walls: diff = (vjump + sign pvel) idr)
+  //   psi_bar    = (dU/dpsi)^T U_bar + (ddiff/dpsi)^T diff_bar   (assembled once: the map is linear
+  //                                                              and state-independent)
+  //   omega_bar += J W z,   As z = psi_bar           (A^-T = -W As^-1, As symmetric)
+  // the probes as coloured, batched launches of the frozen-switch stage (launch_stage mode 2).
   struct Adj {
     bool init = false;
-    int nc1 = 0, nc2 = 0;
-    DBuf<int32_t> col1, col2, r1p, r1i, r2p, r2i, own_el;
+    int nc1 = 0;
+    DBuf<int32_t> col1, r1p, r1i, own_el;
     DBuf<int8_t> own_f;
     DBuf<double> ws[4], Us[4], trws[4], trus[4];  // recorded stages of one step
     DBuf<int8_t> sig[4];
@@ -1018,11 +1020,17 @@ struct Solver {
     DBuf<double> seed, seed_tr, y, p, dU, dvel, dtrw, dtru, lin_diff, lin_vj, pvel_seed;
     DBuf<double> zero_d, zero_tr, zero_e, lin_nvel, lin_vort, lin_adv;
     DBuf<double> lam, xbar, psibar, z, fake, wbar, kbar[4], pvbar;
+    DBuf<double> Mv, Md, Ubar, diffbar, zero_u, zero_tu;  // assembled velocity map and its adjoints
     DBuf<double> ckbuf, segbuf, save_w, save_psi, save_vel;
     int nsteps = 0, every = 0;
     double t0 = 0.;
     bool have_forward = false;
     BCDev Bz{}, Bl{};
+    BCDev Bl_seed() const {  // homogeneous boundary values with the wall flux from the edge seeds
+      BCDev b = Bl;
+      b.diff = pvel_seed;
+      return b;
+    }
   } adj;
 
   void adjoint_check() const {
@@ -1093,13 +1101,11 @@ struct Solver {
   void adjoint_setup() {
     adjoint_check();
     if (adj.init) return;
-    std::vector<int32_t> c1, c2, p1, i1, p2, i2;
+    std::vector<int32_t> c1, p1, i1;
     adj.nc1 = color_within(2, c1);
-    adj.nc2 = color_within(4, c2);
     rings(1, p1, i1);
-    rings(2, p2, i2);
-    adj.col1.upload(c1); adj.col2.upload(c2);
-    adj.r1p.upload(p1); adj.r1i.upload(i1); adj.r2p.upload(p2); adj.r2i.upload(i2);
+    adj.col1.upload(c1);
+    adj.r1p.upload(p1); adj.r1i.upload(i1);
     std::vector<int32_t> oe(std::max(E, 1), 0);
     std::vector<int8_t> of(std::max(E, 1), -1);
     for (int e = 0; e < E; ++e) { oe[e] = mesh.be_elem[e]; of[e] = mesh.be_ijp[e]; }
@@ -1119,6 +1125,7 @@ struct Solver {
     adj.dU.alloc(NB * 2 * D); adj.dvel.alloc(NB * 2 * D); adj.dtru.alloc(NB * TU);
     adj.p.alloc(NB * std::max(N, 1));
     adj.lin_diff.alloc(NB * EK); adj.lin_vj.alloc(NB * EK); adj.pvel_seed.alloc((size_t)4 * K * EK);
+    // seeds of the wall-flux probes are edge arrays like the wall velocity's
     for (DBuf<double>* b : {&adj.zero_e, &adj.lin_vort, &adj.lin_adv, &adj.pvbar}) b->alloc(EK);
     adj.lin_nvel.alloc(EK);
     HIPCHK(hipMemcpy(adj.lin_nvel.p, b_nvel.p, EK * sizeof(double), hipMemcpyDeviceToDevice));
@@ -1130,6 +1137,21 @@ struct Solver {
     adj.Bl.nvel = adj.lin_nvel; adj.Bl.vort = adj.lin_vort; adj.Bl.adv = adj.lin_adv;
     adj.Bz = adj.Bl;
     adj.Bz.diff = adj.zero_e;
+    adj.Mv.alloc((size_t)N * 5 * K2 * 2 * K2);
+    adj.Md.alloc(EK * K2);
+    adj.Ubar.alloc(2 * D); adj.zero_u.alloc(2 * D); adj.zero_tu.alloc(TU); adj.diffbar.alloc(EK);
+    // the velocity map psi -> (U, wall flux): K^2 probes per colour of the distance-2 colouring
+    // (U of an element depends on psi of its face ring), velocity with homogeneous boundary values
+    VelBatch vb;
+    vb.psi = D; vb.trpsi = TR; vb.vel = 2 * D; vb.U = 2 * D; vb.trw = TR; vb.tru = TU; vb.vjump = EK; vb.diff = EK;
+    for (int c = 0; c < adj.nc1; ++c) {
+      HIPCHK(launch_adj_seed_batch(N, K, adj.col1, c, T, adj.seed, adj.seed_tr, stream));
+      HIPCHK(launch_velocity_batch(K, N, T, G, adj.Bl, adj.seed, adj.seed_tr, adj.zero_d, adj.dvel, adj.dU, adj.dtrw,
+                                   adj.dtru, 1. / (cfg.dt * re_inv), I, K2, vb, stream));
+      HIPCHK(launch_adj_assemble_vel(N, E, K, adj.col1, c, d_nb, adj.own_el, adj.dU, adj.lin_diff, adj.Mv, adj.Md,
+                                     stream));
+    }
+    HIPCHK(hipStreamSynchronize(stream));
     adj.init = true;
   }
 
@@ -1177,38 +1199,39 @@ struct Solver {
       HIPCHK(launch_stage(K, 1, N, T, G, adj.Bz, adj.seed, adj.Us[st], adj.seed_tr, adj.trus[st], re_inv, cfg.dt, I,
                           so, stream, 2));
       HIPCHK(launch_adj_dot_batch(N, K2, K2, lam, adj.y, adj.p, stream));
-      HIPCHK(launch_adj_gather_batch(N, K2, adj.col1, c, adj.r1p, adj.r1i, adj.p, xbar, stream));
+      HIPCHK(launch_adj_gather_batch(N, K2, adj.col1, c, adj.r1p, adj.r1i, adj.p, xbar, K2, 0, stream));
     }
-    // (2) psi: velocity and wall flux of the seeds with homogeneous boundary values, then the
-    //     stage advecting the recorded vorticity with those velocities
+    // (2) velocity: K^2 probes per colour and component (the stage depends on U over the face ring),
+    //     the stage advecting the recorded vorticity with the seeded velocity
     so.omega_adv = adj.ws[st]; so.bs_wa = 0;
     so.trw_adv = adj.trws[st]; so.bs_trwa = 0;
-    so.bs_w = 0; so.bs_trw = 0; so.bs_U = 2 * D; so.bs_tru = TU; so.bs_diff = EK;
-    VelBatch vb;
-    vb.psi = D; vb.trpsi = TR; vb.vel = 2 * D; vb.U = 2 * D; vb.trw = TR; vb.tru = TU; vb.vjump = EK; vb.diff = EK;
-    BCDev Bv = adj.Bl;
-    auto psi_probes = [&](int nb, const double* ps, const double* pstr) {
-      so.nbatch = nb;
-      HIPCHK(launch_velocity_batch(K, N, T, G, Bv, ps, pstr, adj.zero_d, adj.dvel, adj.dU, adj.dtrw, adj.dtru, idr, I,
-                                   nb, vb, stream));
-      HIPCHK(launch_stage(K, 1, N, T, G, adj.Bl, adj.zero_d, adj.dU, adj.zero_tr, adj.dtru, re_inv, cfg.dt, I, so,
-                          stream, 2));
-      HIPCHK(launch_adj_dot_batch(N, K2, nb, lam, adj.y, adj.p, stream));
-    };
-    for (int c = 0; c < adj.nc2; ++c) {
-      HIPCHK(launch_adj_seed_batch(N, K, adj.col2, c, T, adj.seed, adj.seed_tr, stream));
-      psi_probes(K2, adj.seed, adj.seed_tr);
-      HIPCHK(launch_adj_gather_batch(N, K2, adj.col2, c, adj.r2p, adj.r2i, adj.p, adj.psibar, stream));
-    }
-    // (3) wall velocity: probe f K + j seeds edge flux point j of the edges owned through face f,
-    //     which reaches only the owner elements' wall flux
+    so.bs_w = 0; so.bs_trw = 0; so.bs_U = 2 * D; so.bs_tru = TU; so.bs_diff = 0;
+    so.nbatch = K2;
+    for (int c = 0; c < adj.nc1; ++c)
+      for (int comp = 0; comp < 2; ++comp) {
+        HIPCHK(launch_adj_seed_u_batch(N, K, adj.col1, c, comp, T, adj.dU, adj.dtru, stream));
+        HIPCHK(launch_stage(K, 1, N, T, G, adj.Bz, adj.zero_d, adj.dU, adj.zero_tr, adj.dtru, re_inv, cfg.dt, I, so,
+                            stream, 2));
+        HIPCHK(launch_adj_dot_batch(N, K2, K2, lam, adj.y, adj.p, stream));
+        HIPCHK(launch_adj_gather_batch(N, K2, adj.col1, c, adj.r1p, adj.r1i, adj.p, adj.Ubar, 2 * K2, comp * K2,
+                                       stream));
+      }
+    // (3) wall flux: probe f K + j seeds edge flux point j of the edges owned through face f, which
+    //     reaches only the owner elements; then the wall velocity's share of it
+    HIPCHK(hipMemsetAsync(adj.diffbar.p, 0, EK * sizeof(double), stream));
     if (E > 0) {
       HIPCHK(launch_adj_edge_seed_batch(E, K, adj.own_f, adj.pvel_seed, stream));
-      vb.psi = 0; vb.trpsi = 0; vb.pvel = EK;
-      Bv.pvel = adj.pvel_seed;
-      psi_probes(4 * K, adj.zero_d, adj.zero_tr);
-      HIPCHK(launch_adj_edge_gather_batch(N, E, K, adj.own_el, adj.own_f, adj.p, adj.pvbar, stream));
+      so.omega_adv = adj.zero_d; so.trw_adv = adj.zero_tr;
+      so.bs_U = 0; so.bs_tru = 0; so.bs_diff = EK;
+      so.nbatch = 4 * K;
+      HIPCHK(launch_stage(K, 1, N, T, G, adj.Bl_seed(), adj.zero_d, adj.zero_u, adj.zero_tr, adj.zero_tu, re_inv,
+                          cfg.dt, I, so, stream, 2));
+      HIPCHK(launch_adj_dot_batch(N, K2, 4 * K, lam, adj.y, adj.p, stream));
+      HIPCHK(launch_adj_edge_gather_batch(N, E, K, adj.own_el, adj.own_f, adj.p, adj.diffbar, stream));
+      HIPCHK(launch_adj_wall(E, K, d_noslip, d_esign, idr, adj.diffbar, adj.pvbar, stream));
     }
+    // psi_bar through the assembled velocity map
+    HIPCHK(launch_adj_vel_transpose(N, K, d_nb, d_nbf, adj.Mv, adj.Md, adj.Ubar, adj.diffbar, adj.psibar, stream));
     // (4) the Poisson solve: As z = psi_bar (a right-hand side -W b = psi_bar comes from the
     //     vorticity psi_bar / (W J) with zero boundary values), then omega_bar += W J z
     HIPCHK(launch_adj_wj(N, K, tab.wgt, d_jac, adj.psibar, adj.fake, 0, stream));
