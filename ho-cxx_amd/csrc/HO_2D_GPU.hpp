@@ -174,6 +174,30 @@ class HO_2D_GPU {
     return 0;
   }
   double current_time() const { return h_ ? hoc_get_time(h_) : 0.; }
+
+  // ---- wall velocity and gradients (no counterpart in the reference) ----
+  // BC_parl_vel at the boundary flux points [E][K] (src/HO_2D.cpp:717-725 sets -1 along the lid).
+  std::vector<double> wall_velocity() const {
+    std::vector<double> v((size_t)hoc_boundary_point_count(h_));
+    check(hoc_get_wall_velocity(h_, (int64_t)v.size(), v.data()), h_);
+    return v;
+  }
+  void set_wall_velocity(const std::vector<double>& v) {
+    check(hoc_set_wall_velocity(h_, (int64_t)v.size(), v.data()), h_);
+  }
+  // nsteps RK4 steps keeping checkpoints every `every` steps, then the gradients of an objective
+  // with respect to the initial vorticity and the wall velocity from its gradient with respect
+  // to the final vorticity (sizes: number of solution points, boundary flux points).
+  void adjoint_forward(int32_t nsteps, int32_t every) { check(hoc_adjoint_forward(h_, nsteps, every), h_); }
+  void adjoint_backward(const std::vector<double>& wbar_final, std::vector<double>& wbar_initial,
+                        std::vector<double>& wall_velocity_bar) {
+    wbar_initial.resize(wbar_final.size());
+    wall_velocity_bar.resize((size_t)hoc_boundary_point_count(h_));
+    check(hoc_adjoint_backward(h_, (int64_t)wbar_final.size(), wbar_final.data(), wbar_initial.data(),
+                               (int64_t)wall_velocity_bar.size(), wall_velocity_bar.data()),
+          h_);
+  }
+
   void clean_up() {
     if (h_) hoc_destroy(h_);
     h_ = nullptr;
