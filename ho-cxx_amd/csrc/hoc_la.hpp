@@ -71,6 +71,10 @@ void mixed_promote_add(hipStream_t s, size_t n, double* x64, const float* e32, c
 // extrap = 0: prev = psi (history shift, psi kept);  extrap = 1: psi' = psi + f (psi - prev)
 // and prev = psi (extrapolation and history shift in one pass).
 void psi_predict(hipStream_t s, size_t n, double f, int extrap, double* psi, double* prev);
+// psi' = sum_j a[j] x_j (x_j may be psi itself, read before the write), and save = psi (the old
+// value) when save is not null; nterm <= 5.
+void psi_combine(hipStream_t s, size_t n, int nterm, const double* a, const double* const* x, double* psi,
+                 double* save);
 
 // Ensembles: out[i] = ||b_i / W||^2 over instance i's elements [off[i], off[i+1]) (K2 values
 // per element), one workgroup per instance (deterministic order).

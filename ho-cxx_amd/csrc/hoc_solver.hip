@@ -269,8 +269,11 @@ struct Solver {
     B = BCDev{b_psi, b_diff, b_nvel, b_pvel, b_vj, b_vort, b_adv, b_psi0, b_psi_r, b_diff0, b_diff_r, b_nvel0, b_nvel_r, b_pvel0, b_pvel_r};
     // fields
     for (auto* f : {&w, &psi, &k1, &k2, &k3, &k4, &vort, &kscratch, &bs}) f->alloc(D);
-    psi_prev.alloc(guess_mode() == 2 ? D : 0);
+    psi_prev.alloc(guess_mode() >= 2 ? D : 0);
+    for (auto& b : rkS) b.alloc(guess_mode() == 3 ? D : 0);
     n_hist = 0;
+    rk_full = false;
+    rk_cur = 0;
     vel.alloc(2 * D); U.alloc(2 * D);
     trw.alloc((size_t)N * 8 * K); tru.alloc((size_t)N * 4 * K);
     mixed = cfg.precision == HOC_MIXED;
@@ -659,16 +662,66 @@ struct Solver {
   int n_hist = 0;               // solutions in the history (0: psi is not a stage solution)
   int64_t step_count = 0;
   double stage_tau = std::nan("");  // stage time of the solve in progress (NaN: not a stage solve)
-  int guess_mode() const { return cfg.psi_guess == 0 ? 2 : cfg.psi_guess; }
+  // default: mode 3 with RK4; with Euler and RK2 the reference's warm start (extrapolating over
+  // their steps measured no better: their stiff modes alternate in sign from step to step)
+  int guess_mode() const {
+    if (cfg.psi_guess != 0) return cfg.psi_guess;
+    return cfg.time_integration_type == 4 ? 3 : 1;
+  }
+  // Mode 3 (RK4): psi(omega_n + c k) = psi(omega_n) + c Psi(k) with Psi linear, so the stage
+  // solutions S1..S4 of a step give Psi(k1) = 2 (S2 - S1), Psi(k2) = 2 (S3 - S1), Psi(k3) = S4 - S1,
+  // and the next step's first solve is S1 + (Psi(k1) + 2 Psi(k2) + 2 Psi(k3) + Psi(k4)) / 6 with
+  // Psi(k4) ~ Psi(k3): (-3 S1 + 2 S2 + 4 S3 + 3 S4) / 6.  Its solution T1 then gives Psi(k4) exactly,
+  // and the second stage, T1 + Psi(k1') / 2 with Psi(k1') ~ Psi(k4): 4 T1 + S1 - S2 - 2 S3 - S4.
+  // Stages 3 and 4 as in mode 2 (S2; 2 S3 - S1).  Other integrators use mode 2.
+  DBuf<double> rkS[4];
+  int rk_stage = -1, rk_cur = 0;  // stage index of the RK4 solve in progress; stages of this step solved
+  bool rk_full = false;           // S1..S4 hold a complete step
+  void rk_guess_before() {
+    const int st = rk_stage;
+    if (st == 0 && rk_full) {
+      const double a[4] = {-0.5, 1. / 3., 2. / 3., 0.5};
+      const double* x[4] = {rkS[0], rkS[1], rkS[2], rkS[3]};
+      psi_combine(stream, D, 4, a, x, psi, nullptr);
+    } else if (st == 1 && rk_cur == 1) {
+      if (rk_full) {
+        const double a[5] = {4., 1., -1., -2., -1.};
+        const double* x[5] = {psi, rkS[0], rkS[1], rkS[2], rkS[3]};
+        psi_combine(stream, D, 5, a, x, psi, rkS[0]);
+      } else {
+        dcopy(rkS[0], psi, D);
+      }
+    } else if (st == 3 && rk_cur == 3) {
+      const double a[2] = {2., -1.};
+      const double* x[2] = {psi, rkS[0]};
+      psi_combine(stream, D, 2, a, x, psi, nullptr);
+    }
+  }
+  void rk_guess_after() {
+    const int st = rk_stage;
+    if (st != rk_cur % 4 || (st > 0 && rk_cur != st)) { rk_full = false; rk_cur = 0; return; }
+    if (st >= 1) dcopy(rkS[st], psi, D);
+    rk_cur = st + 1;
+    if (st == 3) { rk_full = true; rk_cur = 0; }
+  }
+  void invalidate_guess() {
+    n_hist = 0;
+    rk_full = false;
+    rk_cur = 0;
+  }
   void guess_before_solve() {
-    if (guess_mode() != 2 || std::isnan(stage_tau)) return;
+    if (guess_mode() == 3 && rk_stage >= 0 && !std::isnan(stage_tau)) { rk_guess_before(); return; }
+    if (guess_mode() < 2 || std::isnan(stage_tau)) return;
     const double tau = stage_tau;
     if (n_hist == 0 || tau == tau_a) return;  // no history, or the same stage time: the previous solution
     const bool ok = n_hist == 2 && tau > tau_a && tau - tau_a <= 1. && tau_a > tau_b && tau_a - tau_b <= 1.;
     psi_predict(stream, D, ok ? (tau - tau_a) / (tau_a - tau_b) : 0., ok ? 1 : 0, psi, psi_prev);
   }
   void guess_after_solve() {
-    if (guess_mode() != 2 || std::isnan(stage_tau)) { n_hist = 0; return; }
+    if (guess_mode() == 3 && rk_stage >= 0 && !std::isnan(stage_tau)) { rk_guess_after(); return; }
+    rk_full = false;
+    rk_cur = 0;
+    if (guess_mode() < 2 || std::isnan(stage_tau)) { n_hist = 0; return; }
     if (n_hist >= 1 && stage_tau == tau_a) return;
     if (n_hist == 0) { tau_a = stage_tau; n_hist = 1; return; }
     tau_b = tau_a;
@@ -888,7 +941,10 @@ struct Solver {
       stage_body(win, so);
     } catch (...) {
       stage_tau = std::nan("");
+      rk_stage = -1;
       n_hist = 0;
+      rk_full = false;
+      rk_cur = 0;
       throw;
     }
     stage_tau = std::nan("");
@@ -908,11 +964,16 @@ struct Solver {
     } else if (cfg.time_integration_type == 4) {
       const double c[4] = {0., 0.5, 0.5, 1.};
       double* ks[3] = {k1, k2, k3};
+      rk_stage = 0;
       stage_rk4(w, c[0], ks[0], 0.5);
+      rk_stage = 1;
       stage_rk4(vort, c[1], ks[1], 0.5);
+      rk_stage = 2;
       stage_rk4(vort, c[2], ks[2], 1.0);
       StageOut s4; s4.base = w; s4.final_mode = 4; s4.k1 = k1; s4.k2 = k2; s4.k3 = k3;
+      rk_stage = 3;
       stage(vort, c[3], s4);
+      rk_stage = -1;
     } else {
       throw StatusError(HOC_EINVAL, "time_integration_type must be 1, 2 or 4");
     }
@@ -1299,7 +1360,7 @@ struct Solver {
         const int n0 = sg * every, n1 = std::min(n0 + every, nsteps);
         dcopy(w, adj.ckbuf.p + (size_t)sg * D, D);
         time = adj.t0 + n0 * cfg.dt;
-        n_hist = 0;
+        invalidate_guess();
         for (int n = n0; n < n1; ++n) {
           dcopy(adj.segbuf.p + (size_t)(n - n0) * D, w, D);
           if (n + 1 < n1) rk_step();
@@ -1466,7 +1527,7 @@ int hoc_create(hoc_t* out, const hoc_config* cfg) {
   if (!(cfg->mixed_inner_tol >= 0. && cfg->mixed_inner_tol < 1.)) return hoc_fail(nullptr, HOC_EINVAL, "mixed_inner_tol must be in [0, 1)");
   if (cfg->cheb_degree < 0 || cfg->cheb_degree > hoc::kMaxCheb) return hoc_fail(nullptr, HOC_EINVAL, "cheb_degree must be in 0..8");
   if (!(cfg->cheb_lower >= 0. && cfg->cheb_lower < 1.)) return hoc_fail(nullptr, HOC_EINVAL, "cheb_lower must be in [0, 1)");
-  if (cfg->psi_guess < 0 || cfg->psi_guess > 2) return hoc_fail(nullptr, HOC_EINVAL, "psi_guess must be 0, 1 or 2");
+  if (cfg->psi_guess < 0 || cfg->psi_guess > 3) return hoc_fail(nullptr, HOC_EINVAL, "psi_guess must be 0..3");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return hoc_fail(nullptr, HOC_EDEVICE, "no HIP device");
   if (cfg->device < 0 || cfg->device >= ndev) return hoc_fail(nullptr, HOC_EDEVICE, "device ordinal out of range");
@@ -1618,7 +1679,7 @@ int hoc_set_vorticity(hoc_t h, int64_t len, const double* w) {
   return guarded(h, [&] {
     NEED_READY(h); NEED_LEN(len, h->s.D);
     h->s.h2d(h->s.w.p, w, h->s.D * sizeof(double));
-    h->s.n_hist = 0;
+    h->s.invalidate_guess();
   });
 }
 int hoc_get_vorticity(hoc_t h, int64_t len, double* w) {
@@ -1637,7 +1698,7 @@ int hoc_set_streamfunction(hoc_t h, int64_t len, const double* p) {
   return guarded(h, [&] {
     NEED_READY(h); NEED_LEN(len, h->s.D);
     h->s.h2d(h->s.psi.p, p, h->s.D * sizeof(double));
-    h->s.n_hist = 0;
+    h->s.invalidate_guess();
   });
 }
 int hoc_get_velocity(hoc_t h, int64_t len, double* uv) {

@@ -60,8 +60,11 @@ typedef struct {
   int32_t psi_guess;              /* initial guess of the stage Poisson solves: 1 = the
                                      previous solution (the reference's warm start),
                                      2 = linear extrapolation in the stage time from the
-                                     solutions at the last two distinct stage times
-                                     (0 = 2)                                              */
+                                     solutions at the last two distinct stage times,
+                                     3 = with RK4, the first two stages of a step from the
+                                     previous step's stage solutions (psi is affine in
+                                     omega), otherwise as 2  (0 = 3 with RK4, 1 with
+                                     Euler and RK2)                                       */
   double cheb_lower;              /* smoothing interval [cheb_lower, 1.1] x lambda_max(D^-1 As)
                                      (0 = 1/60)                                          */
 } hoc_config;

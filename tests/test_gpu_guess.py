@@ -1,0 +1,64 @@
+"""Initial guesses of the stage Poisson solves (hoc_config.psi_guess): the reference's warm start
+from the previous solution (1), linear extrapolation in the stage time (2) and the RK4 stage
+combination (3, the default).  Only the starting point of each solve differs, so the fields
+agree to the solver tolerance, match the oracle, and the better guesses need fewer iterations."""
+import numpy as np
+import pytest
+
+import hoc
+from hoc import meshes
+from pyoracle import Oracle
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def run(m, K, guess, steps, tol, integ=4, **kw):
+    g = hoc.HO2D(K=K, problem_type=1, Re=400.0, dt=2e-4, time_integration_type=integ, advection_type=1, tol=tol,
+                 maxiter=500, psi_guess=guess, **kw)
+    g.load_mesh(m)
+    g.setup()
+    g.step(steps)
+    st = g.stats()
+    return g.get_vorticity(), g.get_streamfunction(), st["total_iters"] / st["n_solves"]
+
+
+def test_guess_modes_agree_and_save_iterations(gpu_available):
+    m = meshes.cavity(16, 2, seed=1, jitter=0.1)
+    out = {gm: run(m, 4, gm, 12, 1e-13) for gm in (1, 2, 3)}
+    o = Oracle(m, 4, problem_type=1, Re=400.0, dt=2e-4, time_integration_type=4, advection_type=1, tol=1e-13,
+               maxiter=500)
+    o.step(12)
+    for gm, (w, p, it) in out.items():
+        ew, ep = rel_l2(w, o.vorticity()), rel_l2(p, o.streamfunction())
+        print("psi_guess", gm, "iterations per solve", it, "omega vs oracle", ew, "psi", ep)
+        assert ew < 1e-11 and ep < 1e-11
+    assert out[3][2] < out[1][2] and out[2][2] < out[1][2]
+
+
+@pytest.mark.parametrize("integ", [1, 2])
+def test_guess_with_other_integrators(gpu_available, integ):
+    """Euler and RK2 keep the reference's warm start under the default; the extrapolation
+    (mode 2, and mode 3 outside RK4) gives the same fields."""
+    m = meshes.cavity(8, 2)
+    w1, p1, it1 = run(m, 3, 1, 6, 1e-13, integ=integ)
+    w0, p0, it0 = run(m, 3, 0, 6, 1e-13, integ=integ)
+    w2, p2, it2 = run(m, 3, 2, 6, 1e-13, integ=integ)
+    w3, p3, it3 = run(m, 3, 3, 6, 1e-13, integ=integ)
+    print("integrator", integ, "iterations (modes 1, default, 2, 3)", it1, it0, it2, it3)
+    assert np.array_equal(w0, w1) and it0 == it1
+    for w, p in ((w2, p2), (w3, p3)):
+        assert rel_l2(w, w1) < 1e-11 and rel_l2(p, p1) < 1e-11
+    assert it3 == it2
+
+
+def test_guess_mixed_precision(gpu_available):
+    m = meshes.cavity(12, 2)
+    w64, p64, _ = run(m, 5, 3, 6, 1e-11)
+    w32, p32, _ = run(m, 5, 3, 6, 1e-11, precision=hoc.HOC_MIXED)
+    assert rel_l2(w32, w64) < 1e-8 and rel_l2(p32, p64) < 1e-8
+
+
+def test_guess_rejects_bad_mode():
+    with pytest.raises(hoc.HocError):
+        hoc.HO2D(K=3, psi_guess=4)
