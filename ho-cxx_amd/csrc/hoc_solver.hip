@@ -770,7 +770,7 @@ struct Solver {
     } else {
       // The pass decisions are taken on the device (refine_control); the host waits only for
       // the control step's published verdict, while the demotion behind it keeps the GPU busy.
-      const double eta_floor = cfg.mixed_inner_tol > 0. ? cfg.mixed_inner_tol : 1e-5;
+      const double eta_floor = cfg.mixed_inner_tol > 0. ? cfg.mixed_inner_tol : 1e-4;
       constexpr int kMaxOuter = 12;
       HIPCHK(hipMemsetAsync(F.p + F_OUTER, 0, 2 * sizeof(int), stream));  // F_OUTER, F_TOTAL
       for (;;) {

@@ -54,7 +54,7 @@ typedef struct {
   int32_t device;                 /* HIP device ordinal                                  */
   int32_t fail_on_noconv;         /* nonzero: return HOC_ENOCONV when tol is not met     */
   double mixed_inner_tol;         /* HOC_MIXED: floor of the FP32 inner solve's relative
-                                     tolerance per refinement pass (0 = 1e-5)            */
+                                     tolerance per refinement pass (0 = 1e-4)            */
   int32_t cheb_degree;            /* fine-level Chebyshev smoother degree, pre and post
                                      (0 = 3; at most 8)                                   */
   int32_t psi_guess;              /* initial guess of the stage Poisson solves: 1 = the
