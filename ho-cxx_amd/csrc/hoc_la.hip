@@ -430,8 +430,8 @@ __global__ void k_psi_predict(size_t n, double f, int extrap, double* __restrict
   }
 }
 struct Comb5 {
-  double a[5];
-  const double* x[5];
+  double a[8];
+  const double* x[8];
 };
 __global__ void k_psi_combine(size_t n, int nterm, Comb5 C, double* __restrict__ psi, double* __restrict__ save) {
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
@@ -444,7 +444,7 @@ __global__ void k_psi_combine(size_t n, int nterm, Comb5 C, double* __restrict__
 }
 void psi_combine(hipStream_t s, size_t n, int nterm, const double* a, const double* const* x, double* psi,
                  double* save) {
-  if (n == 0 || nterm < 1 || nterm > 5) return;
+  if (n == 0 || nterm < 1 || nterm > 8) return;
   Comb5 C{};
   for (int j = 0; j < nterm; ++j) { C.a[j] = a[j]; C.x[j] = x[j]; }
   hipLaunchKernelGGL(k_psi_combine, dim3(vec_grid(n)), dim3(256), 0, s, n, nterm, C, psi, save);

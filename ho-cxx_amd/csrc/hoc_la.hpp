@@ -72,7 +72,7 @@ void mixed_promote_add(hipStream_t s, size_t n, double* x64, const float* e32, c
 // and prev = psi (extrapolation and history shift in one pass).
 void psi_predict(hipStream_t s, size_t n, double f, int extrap, double* psi, double* prev);
 // psi' = sum_j a[j] x_j (x_j may be psi itself, read before the write), and save = psi (the old
-// value) when save is not null; nterm <= 5.
+// value) when save is not null; nterm <= 8.
 void psi_combine(hipStream_t s, size_t n, int nterm, const double* a, const double* const* x, double* psi,
                  double* save);
 

@@ -270,7 +270,8 @@ struct Solver {
     // fields
     for (auto* f : {&w, &psi, &k1, &k2, &k3, &k4, &vort, &kscratch, &bs}) f->alloc(D);
     psi_prev.alloc(guess_mode() >= 2 ? D : 0);
-    for (auto& b : rkS) b.alloc(guess_mode() == 3 ? D : 0);
+    for (auto& b : rkS) b.alloc(guess_mode() >= 3 ? D : 0);
+    for (auto& b : rkQ) b.alloc(guess_mode() == 4 ? D : 0);
     n_hist = 0;
     rk_full = false;
     rk_cur = 0;
@@ -662,22 +663,77 @@ struct Solver {
   int n_hist = 0;               // solutions in the history (0: psi is not a stage solution)
   int64_t step_count = 0;
   double stage_tau = std::nan("");  // stage time of the solve in progress (NaN: not a stage solve)
-  // default: mode 3 with RK4; with Euler and RK2 the reference's warm start (extrapolating over
+  // default: mode 4 with RK4; with Euler and RK2 the reference's warm start (extrapolating over
   // their steps measured no better: their stiff modes alternate in sign from step to step)
   int guess_mode() const {
     if (cfg.psi_guess != 0) return cfg.psi_guess;
-    return cfg.time_integration_type == 4 ? 3 : 1;
+    return cfg.time_integration_type == 4 ? 4 : 1;
   }
   // Mode 3 (RK4): psi(omega_n + c k) = psi(omega_n) + c Psi(k) with Psi linear, so the stage
   // solutions S1..S4 of a step give Psi(k1) = 2 (S2 - S1), Psi(k2) = 2 (S3 - S1), Psi(k3) = S4 - S1,
   // and the next step's first solve is S1 + (Psi(k1) + 2 Psi(k2) + 2 Psi(k3) + Psi(k4)) / 6 with
   // Psi(k4) ~ Psi(k3): (-3 S1 + 2 S2 + 4 S3 + 3 S4) / 6.  Its solution T1 then gives Psi(k4) exactly,
   // and the second stage, T1 + Psi(k1') / 2 with Psi(k1') ~ Psi(k4): 4 T1 + S1 - S2 - 2 S3 - S4.
-  // Stages 3 and 4 as in mode 2 (S2; 2 S3 - S1).  Other integrators use mode 2.
+  // Stages 3 and 4 as in mode 2 (S2; 2 S3 - S1).  Other integrators use mode 2 under modes 3 and 4.
   DBuf<double> rkS[4];
+  DBuf<double> rkQ[4];            // mode 4: Psi(k1..k4) of the step before S1..S4's
   int rk_stage = -1, rk_cur = 0;  // stage index of the RK4 solve in progress; stages of this step solved
   bool rk_full = false;           // S1..S4 hold a complete step
+  bool rk_qok = false;            // rkQ holds a complete step (mode 4)
+  // Mode 4: as mode 3, with each predicted Psi(k) corrected by the same stage difference one step
+  // earlier (Q = Psi(k) of that step): stage 1 Psi(k4) ~ Psi(k3) + Q4 - Q3; stage 2
+  // Psi(k1') ~ Psi(k4) + Q1 - Q4; stage 3 Psi(k2') ~ Psi(k1') + Q2 - Q1; stage 4 Psi(k3') ~ Psi(k2') + Q3 - Q2.
+  void rk4_guess_before() {
+    const int st = rk_stage;
+    double* S[4] = {rkS[0], rkS[1], rkS[2], rkS[3]};
+    double* Q[4] = {rkQ[0], rkQ[1], rkQ[2], rkQ[3]};
+    const double q = rk_qok ? 1. : 0.;
+    if (st == 0 && rk_full) {
+      const double a[6] = {-0.5, 1. / 3., 2. / 3., 0.5, q / 6., -q / 6.};
+      const double* x[6] = {S[0], S[1], S[2], S[3], Q[3], Q[2]};
+      psi_combine(stream, D, rk_qok ? 6 : 4, a, x, psi, nullptr);
+    } else if (st == 1 && rk_cur == 1) {
+      if (rk_full) {
+        // T1 = psi; guess 4 T1 - 2 S3 - S4 - Q4/2 (Q4 of the older step), then the Q history of the
+        // completed step: Q1 = 2 (S2 - S1), Q2 = 2 (S3 - S1), Q3 = S4 - S1, Q4 = 6 T1 + 2 S1 - 2 S2 - 4 S3 - 2 S4
+        dcopy(adj_tmp(), psi, D);  // T1
+        if (rk_qok) {
+          const double a[4] = {4., -2., -1., -0.5};
+          const double* x[4] = {psi, S[2], S[3], Q[3]};
+          psi_combine(stream, D, 4, a, x, psi, nullptr);
+        } else {  // mode 3's guess
+          const double a[5] = {4., 1., -1., -2., -1.};
+          const double* x[5] = {psi, S[0], S[1], S[2], S[3]};
+          psi_combine(stream, D, 5, a, x, psi, nullptr);
+        }
+        const double* T1 = adj_tmp();
+        { const double b[2] = {2., -2.}; const double* y[2] = {S[1], S[0]}; psi_combine(stream, D, 2, b, y, Q[0], nullptr); }
+        { const double b[2] = {2., -2.}; const double* y[2] = {S[2], S[0]}; psi_combine(stream, D, 2, b, y, Q[1], nullptr); }
+        { const double b[5] = {6., 2., -2., -4., -2.}; const double* y[5] = {T1, S[0], S[1], S[2], S[3]};
+          psi_combine(stream, D, 5, b, y, Q[3], nullptr); }
+        { const double b[2] = {1., -1.}; const double* y[2] = {S[3], S[0]}; psi_combine(stream, D, 2, b, y, Q[2], nullptr); }
+        dcopy(S[0], T1, D);
+        rk_qok = true;
+      } else {
+        dcopy(rkS[0], psi, D);
+      }
+    } else if (st == 2 && rk_cur == 2 && rk_qok) {
+      const double a[3] = {1., 0.5, -0.5};
+      const double* x[3] = {psi, Q[1], Q[0]};
+      psi_combine(stream, D, 3, a, x, psi, nullptr);
+    } else if (st == 3 && rk_cur == 3) {
+      const double a[4] = {2., -1., q, -q};
+      const double* x[4] = {psi, rkS[0], Q[2], Q[1]};
+      psi_combine(stream, D, rk_qok ? 4 : 2, a, x, psi, nullptr);
+    }
+  }
+  DBuf<double> guess_tmp;
+  double* adj_tmp() {
+    if (guess_tmp.n != D) guess_tmp.alloc(D);
+    return guess_tmp.p;
+  }
   void rk_guess_before() {
+    if (guess_mode() == 4) { rk4_guess_before(); return; }
     const int st = rk_stage;
     if (st == 0 && rk_full) {
       const double a[4] = {-0.5, 1. / 3., 2. / 3., 0.5};
@@ -699,7 +755,7 @@ struct Solver {
   }
   void rk_guess_after() {
     const int st = rk_stage;
-    if (st != rk_cur % 4 || (st > 0 && rk_cur != st)) { rk_full = false; rk_cur = 0; return; }
+    if (st != rk_cur % 4 || (st > 0 && rk_cur != st)) { rk_full = false; rk_qok = false; rk_cur = 0; return; }
     if (st >= 1) dcopy(rkS[st], psi, D);
     rk_cur = st + 1;
     if (st == 3) { rk_full = true; rk_cur = 0; }
@@ -707,10 +763,11 @@ struct Solver {
   void invalidate_guess() {
     n_hist = 0;
     rk_full = false;
+    rk_qok = false;
     rk_cur = 0;
   }
   void guess_before_solve() {
-    if (guess_mode() == 3 && rk_stage >= 0 && !std::isnan(stage_tau)) { rk_guess_before(); return; }
+    if (guess_mode() >= 3 && rk_stage >= 0 && !std::isnan(stage_tau)) { rk_guess_before(); return; }
     if (guess_mode() < 2 || std::isnan(stage_tau)) return;
     const double tau = stage_tau;
     if (n_hist == 0 || tau == tau_a) return;  // no history, or the same stage time: the previous solution
@@ -718,8 +775,9 @@ struct Solver {
     psi_predict(stream, D, ok ? (tau - tau_a) / (tau_a - tau_b) : 0., ok ? 1 : 0, psi, psi_prev);
   }
   void guess_after_solve() {
-    if (guess_mode() == 3 && rk_stage >= 0 && !std::isnan(stage_tau)) { rk_guess_after(); return; }
+    if (guess_mode() >= 3 && rk_stage >= 0 && !std::isnan(stage_tau)) { rk_guess_after(); return; }
     rk_full = false;
+    rk_qok = false;
     rk_cur = 0;
     if (guess_mode() < 2 || std::isnan(stage_tau)) { n_hist = 0; return; }
     if (n_hist >= 1 && stage_tau == tau_a) return;
@@ -944,6 +1002,7 @@ struct Solver {
       rk_stage = -1;
       n_hist = 0;
       rk_full = false;
+      rk_qok = false;
       rk_cur = 0;
       throw;
     }
@@ -1527,7 +1586,7 @@ int hoc_create(hoc_t* out, const hoc_config* cfg) {
   if (!(cfg->mixed_inner_tol >= 0. && cfg->mixed_inner_tol < 1.)) return hoc_fail(nullptr, HOC_EINVAL, "mixed_inner_tol must be in [0, 1)");
   if (cfg->cheb_degree < 0 || cfg->cheb_degree > hoc::kMaxCheb) return hoc_fail(nullptr, HOC_EINVAL, "cheb_degree must be in 0..8");
   if (!(cfg->cheb_lower >= 0. && cfg->cheb_lower < 1.)) return hoc_fail(nullptr, HOC_EINVAL, "cheb_lower must be in [0, 1)");
-  if (cfg->psi_guess < 0 || cfg->psi_guess > 3) return hoc_fail(nullptr, HOC_EINVAL, "psi_guess must be 0..3");
+  if (cfg->psi_guess < 0 || cfg->psi_guess > 4) return hoc_fail(nullptr, HOC_EINVAL, "psi_guess must be 0..4");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return hoc_fail(nullptr, HOC_EDEVICE, "no HIP device");
   if (cfg->device < 0 || cfg->device >= ndev) return hoc_fail(nullptr, HOC_EDEVICE, "device ordinal out of range");

@@ -63,7 +63,9 @@ typedef struct {
                                      solutions at the last two distinct stage times,
                                      3 = with RK4, the first two stages of a step from the
                                      previous step's stage solutions (psi is affine in
-                                     omega), otherwise as 2  (0 = 3 with RK4, 1 with
+                                     omega), otherwise as 2;  4 = as 3, each predicted
+                                     stage increment corrected by the same stage
+                                     difference one step earlier  (0 = 4 with RK4, 1 with
                                      Euler and RK2)                                       */
   double cheb_lower;              /* smoothing interval [cheb_lower, 1.1] x lambda_max(D^-1 As)
                                      (0 = 1/60)                                          */

@@ -1,6 +1,7 @@
 """Initial guesses of the stage Poisson solves (hoc_config.psi_guess): the reference's warm start
-from the previous solution (1), linear extrapolation in the stage time (2) and the RK4 stage
-combination (3, the default).  Only the starting point of each solve differs, so the fields
+from the previous solution (1), linear extrapolation in the stage time (2), the RK4 stage
+combination (3) and that combination with the previous step's stage differences (4, the default
+with RK4).  Only the starting point of each solve differs, so the fields
 agree to the solver tolerance, match the oracle, and the better guesses need fewer iterations."""
 import numpy as np
 import pytest
@@ -25,7 +26,7 @@ def run(m, K, guess, steps, tol, integ=4, **kw):
 
 def test_guess_modes_agree_and_save_iterations(gpu_available):
     m = meshes.cavity(16, 2, seed=1, jitter=0.1)
-    out = {gm: run(m, 4, gm, 12, 1e-13) for gm in (1, 2, 3)}
+    out = {gm: run(m, 4, gm, 12, 1e-13) for gm in (1, 2, 3, 4)}
     o = Oracle(m, 4, problem_type=1, Re=400.0, dt=2e-4, time_integration_type=4, advection_type=1, tol=1e-13,
                maxiter=500)
     o.step(12)
@@ -33,7 +34,7 @@ def test_guess_modes_agree_and_save_iterations(gpu_available):
         ew, ep = rel_l2(w, o.vorticity()), rel_l2(p, o.streamfunction())
         print("psi_guess", gm, "iterations per solve", it, "omega vs oracle", ew, "psi", ep)
         assert ew < 1e-11 and ep < 1e-11
-    assert out[3][2] < out[1][2] and out[2][2] < out[1][2]
+    assert out[4][2] < out[1][2] and out[3][2] < out[1][2] and out[2][2] < out[1][2]
 
 
 @pytest.mark.parametrize("integ", [1, 2])
@@ -45,11 +46,12 @@ def test_guess_with_other_integrators(gpu_available, integ):
     w0, p0, it0 = run(m, 3, 0, 6, 1e-13, integ=integ)
     w2, p2, it2 = run(m, 3, 2, 6, 1e-13, integ=integ)
     w3, p3, it3 = run(m, 3, 3, 6, 1e-13, integ=integ)
-    print("integrator", integ, "iterations (modes 1, default, 2, 3)", it1, it0, it2, it3)
+    w4, p4, it4 = run(m, 3, 4, 6, 1e-13, integ=integ)
+    print("integrator", integ, "iterations (modes 1, default, 2, 3, 4)", it1, it0, it2, it3, it4)
     assert np.array_equal(w0, w1) and it0 == it1
-    for w, p in ((w2, p2), (w3, p3)):
+    for w, p in ((w2, p2), (w3, p3), (w4, p4)):
         assert rel_l2(w, w1) < 1e-11 and rel_l2(p, p1) < 1e-11
-    assert it3 == it2
+    assert it3 == it2 and it4 == it2
 
 
 def test_guess_mixed_precision(gpu_available):
@@ -61,4 +63,4 @@ def test_guess_mixed_precision(gpu_available):
 
 def test_guess_rejects_bad_mode():
     with pytest.raises(hoc.HocError):
-        hoc.HO2D(K=3, psi_guess=4)
+        hoc.HO2D(K=3, psi_guess=5)
