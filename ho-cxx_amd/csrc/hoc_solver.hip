@@ -986,8 +986,10 @@ struct Solver {
   }
 
   // Euler_time_integrate (src/HO_2D.cpp:1806-1839) on input win; boundary values already set.
+  double* capture_psi = nullptr;  // when set, RK4 stage s copies its solution to capture_psi + s D
   void stage_body(const double* win, const StageOut& so) {
     poisson(win);
+    if (capture_psi && rk_stage >= 0) dcopy(capture_psi + (size_t)rk_stage * D, psi, D);
     HIPCHK(launch_trace(K, PRE_COPY, N, T, psi, nullptr, nullptr, nullptr, 0., nullptr, 0, nullptr, P64.tr.p, stream));
     HIPCHK(launch_velocity(K, N, T, G, B, psi, P64.tr, win, vel, U, trw, tru, 1. / (cfg.dt * re_inv), I, stream));
     HIPCHK(launch_stage(K, cfg.advection_type, N, T, G, B, win, U, trw, tru, re_inv, cfg.dt, I, so, stream));
@@ -1141,7 +1143,12 @@ struct Solver {
     DBuf<double> zero_d, zero_tr, zero_e, lin_nvel, lin_vort, lin_adv;
     DBuf<double> lam, xbar, psibar, z, fake, wbar, kbar[4], pvbar;
     DBuf<double> Mv, Md, Ubar, diffbar, zero_u, zero_tu;  // assembled velocity map and its adjoints
-    DBuf<double> ckbuf, segbuf, save_w, save_psi, save_vel;
+    // adjoint solves: the previous solution z (in z) and the one before (zprev) with their stage
+    // times, for a guess extrapolated in the reversed stage time
+    DBuf<double> zprev;
+    double ztau_a = 0., ztau_b = 0.;
+    int zn = 0;
+    DBuf<double> ckbuf, segbuf, segpsi, save_w, save_psi, save_vel;
     int nsteps = 0, every = 0;
     double t0 = 0.;
     bool have_forward = false;
@@ -1260,6 +1267,7 @@ struct Solver {
     adj.Mv.alloc((size_t)N * 5 * K2 * 2 * K2);
     adj.Md.alloc(EK * K2);
     adj.Ubar.alloc(2 * D); adj.zero_u.alloc(2 * D); adj.zero_tu.alloc(TU); adj.diffbar.alloc(EK);
+    adj.zprev.alloc(D);
     // the velocity map psi -> (U, wall flux): K^2 probes per colour of the distance-2 colouring
     // (U of an element depends on psi of its face ring), velocity with homogeneous boundary values
     VelBatch vb;
@@ -1280,14 +1288,17 @@ struct Solver {
   }
 
   // One RK4 step from w = omega_n recording every stage's input, velocity, traces and upwind
-  // switches (w itself is not advanced).
-  void adj_record_step() {
+  // switches (w itself is not advanced).  stage_psi: the stage solutions of the same step kept by
+  // the segment's recompute (their inputs are recomputed here by the same kernels), or null to
+  // solve.
+  void adj_record_step(const double* stage_psi) {
     const double c[4] = {0., 0.5, 0.5, 1.};
     const double cn[3] = {0.5, 0.5, 1.};
     dcopy(adj.ws[0], w, D);
     for (int st = 0; st < 4; ++st) {
       update_bcs(c[st]);
-      poisson(adj.ws[st]);
+      if (stage_psi) dcopy(psi, stage_psi + (size_t)st * D, D);
+      else poisson(adj.ws[st]);
       HIPCHK(launch_trace(K, PRE_COPY, N, T, psi, nullptr, nullptr, nullptr, 0., nullptr, 0, nullptr, P64.tr.p, stream));
       HIPCHK(launch_velocity(K, N, T, G, B, psi, P64.tr, adj.ws[st], vel, adj.Us[st], adj.trws[st], adj.trus[st],
                              1. / (cfg.dt * re_inv), I, stream));
@@ -1302,7 +1313,7 @@ struct Solver {
 
   // xbar = (dk_s/domega_s)^T lam for recorded stage st, adding (dk_s/dpvel)^T lam to adj.pvbar.
   // The K^2 probes of a colour (and the 4K wall-velocity probes) run as one batched launch each.
-  void adj_stage(int st, const double* lam, double* xbar) {
+  void adj_stage(int st, const double* lam, double* xbar, double tau = std::nan("")) {
     const double idr = 1. / (cfg.dt * re_inv);
     const size_t TR = (size_t)4 * N * 2 * K, TU = (size_t)4 * N * K, EK = std::max<size_t>((size_t)E * K, 1);
     StageOut so;
@@ -1355,6 +1366,13 @@ struct Solver {
     // (4) the Poisson solve: As z = psi_bar (a right-hand side -W b = psi_bar comes from the
     //     vorticity psi_bar / (W J) with zero boundary values), then omega_bar += W J z
     HIPCHK(launch_adj_wj(N, K, tab.wgt, d_jac, adj.psibar, adj.fake, 0, stream));
+    if (std::isnan(tau)) {
+      adj.zn = 0;
+    } else if (adj.zn >= 1 && tau != adj.ztau_a) {
+      const double d1 = tau - adj.ztau_a, d2 = adj.ztau_a - adj.ztau_b;
+      const bool ok = adj.zn == 2 && d1 * d2 > 0. && std::fabs(d1) <= 1. && std::fabs(d2) <= 1.;
+      psi_predict(stream, D, ok ? d1 / d2 : 0., ok ? 1 : 0, adj.z, adj.zprev);
+    }
     rhs_bc = adj.zero_e;
     std::swap(psi.p, adj.z.p);
     try {
@@ -1366,6 +1384,11 @@ struct Solver {
     }
     std::swap(psi.p, adj.z.p);
     rhs_bc = nullptr;
+    if (!std::isnan(tau) && !(adj.zn >= 1 && tau == adj.ztau_a)) {
+      adj.ztau_b = adj.ztau_a;
+      adj.ztau_a = tau;
+      adj.zn = std::min(adj.zn + 1, 2);
+    }
     HIPCHK(launch_adj_wj(N, K, tab.wgt, d_jac, adj.z, xbar, 1, stream));
   }
 
@@ -1377,6 +1400,7 @@ struct Solver {
     const int nck = std::max(1, (nsteps + every - 1) / every);
     adj.ckbuf.alloc((size_t)nck * D);
     adj.segbuf.alloc((size_t)std::min(every, std::max(nsteps, 1)) * D);
+    adj.segpsi.alloc((size_t)std::min(every, std::max(nsteps, 1)) * 4 * D);
     adj.t0 = time;
     for (int n = 0; n < nsteps; ++n) {
       if (n % every == 0) dcopy(adj.ckbuf.p + (size_t)(n / every) * D, w, D);
@@ -1406,7 +1430,9 @@ struct Solver {
     const size_t EK = std::max<size_t>((size_t)E * K, 1);
     HIPCHK(hipMemsetAsync(adj.pvbar.p, 0, EK * sizeof(double), stream));
     const double cn[3] = {0.5, 0.5, 1.};
+    const double cst[4] = {0., 0.5, 0.5, 1.};
     const int every = adj.every, nsteps = adj.nsteps;
+    adj.zn = 0;
     const int nseg = (nsteps + every - 1) / every;
     auto restore = [&] {
       dcopy(w, adj.save_w, D);
@@ -1422,18 +1448,20 @@ struct Solver {
         invalidate_guess();
         for (int n = n0; n < n1; ++n) {
           dcopy(adj.segbuf.p + (size_t)(n - n0) * D, w, D);
-          if (n + 1 < n1) rk_step();
+          capture_psi = adj.segpsi.p + (size_t)(n - n0) * 4 * D;
+          rk_step();
+          capture_psi = nullptr;
         }
         for (int n = n1 - 1; n >= n0; --n) {
           dcopy(w, adj.segbuf.p + (size_t)(n - n0) * D, D);
           time = adj.t0 + n * cfg.dt;
-          adj_record_step();
+          adj_record_step(adj.segpsi.p + (size_t)(n - n0) * 4 * D);
           HIPCHK(launch_adj_axpby(D, 1. / 6., adj.wbar, 0., adj.kbar[3], stream));
           HIPCHK(launch_adj_axpby(D, 1. / 3., adj.wbar, 0., adj.kbar[2], stream));
           HIPCHK(launch_adj_axpby(D, 1. / 3., adj.wbar, 0., adj.kbar[1], stream));
           HIPCHK(launch_adj_axpby(D, 1. / 6., adj.wbar, 0., adj.kbar[0], stream));
           for (int st = 3; st >= 0; --st) {
-            adj_stage(st, adj.kbar[st], adj.xbar);
+            adj_stage(st, adj.kbar[st], adj.xbar, n + cst[st]);
             HIPCHK(launch_adj_axpby(D, 1., adj.xbar, 1., adj.wbar, stream));
             if (st > 0) HIPCHK(launch_adj_axpby(D, cn[st - 1], adj.xbar, 1., adj.kbar[st - 1], stream));
           }
@@ -1442,6 +1470,7 @@ struct Solver {
       d2h(w0bar_h, adj.wbar.p, D * sizeof(double));
       if (pvbar_h && E > 0) d2h(pvbar_h, adj.pvbar.p, (size_t)E * K * sizeof(double));
     } catch (...) {
+      capture_psi = nullptr;
       restore();
       throw;
     }
