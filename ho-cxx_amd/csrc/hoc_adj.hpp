@@ -19,9 +19,9 @@ hipError_t launch_adj_seed_batch(int N, int K, const int32_t* color, int c, cons
                                  hipStream_t s);
 // p[b][e] = sum_r lam[e][r] y[b][e][r], b < nb
 hipError_t launch_adj_dot_batch(int N, int K2, int nb, const double* lam, const double* y, double* p, hipStream_t s);
-// out[i][ooff + m] (row stride ostride) = sum over e in ring(i) of p[m][e], for the elements i of
-// colour c (ring as CSR)
-hipError_t launch_adj_gather_batch(int N, int K2, const int32_t* color, int c, const int32_t* rptr,
+// out[i][ooff + m] (row stride ostride) = sum over e in ring(i) of p[m][e], for the nmem elements
+// i of a colour (members; ring as CSR)
+hipError_t launch_adj_gather_batch(int N, int K2, int nmem, const int32_t* members, const int32_t* rptr,
                                    const int32_t* ridx, const double* p, double* out, int ostride, int ooff,
                                    hipStream_t s);
 // K^2 probes of colour c on velocity component comp of U ([N][2][K2]): U[b] = unit seed at local dof b

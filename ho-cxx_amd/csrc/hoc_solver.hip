@@ -1135,6 +1135,8 @@ struct Solver {
     bool init = false;
     int nc1 = 0;
     DBuf<int32_t> col1, r1p, r1i, own_el;
+    DBuf<int32_t> members;            // elements sorted by colour, colour c at [moff[c], moff[c + 1])
+    std::vector<int> moff;
     DBuf<int8_t> own_f;
     DBuf<double> ws[4], Us[4], trws[4], trus[4];  // recorded stages of one step
     DBuf<int8_t> sig[4];
@@ -1233,6 +1235,16 @@ struct Solver {
     rings(1, p1, i1);
     adj.col1.upload(c1);
     adj.r1p.upload(p1); adj.r1i.upload(i1);
+    {
+      std::vector<int32_t> mem;
+      adj.moff.assign(1, 0);
+      for (int c = 0; c < adj.nc1; ++c) {
+        for (int e = 0; e < N; ++e)
+          if (c1[e] == c) mem.push_back(e);
+        adj.moff.push_back((int)mem.size());
+      }
+      adj.members.upload(mem);
+    }
     std::vector<int32_t> oe(std::max(E, 1), 0);
     std::vector<int8_t> of(std::max(E, 1), -1);
     for (int e = 0; e < E; ++e) { oe[e] = mesh.be_elem[e]; of[e] = mesh.be_ijp[e]; }
@@ -1330,7 +1342,8 @@ struct Solver {
       HIPCHK(launch_stage(K, 1, N, T, G, adj.Bz, adj.seed, adj.Us[st], adj.seed_tr, adj.trus[st], re_inv, cfg.dt, I,
                           so, stream, 2));
       HIPCHK(launch_adj_dot_batch(N, K2, K2, lam, adj.y, adj.p, stream));
-      HIPCHK(launch_adj_gather_batch(N, K2, adj.col1, c, adj.r1p, adj.r1i, adj.p, xbar, K2, 0, stream));
+      HIPCHK(launch_adj_gather_batch(N, K2, adj.moff[c + 1] - adj.moff[c], adj.members.p + adj.moff[c], adj.r1p,
+                                     adj.r1i, adj.p, xbar, K2, 0, stream));
     }
     // (2) velocity: K^2 probes per colour and component (the stage depends on U over the face ring),
     //     the stage advecting the recorded vorticity with the seeded velocity
@@ -1344,7 +1357,8 @@ struct Solver {
         HIPCHK(launch_stage(K, 1, N, T, G, adj.Bz, adj.zero_d, adj.dU, adj.zero_tr, adj.dtru, re_inv, cfg.dt, I, so,
                             stream, 2));
         HIPCHK(launch_adj_dot_batch(N, K2, K2, lam, adj.y, adj.p, stream));
-        HIPCHK(launch_adj_gather_batch(N, K2, adj.col1, c, adj.r1p, adj.r1i, adj.p, adj.Ubar, 2 * K2, comp * K2,
+        HIPCHK(launch_adj_gather_batch(N, K2, adj.moff[c + 1] - adj.moff[c], adj.members.p + adj.moff[c], adj.r1p,
+                                       adj.r1i, adj.p, adj.Ubar, 2 * K2, comp * K2,
                                        stream));
       }
     // (3) wall flux: probe f K + j seeds edge flux point j of the edges owned through face f, which
