@@ -705,10 +705,18 @@ __global__ __launch_bounds__(kWG) void k_stage(int N, Tab T, Geo G, BCDev B, con
   double kk[K2];
   if (live) {
     const size_t base = (size_t)el * K2;
-    // ---- diffusion
+    // ---- diffusion (mode 2 with so.adv_only: zero diffusion inputs, skipped)
+    const bool do_diff = MODE != 2 || !so.adv_only;
     double bv[4][K], bg[4][K], cv[4][K], cg[4][K];
-    traces<K>(T, w, bv, bg);
-    faces2<K, AFF>(T, G, el, trw, G.sw_diff, B.diff, false, bv, bg, cv, cg);
+    if (do_diff) {
+      traces<K>(T, w, bv, bg);
+      faces2<K, AFF>(T, G, el, trw, G.sw_diff, B.diff, false, bv, bg, cv, cg);
+    } else {
+#pragma unroll
+      for (int f = 0; f < 4; ++f)
+#pragma unroll
+        for (int k = 0; k < K; ++k) bv[f][k] = bg[f][k] = cv[f][k] = cg[f][k] = 0.;
+    }
     double bva[MODE == 2 ? 4 : 1][K];
     if constexpr (MODE == 2) {
       double bga[4][K];
@@ -719,13 +727,16 @@ __global__ __launch_bounds__(kWG) void k_stage(int N, Tab T, Geo G, BCDev B, con
     };
     double dif[K2];
     double ej = 0.;
-    if (AFF) {
+    if (AFF) ej = G.gjm[(size_t)el * 5 + 4];
+    if (!do_diff) {
+#pragma unroll
+      for (int m = 0; m < K2; ++m) dif[m] = 0.;
+    } else if (AFF) {
       double ga[K2], gb[K2], gc[K2];
       const double a = G.gel[(size_t)el * 3], b = G.gel[(size_t)el * 3 + 1], c = G.gel[(size_t)el * 3 + 2];
 #pragma unroll
       for (int m = 0; m < K2; ++m) { ga[m] = a; gb[m] = b; gc[m] = c; }
       second_order_core<K>(T, w, bv, cv, cg, ga, gb, gc, dif);
-      ej = G.gjm[(size_t)el * 5 + 4];
     } else {
       second_order_core<K>(T, w, bv, cv, cg, G.Ga + base, G.Gb + base, G.Gc + base, dif);
     }

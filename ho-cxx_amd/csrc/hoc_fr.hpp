@@ -23,6 +23,7 @@ struct StageOut {
   // mode 2 batches: nbatch independent products in one launch, probe b reading/writing its
   // arrays at b * (stride in values); stride 0 shares an array between the probes
   int nbatch = 1;
+  int adv_only = 0;  // mode 2: the diffusion inputs are zero (skip the diffusion term)
   size_t bs_w = 0, bs_trw = 0, bs_U = 0, bs_tru = 0, bs_wa = 0, bs_trwa = 0, bs_k = 0, bs_diff = 0;
 };
 

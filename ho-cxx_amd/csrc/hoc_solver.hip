@@ -1351,6 +1351,7 @@ struct Solver {
     so.trw_adv = adj.trws[st]; so.bs_trwa = 0;
     so.bs_w = 0; so.bs_trw = 0; so.bs_U = 2 * D; so.bs_tru = TU; so.bs_diff = 0;
     so.nbatch = K2;
+    so.adv_only = 1;  // zero vorticity and wall flux for the diffusion term
     for (int c = 0; c < adj.nc1; ++c)
       for (int comp = 0; comp < 2; ++comp) {
         HIPCHK(launch_adj_seed_u_batch(N, K, adj.col1, c, comp, T, adj.dU, adj.dtru, stream));
@@ -1368,6 +1369,7 @@ struct Solver {
       HIPCHK(launch_adj_edge_seed_batch(E, K, adj.own_f, adj.pvel_seed, stream));
       so.omega_adv = adj.zero_d; so.trw_adv = adj.zero_tr;
       so.bs_U = 0; so.bs_tru = 0; so.bs_diff = EK;
+      so.adv_only = 0;
       so.nbatch = 4 * K;
       HIPCHK(launch_stage(K, 1, N, T, G, adj.Bl_seed(), adj.zero_d, adj.zero_u, adj.zero_tr, adj.zero_tu, re_inv,
                           cfg.dt, I, so, stream, 2));
