@@ -1,6 +1,8 @@
 // hoc_la.hip - vector, reduction, CSR and probing kernels of the Poisson solver.
 #include "hoc_la.hpp"
 
+#include <algorithm>
+
 namespace hoc {
 
 __device__ __forceinline__ double wsum(double v) {
@@ -448,6 +450,69 @@ void psi_combine(hipStream_t s, size_t n, int nterm, const double* a, const doub
   Comb5 C{};
   for (int j = 0; j < nterm; ++j) { C.a[j] = a[j]; C.x[j] = x[j]; }
   hipLaunchKernelGGL(k_psi_combine, dim3(vec_grid(n)), dim3(256), 0, s, n, nterm, C, psi, save);
+}
+constexpr int kProjBlocks = 1024;
+int proj_part_len(size_t n) { return (int)std::min<size_t>(kProjBlocks, (n + 255) / 256 + 1); }
+__global__ __launch_bounds__(256) void k_proj_dots(size_t n, int m, const double* __restrict__ X,
+                                                   const double* __restrict__ y, double* __restrict__ part) {
+  double s[kProjMax];
+#pragma unroll
+  for (int j = 0; j < kProjMax; ++j) s[j] = 0.;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    const double yi = y[i];
+#pragma unroll
+    for (int j = 0; j < kProjMax; ++j)
+      if (j < m) s[j] += X[(size_t)j * n + i] * yi;
+  }
+  __shared__ double red[4][kProjMax];  // 4 wavefronts of 64
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int j = 0; j < kProjMax; ++j) {
+    double v = s[j];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if (lane == 0) red[wv][j] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < m) {
+    const int j = threadIdx.x;
+    part[(size_t)blockIdx.x * m + j] = red[0][j] + red[1][j] + red[2][j] + red[3][j];
+  }
+}
+__global__ void k_proj_fin(int nb, int m, const double* __restrict__ part, double* __restrict__ out) {
+  const int j = threadIdx.x;
+  if (j >= m) return;
+  double s = 0.;
+  for (int b = 0; b < nb; ++b) s += part[(size_t)b * m + j];  // fixed order: deterministic
+  out[j] = s;
+}
+__global__ void k_proj_combine(size_t n, int m, const double* __restrict__ X, const double* __restrict__ c,
+                               int accumulate, double* __restrict__ out) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    double v = 0.;
+    for (int j = 0; j < m; ++j) v += c[j] * X[(size_t)j * n + i];
+    out[i] = accumulate ? out[i] - v : v;
+  }
+}
+__global__ void k_proj_normalize(size_t n, const double* __restrict__ d, double* __restrict__ a, double* __restrict__ b) {
+  const double f = (d[0] > 1e-24 * fabs(d[1]) && d[0] > 0.) ? 1. / sqrt(d[0]) : 0.;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    a[i] *= f;
+    b[i] *= f;
+  }
+}
+void proj_dots(hipStream_t s, size_t n, int m, const double* X, const double* y, double* part, double* out) {
+  if (m <= 0 || n == 0) return;
+  const int nb = proj_part_len(n);
+  hipLaunchKernelGGL(k_proj_dots, dim3(nb), dim3(256), 0, s, n, m, X, y, part);
+  hipLaunchKernelGGL(k_proj_fin, dim3(1), dim3(kProjMax), 0, s, nb, m, part, out);
+}
+void proj_combine(hipStream_t s, size_t n, int m, const double* X, const double* c, int accumulate, double* out) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_proj_combine, dim3(vec_grid(n)), dim3(256), 0, s, n, m, X, c, accumulate, out);
+}
+void proj_normalize(hipStream_t s, size_t n, const double* d, double* a, double* b) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_proj_normalize, dim3(vec_grid(n)), dim3(256), 0, s, n, d, a, b);
 }
 void psi_predict(hipStream_t s, size_t n, double f, int extrap, double* psi, double* prev) {
   if (n == 0) return;

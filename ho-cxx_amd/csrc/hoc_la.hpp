@@ -71,6 +71,17 @@ void mixed_promote_add(hipStream_t s, size_t n, double* x64, const float* e32, c
 // extrap = 0: prev = psi (history shift, psi kept);  extrap = 1: psi' = psi + f (psi - prev)
 // and prev = psi (extrapolation and history shift in one pass).
 void psi_predict(hipStream_t s, size_t n, double f, int extrap, double* psi, double* prev);
+
+// Projection onto an As-orthonormal basis of earlier solutions (m <= kProjMax vectors X_j of
+// length n, stored contiguously, with AX_j = As X_j):
+//   proj_dots:    out[j] = X_j . y  (j < m), through per-block partials (part: proj_part_len(n) * m)
+//   proj_combine: out = accumulate ? out - sum_j c[j] X_j : sum_j c[j] X_j
+//   proj_normalize: f = 1/sqrt(d[0]) when d[0] > tiny * d[1] (else 0); a *= f, b *= f
+constexpr int kProjMax = 16;
+int proj_part_len(size_t n);
+void proj_dots(hipStream_t s, size_t n, int m, const double* X, const double* y, double* part, double* out);
+void proj_combine(hipStream_t s, size_t n, int m, const double* X, const double* c, int accumulate, double* out);
+void proj_normalize(hipStream_t s, size_t n, const double* d, double* a, double* b);
 // psi' = sum_j a[j] x_j (x_j may be psi itself, read before the write), and save = psi (the old
 // value) when save is not null; nterm <= 8.
 void psi_combine(hipStream_t s, size_t n, int nterm, const double* a, const double* const* x, double* psi,

@@ -269,12 +269,6 @@ struct Solver {
     B = BCDev{b_psi, b_diff, b_nvel, b_pvel, b_vj, b_vort, b_adv, b_psi0, b_psi_r, b_diff0, b_diff_r, b_nvel0, b_nvel_r, b_pvel0, b_pvel_r};
     // fields
     for (auto* f : {&w, &psi, &k1, &k2, &k3, &k4, &vort, &kscratch, &bs}) f->alloc(D);
-    psi_prev.alloc(guess_mode() >= 2 ? D : 0);
-    for (auto& b : rkS) b.alloc(guess_mode() >= 3 ? D : 0);
-    for (auto& b : rkQ) b.alloc(guess_mode() == 4 ? D : 0);
-    n_hist = 0;
-    rk_full = false;
-    rk_cur = 0;
     vel.alloc(2 * D); U.alloc(2 * D);
     trw.alloc((size_t)N * 8 * K); tru.alloc((size_t)N * 4 * K);
     mixed = cfg.precision == HOC_MIXED;
@@ -305,6 +299,13 @@ struct Solver {
       HIPCHK(hipHostGetDevicePointer((void**)&mirror_dev, mirror, 0));
     }
     build_hierarchy();
+    // initial-guess history (its mode depends on the Krylov method chosen above)
+    psi_prev.alloc(guess_mode() >= 2 ? D : 0);
+    for (auto& b : rkS) b.alloc(guess_mode() >= 3 ? D : 0);
+    for (auto& b : rkQ) b.alloc(guess_mode() >= 4 ? D : 0);
+    pX.alloc(0); pAX.alloc(0);
+    p_m = 0;
+    invalidate_guess();
     time = 0.;
     smooth_ready = false;
     n_sp = 0;
@@ -663,10 +664,12 @@ struct Solver {
   int n_hist = 0;               // solutions in the history (0: psi is not a stage solution)
   int64_t step_count = 0;
   double stage_tau = std::nan("");  // stage time of the solve in progress (NaN: not a stage solve)
-  // default: mode 4 with RK4; with Euler and RK2 the reference's warm start (extrapolating over
-  // their steps measured no better: their stiff modes alternate in sign from step to step)
+  // default: the projection (mode 5) when As is symmetric (PCG); otherwise mode 4 with RK4, and with
+  // Euler and RK2 the reference's warm start (extrapolating over their steps measured no better:
+  // their stiff modes alternate in sign from step to step)
   int guess_mode() const {
     if (cfg.psi_guess != 0) return cfg.psi_guess;
+    if (krylov == HOC_KRYLOV_PCG) return 5;
     return cfg.time_integration_type == 4 ? 4 : 1;
   }
   // Mode 3 (RK4): psi(omega_n + c k) = psi(omega_n) + c Psi(k) with Psi linear, so the stage
@@ -733,7 +736,7 @@ struct Solver {
     return guess_tmp.p;
   }
   void rk_guess_before() {
-    if (guess_mode() == 4) { rk4_guess_before(); return; }
+    if (guess_mode() >= 4) { rk4_guess_before(); return; }
     const int st = rk_stage;
     if (st == 0 && rk_full) {
       const double a[4] = {-0.5, 1. / 3., 2. / 3., 0.5};
@@ -760,13 +763,60 @@ struct Solver {
     rk_cur = st + 1;
     if (st == 3) { rk_full = true; rk_cur = 0; }
   }
+  // Mode 5 (symmetric As, i.e. PCG; otherwise mode 4): projection for successive right-hand sides
+  // (P. F. Fischer, Comput. Methods Appl. Mech. Engrg. 163 (1998) 193-204).  An As-orthonormal basis
+  // X of up to M = 16 earlier stage solutions gives each stage solve the start sum_j (X_j . b) X_j,
+  // the As-norm best approximation of its solution in their span.  The new solution x is then
+  // orthogonalised against X by classical Gram-Schmidt applied twice, using As x = b - r (r: the
+  // solve's final residual) and the stored As X, and appended; a full basis restarts from it.
+  static constexpr int kProjVectors = 16;
+  DBuf<double> pX, pAX, pcoef, ppart, ptmp;
+  int p_m = 0;  // vectors held
+  bool proj_on() const { return guess_mode() >= 5 && krylov == HOC_KRYLOV_PCG; }
+  void proj_before(const double* b) {
+    if (p_m == 0) return;
+    proj_dots(stream, D, p_m, pX, b, ppart, pcoef);
+    proj_combine(stream, D, p_m, pX, pcoef, 0, psi);
+  }
+  void proj_after(const double* b, const double* r) {
+    constexpr int M = kProjVectors;
+    if (pX.n < (size_t)M * D) {
+      pX.alloc((size_t)M * D); pAX.alloc((size_t)M * D); ptmp.alloc(2 * D);
+      pcoef.alloc(kProjMax + 3); ppart.alloc((size_t)proj_part_len(D) * kProjMax);
+      const double one = 1.;
+      h2d(pcoef.p + kProjMax + 2, &one, sizeof(double));
+      p_m = 0;
+    }
+    if (p_m >= M) p_m = 0;  // restart from the newest solution
+    const int j = p_m;
+    double* x = ptmp.p;
+    double* ax = ptmp.p + D;
+    dcopy(x, psi, D);
+    dcopy(ax, b, D);
+    proj_combine(stream, D, 1, r, pcoef.p + kProjMax + 2, 1, ax);  // As x = b - r
+    proj_dots(stream, D, 1, x, ax, ppart, pcoef.p + kProjMax + 1);  // x . As x before orthogonalisation
+    for (int pass = 0; pass < 2 && j > 0; ++pass) {
+      proj_dots(stream, D, j, pX, ax, ppart, pcoef);  // c_i = X_i . As x
+      proj_combine(stream, D, j, pX, pcoef, 1, x);
+      proj_combine(stream, D, j, pAX, pcoef, 1, ax);
+    }
+    proj_dots(stream, D, 1, x, ax, ppart, pcoef.p + kProjMax);
+    proj_normalize(stream, D, pcoef.p + kProjMax, x, ax);  // (a vector left in the span becomes zero)
+    dcopy(pX.p + (size_t)j * D, x, D);
+    dcopy(pAX.p + (size_t)j * D, ax, D);
+    p_m = j + 1;
+  }
+  // Called when the state is set from outside: the histories restart, so that the same state gives
+  // the same iterations (the projection basis would stay valid, holding solutions of As x = b only).
   void invalidate_guess() {
+    p_m = 0;
     n_hist = 0;
     rk_full = false;
     rk_qok = false;
     rk_cur = 0;
   }
   void guess_before_solve() {
+    if (proj_on() && !std::isnan(stage_tau)) return;  // projected after the right-hand side is formed
     if (guess_mode() >= 3 && rk_stage >= 0 && !std::isnan(stage_tau)) { rk_guess_before(); return; }
     if (guess_mode() < 2 || std::isnan(stage_tau)) return;
     const double tau = stage_tau;
@@ -775,6 +825,7 @@ struct Solver {
     psi_predict(stream, D, ok ? (tau - tau_a) / (tau_a - tau_b) : 0., ok ? 1 : 0, psi, psi_prev);
   }
   void guess_after_solve() {
+    if (proj_on() && !std::isnan(stage_tau)) return;
     if (guess_mode() >= 3 && rk_stage >= 0 && !std::isnan(stage_tau)) { rk_guess_after(); return; }
     rk_full = false;
     rk_qok = false;
@@ -794,6 +845,8 @@ struct Solver {
     const double tol2 = cfg.tol * cfg.tol;
     HIPCHK(hipMemcpyAsync(S.p + S_TOL2, &tol2, sizeof(double), hipMemcpyHostToDevice, stream));
     HIPCHK(launch_rhs(K, N, T, G, omega, rhs_bc ? rhs_bc : b_psi.p, bs, part, stream));
+    const bool proj = proj_on() && !std::isnan(stage_tau);
+    if (proj) proj_before(bs);
     finalize(stream, PH_BB, part, lb, S, F, maxit);
     if (n_inst > 1) {
       // ||b - A psi|| <= tol min_i ||b_i|| bounds every instance's own relative residual
@@ -855,6 +908,7 @@ struct Solver {
     if (hf[F_ZERO]) last_rel = 0.;
     total_iters += last_iters;
     ++n_solves;
+    if (proj && !hf[F_ZERO]) proj_after(bs, P64.r);
     guess_after_solve();
     if (!std::isfinite(last_rel)) throw StatusError(HOC_ENAN, "NaN/Inf in the Poisson solve");
     if (cfg.fail_on_noconv && last_rel > cfg.tol)
@@ -1631,7 +1685,7 @@ int hoc_create(hoc_t* out, const hoc_config* cfg) {
   if (!(cfg->mixed_inner_tol >= 0. && cfg->mixed_inner_tol < 1.)) return hoc_fail(nullptr, HOC_EINVAL, "mixed_inner_tol must be in [0, 1)");
   if (cfg->cheb_degree < 0 || cfg->cheb_degree > hoc::kMaxCheb) return hoc_fail(nullptr, HOC_EINVAL, "cheb_degree must be in 0..8");
   if (!(cfg->cheb_lower >= 0. && cfg->cheb_lower < 1.)) return hoc_fail(nullptr, HOC_EINVAL, "cheb_lower must be in [0, 1)");
-  if (cfg->psi_guess < 0 || cfg->psi_guess > 4) return hoc_fail(nullptr, HOC_EINVAL, "psi_guess must be 0..4");
+  if (cfg->psi_guess < 0 || cfg->psi_guess > 5) return hoc_fail(nullptr, HOC_EINVAL, "psi_guess must be 0..5");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return hoc_fail(nullptr, HOC_EDEVICE, "no HIP device");
   if (cfg->device < 0 || cfg->device >= ndev) return hoc_fail(nullptr, HOC_EDEVICE, "device ordinal out of range");
