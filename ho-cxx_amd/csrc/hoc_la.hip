@@ -478,12 +478,17 @@ __global__ __launch_bounds__(256) void k_proj_dots(size_t n, int m, const double
     part[(size_t)blockIdx.x * m + j] = red[0][j] + red[1][j] + red[2][j] + red[3][j];
   }
 }
-__global__ void k_proj_fin(int nb, int m, const double* __restrict__ part, double* __restrict__ out) {
-  const int j = threadIdx.x;
-  if (j >= m) return;
+// one block per dot product: strided partial sums, then a fixed-order tree (deterministic)
+__global__ __launch_bounds__(256) void k_proj_fin(int nb, int m, const double* __restrict__ part,
+                                                  double* __restrict__ out) {
+  const int j = blockIdx.x;
   double s = 0.;
-  for (int b = 0; b < nb; ++b) s += part[(size_t)b * m + j];  // fixed order: deterministic
-  out[j] = s;
+  for (int b = threadIdx.x; b < nb; b += 256) s += part[(size_t)b * m + j];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  __shared__ double red[4];
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) out[j] = red[0] + red[1] + red[2] + red[3];
 }
 __global__ void k_proj_combine(size_t n, int m, const double* __restrict__ X, const double* __restrict__ c,
                                int accumulate, double* __restrict__ out) {
@@ -504,7 +509,7 @@ void proj_dots(hipStream_t s, size_t n, int m, const double* X, const double* y,
   if (m <= 0 || n == 0) return;
   const int nb = proj_part_len(n);
   hipLaunchKernelGGL(k_proj_dots, dim3(nb), dim3(256), 0, s, n, m, X, y, part);
-  hipLaunchKernelGGL(k_proj_fin, dim3(1), dim3(kProjMax), 0, s, nb, m, part, out);
+  hipLaunchKernelGGL(k_proj_fin, dim3(m), dim3(256), 0, s, nb, m, part, out);
 }
 void proj_combine(hipStream_t s, size_t n, int m, const double* X, const double* c, int accumulate, double* out) {
   if (n == 0) return;

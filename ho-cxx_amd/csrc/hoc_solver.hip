@@ -764,19 +764,24 @@ struct Solver {
     if (st == 3) { rk_full = true; rk_cur = 0; }
   }
   // Mode 5 (symmetric As, i.e. PCG; otherwise mode 4): projection for successive right-hand sides
-  // (P. F. Fischer, Comput. Methods Appl. Mech. Engrg. 163 (1998) 193-204).  An As-orthonormal basis
-  // X of up to M = 16 earlier stage solutions gives each stage solve the start sum_j (X_j . b) X_j,
-  // the As-norm best approximation of its solution in their span.  The new solution x is then
-  // orthogonalised against X by classical Gram-Schmidt applied twice, using As x = b - r (r: the
-  // solve's final residual) and the stored As X, and appended; a full basis restarts from it.
+  // (P. F. Fischer, Comput. Methods Appl. Mech. Engrg. 163 (1998) 193-204), applied to the
+  // correction of mode 4's guess x0 (with Euler and RK2: of the previous solution).  An
+  // As-orthonormal basis X of up to M = 16 earlier stage solutions gives the start
+  // x0 + sum_j (X_j . (b - As x0)) X_j: x0 plus the As-norm best approximation of its error in their
+  // span, never worse than x0 in the As-norm.  The new solution x is then orthogonalised against X
+  // by classical Gram-Schmidt applied twice, using As x = b - r (r: the solve's final residual) and
+  // the stored As X, and appended; a full basis restarts from it.
   static constexpr int kProjVectors = 16;
   DBuf<double> pX, pAX, pcoef, ppart, ptmp;
   int p_m = 0;  // vectors held
   bool proj_on() const { return guess_mode() >= 5 && krylov == HOC_KRYLOV_PCG; }
   void proj_before(const double* b) {
     if (p_m == 0) return;
-    proj_dots(stream, D, p_m, pX, b, ppart, pcoef);
-    proj_combine(stream, D, p_m, pX, pcoef, 0, psi);
+    double* t = ptmp.p;
+    apply_As(psi, t);
+    proj_combine(stream, D, 1, b, pcoef.p + kProjMax + 2, 1, t);  // t = As x0 - b
+    proj_dots(stream, D, p_m, pX, t, ppart, pcoef);                // -(X_j . (b - As x0))
+    proj_combine(stream, D, p_m, pX, pcoef, 1, psi);
   }
   void proj_after(const double* b, const double* r) {
     constexpr int M = kProjVectors;
@@ -816,16 +821,14 @@ struct Solver {
     rk_cur = 0;
   }
   void guess_before_solve() {
-    if (proj_on() && !std::isnan(stage_tau)) return;  // projected after the right-hand side is formed
     if (guess_mode() >= 3 && rk_stage >= 0 && !std::isnan(stage_tau)) { rk_guess_before(); return; }
-    if (guess_mode() < 2 || std::isnan(stage_tau)) return;
+    if (guess_mode() < 2 || guess_mode() == 5 || std::isnan(stage_tau)) return;
     const double tau = stage_tau;
     if (n_hist == 0 || tau == tau_a) return;  // no history, or the same stage time: the previous solution
     const bool ok = n_hist == 2 && tau > tau_a && tau - tau_a <= 1. && tau_a > tau_b && tau_a - tau_b <= 1.;
     psi_predict(stream, D, ok ? (tau - tau_a) / (tau_a - tau_b) : 0., ok ? 1 : 0, psi, psi_prev);
   }
   void guess_after_solve() {
-    if (proj_on() && !std::isnan(stage_tau)) return;
     if (guess_mode() >= 3 && rk_stage >= 0 && !std::isnan(stage_tau)) { rk_guess_after(); return; }
     rk_full = false;
     rk_qok = false;
