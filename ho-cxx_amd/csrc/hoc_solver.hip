@@ -664,14 +664,8 @@ struct Solver {
   int n_hist = 0;               // solutions in the history (0: psi is not a stage solution)
   int64_t step_count = 0;
   double stage_tau = std::nan("");  // stage time of the solve in progress (NaN: not a stage solve)
-  // default: the projection (mode 5) when As is symmetric (PCG); otherwise mode 4 with RK4, and with
-  // Euler and RK2 the reference's warm start (extrapolating over their steps measured no better:
-  // their stiff modes alternate in sign from step to step)
-  int guess_mode() const {
-    if (cfg.psi_guess != 0) return cfg.psi_guess;
-    if (krylov == HOC_KRYLOV_PCG) return 5;
-    return cfg.time_integration_type == 4 ? 4 : 1;
-  }
+  // default: the projection (mode 5)
+  int guess_mode() const { return cfg.psi_guess != 0 ? cfg.psi_guess : 5; }
   // Mode 3 (RK4): psi(omega_n + c k) = psi(omega_n) + c Psi(k) with Psi linear, so the stage
   // solutions S1..S4 of a step give Psi(k1) = 2 (S2 - S1), Psi(k2) = 2 (S3 - S1), Psi(k3) = S4 - S1,
   // and the next step's first solve is S1 + (Psi(k1) + 2 Psi(k2) + 2 Psi(k3) + Psi(k4)) / 6 with
@@ -763,24 +757,27 @@ struct Solver {
     rk_cur = st + 1;
     if (st == 3) { rk_full = true; rk_cur = 0; }
   }
-  // Mode 5 (symmetric As, i.e. PCG; otherwise mode 4): projection for successive right-hand sides
-  // (P. F. Fischer, Comput. Methods Appl. Mech. Engrg. 163 (1998) 193-204), applied to the
-  // correction of mode 4's guess x0 (with Euler and RK2: of the previous solution).  An
-  // As-orthonormal basis X of up to M = 16 earlier stage solutions gives the start
-  // x0 + sum_j (X_j . (b - As x0)) X_j: x0 plus the As-norm best approximation of its error in their
-  // span, never worse than x0 in the As-norm.  The new solution x is then orthogonalised against X
-  // by classical Gram-Schmidt applied twice, using As x = b - r (r: the solve's final residual) and
-  // the stored As X, and appended; a full basis restarts from it.
+  // Mode 5: projection for successive right-hand sides (P. F. Fischer, Comput. Methods Appl. Mech.
+  // Engrg. 163 (1998) 193-204), applied to the correction of mode 4's guess x0 (with Euler and RK2,
+  // whose stiff modes alternate in sign from step to step: of the previous solution).  A basis X of
+  // up to M = 16 earlier stage solutions with their images AX = As X gives the start
+  // x0 + sum_j c_j X_j with
+  //   symmetric As (PCG): X As-orthonormal, c_j = X_j . (b - As x0): the As-norm best correction;
+  //   otherwise (BiCGSTAB): AX orthonormal, c_j = AX_j . (b - As x0): the least-squares residual.
+  // Either is never worse than x0 in its norm.  The new solution x is then orthogonalised against
+  // the basis by classical Gram-Schmidt applied twice, using As x = b - r (r: the solve's final
+  // residual) and the stored AX, and appended; a full basis restarts from it.
   static constexpr int kProjVectors = 16;
   DBuf<double> pX, pAX, pcoef, ppart, ptmp;
   int p_m = 0;  // vectors held
-  bool proj_on() const { return guess_mode() >= 5 && krylov == HOC_KRYLOV_PCG; }
+  bool proj_on() const { return guess_mode() >= 5; }
+  bool proj_mr() const { return krylov != HOC_KRYLOV_PCG; }  // minimal-residual form
   void proj_before(const double* b) {
     if (p_m == 0) return;
     double* t = ptmp.p;
     apply_As(psi, t);
     proj_combine(stream, D, 1, b, pcoef.p + kProjMax + 2, 1, t);  // t = As x0 - b
-    proj_dots(stream, D, p_m, pX, t, ppart, pcoef);                // -(X_j . (b - As x0))
+    proj_dots(stream, D, p_m, proj_mr() ? pAX.p : pX.p, t, ppart, pcoef);  // -c_j
     proj_combine(stream, D, p_m, pX, pcoef, 1, psi);
   }
   void proj_after(const double* b, const double* r) {
@@ -799,13 +796,14 @@ struct Solver {
     dcopy(x, psi, D);
     dcopy(ax, b, D);
     proj_combine(stream, D, 1, r, pcoef.p + kProjMax + 2, 1, ax);  // As x = b - r
-    proj_dots(stream, D, 1, x, ax, ppart, pcoef.p + kProjMax + 1);  // x . As x before orthogonalisation
+    const double* nx = proj_mr() ? ax : x;  // the norm: |x|_As^2 = x . As x, or |As x|^2
+    proj_dots(stream, D, 1, nx, ax, ppart, pcoef.p + kProjMax + 1);  // before orthogonalisation
     for (int pass = 0; pass < 2 && j > 0; ++pass) {
-      proj_dots(stream, D, j, pX, ax, ppart, pcoef);  // c_i = X_i . As x
+      proj_dots(stream, D, j, proj_mr() ? pAX.p : pX.p, ax, ppart, pcoef);  // c_i
       proj_combine(stream, D, j, pX, pcoef, 1, x);
       proj_combine(stream, D, j, pAX, pcoef, 1, ax);
     }
-    proj_dots(stream, D, 1, x, ax, ppart, pcoef.p + kProjMax);
+    proj_dots(stream, D, 1, nx, ax, ppart, pcoef.p + kProjMax);
     proj_normalize(stream, D, pcoef.p + kProjMax, x, ax);  // (a vector left in the span becomes zero)
     dcopy(pX.p + (size_t)j * D, x, D);
     dcopy(pAX.p + (size_t)j * D, ax, D);

@@ -66,12 +66,10 @@ typedef struct {
                                      omega), otherwise as 2;  4 = as 3, each predicted
                                      stage increment corrected by the same stage
                                      difference one step earlier;  5 = mode 4's guess (with
-                                     Euler and RK2 the previous solution) plus its
-                                     correction projected onto an As-orthonormal basis
-                                     of the last 16 stage solutions (symmetric operators;
-                                     otherwise as 4)
-                                     (0 = 5 when the Poisson operator is symmetric, else 4
-                                     with RK4 and 1 with Euler and RK2)                   */
+                                     Euler and RK2 the previous solution) plus its best
+                                     correction in the span of the last 16 stage
+                                     solutions: in the As-norm for a symmetric operator,
+                                     least-squares residual otherwise  (0 = 5)            */
   double cheb_lower;              /* smoothing interval [cheb_lower, 1.1] x lambda_max(D^-1 As)
                                      (0 = 1/60)                                          */
 } hoc_config;
