@@ -303,8 +303,8 @@ struct Solver {
     psi_prev.alloc(guess_mode() >= 2 ? D : 0);
     for (auto& b : rkS) b.alloc(guess_mode() >= 3 ? D : 0);
     for (auto& b : rkQ) b.alloc(guess_mode() >= 4 ? D : 0);
-    pX.alloc(0); pAX.alloc(0);
-    p_m = 0;
+    proj_fwd.X.alloc(0); proj_fwd.AX.alloc(0); proj_fwd.m = 0;
+    proj_adj.X.alloc(0); proj_adj.AX.alloc(0); proj_adj.m = 0;
     invalidate_guess();
     time = 0.;
     smooth_ready = false;
@@ -768,29 +768,40 @@ struct Solver {
   // the basis by classical Gram-Schmidt applied twice, using As x = b - r (r: the solve's final
   // residual) and the stored AX, and appended; a full basis restarts from it.
   static constexpr int kProjVectors = 16;
-  DBuf<double> pX, pAX, pcoef, ppart, ptmp;
-  int p_m = 0;  // vectors held
+  struct ProjBasis {
+    DBuf<double> X, AX;
+    int m = 0;  // vectors held
+  };
+  ProjBasis proj_fwd, proj_adj;     // stage solves; adjoint (transposed) solves
+  ProjBasis* proj_cur = &proj_fwd;  // the basis of the solve in progress
+  bool proj_force = false;          // project a solve that is not a stage solve (adjoint solves)
+  DBuf<double> pcoef, ppart, ptmp;
   bool proj_on() const { return guess_mode() >= 5; }
   bool proj_mr() const { return krylov != HOC_KRYLOV_PCG; }  // minimal-residual form
   void proj_before(const double* b) {
-    if (p_m == 0) return;
+    ProjBasis& P = *proj_cur;
+    if (P.m == 0) return;
     double* t = ptmp.p;
     apply_As(psi, t);
     proj_combine(stream, D, 1, b, pcoef.p + kProjMax + 2, 1, t);  // t = As x0 - b
-    proj_dots(stream, D, p_m, proj_mr() ? pAX.p : pX.p, t, ppart, pcoef);  // -c_j
-    proj_combine(stream, D, p_m, pX, pcoef, 1, psi);
+    proj_dots(stream, D, P.m, proj_mr() ? P.AX.p : P.X.p, t, ppart, pcoef);  // -c_j
+    proj_combine(stream, D, P.m, P.X.p, pcoef, 1, psi);
   }
   void proj_after(const double* b, const double* r) {
     constexpr int M = kProjVectors;
-    if (pX.n < (size_t)M * D) {
-      pX.alloc((size_t)M * D); pAX.alloc((size_t)M * D); ptmp.alloc(2 * D);
+    ProjBasis& P = *proj_cur;
+    if (ptmp.n < 2 * D) {
+      ptmp.alloc(2 * D);
       pcoef.alloc(kProjMax + 3); ppart.alloc((size_t)proj_part_len(D) * kProjMax);
       const double one = 1.;
       h2d(pcoef.p + kProjMax + 2, &one, sizeof(double));
-      p_m = 0;
     }
-    if (p_m >= M) p_m = 0;  // restart from the newest solution
-    const int j = p_m;
+    if (P.X.n < (size_t)M * D) {
+      P.X.alloc((size_t)M * D); P.AX.alloc((size_t)M * D);
+      P.m = 0;
+    }
+    if (P.m >= M) P.m = 0;  // restart from the newest solution
+    const int j = P.m;
     double* x = ptmp.p;
     double* ax = ptmp.p + D;
     dcopy(x, psi, D);
@@ -799,20 +810,20 @@ struct Solver {
     const double* nx = proj_mr() ? ax : x;  // the norm: |x|_As^2 = x . As x, or |As x|^2
     proj_dots(stream, D, 1, nx, ax, ppart, pcoef.p + kProjMax + 1);  // before orthogonalisation
     for (int pass = 0; pass < 2 && j > 0; ++pass) {
-      proj_dots(stream, D, j, proj_mr() ? pAX.p : pX.p, ax, ppart, pcoef);  // c_i
-      proj_combine(stream, D, j, pX, pcoef, 1, x);
-      proj_combine(stream, D, j, pAX, pcoef, 1, ax);
+      proj_dots(stream, D, j, proj_mr() ? P.AX.p : P.X.p, ax, ppart, pcoef);  // c_i
+      proj_combine(stream, D, j, P.X.p, pcoef, 1, x);
+      proj_combine(stream, D, j, P.AX.p, pcoef, 1, ax);
     }
     proj_dots(stream, D, 1, nx, ax, ppart, pcoef.p + kProjMax);
     proj_normalize(stream, D, pcoef.p + kProjMax, x, ax);  // (a vector left in the span becomes zero)
-    dcopy(pX.p + (size_t)j * D, x, D);
-    dcopy(pAX.p + (size_t)j * D, ax, D);
-    p_m = j + 1;
+    dcopy(P.X.p + (size_t)j * D, x, D);
+    dcopy(P.AX.p + (size_t)j * D, ax, D);
+    P.m = j + 1;
   }
   // Called when the state is set from outside: the histories restart, so that the same state gives
   // the same iterations (the projection basis would stay valid, holding solutions of As x = b only).
   void invalidate_guess() {
-    p_m = 0;
+    proj_fwd.m = 0;
     n_hist = 0;
     rk_full = false;
     rk_qok = false;
@@ -846,7 +857,7 @@ struct Solver {
     const double tol2 = cfg.tol * cfg.tol;
     HIPCHK(hipMemcpyAsync(S.p + S_TOL2, &tol2, sizeof(double), hipMemcpyHostToDevice, stream));
     HIPCHK(launch_rhs(K, N, T, G, omega, rhs_bc ? rhs_bc : b_psi.p, bs, part, stream));
-    const bool proj = proj_on() && !std::isnan(stage_tau);
+    const bool proj = proj_on() && (!std::isnan(stage_tau) || proj_force);
     if (proj) proj_before(bs);
     finalize(stream, PH_BB, part, lb, S, F, maxit);
     if (n_inst > 1) {
@@ -1446,15 +1457,21 @@ struct Solver {
     }
     rhs_bc = adj.zero_e;
     std::swap(psi.p, adj.z.p);
+    proj_cur = &proj_adj;
+    proj_force = true;
     try {
       poisson(adj.fake);
     } catch (...) {
       std::swap(psi.p, adj.z.p);
       rhs_bc = nullptr;
+      proj_cur = &proj_fwd;
+      proj_force = false;
       throw;
     }
     std::swap(psi.p, adj.z.p);
     rhs_bc = nullptr;
+    proj_cur = &proj_fwd;
+    proj_force = false;
     if (!std::isnan(tau) && !(adj.zn >= 1 && tau == adj.ztau_a)) {
       adj.ztau_b = adj.ztau_a;
       adj.ztau_a = tau;
@@ -1504,6 +1521,7 @@ struct Solver {
     const double cst[4] = {0., 0.5, 0.5, 1.};
     const int every = adj.every, nsteps = adj.nsteps;
     adj.zn = 0;
+    proj_adj.m = 0;
     const int nseg = (nsteps + every - 1) / every;
     auto restore = [&] {
       dcopy(w, adj.save_w, D);
@@ -1552,6 +1570,7 @@ struct Solver {
   // (dk/domega)^T lam and (dk/dpvel)^T lam of one stage at time t (hoc_stage_residual's map).
   void stage_vjp(const double* omega_h, double t, const double* lam_h, double* out_h, double* pvbar_h) {
     adjoint_setup();
+    proj_adj.m = 0;
     const size_t EK = std::max<size_t>((size_t)E * K, 1);
     h2d(adj.ws[0].p, omega_h, D * sizeof(double));
     h2d(adj.lam.p, lam_h, D * sizeof(double));
