@@ -1,8 +1,7 @@
 """Throughput of the HO-CXX time-step hot path on one MI355X.
 
 Metric: Mdof-steps/s per RK4 step including the four Poisson solves, where
-Mdof = N_el * K^2 / 1e6.  Workload (the north-star configuration): lid-driven
-cavity on the unit square, 512 x 512 = 2^18 bilinear quads stored as
+Mdof = N_el * K^2 / 1e6.  Workload: lid-driven cavity on the unit square, 512 x 512 = 2^18 bilinear quads stored as
 unstructured element/edge lists, K = 5 (P = 4), Re = 100, zero initial
 vorticity, RK4 with dt = half the measured explicit-RK4 diffusion limit, Krylov
 relative tolerance 1e-10.  Inputs live in HBM for the whole timed region.
