@@ -2580,6 +2580,29 @@ int ho_oracle_velocity_from_psi(ho_oracle* h, double* uv) {
   return 0;
 }
 
+int ho_oracle_set_time(ho_oracle* h, double t) { h->current_time = t; return 0; }
+
+int ho_oracle_update_bcs(ho_oracle* h, double t) {
+  try {
+    h->update_BCs(t);
+  } catch (const std::exception& e) { return fail(h, e); }
+  return 0;
+}
+
+int ho_oracle_advection(ho_oracle* h, const double* w, const double* uv, const double* bc_vort,
+                        const double* bc_nvel, double* out) {
+  try {
+    std::copy(uv, uv + h->vel.size(), h->vel.begin());
+    std::copy(bc_vort, bc_vort + h->BC_vorticity.size(), h->BC_vorticity.begin());
+    std::copy(bc_nvel, bc_nvel + h->BC_normal_vel.size(), h->BC_normal_vel.begin());
+    h->update_advection_BC();
+    if (h->advection_type == 1) h->calc_RHS_advection(w);
+    else h->calc_RHS_advection_continuous(w);
+    std::copy(h->RHS_adv.begin(), h->RHS_adv.end(), out);
+  } catch (const std::exception& e) { return fail(h, e); }
+  return 0;
+}
+
 int ho_oracle_apply_laplacian_mf(ho_oracle* h, const double* x, double* y) {
   try {
     std::vector<double> zeros((size_t)h->N_eb * h->K, 0.);

@@ -128,6 +128,17 @@ int ho_oracle_sample_values(ho_oracle*, double* out);
  * before each dump (src/HO_2D.cpp:1411-1414). */
 int ho_oracle_refresh(ho_oracle*);
 
+/* Test hooks for the pins: set the clock; run update_BCs(t) alone (src/HO_2D.cpp:1979-2217) so
+ * the boundary tables can be read; evaluate the advective residual (update_advection_BC +
+ * calc_RHS_advection or calc_RHS_advection_continuous, src/HO_2D.cpp:2219-2645) of w with a
+ * prescribed velocity uv (2*dofs, interleaved), boundary vorticity BC_vorticity and boundary
+ * normal velocity BC_normal_vel (both N_eb*K, edge flux-point order, the latter with the
+ * element-side sign update_BCs applies). */
+int ho_oracle_set_time(ho_oracle*, double t);
+int ho_oracle_update_bcs(ho_oracle*, double t);
+int ho_oracle_advection(ho_oracle*, const double* w, const double* uv, const double* bc_vort,
+                        const double* bc_nvel, double* out);
+
 #ifdef __cplusplus
 }
 #endif

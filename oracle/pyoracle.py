@@ -66,6 +66,9 @@ def lib():
         L.ho_oracle_sample_location.argtypes = [C.c_void_p, _ip, _dp]
         L.ho_oracle_sample_values.argtypes = [C.c_void_p, _dp]
         L.ho_oracle_refresh.argtypes = [C.c_void_p]
+        L.ho_oracle_set_time.argtypes = [C.c_void_p, C.c_double]
+        L.ho_oracle_update_bcs.argtypes = [C.c_void_p, C.c_double]
+        L.ho_oracle_advection.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, _dp]
         L.ho_oracle_set_threads.argtypes = [C.c_int]
         L.ho_oracle_set_threads.restype = C.c_int
         L.ho_oracle_set_threads(1)   # deterministic single-thread checker unless asked otherwise
@@ -154,6 +157,21 @@ class Oracle:
 
     def time(self):
         return lib().ho_oracle_time(self.h)
+
+    def set_time(self, t):
+        self._chk(lib().ho_oracle_set_time(self.h, float(t)))
+
+    def update_bcs(self, t):
+        """update_BCs(t) alone, so the boundary tables can be read."""
+        self._chk(lib().ho_oracle_update_bcs(self.h, float(t)))
+
+    def advection(self, w, uv, bc_vort, bc_nvel):
+        """Advective residual of w (the configured advection type) with a prescribed velocity uv
+        [dofs, 2] and boundary values BC_vorticity / BC_normal_vel ([n_bedges, K], edge order)."""
+        out = np.zeros(self.D)
+        f = lambda a: np.ascontiguousarray(a, dtype=np.float64).ravel()
+        self._chk(lib().ho_oracle_advection(self.h, f(w), f(uv), f(bc_vort), f(bc_nvel), out))
+        return out
 
     def step(self, n=1):
         self._chk(lib().ho_oracle_step(self.h, n))
