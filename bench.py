@@ -11,8 +11,9 @@ workload is then run in mixed precision (FP64 iterative refinement around an FP3
 Krylov + V-cycle) from the same initial state, and the line carries its throughput,
 speed-up and relative L2 field error against the FP64 run after the same steps
 (valid at <= 1e-8).  The cpu_baseline leg times the FP64 CPU oracle (the
-restatement of the reference algorithm, single thread) on a bounded sample of
-the same workload (a 64 x 64 cavity at the same K, Re, dt rule and tolerance).
+restatement of the reference algorithm) on a bounded sample of the same workload
+(a 64 x 64 cavity at the same K, Re, dt rule and tolerance), on 1 thread and on all
+host threads, median of 3 runs, with its Krylov iterations, the CPU model and nproc.
 """
 import argparse
 import json
@@ -51,10 +52,25 @@ def cpu_threads():
     return max(1, min(16, n))
 
 
-def cpu_baseline(K, Re, tol, n=64, warm=2, steps_1=6, steps_n=16):
-    """The FP64 oracle on a bounded sample of the workload (about 20 s of CPU work), on 1 thread
-    (the reference's default serial build; steps_1 timed steps) and on all host threads (OpenMP
-    linear algebra; the FR residual loops, < 2% of the oracle's step, stay serial; steps_n)."""
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    import platform
+    return platform.processor() or "unknown"
+
+
+def cpu_baseline(K, Re, tol, n=64, warm=2, runs=3, steps_1=3, steps_n=20, n_bench=512):
+    """The FP64 oracle on a bounded sample of the workload (about 30 s of CPU work): the same
+    cavity problem at n x n, on 1 thread (the reference's default serial build) and on all host
+    threads (OpenMP linear algebra; the FR residual loops, < 2% of the oracle's step, stay
+    serial).  After `warm` warm-up steps, `runs` consecutive timed runs of steps_1 / steps_n RK4
+    steps each; the median run is reported, with the oracle's Krylov iterations per solve (its
+    solver is the reference's: BiCGSTAB warm-started from the previous solution)."""
     import pyoracle
     from hoc import meshes
     from pyoracle import Oracle
@@ -69,20 +85,39 @@ def cpu_baseline(K, Re, tol, n=64, warm=2, steps_1=6, steps_n=16):
         o = Oracle(m, K, problem_type=1, Re=Re, dt=dt, time_integration_type=4, advection_type=1, tol=tol, maxiter=150)
         setup = time.perf_counter() - t0
         o.step(warm)
-        t0 = time.perf_counter()
-        o.step(steps)
-        el = time.perf_counter() - t0
-        res[nt] = (D * steps / el / 1e6, el, setup, steps)
+        vals, secs, its = [], [], []
+        for _ in range(runs):
+            c0 = o.solve_counts()
+            t0 = time.perf_counter()
+            o.step(steps)
+            el = time.perf_counter() - t0
+            c1 = o.solve_counts()
+            vals.append(D * steps / el / 1e6)
+            secs.append(el)
+            its.append((c1[1] - c0[1]) / max(1, c1[0] - c0[0]))
+        k = int(np.argsort(vals)[len(vals) // 2])
+        res[nt] = {"value": vals[k], "runs": vals, "timed_s": secs[k], "steps_per_run": steps,
+                   "krylov_iters_per_solve": its[k], "setup_s": setup}
+        log(f"cpu baseline, {nt} threads: runs {['%.4f' % v for v in vals]} Mdof-steps/s, "
+            f"{its[k]:.1f} Krylov its/solve")
     pyoracle.set_threads(1)
     nt = cpu_threads()
-    v1, el1, _, s1 = res[1]
-    vn, eln, setup, sn = res[nt]
-    return {"value": vn, "unit": "Mdof-steps/s", "cores": nt, "kind": "port",
+    r1, rn = res[1], res[nt]
+    try:
+        nproc = os.cpu_count()
+    except Exception:
+        nproc = None
+    return {"value": rn["value"], "unit": "Mdof-steps/s", "cores": nt, "kind": "port",
             "sample": f"oracle (FP64 CPU restatement of the reference algorithm; OpenMP SpMV/vector/dot in the "
-                      f"AMG-preconditioned Krylov solve) on a {n}x{n} cavity, K={K}, Re={Re}, dt={dt:.3g}, "
-                      f"tol={tol:g}: {warm} warm-up + {sn} timed RK4 steps ({eln:.1f} s timed on {nt} threads, "
-                      f"setup {setup:.1f} s)",
-            "single_thread": {"value": v1, "cores": 1, "timed_s": el1, "timed_steps": s1}}
+                      f"AMG-preconditioned Krylov solve) on a {n}x{n} cavity ({D} dofs, 1/{(n_bench // n) ** 2} of "
+                      f"the {n_bench}x{n_bench} workload), K={K}, Re={Re}, dt={dt:.3g}, tol={tol:g}: {warm} warm-up "
+                      f"steps, then the median of {runs} runs of {rn['steps_per_run']} RK4 steps on {nt} threads "
+                      f"({rn['krylov_iters_per_solve']:.1f} BiCGSTAB iterations per solve)",
+            "cpu_model": cpu_model(), "nproc": nproc, "threads_used": nt,
+            "runs": rn["runs"], "krylov_iters_per_solve": rn["krylov_iters_per_solve"], "setup_s": rn["setup_s"],
+            "single_thread": {"value": r1["value"], "cores": 1, "runs": r1["runs"], "timed_s": r1["timed_s"],
+                              "steps_per_run": r1["steps_per_run"],
+                              "krylov_iters_per_solve": r1["krylov_iters_per_solve"]}}
 
 
 def measured_traffic(kernel, n, K):
@@ -102,17 +137,20 @@ def measured_traffic(kernel, n, K):
     return None
 
 
-def roofline(prof, suffix=""):
+def roofline(prof, suffix="", noop=0):
     """Dominant kernel (largest total time among the fine-operator variants of one
     precision): algorithmic bytes per launch over its average launch duration, from HIP
-    events recorded on the library's stream around every launch (see run / profiled_step)."""
-    cand = {k: v for k, v in prof.items() if v["launches"] > 0 and k.endswith("_f32") == (suffix == "_f32")}
+    events recorded on the library's stream around sampled launches; launches predicated off
+    after convergence (they return at once) are left out by the library and counted in noop."""
+    cand = {k: v for k, v in prof.items()
+            if not k.startswith("_") and v["launches"] > 0 and k.endswith("_f32") == (suffix == "_f32")}
     name, v = max(cand.items(), key=lambda kv: kv[1]["total_ms"])
     avg_s = v["total_ms"] / max(1, v["launches"]) / 1e3
     gbs = v["bytes_per_launch"] / avg_s / 1e9
     return {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
             "traffic": None, "traffic_unit": "bytes per launch", "kernel": name, "avg_launch_us": avg_s * 1e6,
             "bytes_per_launch": v["bytes_per_launch"], "launches": v["launches"],
+            "noop_launches_excluded": noop,
             "all_variants": {k: {"avg_us": 1e3 * x["total_ms"] / max(1, x["launches"]),
                                  "GBs": x["bytes_per_launch"] / max(1e-12, x["total_ms"] / max(1, x["launches"]) / 1e3) / 1e9,
                                  "launches": x["launches"]} for k, x in cand.items()}}
@@ -146,6 +184,7 @@ def run(hoc, m, n, K, Re, dt, tol, precision, warmup, steps):
     g.step(steps)   # synchronizes the device at the end
     el = time.perf_counter() - t0
     prof = g.profile_read()
+    prof["_noop"] = g.profile_noop_launches
     g.profile(False)
     st = g.stats()
     ns = max(1, st["n_solves"] - s0["n_solves"])
@@ -213,8 +252,9 @@ def main():
         "roofline": None,
         "cpu_baseline": None,
     }
-    timing = f"HIP events on the library stream around every {PROFILE_EVERY}th launch of the timed steps"
-    out["roofline"] = roofline(prof, "" if primary == hoc.HOC_FP64 else "_f32")
+    timing = (f"HIP events on the library stream around every {PROFILE_EVERY}th launch of the timed steps; "
+              f"launches predicated off after convergence excluded")
+    out["roofline"] = roofline(prof, "" if primary == hoc.HOC_FP64 else "_f32", prof["_noop"])
     out["roofline"]["traffic"] = measured_traffic(out["roofline"]["kernel"], n, K)
     out["roofline"]["timing"] = timing
     if args.precision == "both":
@@ -233,7 +273,7 @@ def main():
                         "field_error_vs_fp64": err, "valid_at_1e-8": worst <= 1e-8,
                         "krylov_iters_per_solve": res2["krylov_iters_per_solve"],
                         "refinements_last_solve": res2["last_refinements"],
-                        "roofline": dict(roofline(prof2, "_f32")),
+                        "roofline": dict(roofline(prof2, "_f32", prof2["_noop"])),
                         "dtype": "f64 outer residual + f32 Krylov/V-cycle"}
         out["mixed"]["roofline"]["traffic"] = measured_traffic(out["mixed"]["roofline"]["kernel"], n, K)
         out["mixed"]["roofline"]["timing"] = timing

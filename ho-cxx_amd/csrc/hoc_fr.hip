@@ -312,8 +312,11 @@ __global__ __launch_bounds__(kWG, (AFF && sizeof(R) == 4 && K <= 5) ? 4 : 1) voi
                                               R* __restrict__ y, const R* __restrict__ r,
                                               const R* __restrict__ dinv, R* __restrict__ d,
                                               R c1, R c2, double* __restrict__ part,
-                                              const int* __restrict__ active, R* __restrict__ tr_out) {
-  if (active && !*active) return;
+                                              const int* __restrict__ active, R* __restrict__ tr_out,
+                                              int* __restrict__ ran) {
+  const bool go = !active || *active;
+  if (ran && blockIdx.x == 0 && threadIdx.x == 0) *ran = go ? 1 : 0;  // timed launches: did it work?
+  if (!go) return;
   constexpr int K2 = K * K;
   __shared__ __align__(16) R lds[kWG * K2];
   const int e0 = blockIdx.x * kWG;
@@ -1015,15 +1018,15 @@ static hipError_t launch_trace_t(int K, int pre, int N, const TabT<R>& T, R* x, 
 template <class R>
 static hipError_t launch_lap_t(int K, int epi, int N, const TabT<R>& T, const GeoT<R>& G, R* x, const R* tr, R* y,
                                const R* r, const R* diag, R* d, double c1, double c2, double* part, const int* active,
-                               hipStream_t s, R* tr_out) {
+                               hipStream_t s, R* tr_out, int* ran) {
 #define LL(E)                                                                                                        \
   do {                                                                                                               \
     if (G.gel)                                                                                                       \
       hipLaunchKernelGGL((k_lap<KK, E, true, R>), dim3(nblk(N)), dim3(kWG), 0, s, N, T, G, x, tr, y, r, diag, d,      \
-                         (R)c1, (R)c2, part, active, tr_out);                                                        \
+                         (R)c1, (R)c2, part, active, tr_out, ran);                                                   \
     else                                                                                                             \
       hipLaunchKernelGGL((k_lap<KK, E, false, R>), dim3(nblk(N)), dim3(kWG), 0, s, N, T, G, x, tr, y, r, diag, d,     \
-                         (R)c1, (R)c2, part, active, tr_out);                                                        \
+                         (R)c1, (R)c2, part, active, tr_out, ran);                                                   \
   } while (0)
   HOC_K_SWITCH(K, {
     switch (epi) {
@@ -1051,13 +1054,13 @@ hipError_t launch_trace(int K, int pre, int N, const TabT<float>& T, float* x, c
 }
 hipError_t launch_lap(int K, int epi, int N, const TabT<double>& T, const GeoT<double>& G, double* x, const double* tr,
                       double* y, const double* r, const double* diag, double* d, double c1, double c2, double* part,
-                      const int* active, hipStream_t s, double* tr_out) {
-  return launch_lap_t<double>(K, epi, N, T, G, x, tr, y, r, diag, d, c1, c2, part, active, s, tr_out);
+                      const int* active, hipStream_t s, double* tr_out, int* ran) {
+  return launch_lap_t<double>(K, epi, N, T, G, x, tr, y, r, diag, d, c1, c2, part, active, s, tr_out, ran);
 }
 hipError_t launch_lap(int K, int epi, int N, const TabT<float>& T, const GeoT<float>& G, float* x, const float* tr,
                       float* y, const float* r, const float* diag, float* d, double c1, double c2, double* part,
-                      const int* active, hipStream_t s, float* tr_out) {
-  return launch_lap_t<float>(K, epi, N, T, G, x, tr, y, r, diag, d, c1, c2, part, active, s, tr_out);
+                      const int* active, hipStream_t s, float* tr_out, int* ran) {
+  return launch_lap_t<float>(K, epi, N, T, G, x, tr, y, r, diag, d, c1, c2, part, active, s, tr_out, ran);
 }
 
 hipError_t launch_rhs(int K, int N, const Tab& T, const Geo& G, const double* omega, const double* bcpsi, double* bs,

@@ -40,13 +40,14 @@ hipError_t launch_trace(int K, int pre, int N, const TabT<float>& T, float* x, c
                         float* d, double coef, const double* scal, int scal_idx, const int* active, float* tr,
                         hipStream_t s, const int32_t* ev = nullptr);
 // tr: traces of x (input); tr_out (EPI_CHEB/EPI_CHEBF only, may be null): traces of the
-// updated x, written for the next application.  tr and tr_out must differ.
+// updated x, written for the next application.  tr and tr_out must differ.  ran (may be null):
+// set to 1 when the launch worked, 0 when the active flag made it return at once.
 hipError_t launch_lap(int K, int epi, int N, const TabT<double>& T, const GeoT<double>& G, double* x, const double* tr,
                       double* y, const double* r, const double* diag, double* d, double c1, double c2, double* part,
-                      const int* active, hipStream_t s, double* tr_out = nullptr);
+                      const int* active, hipStream_t s, double* tr_out = nullptr, int* ran = nullptr);
 hipError_t launch_lap(int K, int epi, int N, const TabT<float>& T, const GeoT<float>& G, float* x, const float* tr,
                       float* y, const float* r, const float* diag, float* d, double c1, double c2, double* part,
-                      const int* active, hipStream_t s, float* tr_out = nullptr);
+                      const int* active, hipStream_t s, float* tr_out = nullptr, int* ran = nullptr);
 hipError_t launch_rhs(int K, int N, const Tab& T, const Geo& G, const double* omega, const double* bcpsi, double* bs,
                       double* part, hipStream_t s);
 hipError_t launch_velocity(int K, int N, const Tab& T, const Geo& G, const BCDev& B, const double* psi,

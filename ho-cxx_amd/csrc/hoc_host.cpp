@@ -2,6 +2,7 @@
 #include "hoc_host.hpp"
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdlib>
 #include <fstream>
@@ -713,6 +714,7 @@ void read_gmsh41(const std::string& path, int occ, Mesh& m) {
   i = L.find("$Nodes") + 1;
   const auto nh = ints(at(i++));
   if (nh.size() < 4) throw std::runtime_error("gmsh: bad $Nodes header");
+  if (nh[0] < 0 || nh[1] < 0 || nh[1] > INT32_MAX / 2) throw std::runtime_error("gmsh: bad $Nodes counts");
   std::unordered_map<long long, int32_t> tag2idx;
   m.xy.assign(2 * (size_t)nh[1], 0.);
   int32_t c = 0;
@@ -720,6 +722,7 @@ void read_gmsh41(const std::string& path, int occ, Mesh& m) {
     const auto bh = ints(at(i++));
     if (bh.size() < 4) throw std::runtime_error("gmsh: bad node block");
     const long long nb = bh[3];
+    if (nb < 0 || c + nb > nh[1]) throw std::runtime_error("gmsh: node block exceeds the $Nodes count");
     for (long long k = 0; k < nb; ++k) tag2idx[ints(at(i + k)).at(0)] = c + (int32_t)k;
     i += nb;
     for (long long k = 0; k < nb; ++k) {
@@ -745,6 +748,7 @@ void read_gmsh41(const std::string& path, int occ, Mesh& m) {
     const auto bh = ints(at(i++));
     if (bh.size() < 4) throw std::runtime_error("gmsh: bad element block");
     const long long dim = bh[0], etag = bh[1], etype = bh[2], nb = bh[3];
+    if (nb < 0) throw std::runtime_error("gmsh: negative element count");
     if (dim == 1) {
       const int en = etype == 1 ? 2 : etype == 8 ? 3 : etype == 26 ? 4 : 0;
       if (!en) throw std::runtime_error("gmsh: unsupported line element type " + std::to_string(etype));

@@ -135,6 +135,8 @@ struct Solver {
     std::vector<hipEvent_t> e0, e1;
     std::vector<int> var;
     std::vector<double> nbytes;
+    DBuf<int> ran;        // per recorded launch: 1 when it worked, 0 when it was a predicated no-op
+    int64_t skipped = 0;  // recorded launches left out as no-ops
     int n = 0;
     double ms[HOC_PROF_VARIANTS] = {0};
     double bytes[HOC_PROF_VARIANTS] = {0};
@@ -305,6 +307,7 @@ struct Solver {
     for (auto& b : rkQ) b.alloc(guess_mode() >= 4 ? D : 0);
     proj_fwd.X.alloc(0); proj_fwd.AX.alloc(0); proj_fwd.m = 0;
     proj_adj.X.alloc(0); proj_adj.AX.alloc(0); proj_adj.m = 0;
+    reset_adjoint();
     invalidate_guess();
     time = 0.;
     smooth_ready = false;
@@ -320,7 +323,8 @@ struct Solver {
            const int* active, const R* tin = nullptr, R* tout = nullptr) {
     const bool rec = prof.on && (prof.seen++ % prof.every) == 0;
     if (rec) HIPCHK(hipEventRecord(prof.e0[prof.n], stream));
-    HIPCHK(launch_lap(K, epi, N, P.T, P.G, x, tin ? tin : P.tr.p, y, rr, dg, d, c1, c2, partp, active, stream, tout));
+    HIPCHK(launch_lap(K, epi, N, P.T, P.G, x, tin ? tin : P.tr.p, y, rr, dg, d, c1, c2, partp, active, stream, tout,
+                      rec ? prof.ran.p + prof.n : nullptr));
     if (rec) {
       HIPCHK(hipEventRecord(prof.e1[prof.n], stream));
       prof.var[prof.n] = P.prof_off + epi;
@@ -328,10 +332,15 @@ struct Solver {
       if (++prof.n == (int)prof.e0.size()) prof_flush();
     }
   }
+  // Only launches that worked count: a launch predicated off after convergence returns at once
+  // and would book its algorithmic bytes against a near-zero duration.
   void prof_flush() {
     if (prof.n == 0) return;
     HIPCHK(hipEventSynchronize(prof.e1[prof.n - 1]));
+    std::vector<int> ran(prof.n);
+    HIPCHK(hipMemcpy(ran.data(), prof.ran.p, prof.n * sizeof(int), hipMemcpyDeviceToHost));
     for (int i = 0; i < prof.n; ++i) {
+      if (!ran[i]) { ++prof.skipped; continue; }
       float ms = 0.f;
       HIPCHK(hipEventElapsedTime(&ms, prof.e0[i], prof.e1[i]));
       prof.ms[prof.var[i]] += ms;
@@ -820,6 +829,14 @@ struct Solver {
     dcopy(P.AX.p + (size_t)j * D, ax, D);
     P.m = j + 1;
   }
+  // The adjoint's colouring, velocity map, buffers and recorded trajectory belong to one mesh and
+  // one set-up: a new mesh or set-up rebuilds them on the next adjoint call.
+  void reset_adjoint() {
+    adj.init = false;
+    adj.have_forward = false;
+    adj.zn = 0;
+    proj_adj.m = 0;
+  }
   // Called when the state is set from outside: the histories restart, so that the same state gives
   // the same iterations (the projection basis would stay valid, holding solutions of As x = b only).
   void invalidate_guess() {
@@ -1021,6 +1038,7 @@ struct Solver {
     mesh = std::move(m);
     mesh_loaded = true;
     ready = false;
+    reset_adjoint();
     n_inst = 1;
     inst_off.clear(); inst_re.clear(); inst_dt.clear(); inst_time.clear();
     edge_nvel_in.clear(); edge_has_nvel.clear();
@@ -1528,6 +1546,7 @@ struct Solver {
       dcopy(psi, adj.save_psi, D);
       dcopy(vel, adj.save_vel, 2 * D);
       time = time_end_;
+      invalidate_guess();   // the recompute left the histories of the trajectory's early steps
     };
     try {
       for (int sg = nseg - 1; sg >= 0; --sg) {
@@ -1997,7 +2016,12 @@ int hoc_write_vtk(hoc_t h, const char* path) {
 }
 
 double hoc_get_time(hoc_t h) { return h ? h->s.time : 0.; }
-int hoc_set_time(hoc_t h, double t) { return guarded(h, [&] { h->s.time = t; }); }
+int hoc_set_time(hoc_t h, double t) {
+  return guarded(h, [&] {
+    h->s.time = t;
+    std::fill(h->s.inst_time.begin(), h->s.inst_time.end(), t);
+  });
+}
 
 int hoc_step(hoc_t h, int32_t nsteps) {
   return guarded(h, [&] {
@@ -2150,8 +2174,12 @@ int hoc_profile_enable(hoc_t h, int32_t on) {
       const int cap = 4096;
       s.prof.e0.resize(cap); s.prof.e1.resize(cap); s.prof.var.resize(cap); s.prof.nbytes.resize(cap);
       for (int i = 0; i < cap; ++i) { HIPCHK(hipEventCreate(&s.prof.e0[i])); HIPCHK(hipEventCreate(&s.prof.e1[i])); }
+      s.prof.ran.alloc(cap);
     }
-    if (on) for (int v = 0; v < HOC_PROF_VARIANTS; ++v) { s.prof.ms[v] = 0.; s.prof.cnt[v] = 0; s.prof.bytes[v] = 0.; }
+    if (on) {
+      for (int v = 0; v < HOC_PROF_VARIANTS; ++v) { s.prof.ms[v] = 0.; s.prof.cnt[v] = 0; s.prof.bytes[v] = 0.; }
+      s.prof.skipped = 0;
+    }
     s.prof.on = on != 0;
     s.prof.every = on > 1 ? on : 1;
     s.prof.seen = 0;
@@ -2175,6 +2203,7 @@ int hoc_profile_get(hoc_t h, hoc_kernel_profile* p) {
                                                  : (v < half ? s.lap_bytes(v, 8.) : s.lap_bytes(v - half, 4.));
       std::strncpy(p->names[v], names[v], sizeof(p->names[v]) - 1);
     }
+    p->noop_launches = s.prof.skipped;
   });
 }
 

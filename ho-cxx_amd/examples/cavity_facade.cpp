@@ -1,6 +1,9 @@
 // cavity_facade.cpp - drives the GPU path through the HO_2D-shaped C++ facade the way the
 // reference's hybrid driver drives HO_2D: arrays in, process_mesh_input, solveto_d,
-// getallvorts_d.  Usage: cavity_facade n K nsteps dt out.bin
+// getallvorts_d.  Usage: cavity_facade n K nsteps dt [out.bin [sample_points.dat]]
+// out.bin holds, as doubles: vorticity (D), stream function (D), velocity (2D), solution points
+// (2D), solution areas (D), hoquad weights (K*K).  With a sample-point file, trigger_write(0) also
+// writes the reference's output files into the working directory.
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -14,6 +17,7 @@ int main(int argc, char** argv) {
   const int nsteps = argc > 3 ? std::atoi(argv[3]) : 5;
   const double dt = argc > 4 ? std::atof(argv[4]) : 1e-4;
   const char* out = argc > 5 ? argv[5] : nullptr;
+  const char* samples = argc > 6 ? argv[6] : nullptr;
   // unit-square n x n bilinear quads: nodes row-major, elements CCW from the SW corner
   std::vector<double> xy;
   for (int j = 0; j <= n; ++j)
@@ -37,15 +41,24 @@ int main(int argc, char** argv) {
                           wall.data(), 0, nullptr);
     ho.add_boundary("top", (int32_t)top.size(), top.data());
     ho.process_mesh_input();
+    if (samples) ho.read_process_sample_points(samples);
     ho.solveto_d(nsteps * dt, nsteps, 4, 100.0);
-    std::vector<double> w((size_t)n * n * K * K);
-    ho.getallvorts_d((int32_t)w.size(), w.data());
+    const size_t D = (size_t)n * n * K * K;
+    std::vector<double> w(D), psi(D), uv(2 * D), xyp(2 * D), area(D), hq((size_t)K * K);
+    ho.getallvorts_d((int32_t)D, w.data());
+    ho.getstreamfunction_d((int32_t)D, psi.data());
+    ho.getvelocity_d((int32_t)(2 * D), uv.data());
+    ho.getsolnpts_d(ho.getsolnptlen(), xyp.data());
+    ho.getsolnareas_d((int32_t)D, area.data());
+    ho.get_hoquad_weights_d(K * K, hq.data());
     double s = 0.;
     for (double v : w) s += v * v;
     std::printf("time %.6g |omega| %.12e\n", ho.current_time(), std::sqrt(s));
+    if (samples) ho.trigger_write(0);
     if (out) {
       std::FILE* f = std::fopen(out, "wb");
-      std::fwrite(w.data(), sizeof(double), w.size(), f);
+      if (!f) { std::printf("cannot open %s\n", out); return 2; }
+      for (const auto* v : {&w, &psi, &uv, &xyp, &area, &hq}) std::fwrite(v->data(), sizeof(double), v->size(), f);
       std::fclose(f);
     }
   } catch (const hoc_gpu_error& e) {

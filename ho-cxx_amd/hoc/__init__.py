@@ -63,7 +63,8 @@ class hoc_stats(C.Structure):
 
 class hoc_kernel_profile(C.Structure):
     _fields_ = [("launches", C.c_int64 * PROF_VARIANTS), ("total_ms", C.c_double * PROF_VARIANTS),
-                ("bytes_per_launch", C.c_double * PROF_VARIANTS), ("names", (C.c_char * 24) * PROF_VARIANTS)]
+                ("bytes_per_launch", C.c_double * PROF_VARIANTS), ("names", (C.c_char * 24) * PROF_VARIANTS),
+                ("noop_launches", C.c_int64)]
 
 
 class HocError(RuntimeError):
@@ -325,6 +326,10 @@ class HO2D:
     def time(self):
         return lib().hoc_get_time(self.h)
 
+    def set_time(self, t):
+        """Set the clock (every instance's clock for an ensemble)."""
+        self._chk(lib().hoc_set_time(self.h, float(t)))
+
     def step(self, n=1):
         self._chk(lib().hoc_step(self.h, n))
 
@@ -406,6 +411,7 @@ class HO2D:
         for v in range(PROF_VARIANTS):
             out[p.names[v].value.decode()] = {"launches": p.launches[v], "total_ms": p.total_ms[v],
                                               "bytes_per_launch": p.bytes_per_launch[v]}
+        self.profile_noop_launches = int(p.noop_launches)
         return out
 
 

@@ -173,6 +173,9 @@ int hoc_get_sample_location(hoc_t h, int32_t n, int32_t* elem, double* local_xy)
 int hoc_get_sample_values(hoc_t h, int32_t refresh, int32_t n, double* out);
 /* The same values as a text file of lines "x y vorticity u v" (src/HO_2D.cpp:3593-3612). */
 int hoc_write_samples(hoc_t h, int32_t refresh, const char* path);
+/* The clock (current_time, src/HO_2D.cpp:1406-1418) that time-dependent boundary values follow
+ * (the inflow ramps of problem types 2 and 3, src/HO_2D.cpp:2014, 2062).  Setting it also sets
+ * every ensemble instance's clock. */
 double hoc_get_time(hoc_t h);
 int hoc_set_time(hoc_t h, double t);
 
@@ -238,7 +241,8 @@ int hoc_stage_residual_vjp(hoc_t h, const double* omega, double t, const double*
 int hoc_stats_get(hoc_t h, hoc_stats* s);
 
 /* Live timing of the fine-level Poisson operator kernel (the dominant kernel): with
- * profiling on, launches are bracketed by HIP events recorded on the handle's own stream
+ * profiling on, launches are bracketed by HIP events recorded on the handle's own stream, and
+ * only launches that did the work are counted
  * (every launch for on == 1, every on-th launch for on > 1: each event pair costs a little
  * device time, so sampling keeps a timed run's throughput intact).  One entry per fused
  * epilogue variant: entries 0-4 FP64, 5-9 FP32. */
@@ -248,6 +252,8 @@ typedef struct {
   double total_ms[HOC_PROF_VARIANTS];
   double bytes_per_launch[HOC_PROF_VARIANTS];  /* algorithmic HBM bytes per launch (average) */
   char names[HOC_PROF_VARIANTS][24];
+  int64_t noop_launches;   /* sampled launches left out: predicated off (the iteration after
+                              convergence), they returned at once without doing the work */
 } hoc_kernel_profile;
 int hoc_profile_enable(hoc_t h, int32_t on);   /* on: 0 off, 1 every launch, n > 1 every n-th;
                                                   on != 0 also clears the counters */

@@ -654,6 +654,7 @@ struct ho_oracle {
   AMG amg;
   int last_iters = 0;
   double last_rel = 0.;
+  int64_t n_solves = 0, total_iters = 0;
   std::string err;
 
   int K2() const { return K * K; }
@@ -1552,6 +1553,8 @@ struct ho_oracle {
       last_iters = bicgstab(A, amg, RHS, x, tol, maxiter, last_rel);
     }
     stream_function = x;
+    ++n_solves;
+    total_iters += last_iters;
   }
 
   // --- src/HO_2D.cpp:2833-2870 ----------------------------------------------------------------
@@ -2427,6 +2430,11 @@ int ho_oracle_poisson(ho_oracle* h, const double* w, double t, double* psi) {
 }
 
 int ho_oracle_last_solve(ho_oracle* h, int* iters, double* rel) { *iters = h->last_iters; *rel = h->last_rel; return 0; }
+int ho_oracle_solve_counts(ho_oracle* h, int64_t* n_solves, int64_t* total_iters) {
+  *n_solves = h->n_solves;
+  *total_iters = h->total_iters;
+  return 0;
+}
 
 int64_t ho_oracle_nnz(ho_oracle* h) { return (int64_t)h->A.val.size(); }
 int ho_oracle_get_csr(ho_oracle* h, int32_t* rowptr, int32_t* col, double* val) {

@@ -85,6 +85,8 @@ int ho_oracle_stage(ho_oracle*, const double* w, double t, double* k_out);
 int ho_oracle_poisson(ho_oracle*, const double* w, double t, double* psi_out);
 /* Last Poisson solve statistics. */
 int ho_oracle_last_solve(ho_oracle*, int* iters, double* rel_residual);
+/* Poisson solves so far and their summed Krylov iterations. */
+int ho_oracle_solve_counts(ho_oracle*, int64_t* n_solves, int64_t* total_iters);
 
 /* Laplacian in CSR (after the |v|>1e-14 filter of src/HO_2D.cpp:3029,3043). */
 int64_t ho_oracle_nnz(ho_oracle*);

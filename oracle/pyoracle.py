@@ -42,6 +42,7 @@ def lib():
         L.ho_oracle_stage.argtypes = [C.c_void_p, _dp, C.c_double, _dp]
         L.ho_oracle_poisson.argtypes = [C.c_void_p, _dp, C.c_double, _dp]
         L.ho_oracle_last_solve.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_double)]
+        L.ho_oracle_solve_counts.argtypes = [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
         L.ho_oracle_nnz.argtypes = [C.c_void_p]
         L.ho_oracle_nnz.restype = C.c_int64
         L.ho_oracle_get_csr.argtypes = [C.c_void_p, _ip, _ip, _dp]
@@ -190,6 +191,12 @@ class Oracle:
         it, rel = C.c_int(), C.c_double()
         lib().ho_oracle_last_solve(self.h, C.byref(it), C.byref(rel))
         return it.value, rel.value
+
+    def solve_counts(self):
+        """(Poisson solves so far, their summed Krylov iterations)."""
+        n, it = C.c_int64(), C.c_int64()
+        lib().ho_oracle_solve_counts(self.h, C.byref(n), C.byref(it))
+        return n.value, it.value
 
     def csr(self):
         L = lib()

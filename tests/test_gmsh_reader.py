@@ -134,3 +134,22 @@ def test_reference_meshes_match_fixtures(dump_exe, case, fname, occ):
     assert np.array_equal(got.elem, z["elem"])
     assert np.array_equal(got.bedges, z["bedges"]) and np.array_equal(got.bedge_bdry, z["bedge_bdry"])
     assert got.bnames == [str(s) for s in z["bnames"]]
+
+
+@pytest.mark.parametrize("block_count", [7, -1])
+def test_malformed_node_blocks_rejected(dump_exe, tmp_path, block_count):
+    """A node block larger than the $Nodes header's count (or negative) is an error, not a write
+    past the coordinate array.  The file is a valid one with its first node block's count edited."""
+    m = meshes.cavity(2, 2)
+    good = tmp_path / "g.msh"
+    write_gmsh41(m, str(good))
+    lines = good.read_text().splitlines()
+    k = lines.index("$Nodes")
+    n = m.n_nodes
+    assert lines[k + 1] == f"1 {n} 10 {n + 9}" and lines[k + 2] == f"2 1 0 {n}"
+    lines[k + 1] = f"1 {n - 2} 10 {n + 9}" if block_count > 0 else lines[k + 1]
+    lines[k + 2] = f"2 1 0 {n if block_count > 0 else block_count}"
+    bad = tmp_path / "b.msh"
+    bad.write_text("\n".join(lines) + "\n")
+    with pytest.raises(ValueError, match="node block exceeds"):   # checked before any write
+        read_cpp(dump_exe, str(bad), 0)

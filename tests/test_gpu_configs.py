@@ -71,6 +71,7 @@ def test_bfs_full_mixed_vs_fp64(gpu_available, bfs_full):
         g = hoc.HO2D(precision=prec, **kw)
         g.load_mesh(bfs_full)
         g.setup()
+        g.set_time(20.0 / 3.0)         # inflow ramp 2|sin(pi t/40)| = 1: the full parabolic inflow
         g.step(20)
         out[prec] = (g.get_vorticity(), g.get_streamfunction(), g.solution_areas())
     (w64, p64, a), (w32, p32, _) = out[hoc.HOC_FP64], out[hoc.HOC_MIXED]

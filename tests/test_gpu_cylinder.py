@@ -13,6 +13,11 @@ from conftest import rel_l2
 pytestmark = pytest.mark.gpu
 
 
+# Start where the inflow ramp 2|sin(pi t/40)| of problem_type 3 (src/HO_2D.cpp:2062) is 1: the
+# uniform inflow u = 1 at full strength.
+T0 = 20.0 / 3.0
+
+
 def omega0(xy, seed=3):
     """Seeded vorticity perturbation of amplitude 1e-3 at the solution points."""
     return 1e-3 * np.random.default_rng(seed).uniform(-1.0, 1.0, len(xy))
@@ -29,8 +34,11 @@ def test_ogrid_K6_vs_oracle(gpu_available, precision):
     w0 = omega0(g.solution_points())
     o.set_vorticity(w0)
     g.set_vorticity(w0)
+    o.set_time(T0)
+    g.set_time(T0)
     o.step(2)
     g.step(2)
+    assert abs(g.time - o.time()) < 1e-14
     ew = rel_l2(g.get_vorticity(), o.vorticity())
     ep = rel_l2(g.get_streamfunction(), o.streamfunction())
     st = g.stats()
@@ -50,6 +58,7 @@ def test_ogrid_full_mixed_vs_fp64(gpu_available):
         g.load_mesh(m)
         g.setup()
         g.set_vorticity(omega0(g.solution_points()))
+        g.set_time(T0)
         g.step(3)
         out[prec] = (g.get_vorticity(), g.get_streamfunction(), g.stats())
         g.close()

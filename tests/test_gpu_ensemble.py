@@ -11,13 +11,15 @@ from conftest import rel_l2
 pytestmark = pytest.mark.gpu
 
 
-def run_ensemble(parts, K, Re, dt, steps, problem_type=1, precision=hoc.HOC_FP64, tol=1e-13):
+def run_ensemble(parts, K, Re, dt, steps, problem_type=1, precision=hoc.HOC_FP64, tol=1e-13, t0=0.0):
     m, off = meshes.concat(parts)
     g = hoc.HO2D(K=K, problem_type=problem_type, Re=Re[0], dt=dt[0], time_integration_type=4, advection_type=1,
                  tol=tol, maxiter=800, precision=precision)
     g.load_mesh(m)
     g.set_instances(off, Re, dt)
     g.setup()
+    if t0:
+        g.set_time(t0)
     g.step(steps)
     return g, off
 
@@ -42,18 +44,22 @@ def test_uneven_cavity_ensemble_vs_oracle(gpu_available):
 
 
 def test_ensemble_with_ramped_channels(gpu_available):
-    """Backward-facing steps with time-dependent inflow: each instance ramps on its own clock."""
+    """Backward-facing steps with time-dependent inflow: each instance ramps on its own clock,
+    from t = 20/3 where the ramp 2|sin(pi t/40)| (src/HO_2D.cpp:2014) is 1."""
     parts = [meshes.backward_facing_step(16, 6, 2, jitter=0.05, seed=1),
              meshes.backward_facing_step(20, 8, 2, jitter=0.05, seed=2)]
     K = 3
     Re = np.array([200.0, 400.0])
     dt = np.array([5e-3, 2e-3])
     steps = 3
-    g, off = run_ensemble(parts, K, Re, dt, steps, problem_type=2)
+    t0 = 20.0 / 3.0
+    g, off = run_ensemble(parts, K, Re, dt, steps, problem_type=2, t0=t0)
+    assert np.allclose(g.instance_times(), t0 + steps * dt, rtol=1e-14, atol=0)
     w = g.get_vorticity()
     for i, part in enumerate(parts):
         o = Oracle(part, K, problem_type=2, Re=Re[i], dt=dt[i], time_integration_type=4, advection_type=1,
                    tol=1e-13, maxiter=800)
+        o.set_time(t0)
         o.step(steps)
         sl = slice(off[i] * K * K, off[i + 1] * K * K)
         ew = rel_l2(w[sl], o.vorticity())

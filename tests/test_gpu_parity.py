@@ -92,9 +92,17 @@ def test_bfs_poisson_stage(gpu_available, K, adv):
     assert rel_l2(k_got, k_ref) < 1e-10
 
 
-def test_bfs_steps(gpu_available):
+# Time at which the inflow ramp 2|sin(pi t/40)| of problem types 2 and 3 (src/HO_2D.cpp:2014,
+# 2062) is 1: multi-step runs of those problems start here, with the inflow at full strength.
+T_FULL_INFLOW = 20.0 / 3.0
+
+
+@pytest.mark.parametrize("adv", [1, 2])
+def test_bfs_steps(gpu_available, adv):
     m = bfs_small()
-    o, g = make_pair(m, 3, problem_type=2, Re=400.0, dt=5e-4, adv=1)
+    o, g = make_pair(m, 3, problem_type=2, Re=400.0, dt=5e-4, adv=adv)
+    o.set_time(T_FULL_INFLOW)
+    g.set_time(T_FULL_INFLOW)
     o.step(6)
     g.step(6)
     ew = rel_l2(g.get_vorticity(), o.vorticity())
@@ -132,6 +140,9 @@ def shipped(case):
 def test_shipped_mesh_steps(gpu_available, case, ptype, Re, dt):
     m = shipped(case)
     o, g = make_pair(m, 3, problem_type=ptype, Re=Re, dt=dt, tol=1e-14, maxiter=800)
+    if ptype in (2, 3):
+        o.set_time(T_FULL_INFLOW)
+        g.set_time(T_FULL_INFLOW)
     o.step(4)
     g.step(4)
     ew = rel_l2(g.get_vorticity(), o.vorticity())
