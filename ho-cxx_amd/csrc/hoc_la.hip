@@ -319,6 +319,15 @@ __global__ void k_dense(int n, const R* __restrict__ Ainv, const R* __restrict__
   if (lane == 0) x[row] = s;
 }
 
+// y = -y / (w_j w_i) at every solution point (el K^2 + j K + i): As = -W A turned back into A.
+struct Weights6 { double w[6]; };
+__global__ void k_unweight(size_t n, int K, Weights6 W, double* __restrict__ y) {
+  const size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n) return;
+  const int m = (int)(v % (size_t)(K * K));
+  y[v] = -y[v] / (W.w[m / K] * W.w[m % K]);
+}
+
 // ---- probing ---------------------------------------------------------------------------------
 // x[el][m] = (color[el] == c && (m == rs || rs < 0)) ? 1 : 0
 __global__ void k_probe_set(int N, int K2, const int32_t* __restrict__ color, int c, int rs, double* __restrict__ x) {
@@ -583,6 +592,11 @@ void mixed_promote_add(hipStream_t s, size_t n, double* x64, const float* e32, c
 }
 void inst_norms(hipStream_t s, const VecArgs& A, int n_inst, const int32_t* off, int K2, const double* b, double* out) {
   hipLaunchKernelGGL(k_inst_norms, dim3(n_inst), dim3(256), 0, s, A, off, K2, b, out);
+}
+void unweight(hipStream_t s, size_t n, int K, const double* w, double* y) {
+  Weights6 W{};
+  for (int i = 0; i < K && i < 6; ++i) W.w[i] = w[i];
+  if (n) hipLaunchKernelGGL(k_unweight, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, K, W, y);
 }
 void probe_set(hipStream_t s, int N, int K2, const int32_t* color, int c, int rs, double* x) {
   size_t n = (size_t)N * K2;

@@ -91,6 +91,9 @@ void psi_combine(hipStream_t s, size_t n, int nterm, const double* a, const doub
 // per element), one workgroup per instance (deterministic order).
 void inst_norms(hipStream_t s, const VecArgs& A, int n_inst, const int32_t* off, int K2, const double* b, double* out);
 
+// y = -y / (w[j] w[i]) at every solution point (v = el K^2 + j K + i); w: K host values (K <= 6).
+void unweight(hipStream_t s, size_t n, int K, const double* w, double* y);
+
 // Setup-time probing (FP64).
 void probe_set(hipStream_t s, int N, int K2, const int32_t* color, int c, int rs, double* x);
 void probe_diag(hipStream_t s, int N, int K2, const int32_t* color, int c, int rs, const double* y, double* diag);

@@ -17,6 +17,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -30,6 +31,34 @@
 #include "hoc_out.hpp"
 
 namespace hoc {
+
+// ---- scratch of the no-allocation operator apply (hoc_apply_laplacian_device) ---------------
+// The face traces of that entry point live in this library-wide array instead of a buffer of
+// the handle: every GPU the library runs on holds one copy of kNoAllocScratchBytes until the
+// process ends, shared by every handle on that GPU (calls that overlap on one GPU race on it).
+// Host code never passes the array's name to a kernel: it uses the per-device address that
+// noalloc_scratch_init() takes from hipGetSymbolAddress.
+constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) double scratch[kNoAllocScratchBytes / sizeof(double)];
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static double* g_scratch_ptr[kMaxDevices] = {nullptr};
+
+// Once per device, with that device current: the device address of `scratch`.
+static void noalloc_scratch_init(int dev) {
+  if (dev < 0 || dev >= kMaxDevices) throw StatusError(HOC_EINVAL, "device ordinal out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[dev]) return;
+  void* p = nullptr;
+  HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  if (!p) throw HipError("hipGetSymbolAddress returned a null address for the scratch array");
+  g_scratch_ptr[dev] = static_cast<double*>(p);
+}
+static double* noalloc_scratch(int dev) {
+  if (dev < 0 || dev >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  return g_scratch_ptr[dev];
+}
 
 // Preconditioned-Krylov state in working precision R: the fine operator's tables and
 // metrics, the Krylov and smoother vectors, face traces and the V-cycle hierarchy.
@@ -177,6 +206,7 @@ struct Solver {
     auto t0 = std::chrono::steady_clock::now();
     HIPCHK(hipSetDevice(device));
     if (!stream) HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    noalloc_scratch_init(device);
     connect(mesh);
     build_tables(cfg.K, mesh.lnod, tab);
     build_geometry(mesh, tab, geo);
@@ -2143,6 +2173,30 @@ int hoc_apply_laplacian(hoc_t h, const double* x, double* y) {
       const int m = (int)(v % s.K2);
       y[v] = -hy[v] / (s.tab.wgt[m / s.K] * s.tab.wgt[m % s.K]);
     }
+  });
+}
+
+int hoc_apply_laplacian_device(hoc_t h, const double* x, double* y, void* stream) {
+  return guarded(h, [&] {
+    NEED_READY(h);
+    Solver& s = h->s;
+    if (!x || !y) throw hoc::StatusError(HOC_EINVAL, "null buffer");
+    if (x == y) throw hoc::StatusError(HOC_EINVAL, "x and y must not alias");
+    const size_t need = (size_t)s.N * 8 * s.K * sizeof(double);  // face traces: 4 faces x (value, derivative) x K
+    if (need > hoc::kNoAllocScratchBytes)
+      throw hoc::StatusError(HOC_EINVAL, "mesh too large for the no-allocation apply: its face traces need " +
+                                             std::to_string(need) + " bytes, the scratch array holds " +
+                                             std::to_string(hoc::kNoAllocScratchBytes));
+    double* tr = hoc::noalloc_scratch(s.device);  // set at hoc_setup on this device
+    if (!tr) throw hoc::StatusError(HOC_ESTATE, "scratch array not set up on this device");
+    HIPCHK(hipSetDevice(s.device));
+    hipStream_t st = stream ? static_cast<hipStream_t>(stream) : s.stream;
+    double* xm = const_cast<double*>(x);  // read only: PRE_COPY and EPI_STORE do not write x
+    HIPCHK(hoc::launch_trace(s.K, hoc::PRE_COPY, s.N, s.T, xm, nullptr, nullptr, nullptr, 0., nullptr, 0, nullptr, tr, st));
+    HIPCHK(hoc::launch_lap(s.K, hoc::EPI_STORE, s.N, s.P64.T, s.P64.G, xm, tr, y, nullptr, nullptr, nullptr, 0., 0.,
+                           nullptr, nullptr, st));
+    hoc::unweight(st, s.D, s.K, s.tab.wgt, y);  // As = -W A  =>  A x = -(As x) / W
+    HIPCHK(hipGetLastError());
   });
 }
 

@@ -34,7 +34,7 @@ EXPORTED = [
     "hoc_set_p2g_weights", "hoc_set_vorticity", "hoc_get_vorticity",
     "hoc_get_streamfunction", "hoc_set_streamfunction", "hoc_get_velocity", "hoc_get_solution_points",
     "hoc_get_solution_areas", "hoc_get_time", "hoc_set_time", "hoc_step", "hoc_solveto", "hoc_poisson",
-    "hoc_stage_residual", "hoc_apply_laplacian", "hoc_boundary_point_count", "hoc_get_boundary_points",
+    "hoc_stage_residual", "hoc_apply_laplacian", "hoc_apply_laplacian_device", "hoc_boundary_point_count", "hoc_get_boundary_points",
     "hoc_get_wall_velocity", "hoc_set_wall_velocity", "hoc_adjoint_forward", "hoc_adjoint_backward",
     "hoc_stage_residual_vjp", "hoc_stats_get", "hoc_profile_enable", "hoc_profile_get",
     "hoc_last_error", "hoc_destroy",
@@ -118,6 +118,7 @@ def lib():
         L.hoc_poisson.argtypes = [C.c_void_p, _dp, C.c_double, _dp]
         L.hoc_stage_residual.argtypes = [C.c_void_p, _dp, C.c_double, _dp]
         L.hoc_apply_laplacian.argtypes = [C.c_void_p, _dp, _dp]
+        L.hoc_apply_laplacian_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         L.hoc_boundary_point_count.argtypes = [C.c_void_p]
         L.hoc_boundary_point_count.restype = C.c_int64
         for nm in ("hoc_get_boundary_points", "hoc_get_wall_velocity", "hoc_set_wall_velocity"):
@@ -393,6 +394,12 @@ class HO2D:
         out = np.zeros(self.D)
         self._chk(lib().hoc_apply_laplacian(self.h, np.ascontiguousarray(x, dtype=np.float64), out))
         return out
+
+    def apply_laplacian_device(self, x_ptr, y_ptr, stream=0):
+        """y = A x for device buffers (raw addresses of D doubles each); allocates nothing and
+        returns without waiting on `stream` (0 = the handle's stream)."""
+        self._chk(lib().hoc_apply_laplacian_device(self.h, C.c_void_p(x_ptr), C.c_void_p(y_ptr),
+                                                   C.c_void_p(stream or None)))
 
     def stats(self):
         s = hoc_stats()

@@ -207,6 +207,16 @@ int hoc_stage_residual(hoc_t h, const double* omega, double t, double* k);
 /* y = A x with the Poisson matrix of form_Laplace_operator_matrix (src/HO_2D.cpp:1533-1804),
  * applied matrix-free. */
 int hoc_apply_laplacian(hoc_t h, const double* x, double* y);
+/* No counterpart in the reference: hoc_apply_laplacian for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * x and y are device buffers of n_dofs doubles (distinct); the call allocates nothing: its
+ * scratch (the face traces, n_elem * 8 * K doubles) is a library-wide device array of 64 MiB per
+ * GPU, held until the process ends, and larger meshes are rejected with HOC_EINVAL.  The work is
+ * queued on `stream` (a hipStream_t; NULL = the handle's stream) and the call returns without
+ * waiting; the caller synchronizes that stream before reading y.  Every call on one GPU shares
+ * the array, so calls that overlap on one GPU (from any handles or streams) race.  Needs
+ * hoc_setup on the handle (which takes the array's address on its device). */
+int hoc_apply_laplacian_device(hoc_t h, const double* x, double* y, void* stream);
 
 /* Boundary flux points and the tangential wall velocity.  Boundary arrays are [E][K] in the
  * edge flux-point order of the BC_* arrays (src/HO_2D.cpp:2229).  The wall velocity is
