@@ -241,9 +241,11 @@ __device__ __forceinline__ void second_order_core(const TabT<R>& T, const R (&u)
 //   PRE_CHEB0  u = r * dinv * coef (coef = 1/theta); writes x = d = u   (Chebyshev step from zero)
 //   PRE_PROL   u = x + Pc xc (bilinear from the corner vertices); writes x = u
 //   PRE_PUPD   u = z + beta * p; writes p = u                     (PCG direction)
+// RA: the type of a (R, or float for the FP64 direction update of the mixed-precision solve,
+// whose V-cycle leaves z in FP32).
 // ==========================================================================================
-template <int K, int PRE, class R>
-__global__ __launch_bounds__(kWG) void k_trace(int N, TabT<R> T, R* __restrict__ x, const R* __restrict__ a,
+template <int K, int PRE, class R, class RA = R>
+__global__ __launch_bounds__(kWG) void k_trace(int N, TabT<R> T, R* __restrict__ x, const RA* __restrict__ a,
                                                 const R* __restrict__ b, R* __restrict__ d, R coef,
                                                 const double* __restrict__ scal, int scal_idx, const int* __restrict__ active,
                                                 R* __restrict__ tr, const int32_t* __restrict__ ev) {
@@ -259,12 +261,14 @@ __global__ __launch_bounds__(kWG) void k_trace(int N, TabT<R> T, R* __restrict__
   R xc[4] = {0, 0, 0, 0};
   if (PRE == PRE_PROL && live)
 #pragma unroll
-    for (int c = 0; c < 4; ++c) xc[c] = a[ev[(size_t)el * 4 + c]];
-  tile_load<K2>(PRE == PRE_COPY || PRE == PRE_PROL ? (const R*)x : a, lds, e0, N);
+    for (int c = 0; c < 4; ++c) xc[c] = (R)a[ev[(size_t)el * 4 + c]];
+  RA* la = reinterpret_cast<RA*>(lds);  // sizeof(RA) <= sizeof(R): a fits the tile
+  if constexpr (PRE == PRE_COPY || PRE == PRE_PROL) tile_load<K2>((const R*)x, lds, e0, N);
+  else tile_load<K2>(a, la, e0, N);
 #pragma unroll
   for (int m = 0; m < K2; ++m) {
     const int li = threadIdx.x * K2 + m;
-    R v = lds[li];
+    R v = (PRE == PRE_COPY || PRE == PRE_PROL) ? lds[li] : (R)la[li];
     if (PRE == PRE_PROL) {
       const int j = m / K, i = m % K;
       v += q1w(T, 0, j, i) * xc[0] + q1w(T, 1, j, i) * xc[1] + q1w(T, 2, j, i) * xc[2] + q1w(T, 3, j, i) * xc[3];
@@ -1051,6 +1055,13 @@ hipError_t launch_trace(int K, int pre, int N, const TabT<float>& T, float* x, c
                         float* d, double coef, const double* scal, int scal_idx, const int* active, float* tr,
                         hipStream_t s, const int32_t* ev) {
   return launch_trace_t<float>(K, pre, N, T, x, a, b, d, coef, scal, scal_idx, active, tr, s, ev);
+}
+hipError_t launch_trace_pupd_f32z(int K, int N, const TabT<double>& T, double* p, const float* z, const double* scal,
+                                  int scal_idx, const int* active, double* tr, hipStream_t s) {
+  HOC_K_SWITCH(K, hipLaunchKernelGGL((k_trace<KK, PRE_PUPD, double, float>), dim3(nblk(N)), dim3(kWG), 0, s, N, T, p,
+                                     z, (const double*)nullptr, (double*)nullptr, 0., scal, scal_idx, active, tr,
+                                     (const int32_t*)nullptr));
+  return hipGetLastError();
 }
 hipError_t launch_lap(int K, int epi, int N, const TabT<double>& T, const GeoT<double>& G, double* x, const double* tr,
                       double* y, const double* r, const double* diag, double* d, double c1, double c2, double* part,

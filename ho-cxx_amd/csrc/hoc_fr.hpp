@@ -39,6 +39,9 @@ hipError_t launch_trace(int K, int pre, int N, const TabT<double>& T, double* x,
 hipError_t launch_trace(int K, int pre, int N, const TabT<float>& T, float* x, const float* a, const float* b,
                         float* d, double coef, const double* scal, int scal_idx, const int* active, float* tr,
                         hipStream_t s, const int32_t* ev = nullptr);
+// PRE_PUPD of the FP64 Krylov method with the FP32 V-cycle: p = z + beta p with z in FP32.
+hipError_t launch_trace_pupd_f32z(int K, int N, const TabT<double>& T, double* p, const float* z, const double* scal,
+                                  int scal_idx, const int* active, double* tr, hipStream_t s);
 // tr: traces of x (input); tr_out (EPI_CHEB/EPI_CHEBF only, may be null): traces of the
 // updated x, written for the next application.  tr and tr_out must differ.  ran (may be null):
 // set to 1 when the launch worked, 0 when the active flag made it return at once.

@@ -120,9 +120,12 @@ __global__ void k_finalize(int phase, const double* __restrict__ pa, int np, dou
   } else if (half && phase == PH_RNORM && threadIdx.x == 0) {
     F[F_ACTIVE] = 0;
   }
-  if (mirror && threadIdx.x == 0)
-    __hip_atomic_store(mirror, seq * 2 + (F[F_ACTIVE] ? 1 : 0), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  if (mirror && threadIdx.x == 0) {
+    const int sq = seq >= 0 ? seq : ++F[F_SEQ];
+    __hip_atomic_store(mirror, sq * 2 + (F[F_ACTIVE] ? 1 : 0), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  }
 }
+__global__ void k_set_int(int* __restrict__ p, int v) { *p = v; }
 __device__ void finalize_phase(int phase, double a, double b, double* __restrict__ S, int* __restrict__ F,
                                int maxiter) {
   const double tol2bb = S[S_TOL2] * S[S_BB];
@@ -178,32 +181,6 @@ __device__ void finalize_phase(int phase, double a, double b, double* __restrict
       F[F_ACTIVE] = (b > tol2bb && F[F_ITER] < maxiter && S[S_OMEGA] != 0. && isfinite(b)) ? 1 : 0;
     } break;
   }
-}
-
-__global__ void k_refine_control(double* __restrict__ S, int* __restrict__ F, double* __restrict__ S2,
-                                 int* __restrict__ F2, double tol, double eta_floor, int maxit, int max_outer,
-                                 int* __restrict__ mirror, int seq) {
-  if (threadIdx.x != 0) return;
-  const int outer = F[F_OUTER];
-  if (outer > 0) F[F_TOTAL] += F2[F_ITER];  // iterations of the previous pass
-  const double rr = S[S_RR], bb = S[S_BB];
-  const bool go = F[F_ACTIVE] && isfinite(rr) && F[F_TOTAL] < maxit && outer < max_outer &&
-                  !(outer > 0 && rr > 0.25 * S[S_RRPREV]);
-  if (go) {
-    S[S_RRPREV] = rr;
-    // reduction still needed, with a margin for the FP32 residual gap
-    const double need = tol * sqrt(bb / rr);
-    const double eta = fmin(0.5, fmax(eta_floor, 0.3 * need));
-    S2[S_TOL2] = eta * eta;
-    F2[F_MAXIT] = maxit - F[F_TOTAL];
-    finalize_phase(PH_BB, rr, 0., S2, F2, F2[F_MAXIT]);
-    finalize_phase(PH_INIT, 0., rr, S2, F2, F2[F_MAXIT]);
-    F[F_OUTER] = outer + 1;
-  } else {
-    F2[F_ACTIVE] = 0;
-    F2[F_ITER] = 0;
-  }
-  __hip_atomic_store(mirror, seq * 2 + (go ? 1 : 0), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 // ---- Sparse levels -------------------------------------------------------------------------
@@ -372,31 +349,27 @@ __global__ void k_q1_probe_accum(int N, int K, Tab T, int rs, const double* __re
 }
 
 // ---- mixed precision ---------------------------------------------------------------------------
-// r32 = (float) r64, e32 = 0; both partial slots receive ||r64 / W||^2 (the unscaled norm the
-// inner solve's tolerance is relative to).
-__global__ void k_mixed_demote(VecArgs A, size_t n, const double* __restrict__ r64, float* __restrict__ r32,
-                               float* __restrict__ e32, double* __restrict__ part, const int* __restrict__ F) {
-  if (F && !F[F_ACTIVE]) return;
-  double p0 = 0.;
-  const size_t v0 = (size_t)blockIdx.x * blockDim.x + threadIdx.x, stride = (size_t)gridDim.x * blockDim.x;
-  DofSlot ds(A, v0, stride);
-  for (size_t v = v0; v < n; v += stride, ds.next()) {
-    const double r = r64[v];
-    r32[v] = (float)r;
-    e32[v] = 0.f;
-    const double ru = r * A.winv[ds.m];
-    p0 += ru * ru;
-  }
-  double p1 = p0;
-  block_sum2(p0, p1);
-  if (threadIdx.x == 0) { part[blockIdx.x] = p0; part[gridDim.x + blockIdx.x] = p1; }
-}
-// x64 += (double) e32
-__global__ void k_mixed_promote_add(size_t n, double* __restrict__ x64, const float* __restrict__ e32,
-                                    const int* __restrict__ F) {
+// FP64 Krylov method with the FP32 V-cycle as its preconditioner: the V-cycle's input
+// r32 = (float) r64, and its output z64 = (double) z32 with partials of r64 . z64 (slot 0; slot 1
+// zero) when part is set.
+__global__ void k_demote(size_t n, const double* __restrict__ r64, float* __restrict__ r32, const int* __restrict__ F) {
   if (F && !F[F_ACTIVE]) return;
   for (size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (size_t)gridDim.x * blockDim.x)
-    x64[v] += (double)e32[v];
+    r32[v] = (float)r64[v];
+}
+__global__ void k_promote_dot(size_t n, const float* __restrict__ z32, double* __restrict__ z64,
+                              const double* __restrict__ r64, double* __restrict__ part, const int* __restrict__ F) {
+  if (F && !F[F_ACTIVE]) return;
+  double p0 = 0., p1 = 0.;
+  for (size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += (size_t)gridDim.x * blockDim.x) {
+    const double z = (double)z32[v];
+    z64[v] = z;
+    if (part) p0 += r64[v] * z;
+  }
+  if (part) {
+    block_sum2(p0, p1);
+    if (threadIdx.x == 0) { part[blockIdx.x] = p0; part[gridDim.x + blockIdx.x] = p1; }
+  }
 }
 
 // ---- ensembles ---------------------------------------------------------------------------------
@@ -504,14 +477,45 @@ __global__ void k_proj_combine(size_t n, int m, const double* __restrict__ X, co
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
     double v = 0.;
     for (int j = 0; j < m; ++j) v += c[j] * X[(size_t)j * n + i];
-    out[i] = accumulate ? out[i] - v : v;
+    out[i] = accumulate == 1 ? out[i] - v : accumulate == 2 ? out[i] + v : v;
   }
+}
+// x -= sum_j c_j X_j and ax -= sum_j c_j AX_j in one pass (a Gram-Schmidt step on a vector and its image)
+__global__ void k_proj_combine2(size_t n, int m, const double* __restrict__ X, const double* __restrict__ AX,
+                                const double* __restrict__ c, double* __restrict__ x, double* __restrict__ ax) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    double v = 0., w = 0.;
+    for (int j = 0; j < m; ++j) {
+      v += c[j] * X[(size_t)j * n + i];
+      w += c[j] * AX[(size_t)j * n + i];
+    }
+    x[i] -= v;
+    ax[i] -= w;
+  }
+}
+// x = psi, ax = b - r (As psi, r the solve's final residual), with per-block partials of
+// (mr ? ax : x) . ax in the k_proj_dots layout (m = 1)
+__global__ __launch_bounds__(256) void k_proj_start(size_t n, const double* __restrict__ psi, const double* __restrict__ b,
+                                                    const double* __restrict__ r, double* __restrict__ x,
+                                                    double* __restrict__ ax, int mr, double* __restrict__ part) {
+  double s = 0.;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    const double xv = psi[i], av = b[i] - r[i];
+    x[i] = xv;
+    ax[i] = av;
+    s += (mr ? av : xv) * av;
+  }
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  __shared__ double red[4];
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
 }
 __global__ void k_proj_normalize(size_t n, const double* __restrict__ d, double* __restrict__ a, double* __restrict__ b) {
   const double f = (d[0] > 1e-24 * fabs(d[1]) && d[0] > 0.) ? 1. / sqrt(d[0]) : 0.;
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
     a[i] *= f;
-    b[i] *= f;
+    if (b) b[i] *= f;
   }
 }
 void proj_dots(hipStream_t s, size_t n, int m, const double* X, const double* y, double* part, double* out) {
@@ -523,6 +527,18 @@ void proj_dots(hipStream_t s, size_t n, int m, const double* X, const double* y,
 void proj_combine(hipStream_t s, size_t n, int m, const double* X, const double* c, int accumulate, double* out) {
   if (n == 0) return;
   hipLaunchKernelGGL(k_proj_combine, dim3(vec_grid(n)), dim3(256), 0, s, n, m, X, c, accumulate, out);
+}
+void proj_combine2(hipStream_t s, size_t n, int m, const double* X, const double* AX, const double* c, double* x,
+                   double* ax) {
+  if (n == 0 || m <= 0) return;
+  hipLaunchKernelGGL(k_proj_combine2, dim3(vec_grid(n)), dim3(256), 0, s, n, m, X, AX, c, x, ax);
+}
+void proj_start(hipStream_t s, size_t n, const double* psi, const double* b, const double* r, double* x, double* ax,
+                int mr, double* part, double* out) {
+  if (n == 0) return;
+  const int nb = proj_part_len(n);
+  hipLaunchKernelGGL(k_proj_start, dim3(nb), dim3(256), 0, s, n, psi, b, r, x, ax, mr, part);
+  hipLaunchKernelGGL(k_proj_fin, dim3(1), dim3(256), 0, s, nb, 1, part, out);
 }
 void proj_normalize(hipStream_t s, size_t n, const double* d, double* a, double* b) {
   if (n == 0) return;
@@ -536,14 +552,10 @@ template <class R>
 void zero_if_flag(hipStream_t s, size_t n, R* x, const int* F) {
   hipLaunchKernelGGL(k_zero_if_flag<R>, dim3(vec_grid(n)), dim3(256), 0, s, n, x, F);
 }
+void set_int(hipStream_t s, int* p, int v) { hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, s, p, v); }
 void finalize(hipStream_t s, int phase, const double* part, int np, double* S, int* F, int maxiter, int* mirror,
               int seq) {
   hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, phase, part, np, S, F, maxiter, mirror, seq);
-}
-void refine_control(hipStream_t s, double* S, int* F, double* S2, int* F2, double tol, double eta_floor, int maxit,
-                    int max_outer, int* mirror, int seq) {
-  hipLaunchKernelGGL(k_refine_control, dim3(1), dim3(64), 0, s, S, F, S2, F2, tol, eta_floor, maxit, max_outer,
-                     mirror, seq);
 }
 template <class R>
 void csr(hipStream_t s, int mode, int n, const int32_t* ptr, const int32_t* col, const R* val, const R* x, const R* f,
@@ -583,12 +595,12 @@ template <class R>
 void q1_gather(hipStream_t s, int nv, const int32_t* vptr, const int32_t* vinc, const R* ec, R* rc, const int* F) {
   hipLaunchKernelGGL(k_q1_gather<R>, dim3((nv + 255) / 256), dim3(256), 0, s, nv, vptr, vinc, ec, rc, F);
 }
-void mixed_demote(hipStream_t s, const VecArgs& A, size_t n, const double* r64, float* r32, float* e32, double* part,
-                  const int* F) {
-  hipLaunchKernelGGL(k_mixed_demote, dim3(vec_grid(n)), dim3(256), 0, s, A, n, r64, r32, e32, part, F);
+void demote(hipStream_t s, size_t n, const double* r64, float* r32, const int* F) {
+  hipLaunchKernelGGL(k_demote, dim3(vec_grid(n)), dim3(256), 0, s, n, r64, r32, F);
 }
-void mixed_promote_add(hipStream_t s, size_t n, double* x64, const float* e32, const int* F) {
-  hipLaunchKernelGGL(k_mixed_promote_add, dim3(vec_grid(n)), dim3(256), 0, s, n, x64, e32, F);
+void promote_dot(hipStream_t s, size_t n, const float* z32, double* z64, const double* r64, double* part,
+                 const int* F) {
+  hipLaunchKernelGGL(k_promote_dot, dim3(vec_grid(n)), dim3(256), 0, s, n, z32, z64, r64, part, F);
 }
 void inst_norms(hipStream_t s, const VecArgs& A, int n_inst, const int32_t* off, int K2, const double* b, double* out) {
   hipLaunchKernelGGL(k_inst_norms, dim3(n_inst), dim3(256), 0, s, A, off, K2, b, out);

@@ -13,10 +13,9 @@
 namespace hoc {
 
 // Device scalar slots (double) and flags (int) of the Krylov solvers.
-// S_RRPREV, F_MAXIT, F_OUTER, F_TOTAL: state of the mixed-precision refinement loop
-// (F_MAXIT: iteration budget read by finalize kernels launched with maxiter < 0).
-enum { S_BB = 0, S_RR, S_RZ, S_PQ, S_ALPHA, S_BETA, S_OMEGA, S_RHO, S_TOL2, S_RRPREV, S_COUNT = 16 };
-enum { F_ACTIVE = 0, F_ITER, F_ZERO, F_HALF, F_MAXIT, F_OUTER, F_TOTAL, F_COUNT = 8 };
+// F_MAXIT: iteration budget read by finalize kernels launched with maxiter < 0.
+enum { S_BB = 0, S_RR, S_RZ, S_PQ, S_ALPHA, S_BETA, S_OMEGA, S_RHO, S_TOL2, S_COUNT = 16 };
+enum { F_ACTIVE = 0, F_ITER, F_ZERO, F_HALF, F_MAXIT, F_SEQ, F_COUNT = 8 };
 enum { PH_BB = 0, PH_INIT, PH_RZ, PH_PQ, PH_UPD, PH_RHO, PH_ALPHA, PH_SNORM, PH_OMEGA, PH_RNORM };
 enum { VOP_PCG_UPD = 0, VOP_BICG_P, VOP_BICG_S, VOP_DOT2, VOP_COPY, VOP_BICG_X };
 
@@ -35,18 +34,13 @@ template <class R>
 void bicg_half(hipStream_t s, size_t n, R* x, const R* ph, const double* S, int* F);
 template <class R>
 void zero_if_flag(hipStream_t s, size_t n, R* x, const int* F);
-// mirror (optional): pinned host int that receives seq * 2 + (active flag) when the kernel ends.
+// mirror (optional): pinned host int that receives seq * 2 + (active flag) when the kernel ends;
+// seq < 0: the next number of the device counter F[F_SEQ] (incremented) instead of seq.
 // maxiter < 0: the budget is read from F[F_MAXIT].
 void finalize(hipStream_t s, int phase, const double* part, int np, double* S, int* F, int maxiter,
               int* mirror = nullptr, int seq = 0);
-// Control step of the mixed-precision refinement loop (one thread), run after each FP64
-// residual (S, F of the outer solve).  The loop goes on while the residual test fails, the
-// residual is finite, the previous pass at least halved it, and the iteration and pass
-// budgets last; then it sets the inner solve's tolerance (eta^2 in S2), budget and initial
-// state (its right-hand side norm is the outer residual norm) and counts the pass; otherwise
-// it leaves the inner solve inactive.  Publishes seq * 2 + (go) to mirror.
-void refine_control(hipStream_t s, double* S, int* F, double* S2, int* F2, double tol, double eta_floor, int maxit,
-                    int max_outer, int* mirror, int seq);
+// *p = v, in stream order (a kernel: usable where a memset or copy may not be recorded).
+void set_int(hipStream_t s, int* p, int v);
 template <class R>
 void csr(hipStream_t s, int mode, int n, const int32_t* ptr, const int32_t* col, const R* val, const R* x, const R* f,
          R* y, const R* M, R* xio, const int* F, int lanes_per_row);
@@ -61,11 +55,11 @@ void dense(hipStream_t s, int n, const R* Ainv, const R* f, R* x, const int* F);
 template <class R>
 void q1_gather(hipStream_t s, int nv, const int32_t* vptr, const int32_t* vinc, const R* ec, R* rc, const int* F);
 
-// Mixed precision: r32 = (float) r64, e32 = 0, partials of ||r64 / W||^2 (both slots);
-// and x64 += (double) e32.
-void mixed_demote(hipStream_t s, const VecArgs& A, size_t n, const double* r64, float* r32, float* e32, double* part,
-                  const int* F);
-void mixed_promote_add(hipStream_t s, size_t n, double* x64, const float* e32, const int* F);
+// The FP32 V-cycle inside the FP64 Krylov method: r32 = (float) r64; z64 = (double) z32 with
+// partials of r64 . z64 in part (vec_grid(n) blocks; may be null).
+void demote(hipStream_t s, size_t n, const double* r64, float* r32, const int* F);
+void promote_dot(hipStream_t s, size_t n, const float* z32, double* z64, const double* r64, double* part,
+                 const int* F);
 
 // Initial guess of a stage solve from the solutions at the last two distinct stage times:
 // extrap = 0: prev = psi (history shift, psi kept);  extrap = 1: psi' = psi + f (psi - prev)
@@ -75,12 +69,18 @@ void psi_predict(hipStream_t s, size_t n, double f, int extrap, double* psi, dou
 // Projection onto an As-orthonormal basis of earlier solutions (m <= kProjMax vectors X_j of
 // length n, stored contiguously, with AX_j = As X_j):
 //   proj_dots:    out[j] = X_j . y  (j < m), through per-block partials (part: proj_part_len(n) * m)
-//   proj_combine: out = accumulate ? out - sum_j c[j] X_j : sum_j c[j] X_j
-//   proj_normalize: f = 1/sqrt(d[0]) when d[0] > tiny * d[1] (else 0); a *= f, b *= f
+//   proj_combine: out = sum_j c[j] X_j (accumulate 0), out - sum (1) or out + sum (2)
+//   proj_combine2: x -= sum_j c[j] X_j and ax -= sum_j c[j] AX_j in one pass
+//   proj_start:   x = psi, ax = b - r, out[0] = (mr ? ax : x) . ax (deterministic)
+//   proj_normalize: f = 1/sqrt(d[0]) when d[0] > tiny * d[1] (else 0); a *= f, b *= f (b may be null)
 constexpr int kProjMax = 16;
 int proj_part_len(size_t n);
 void proj_dots(hipStream_t s, size_t n, int m, const double* X, const double* y, double* part, double* out);
 void proj_combine(hipStream_t s, size_t n, int m, const double* X, const double* c, int accumulate, double* out);
+void proj_combine2(hipStream_t s, size_t n, int m, const double* X, const double* AX, const double* c, double* x,
+                   double* ax);
+void proj_start(hipStream_t s, size_t n, const double* psi, const double* b, const double* r, double* x, double* ax,
+                int mr, double* part, double* out);
 void proj_normalize(hipStream_t s, size_t n, const double* d, double* a, double* b);
 // psi' = sum_j a[j] x_j (x_j may be psi itself, read before the write), and save = psi (the old
 // value) when save is not null; nterm <= 8.

@@ -20,6 +20,7 @@
 #include <mutex>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/hoc.h"
@@ -135,13 +136,12 @@ struct Solver {
   // fields
   DBuf<double> w, psi, vel, U, k1, k2, k3, k4, vort, kscratch, bs;
   DBuf<double> trw, tru;
-  // Poisson solver: FP64 state (plain solve, outer residual, probing) and FP32 state
-  // (mixed-precision inner solve); S/F = scalars and flags of the FP64 (outer) solve,
-  // S2/F2 those of the inner solve.
+  // Poisson solver: FP64 state (the Krylov method, the initial residual, probing) and FP32
+  // state (the mixed-precision V-cycle); S/F = scalars and flags of the Krylov solve.
   Prec<double> P64;
   Prec<float> P32;
-  DBuf<double> part, S, S2;
-  DBuf<int> F, F2;
+  DBuf<double> part, S;
+  DBuf<int> F;
   // Krylov progress published by the end-of-iteration finalize kernel (seq * 2 + active), in
   // pinned coherent host memory; seq counts iterations over the handle's lifetime.
   int* mirror = nullptr;
@@ -195,6 +195,7 @@ struct Solver {
   double setup_seconds = 0.;
 
   ~Solver() {
+    clear_graphs();
     for (auto e : prof.e0) (void)hipEventDestroy(e);
     for (auto e : prof.e1) (void)hipEventDestroy(e);
     if (mirror) (void)hipHostFree(mirror);
@@ -206,6 +207,7 @@ struct Solver {
     auto t0 = std::chrono::steady_clock::now();
     HIPCHK(hipSetDevice(device));
     if (!stream) HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    clear_graphs();
     noalloc_scratch_init(device);
     connect(mesh);
     build_tables(cfg.K, mesh.lnod, tab);
@@ -324,7 +326,7 @@ struct Solver {
     }
     nparts = std::max(lap_blocks(N), vec_grid(D));
     part.alloc(2 * (size_t)nparts);
-    S.alloc(S_COUNT); F.alloc(F_COUNT); S2.alloc(S_COUNT); F2.alloc(F_COUNT);
+    S.alloc(S_COUNT); F.alloc(F_COUNT);
     if (!mirror) {
       HIPCHK(hipHostMalloc((void**)&mirror, 64, hipHostMallocCoherent | hipHostMallocMapped));
       __atomic_store_n(mirror, -2, __ATOMIC_RELEASE);
@@ -338,6 +340,7 @@ struct Solver {
     proj_fwd.X.alloc(0); proj_fwd.AX.alloc(0); proj_fwd.m = 0;
     proj_adj.X.alloc(0); proj_adj.AX.alloc(0); proj_adj.m = 0;
     reset_adjoint();
+    tune_launch_mode();
     invalidate_guess();
     time = 0.;
     smooth_ready = false;
@@ -597,13 +600,30 @@ struct Solver {
     }
   }
 
-  // Wait, with a one-iteration lag (none when final_wait), until the finalize kernel of the
+  // z = M r: the V-cycle in the precision of P; for the FP64 Krylov method of a mixed-precision
+  // handle, the FP32 V-cycle between a demotion of r and a promotion of z.  With partp set, the
+  // partials of r . z are left there; returns their count (0 when partp is null).
+  template <class R>
+  int precond(Prec<R>& P, const R* rin, R* zout, double* partp, const int* Fp) {
+    if constexpr (std::is_same<R, double>::value) {
+      if (mixed) {
+        demote(stream, D, rin, P32.r.p, Fp);
+        vcycle(P32, P32.r.p, P32.z.p, nullptr, Fp);
+        promote_dot(stream, D, P32.z.p, zout, rin, partp, Fp);
+        return partp ? vec_grid(D) : 0;
+      }
+    }
+    vcycle(P, rin, zout, partp, Fp);
+    return partp ? lap_blocks(N) : 0;
+  }
+
+  // Wait, lag iterations behind (none when final_wait), until the finalize kernel of the
   // iteration numbered seq has published its status; returns false when the solve ended.
   // seq_first: the first iteration of this solve (no wait before it has run).  The wait spins
   // on the pinned word and checks the stream every 4096 reads, so an idle stream without the
   // expected status (or a device error) ends it with an error instead of a hang.
-  bool poll(int seq, int seq_first, bool final_wait) {
-    const int target = final_wait ? seq : seq - 1;
+  bool poll(int seq, int seq_first, bool final_wait, int lag = 1) {
+    const int target = final_wait ? seq : seq - lag;
     if (target < seq_first) return true;
     for (unsigned spin = 1;; ++spin) {
       const int v = __atomic_load_n(mirror, __ATOMIC_ACQUIRE);
@@ -624,54 +644,181 @@ struct Solver {
   // PH_INIT in (Sp, Fp): r holds the residual of x.  PCG when As is symmetric, else
   // right-preconditioned BiCGSTAB.  Every kernel is predicated on Fp's active flag.
   // maxit bounds the host loop; maxit_dev is the device budget (< 0: Fp[F_MAXIT]).
+  //
+  // Launch modes.  Direct: the host queues each iteration's kernels and waits for the status of
+  // the iteration before the last one it queued.  Graph: a hipGraph holding kGraphChunk
+  // iterations, captured once per set of buffers, is replayed; its end-of-iteration finalize
+  // kernels number themselves from the device counter F[F_SEQ], and the host waits one chunk
+  // behind.  The kernels and their order are the same, so both modes give the same iterates;
+  // graphs pay where the host's launch cost exceeds the kernels' run time (small meshes), and
+  // setup chooses by timing both (launch_mode 0).
+  static constexpr int kGraphChunk = 4;
+  struct KGraph {
+    hipGraph_t g = nullptr;
+    hipGraphExec_t e = nullptr;
+    const void* key[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    int maxit = 0;
+  };
+  std::vector<KGraph> kgraphs;
+  bool use_graphs = false;
+  double launch_tune_ratio = 0.;  // graph / direct time of the same iterations at setup (0: not timed)
+  void clear_graphs() {
+    for (auto& k : kgraphs) {
+      if (k.e) (void)hipGraphExecDestroy(k.e);
+      if (k.g) (void)hipGraphDestroy(k.g);
+    }
+    kgraphs.clear();
+  }
+  // Before the first iteration.
   template <class R>
-  void krylov_solve(Prec<R>& P, R* x, R* r, double* Sp, int* Fp, int maxit_host, int maxit_dev) {
-    const int maxit = maxit_dev;
-    const int lb = lap_blocks(N);
-    const int vg = vec_grid(D);
-    const int seq0 = seq_next;
+  void krylov_begin(Prec<R>& P, R* r) {
     if (krylov == HOC_KRYLOV_PCG) {
       HIPCHK(hipMemsetAsync(P.p.p, 0, D * sizeof(R), stream));
-      for (int it = 0;; ++it) {
-        vcycle(P, r, P.z.p, part.p, Fp);
-        finalize(stream, PH_RZ, part, lb, Sp, Fp, maxit);
-        HIPCHK(launch_trace(K, PRE_PUPD, N, P.T, P.p.p, P.z.p, (const R*)nullptr, (R*)nullptr, 0., Sp, S_BETA, Fp,
-                            P.tr.p, stream));
-        lap(P, EPI_STORE, P.p.p, P.q.p, (const R*)nullptr, (const R*)nullptr, (R*)nullptr, 0., 0., part.p, Fp);
-        finalize(stream, PH_PQ, part, lb, Sp, Fp, maxit);
-        vec<R>(stream, VOP_PCG_UPD, VA, D, x, r, P.p, P.q, nullptr, Sp, Fp, part);
-        const int seq = seq_next++;
-        finalize(stream, PH_UPD, part, vg, Sp, Fp, maxit, mirror_dev, seq);
-        if (!poll(seq, seq0, it > maxit_host + 2) || it > maxit_host + 4) break;
-      }
     } else {
       vec<R>(stream, VOP_COPY, VA, D, P.rh, nullptr, r, nullptr, nullptr, nullptr, nullptr, nullptr);
       HIPCHK(hipMemsetAsync(P.p.p, 0, D * sizeof(R), stream));
       HIPCHK(hipMemsetAsync(P.v.p, 0, D * sizeof(R), stream));
-      const R* none = nullptr;
-      for (int it = 0;; ++it) {
-        vec<R>(stream, VOP_DOT2, VA, D, P.rh, P.rh, r, r, nullptr, Sp, Fp, part);
-        finalize(stream, PH_RHO, part, vg, Sp, Fp, maxit);
-        vec<R>(stream, VOP_BICG_P, VA, D, P.p, nullptr, r, P.v, nullptr, Sp, Fp, nullptr);
-        vcycle(P, P.p.p, P.ph.p, nullptr, Fp);
-        HIPCHK(launch_trace(K, PRE_COPY, N, P.T, P.ph.p, none, none, (R*)nullptr, 0., nullptr, 0, Fp, P.tr.p, stream));
-        lap(P, EPI_STORE, P.ph.p, P.v.p, none, none, (R*)nullptr, 0., 0., nullptr, Fp);
-        vec<R>(stream, VOP_DOT2, VA, D, P.rh, P.rh, P.v, P.v, nullptr, Sp, Fp, part);
-        finalize(stream, PH_ALPHA, part, vg, Sp, Fp, maxit);
-        vec<R>(stream, VOP_BICG_S, VA, D, nullptr, r, nullptr, P.v, nullptr, Sp, Fp, part);
-        finalize(stream, PH_SNORM, part, vg, Sp, Fp, maxit);
-        bicg_half<R>(stream, D, x, P.ph, Sp, Fp);
-        vcycle(P, r, P.sh.p, nullptr, Fp);  // gated by F_ACTIVE only; harmless when converged at the half step
-        HIPCHK(launch_trace(K, PRE_COPY, N, P.T, P.sh.p, none, none, (R*)nullptr, 0., nullptr, 0, Fp, P.tr.p, stream));
-        lap(P, EPI_STORE, P.sh.p, P.t.p, none, none, (R*)nullptr, 0., 0., nullptr, Fp);
-        vec<R>(stream, VOP_DOT2, VA, D, P.t, P.t, r, P.t, nullptr, Sp, Fp, part);
-        finalize(stream, PH_OMEGA, part, vg, Sp, Fp, maxit);
-        vec<R>(stream, VOP_BICG_X, VA, D, x, r, P.ph, P.sh, P.t, Sp, Fp, part);
-        const int seq = seq_next++;
-        finalize(stream, PH_RNORM, part, vg, Sp, Fp, maxit, mirror_dev, seq);
-        if (!poll(seq, seq0, it > maxit_host + 2) || it > maxit_host + 4) break;
-      }
     }
+  }
+  // One iteration's launches.  seq >= 0: the end-of-iteration finalize publishes seq; seq < 0:
+  // it takes the next number from F[F_SEQ].
+  template <class R>
+  void krylov_iter(Prec<R>& P, R* x, R* r, double* Sp, int* Fp, int maxit, int seq) {
+    const int lb = lap_blocks(N);
+    const int vg = vec_grid(D);
+    if (krylov == HOC_KRYLOV_PCG) {
+      bool zf = false;  // z left in FP32 by the mixed-precision V-cycle
+      if constexpr (std::is_same<R, double>::value) {
+        if (mixed) {  // r.z from the FP32 V-cycle's last sweep; p = z + beta p reads z in FP32
+          demote(stream, D, r, P32.r.p, Fp);
+          vcycle(P32, P32.r.p, P32.z.p, part.p, Fp);
+          finalize(stream, PH_RZ, part, lb, Sp, Fp, maxit);
+          HIPCHK(launch_trace_pupd_f32z(K, N, P.T, P.p.p, P32.z.p, Sp, S_BETA, Fp, P.tr.p, stream));
+          zf = true;
+        }
+      }
+      if (!zf) {
+        const int np = precond(P, r, P.z.p, part.p, Fp);
+        finalize(stream, PH_RZ, part, np, Sp, Fp, maxit);
+        HIPCHK(launch_trace(K, PRE_PUPD, N, P.T, P.p.p, P.z.p, (const R*)nullptr, (R*)nullptr, 0., Sp, S_BETA, Fp,
+                            P.tr.p, stream));
+      }
+      lap(P, EPI_STORE, P.p.p, P.q.p, (const R*)nullptr, (const R*)nullptr, (R*)nullptr, 0., 0., part.p, Fp);
+      finalize(stream, PH_PQ, part, lb, Sp, Fp, maxit);
+      vec<R>(stream, VOP_PCG_UPD, VA, D, x, r, P.p, P.q, nullptr, Sp, Fp, part);
+      finalize(stream, PH_UPD, part, vg, Sp, Fp, maxit, mirror_dev, seq);
+    } else {
+      const R* none = nullptr;
+      vec<R>(stream, VOP_DOT2, VA, D, P.rh, P.rh, r, r, nullptr, Sp, Fp, part);
+      finalize(stream, PH_RHO, part, vg, Sp, Fp, maxit);
+      vec<R>(stream, VOP_BICG_P, VA, D, P.p, nullptr, r, P.v, nullptr, Sp, Fp, nullptr);
+      precond(P, P.p.p, P.ph.p, nullptr, Fp);
+      HIPCHK(launch_trace(K, PRE_COPY, N, P.T, P.ph.p, none, none, (R*)nullptr, 0., nullptr, 0, Fp, P.tr.p, stream));
+      lap(P, EPI_STORE, P.ph.p, P.v.p, none, none, (R*)nullptr, 0., 0., nullptr, Fp);
+      vec<R>(stream, VOP_DOT2, VA, D, P.rh, P.rh, P.v, P.v, nullptr, Sp, Fp, part);
+      finalize(stream, PH_ALPHA, part, vg, Sp, Fp, maxit);
+      vec<R>(stream, VOP_BICG_S, VA, D, nullptr, r, nullptr, P.v, nullptr, Sp, Fp, part);
+      finalize(stream, PH_SNORM, part, vg, Sp, Fp, maxit);
+      bicg_half<R>(stream, D, x, P.ph, Sp, Fp);
+      precond(P, r, P.sh.p, nullptr, Fp);  // gated by F_ACTIVE only; harmless when converged at the half step
+      HIPCHK(launch_trace(K, PRE_COPY, N, P.T, P.sh.p, none, none, (R*)nullptr, 0., nullptr, 0, Fp, P.tr.p, stream));
+      lap(P, EPI_STORE, P.sh.p, P.t.p, none, none, (R*)nullptr, 0., 0., nullptr, Fp);
+      vec<R>(stream, VOP_DOT2, VA, D, P.t, P.t, r, P.t, nullptr, Sp, Fp, part);
+      finalize(stream, PH_OMEGA, part, vg, Sp, Fp, maxit);
+      vec<R>(stream, VOP_BICG_X, VA, D, x, r, P.ph, P.sh, P.t, Sp, Fp, part);
+      finalize(stream, PH_RNORM, part, vg, Sp, Fp, maxit, mirror_dev, seq);
+    }
+  }
+  // The instantiated graph of kGraphChunk iterations on these buffers (captured on first use).
+  template <class R>
+  hipGraphExec_t krylov_graph(Prec<R>& P, R* x, R* r, double* Sp, int* Fp, int maxit) {
+    const void* key[6] = {&P, x, r, Sp, Fp, mixed ? (const void*)&P32 : nullptr};
+    for (auto& k : kgraphs)
+      if (k.maxit == maxit && std::equal(key, key + 6, k.key)) return k.e;
+    KGraph kg;
+    std::copy(key, key + 6, kg.key);
+    kg.maxit = maxit;
+    HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    try {
+      for (int i = 0; i < kGraphChunk; ++i) krylov_iter(P, x, r, Sp, Fp, maxit, -1);
+    } catch (...) {
+      hipGraph_t g = nullptr;
+      (void)hipStreamEndCapture(stream, &g);
+      if (g) (void)hipGraphDestroy(g);
+      throw;
+    }
+    HIPCHK(hipStreamEndCapture(stream, &kg.g));
+    HIPCHK(hipGraphInstantiate(&kg.e, kg.g, nullptr, nullptr, 0));
+    kgraphs.push_back(kg);
+    return kg.e;
+  }
+  template <class R>
+  void krylov_solve(Prec<R>& P, R* x, R* r, double* Sp, int* Fp, int maxit_host, int maxit_dev) {
+    const int maxit = maxit_dev;
+    const int seq0 = seq_next;
+    krylov_begin(P, r);
+    if (use_graphs && !prof.on) {
+      hipGraphExec_t ge = krylov_graph(P, x, r, Sp, Fp, maxit);
+      set_int(stream, Fp + F_SEQ, seq_next - 1);
+      for (int done = 0;;) {
+        HIPCHK(hipGraphLaunch(ge, stream));
+        const int last = seq_next + kGraphChunk - 1;
+        seq_next += kGraphChunk;
+        done += kGraphChunk;
+        const bool fin = done > maxit_host + 2;
+        if (!poll(last, seq0, fin, kGraphChunk) || done > maxit_host + 4) break;
+      }
+      return;
+    }
+    for (int it = 0;; ++it) {
+      const int seq = seq_next++;
+      krylov_iter(P, x, r, Sp, Fp, maxit, seq);
+      if (!poll(seq, seq0, it > maxit_host + 2) || it > maxit_host + 4) break;
+    }
+  }
+
+  // Launch mode (launch_mode 0): the same 2 kGraphChunk iterations of a solve on a random
+  // right-hand side, queued directly and replayed from the captured graph; graphs are used when
+  // they take under 85% of the direct time.  The solve's state is reset afterwards.
+  void tune_launch_mode() {
+    clear_graphs();
+    use_graphs = cfg.launch_mode == 2;
+    launch_tune_ratio = 0.;
+    if (cfg.launch_mode != 0) return;
+    std::vector<double> hb(D);
+    uint64_t st = 0x9E3779B97F4A7C15ull;
+    for (size_t i = 0; i < D; ++i) { st ^= st << 13; st ^= st >> 7; st ^= st << 17; hb[i] = (double)(st % 2000001) / 1e6 - 1.; }
+    h2d(bs.p, hb.data(), D * sizeof(double));
+    const double sv[2] = {1., 0.};  // S_BB = 1, S_TOL2 = 0: every tuning iteration stays active
+    h2d(S.p + S_BB, &sv[0], sizeof(double));
+    h2d(S.p + S_TOL2, &sv[1], sizeof(double));
+    const int maxit = std::max(cfg.maxiter, 4 * kGraphChunk);
+    auto run = [&](bool graph) {
+      HIPCHK(hipMemsetAsync(psi.p, 0, D * sizeof(double), stream));
+      outer_residual();
+      krylov_begin(P64, P64.r.p);
+      hipGraphExec_t ge = graph ? krylov_graph(P64, psi.p, P64.r.p, S.p, F.p, maxit) : nullptr;
+      HIPCHK(hipStreamSynchronize(stream));
+      const auto t0 = std::chrono::steady_clock::now();
+      if (graph) {
+        set_int(stream, F.p + F_SEQ, seq_next - 1);
+        for (int c = 0; c < 2; ++c) HIPCHK(hipGraphLaunch(ge, stream));
+        seq_next += 2 * kGraphChunk;
+      } else {
+        for (int i = 0; i < 2 * kGraphChunk; ++i) krylov_iter(P64, psi.p, P64.r.p, S.p, F.p, maxit, seq_next++);
+      }
+      HIPCHK(hipStreamSynchronize(stream));
+      return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    };
+    run(false);  // warm-up
+    const double td = run(false);
+    run(true);   // capture and warm-up
+    const double tg = run(true);
+    launch_tune_ratio = td > 0. ? tg / td : 0.;
+    use_graphs = tg < 0.85 * td;
+    HIPCHK(hipMemsetAsync(psi.p, 0, D * sizeof(double), stream));
+    HIPCHK(hipMemsetAsync(bs.p, 0, D * sizeof(double), stream));
+    HIPCHK(hipStreamSynchronize(stream));
   }
 
   // FP64 residual r = bs - As psi with the reference's unscaled norm, and the solve's
@@ -686,10 +833,10 @@ struct Solver {
   // solve_Poisson (src/HO_2D.cpp:1841-1977) for omega at the current boundary values.
   //  FP64:  PCG/BiCGSTAB with the FP64 V-cycle on the scaled system, stopping on the
   //         reference's ||b - A psi|| <= tol ||b||.
-  //  mixed: iterative refinement.  The FP64 residual r = b - A psi is demoted to FP32, the
-  //         correction e solves As e = r with FP32 Krylov + FP32 V-cycle to the relative
-  //         tolerance eta, psi += e in FP64, and the FP64 residual is recomputed; the loop
-  //         ends on the same FP64 test as the plain solve.
+  //  mixed: the same FP64 Krylov method, the same operator and the same stopping test, with
+  //         the V-cycle in FP32 as its preconditioner: the residual is demoted at the cycle's
+  //         entry and the cycle's output enters the FP64 direction update (precond, and the
+  //         PCG loop's FP32-input direction update).
   const double* rhs_bc = nullptr;  // boundary stream-function values of the next solve (null: b_psi)
 
   // Initial guess of the stage solves.  Mode 1: the previous solution, as the reference
@@ -821,19 +968,17 @@ struct Solver {
     ProjBasis& P = *proj_cur;
     if (P.m == 0) return;
     double* t = ptmp.p;
-    apply_As(psi, t);
-    proj_combine(stream, D, 1, b, pcoef.p + kProjMax + 2, 1, t);  // t = As x0 - b
-    proj_dots(stream, D, P.m, proj_mr() ? P.AX.p : P.X.p, t, ppart, pcoef);  // -c_j
-    proj_combine(stream, D, P.m, P.X.p, pcoef, 1, psi);
+    HIPCHK(launch_trace(K, PRE_COPY, N, T, psi.p, nullptr, nullptr, nullptr, 0., nullptr, 0, nullptr, P64.tr.p, stream));
+    lap<double>(P64, EPI_RESID, psi.p, t, b, nullptr, nullptr, 0., 0., nullptr, nullptr);  // t = b - As x0
+    proj_dots(stream, D, P.m, proj_mr() ? P.AX.p : P.X.p, t, ppart, pcoef);  // c_j
+    proj_combine(stream, D, P.m, P.X.p, pcoef, 2, psi);
   }
   void proj_after(const double* b, const double* r) {
     constexpr int M = kProjVectors;
     ProjBasis& P = *proj_cur;
-    if (ptmp.n < 2 * D) {
-      ptmp.alloc(2 * D);
-      pcoef.alloc(kProjMax + 3); ppart.alloc((size_t)proj_part_len(D) * kProjMax);
-      const double one = 1.;
-      h2d(pcoef.p + kProjMax + 2, &one, sizeof(double));
+    if (ptmp.n < D) {  // t of proj_before; coefficients and norms; partial sums
+      ptmp.alloc(D);
+      pcoef.alloc(kProjMax + 2); ppart.alloc((size_t)proj_part_len(D) * kProjMax);
     }
     if (P.X.n < (size_t)M * D) {
       P.X.alloc((size_t)M * D); P.AX.alloc((size_t)M * D);
@@ -841,22 +986,17 @@ struct Solver {
     }
     if (P.m >= M) P.m = 0;  // restart from the newest solution
     const int j = P.m;
-    double* x = ptmp.p;
-    double* ax = ptmp.p + D;
-    dcopy(x, psi, D);
-    dcopy(ax, b, D);
-    proj_combine(stream, D, 1, r, pcoef.p + kProjMax + 2, 1, ax);  // As x = b - r
+    double* x = P.X.p + (size_t)j * D;   // the new vector is orthogonalised in its slot
+    double* ax = P.AX.p + (size_t)j * D;
     const double* nx = proj_mr() ? ax : x;  // the norm: |x|_As^2 = x . As x, or |As x|^2
-    proj_dots(stream, D, 1, nx, ax, ppart, pcoef.p + kProjMax + 1);  // before orthogonalisation
+    // x = psi, As x = b - r, and the norm before orthogonalisation
+    proj_start(stream, D, psi, b, r, x, ax, proj_mr() ? 1 : 0, ppart, pcoef.p + kProjMax + 1);
     for (int pass = 0; pass < 2 && j > 0; ++pass) {
       proj_dots(stream, D, j, proj_mr() ? P.AX.p : P.X.p, ax, ppart, pcoef);  // c_i
-      proj_combine(stream, D, j, P.X.p, pcoef, 1, x);
-      proj_combine(stream, D, j, P.AX.p, pcoef, 1, ax);
+      proj_combine2(stream, D, j, P.X.p, P.AX.p, pcoef, x, ax);
     }
     proj_dots(stream, D, 1, nx, ax, ppart, pcoef.p + kProjMax);
     proj_normalize(stream, D, pcoef.p + kProjMax, x, ax);  // (a vector left in the span becomes zero)
-    dcopy(P.X.p + (size_t)j * D, x, D);
-    dcopy(P.AX.p + (size_t)j * D, ax, D);
     P.m = j + 1;
   }
   // The adjoint's colouring, velocity map, buffers and recorded trajectory belong to one mesh and
@@ -932,30 +1072,11 @@ struct Solver {
       HIPCHK(hipMemcpyAsync(hf, Fp, sizeof(hf), hipMemcpyDeviceToHost, stream));
       HIPCHK(hipStreamSynchronize(stream));
     };
-    int iters = 0, outer = 0;
-    if (!mixed) {
-      krylov_solve(P64, psi.p, P64.r.p, S.p, F.p, maxit, maxit);
-      read_state(S, F);
-      iters = hf[F_ITER];
-    } else {
-      // The pass decisions are taken on the device (refine_control); the host waits only for
-      // the control step's published verdict, while the demotion behind it keeps the GPU busy.
-      const double eta_floor = cfg.mixed_inner_tol > 0. ? cfg.mixed_inner_tol : 1e-4;
-      constexpr int kMaxOuter = 12;
-      HIPCHK(hipMemsetAsync(F.p + F_OUTER, 0, 2 * sizeof(int), stream));  // F_OUTER, F_TOTAL
-      for (;;) {
-        const int seq_c = seq_next++;
-        refine_control(stream, S, F, S2, F2, cfg.tol, eta_floor, maxit, kMaxOuter, mirror_dev, seq_c);
-        mixed_demote(stream, VA, D, P64.r, P32.r, P32.x, part, F);
-        if (!poll(seq_c, seq_c, true)) break;
-        krylov_solve(P32, P32.x.p, P32.r.p, S2.p, F2.p, maxit, -1);
-        mixed_promote_add(stream, D, psi, P32.x, F);
-        outer_residual();
-      }
-      read_state(S, F);
-      iters = hf[F_TOTAL];
-      outer = hf[F_OUTER];
-    }
+    // FP64: the FP64 V-cycle; mixed: the FP32 V-cycle (precond), the Krylov vectors, the operator
+    // application and the stopping test in FP64 either way.
+    krylov_solve(P64, psi.p, P64.r.p, S.p, F.p, maxit, maxit);
+    read_state(S, F);
+    const int iters = hf[F_ITER], outer = 0;
     // statistics
     last_iters = iters;
     last_outer = outer;
@@ -1738,6 +1859,7 @@ void hoc_default_config(hoc_config* c) {
   c->mixed_inner_tol = 0.;
   c->cheb_degree = 0;
   c->cheb_lower = 0.;
+  c->launch_mode = 0;
 }
 
 int hoc_create(hoc_t* out, const hoc_config* cfg) {
@@ -1754,6 +1876,7 @@ int hoc_create(hoc_t* out, const hoc_config* cfg) {
   if (!(cfg->mixed_inner_tol >= 0. && cfg->mixed_inner_tol < 1.)) return hoc_fail(nullptr, HOC_EINVAL, "mixed_inner_tol must be in [0, 1)");
   if (cfg->cheb_degree < 0 || cfg->cheb_degree > hoc::kMaxCheb) return hoc_fail(nullptr, HOC_EINVAL, "cheb_degree must be in 0..8");
   if (!(cfg->cheb_lower >= 0. && cfg->cheb_lower < 1.)) return hoc_fail(nullptr, HOC_EINVAL, "cheb_lower must be in [0, 1)");
+  if (cfg->launch_mode < 0 || cfg->launch_mode > 2) return hoc_fail(nullptr, HOC_EINVAL, "launch_mode must be 0, 1 or 2");
   if (cfg->psi_guess < 0 || cfg->psi_guess > 5) return hoc_fail(nullptr, HOC_EINVAL, "psi_guess must be 0..5");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return hoc_fail(nullptr, HOC_EDEVICE, "no HIP device");
@@ -2209,6 +2332,8 @@ int hoc_stats_get(hoc_t h, hoc_stats* st) {
     st->last_refinements = s.last_outer;
     st->precision = s.cfg.precision;
     st->n_nodes = s.mesh.n_nodes;
+    st->launch_graphs = s.use_graphs ? 1 : 0;
+    st->launch_tune_ratio = s.launch_tune_ratio;
     st->krylov = s.krylov;
     st->last_iters = s.last_iters;
     st->last_rel_residual = s.last_rel;

@@ -50,7 +50,7 @@ class hoc_config(C.Structure):
                 ("time_integration_type", C.c_int32), ("advection_type", C.c_int32), ("tol", C.c_double),
                 ("maxiter", C.c_int32), ("precision", C.c_int32), ("krylov", C.c_int32), ("device", C.c_int32),
                 ("fail_on_noconv", C.c_int32), ("mixed_inner_tol", C.c_double), ("cheb_degree", C.c_int32),
-                ("psi_guess", C.c_int32), ("cheb_lower", C.c_double)]
+                ("psi_guess", C.c_int32), ("cheb_lower", C.c_double), ("launch_mode", C.c_int32)]
 
 
 class hoc_stats(C.Structure):
@@ -58,7 +58,7 @@ class hoc_stats(C.Structure):
                 ("krylov", C.c_int32), ("last_iters", C.c_int32), ("last_rel_residual", C.c_double),
                 ("total_iters", C.c_int64), ("n_solves", C.c_int64), ("setup_seconds", C.c_double),
                 ("fine_op_bytes", C.c_double), ("last_refinements", C.c_int32), ("precision", C.c_int32),
-                ("n_nodes", C.c_int64)]
+                ("n_nodes", C.c_int64), ("launch_graphs", C.c_int32), ("launch_tune_ratio", C.c_double)]
 
 
 class hoc_kernel_profile(C.Structure):
@@ -162,12 +162,13 @@ class HO2D:
 
     def __init__(self, K=3, problem_type=1, Re=100.0, dt=1e-3, time_integration_type=4, advection_type=1,
                  tol=1e-15, maxiter=150, precision=HOC_FP64, krylov=KRYLOV_AUTO, device=0, fail_on_noconv=False,
-                 mixed_inner_tol=0.0, cheb_degree=0, cheb_lower=0.0, psi_guess=0):
+                 mixed_inner_tol=0.0, cheb_degree=0, cheb_lower=0.0, psi_guess=0, launch_mode=0):
         self.cfg = default_config(K=K, problem_type=problem_type, Re=Re, dt=dt,
                                   time_integration_type=time_integration_type, advection_type=advection_type,
                                   tol=tol, maxiter=maxiter, precision=precision, krylov=krylov, device=device,
                                   fail_on_noconv=int(fail_on_noconv), mixed_inner_tol=mixed_inner_tol,
-                                  cheb_degree=cheb_degree, cheb_lower=cheb_lower, psi_guess=psi_guess)
+                                  cheb_degree=cheb_degree, cheb_lower=cheb_lower, psi_guess=psi_guess,
+                                  launch_mode=launch_mode)
         self.h = C.c_void_p()
         rc = lib().hoc_create(C.byref(self.h), C.byref(self.cfg))
         if rc != HOC_OK:

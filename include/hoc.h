@@ -48,13 +48,12 @@ typedef struct {
   int32_t advection_type;         /* 1 discontinuous mass flux, 2 continuous             */
   double tol;                     /* Krylov relative residual tolerance                  */
   int32_t maxiter;                /* Krylov iteration cap                                */
-  int32_t precision;              /* HOC_FP64, or HOC_MIXED: FP64 iterative refinement
-                                     around an FP32 Krylov + V-cycle inner solve         */
+  int32_t precision;              /* HOC_FP64, or HOC_MIXED: the FP64 Krylov method with
+                                     the V-cycle preconditioner in FP32                  */
   int32_t krylov;                 /* HOC_KRYLOV_*; AUTO = PCG when W*A is symmetric     */
   int32_t device;                 /* HIP device ordinal                                  */
   int32_t fail_on_noconv;         /* nonzero: return HOC_ENOCONV when tol is not met     */
-  double mixed_inner_tol;         /* HOC_MIXED: floor of the FP32 inner solve's relative
-                                     tolerance per refinement pass (0 = 1e-4)            */
+  double mixed_inner_tol;         /* unused (kept for layout); must lie in [0, 1)       */
   int32_t cheb_degree;            /* fine-level Chebyshev smoother degree, pre and post
                                      (0 = 3; at most 8)                                   */
   int32_t psi_guess;              /* initial guess of the stage Poisson solves: 1 = the
@@ -72,6 +71,9 @@ typedef struct {
                                      least-squares residual otherwise  (0 = 5)            */
   double cheb_lower;              /* smoothing interval [cheb_lower, 1.1] x lambda_max(D^-1 As)
                                      (0 = 1/60)                                          */
+  int32_t launch_mode;            /* Krylov iterations: 0 = chosen at hoc_setup by timing
+                                     both, 1 = kernels queued one by one, 2 = replays of a
+                                     captured hipGraph of 4 iterations (same iterates)   */
 } hoc_config;
 
 typedef struct {
@@ -84,9 +86,12 @@ typedef struct {
   int64_t n_solves;
   double setup_seconds;
   double fine_op_bytes;           /* algorithmic bytes of one fine-operator application  */
-  int32_t last_refinements;       /* HOC_MIXED: FP64 refinement passes of the last solve */
+  int32_t last_refinements;       /* 0 (the mixed solve is one FP64 Krylov solve)       */
   int32_t precision;              /* HOC_FP64 or HOC_MIXED                               */
   int64_t n_nodes;                /* mesh nodes (the node-averaged output's length)      */
+  int32_t launch_graphs;          /* 1 when Krylov iterations replay a captured hipGraph */
+  double launch_tune_ratio;       /* graph / direct time of the same iterations measured
+                                     at hoc_setup (0 when launch_mode was set)           */
 } hoc_stats;
 
 /* Fill cfg with the reference defaults (src/HO_2D.cpp:3698-3713, 3066-3069). */

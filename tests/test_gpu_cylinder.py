@@ -26,7 +26,11 @@ def omega0(xy, seed=3):
 @pytest.mark.parametrize("precision", [hoc.HOC_FP64, hoc.HOC_MIXED])
 def test_ogrid_K6_vs_oracle(gpu_available, precision):
     m = meshes.cylinder_ogrid(8, 24, 3)
-    kw = dict(problem_type=3, Re=100.0, dt=2e-5, time_integration_type=4, advection_type=1, tol=1e-12, maxiter=600)
+    # With the inflow on, the wall vorticity flux slip * Re / dt (src/HO_2D.cpp:2656) turns the
+    # O(1) velocity's solve-tolerance differences into vorticity changes that the 1e-3 field
+    # shows relative to itself: tol 1e-12 leaves 3e-10 between two converged solvers, so both
+    # sides solve to 1e-14.
+    kw = dict(problem_type=3, Re=100.0, dt=2e-5, time_integration_type=4, advection_type=1, tol=1e-14, maxiter=600)
     o = Oracle(m, 6, **kw)
     g = hoc.HO2D(K=6, precision=precision, **kw)
     g.load_mesh(m)

@@ -39,7 +39,7 @@ def test_facade_example_vs_oracle(gpu_available, tmp_path):
     n, K, steps, dt = 8, 4, 5, 1e-4
     pts = np.array([[0.5, 0.5], [0.25, 0.8], [0.9, 0.1]])
     sp = tmp_path / "sample_points.dat"
-    sp.write_text("".join(f"{x!r} {y!r}\n" for x, y in pts))
+    sp.write_text("".join(f"{x:.17g} {y:.17g}\n" for x, y in pts))
     out = tmp_path / "fields.bin"
     r = subprocess.run([EXE, str(n), str(K), str(steps), repr(dt), str(out), str(sp)], cwd=tmp_path,
                        capture_output=True, text=True, timeout=120)

@@ -1,6 +1,7 @@
-"""Mixed precision (HOC_MIXED): FP64 iterative refinement around an FP32 Krylov +
-FP32 V-cycle inner solve.  The outer residual and the stopping test are FP64, so
-fields must match the FP64 oracle to the same tolerance as the plain path; the
+"""Mixed precision (HOC_MIXED): the FP64 Krylov method (PCG or BiCGSTAB) with the FP32
+V-cycle as its preconditioner.  The Krylov vectors, the operator application, the residual
+and the stopping test are FP64, so fields must match the FP64 oracle to the same tolerance
+as the plain path; the
 acceptance bar for mixed mode is a relative L2 field error <= 1e-8 (plain and
 quadrature-weighted norms), asserted here at 1e-10 on the small cases."""
 import numpy as np
@@ -50,11 +51,8 @@ def test_mixed_poisson(gpu_available, name, mk, K):
     print(name, "rel", rel_l2(p_got, p_ref), "its", st["last_iters"], "refinements", st["last_refinements"],
           "res", st["last_rel_residual"], "krylov", st["krylov"])
     assert st["precision"] == hoc.HOC_MIXED
-    assert st["last_refinements"] >= 1
-    # the FP64 residual b - A psi itself carries rounding of ~1e-14 relative: the refinement
-    # stops there (the plain path's recursively updated residual can read lower)
-    assert st["last_rel_residual"] <= 5e-14
-    assert st["last_refinements"] < 12
+    assert st["last_refinements"] == 0          # no refinement passes: one FP64 Krylov solve
+    assert st["last_rel_residual"] <= 1e-14
     assert rel_l2(p_got, p_ref) < 1e-11
 
 
@@ -85,7 +83,7 @@ def test_mixed_cavity_steps(gpu_available, integ):
 
 
 def test_mixed_bfs_steps(gpu_available):
-    """Jittered backward-facing step: non-symmetric W A -> FP32 BiCGSTAB inner solve."""
+    """Jittered backward-facing step: non-symmetric W A -> FP64 BiCGSTAB with the FP32 V-cycle."""
     m = meshes.backward_facing_step(24, 8, 2, jitter=0.08, seed=2)
     o = Oracle(m, 3, problem_type=2, Re=400.0, dt=5e-4, time_integration_type=4, advection_type=1, tol=1e-12,
                maxiter=500)

@@ -26,7 +26,7 @@ CASES = {
 def pair(name, steps=3):
     mk, K, kw = CASES[name]
     m = mk()
-    common = dict(time_integration_type=4, advection_type=1, tol=1e-13, maxiter=400, **kw)
+    common = dict(time_integration_type=4, advection_type=1, tol=1e-14, maxiter=400, **kw)
     o = Oracle(m, K, **common)
     g = hoc.HO2D(K=K, **common)
     g.load_mesh(m)
