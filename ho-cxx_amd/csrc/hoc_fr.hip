@@ -524,8 +524,10 @@ __global__ __launch_bounds__(kWG) void k_rhs(int N, Tab T, Geo G, const double* 
 // stage kernel needs from neighbours: omega value/derivative traces and the contravariant
 // normal-velocity trace.  AFF: metrics constant per element (G.gel, G.gface, G.gjm).
 // ==========================================================================================
+// Affine instances with K <= 5 are held to 2 waves per SIMD (256 registers; a few spill to
+// scratch), which beats one wave with no spills.
 template <int K, bool AFF, bool BATCH = false>
-__global__ __launch_bounds__(kWG) void k_velocity(int N, Tab T, Geo G, BCDev B, const double* __restrict__ psi,
+__global__ __launch_bounds__(kWG, (AFF && K <= 5) ? 2 : 1) void k_velocity(int N, Tab T, Geo G, BCDev B, const double* __restrict__ psi,
                                                    const double* __restrict__ trpsi, const double* __restrict__ omega,
                                                    double* __restrict__ vel, double* __restrict__ U,
                                                    double* __restrict__ trw, double* __restrict__ tru,
@@ -636,7 +638,8 @@ __global__ __launch_bounds__(kWG) void k_velocity(int N, Tab T, Geo G, BCDev B, 
   __syncthreads();
 #pragma unroll
   for (int m = 0; m < K2; ++m) { lds[threadIdx.x * 2 * K2 + 2 * m] = vx[m]; lds[threadIdx.x * 2 * K2 + 2 * m + 1] = vy[m]; }
-  tile_store<2 * K2>(vel, lds, e0, N);
+  if (vel) tile_store<2 * K2>(vel, lds, e0, N);  // (uniform) intermediate RK stages skip it
+  else __syncthreads();
 #pragma unroll
   for (int m = 0; m < K2; ++m) { lds[threadIdx.x * 2 * K2 + m] = U1[m]; lds[threadIdx.x * 2 * K2 + K2 + m] = U2[m]; }
   tile_store<2 * K2>(U, lds, e0, N);

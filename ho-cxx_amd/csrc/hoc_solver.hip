@@ -16,6 +16,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <memory>
 #include <mutex>
 #include <stdexcept>
@@ -1226,7 +1227,10 @@ struct Solver {
     poisson(win);
     if (capture_psi && rk_stage >= 0) dcopy(capture_psi + (size_t)rk_stage * D, psi, D);
     HIPCHK(launch_trace(K, PRE_COPY, N, T, psi, nullptr, nullptr, nullptr, 0., nullptr, 0, nullptr, P64.tr.p, stream));
-    HIPCHK(launch_velocity(K, N, T, G, B, psi, P64.tr, win, vel, U, trw, tru, 1. / (cfg.dt * re_inv), I, stream));
+    // the Cartesian velocity is an output of the step's last stage only (what the getters and
+    // writers read); intermediate RK stages need the contravariant fluxes U alone
+    double* v_out = (so.next && !so.final_mode) ? nullptr : vel.p;
+    HIPCHK(launch_velocity(K, N, T, G, B, psi, P64.tr, win, v_out, U, trw, tru, 1. / (cfg.dt * re_inv), I, stream));
     HIPCHK(launch_stage(K, cfg.advection_type, N, T, G, B, win, U, trw, tru, re_inv, cfg.dt, I, so, stream));
   }
   void stage(const double* win, double c, const StageOut& so) {
@@ -1386,6 +1390,14 @@ struct Solver {
     double ztau_a = 0., ztau_b = 0.;
     int zn = 0;
     DBuf<double> ckbuf, segbuf, segpsi, save_w, save_psi, save_vel;
+    // schedule: 0 uniform segments, 1 every step recorded by the forward run, 2 binomial
+    int sched = 0;
+    DBuf<double> trajw, trajpsi;   // sched 1: omega_n and the four stage solutions of every step
+    DBuf<double> snap;             // sched 2: omega snapshots
+    int nsnap = 0;
+    std::vector<int> snap_pos;     // sched 2: step held by each snapshot slot (-1: free)
+    std::vector<int32_t> bsplit;   // sched 2: optimal first advance for (range, free slots)
+    int bsplit_L = 0, bsplit_S = 0;
     int nsteps = 0, every = 0;
     double t0 = 0.;
     bool have_forward = false;
@@ -1649,19 +1661,115 @@ struct Solver {
     HIPCHK(launch_adj_wj(N, K, tab.wgt, d_jac, adj.z, xbar, 1, stream));
   }
 
-  // Forward run of nsteps RK4 steps keeping the vorticity every `every` steps.
+  // Binomial checkpointing: F(L, s) = the fewest step advances that reverse L steps from a
+  // snapshot at the range's start with s free snapshot slots, each step then recorded once:
+  // F(1, s) = 0, F(L, 0) = L (L - 1) / 2, F(L, s) = min_d [d + F(L - d, s - 1) + F(d, s)]
+  // (advance d steps and snapshot, reverse the tail with one slot less, then the head).  The
+  // table of optimal first advances d is built once per (L, s); above kBinoMaxL steps the range
+  // is halved instead.
+  static constexpr int kBinoMaxL = 2048;
+  void binomial_table(int L, int S) {
+    if (L <= adj.bsplit_L && S <= adj.bsplit_S) return;
+    L = std::min(std::max(L, adj.bsplit_L), kBinoMaxL);
+    S = std::max(S, adj.bsplit_S);
+    std::vector<int64_t> F((size_t)(L + 1) * (S + 1), 0);
+    adj.bsplit.assign((size_t)(L + 1) * (S + 1), 1);
+    auto at = [&](int l, int sl) -> int64_t& { return F[(size_t)l * (S + 1) + sl]; };
+    for (int l = 1; l <= L; ++l) at(l, 0) = (int64_t)l * (l - 1) / 2;
+    for (int sl = 1; sl <= S; ++sl)
+      for (int l = 2; l <= L; ++l) {
+        int64_t best = INT64_MAX;
+        int bd = 1;
+        for (int d = 1; d < l; ++d) {
+          const int64_t c = d + at(l - d, sl - 1) + at(d, sl);
+          if (c < best) { best = c; bd = d; }
+        }
+        at(l, sl) = best;
+        adj.bsplit[(size_t)l * (S + 1) + sl] = bd;
+      }
+    adj.bsplit_L = L;
+    adj.bsplit_S = S;
+  }
+  int binomial_split(int L, int S) const {
+    if (L > adj.bsplit_L || S > adj.bsplit_S) return std::max(1, L / 2);
+    return adj.bsplit[(size_t)L * (adj.bsplit_S + 1) + S];
+  }
+  int snap_take(int pos) {
+    for (int i = 0; i < adj.nsnap; ++i)
+      if (adj.snap_pos[i] < 0) {
+        adj.snap_pos[i] = pos;
+        dcopy(adj.snap.p + (size_t)i * D, w, D);
+        return i;
+      }
+    throw StatusError(HOC_EINVAL, "binomial schedule ran out of snapshot slots");
+  }
+  int snap_find(int pos) const {
+    for (int i = 0; i < adj.nsnap; ++i)
+      if (adj.snap_pos[i] == pos) return i;
+    return -1;
+  }
+
+  // Forward run of nsteps RK4 steps recording what the reverse sweep needs (hoc_adjoint_forward):
+  //  every >= 1: the vorticity every `every` steps (uniform segments, each recomputed once);
+  //  every == 0: the vorticity and the four stage solutions of every step when they fit in half
+  //              the free device memory (no recompute), otherwise binomial checkpointing with the
+  //              snapshots that fit;
+  //  every < 0:  binomial checkpointing with -every vorticity snapshots.
   void adjoint_forward(int nsteps, int every) {
     adjoint_setup();
-    if (nsteps < 0 || every < 1) throw StatusError(HOC_EINVAL, "nsteps must be >= 0 and every >= 1");
+    if (nsteps < 0) throw StatusError(HOC_EINVAL, "nsteps must be >= 0");
     adj.have_forward = false;
-    const int nck = std::max(1, (nsteps + every - 1) / every);
-    adj.ckbuf.alloc((size_t)nck * D);
-    adj.segbuf.alloc((size_t)std::min(every, std::max(nsteps, 1)) * D);
-    adj.segpsi.alloc((size_t)std::min(every, std::max(nsteps, 1)) * 4 * D);
+    adj.trajw.alloc(0); adj.trajpsi.alloc(0); adj.snap.alloc(0);
     adj.t0 = time;
-    for (int n = 0; n < nsteps; ++n) {
-      if (n % every == 0) dcopy(adj.ckbuf.p + (size_t)(n / every) * D, w, D);
-      rk_step();
+    int sched = every >= 1 ? 0 : 2;
+    int nsnap = every < 0 ? -every : 0;
+    if (every == 0) {
+      size_t fr = 0, tot = 0;
+      HIPCHK(hipMemGetInfo(&fr, &tot));
+      const double budget = 0.5 * (double)fr;
+      const double per = (double)D * sizeof(double);
+      if (5. * per * std::max(nsteps, 1) <= budget) sched = 1;
+      else nsnap = (int)std::min<double>(budget / per, (double)std::max(nsteps, 1));
+      if (sched == 2 && nsnap < 1) throw StatusError(HOC_EDEVICE, "no device memory for adjoint snapshots");
+    }
+    adj.sched = sched;
+    if (sched == 0) {
+      const int nck = std::max(1, (nsteps + every - 1) / every);
+      adj.ckbuf.alloc((size_t)nck * D);
+      adj.segbuf.alloc((size_t)std::min(every, std::max(nsteps, 1)) * D);
+      adj.segpsi.alloc((size_t)std::min(every, std::max(nsteps, 1)) * 4 * D);
+      for (int n = 0; n < nsteps; ++n) {
+        if (n % every == 0) dcopy(adj.ckbuf.p + (size_t)(n / every) * D, w, D);
+        rk_step();
+      }
+    } else if (sched == 1) {
+      adj.trajw.alloc((size_t)std::max(nsteps, 1) * D);
+      adj.trajpsi.alloc((size_t)std::max(nsteps, 1) * 4 * D);
+      for (int n = 0; n < nsteps; ++n) {
+        dcopy(adj.trajw.p + (size_t)n * D, w, D);
+        capture_psi = adj.trajpsi.p + (size_t)n * 4 * D;
+        rk_step();
+        capture_psi = nullptr;
+      }
+    } else {
+      // snapshot slots: the start and the first descent of the schedule, taken on the way
+      nsnap = std::max(1, std::min(nsnap, std::max(nsteps, 1)));
+      adj.nsnap = nsnap;
+      adj.snap.alloc((size_t)nsnap * D);
+      adj.snap_pos.assign(nsnap, -1);
+      binomial_table(std::max(nsteps, 1), nsnap - 1);
+      snap_take(0);
+      int a = 0, free_slots = nsnap - 1;
+      int next = (nsteps - a > 1 && free_slots > 0) ? a + binomial_split(nsteps - a, free_slots) : -1;
+      for (int n = 0; n < nsteps; ++n) {
+        rk_step();
+        if (n + 1 == next) {
+          snap_take(next);
+          a = next;
+          --free_slots;
+          next = (nsteps - a > 1 && free_slots > 0) ? a + binomial_split(nsteps - a, free_slots) : -1;
+        }
+      }
     }
     HIPCHK(hipStreamSynchronize(stream));
     adj.nsteps = nsteps;
@@ -1669,13 +1777,15 @@ struct Solver {
     adj.have_forward = true;
   }
 
-  // Reverse sweep over the recorded trajectory: segments of `every` steps are recomputed from
-  // their checkpoint (keeping each step's vorticity), then each step is re-run recording its
-  // stages and transposed stage by stage:
+  // Reverse sweep over the recorded trajectory.  Each step is re-run from its vorticity recording
+  // its stages (with the stage solutions kept by the forward run or a recompute, or solving them)
+  // and transposed stage by stage:
   //   k4_bar = w_bar/6, k3_bar = k2_bar = w_bar/3, k1_bar = w_bar/6,
   //   for s = 4..1: x = (dk_s/domega_s)^T k_s_bar;  w_bar += x;  k_{s-1}_bar += c_s x
-  // with omega_s = omega_n + c_s k_{s-1}, c = (1/2, 1/2, 1).  The handle's state (vorticity,
-  // stream function, velocity, time) is restored afterwards.
+  // with omega_s = omega_n + c_s k_{s-1}, c = (1/2, 1/2, 1).  Uniform segments are recomputed from
+  // their checkpoint keeping every step's vorticity and stage solutions; recorded trajectories
+  // need no recompute; the binomial schedule advances from its snapshots as it goes.  The
+  // handle's state (vorticity, stream function, velocity, time) is restored afterwards.
   void adjoint_backward(const double* wbar_h, double* w0bar_h, double* pvbar_h) {
     if (!adj.have_forward) throw StatusError(HOC_ESTATE, "call hoc_adjoint_forward first");
     adjoint_check();
@@ -1691,7 +1801,6 @@ struct Solver {
     const int every = adj.every, nsteps = adj.nsteps;
     adj.zn = 0;
     proj_adj.m = 0;
-    const int nseg = (nsteps + every - 1) / every;
     auto restore = [&] {
       dcopy(w, adj.save_w, D);
       dcopy(psi, adj.save_psi, D);
@@ -1699,32 +1808,69 @@ struct Solver {
       time = time_end_;
       invalidate_guess();   // the recompute left the histories of the trajectory's early steps
     };
+    // w = omega_n on entry
+    auto reverse_step = [&](int n, const double* stage_psi) {
+      time = adj.t0 + n * cfg.dt;
+      adj_record_step(stage_psi);
+      HIPCHK(launch_adj_axpby(D, 1. / 6., adj.wbar, 0., adj.kbar[3], stream));
+      HIPCHK(launch_adj_axpby(D, 1. / 3., adj.wbar, 0., adj.kbar[2], stream));
+      HIPCHK(launch_adj_axpby(D, 1. / 3., adj.wbar, 0., adj.kbar[1], stream));
+      HIPCHK(launch_adj_axpby(D, 1. / 6., adj.wbar, 0., adj.kbar[0], stream));
+      for (int st = 3; st >= 0; --st) {
+        adj_stage(st, adj.kbar[st], adj.xbar, n + cst[st]);
+        HIPCHK(launch_adj_axpby(D, 1., adj.xbar, 1., adj.wbar, stream));
+        if (st > 0) HIPCHK(launch_adj_axpby(D, cn[st - 1], adj.xbar, 1., adj.kbar[st - 1], stream));
+      }
+    };
+    // binomial schedule: w = omega_{a+d}, advanced from the snapshot at a
+    auto advance_from = [&](int a, int d) {
+      const int sl = snap_find(a);
+      if (sl < 0) throw StatusError(HOC_ESTATE, "binomial schedule lost a snapshot");
+      dcopy(w, adj.snap.p + (size_t)sl * D, D);
+      time = adj.t0 + a * cfg.dt;
+      invalidate_guess();
+      for (int i = 0; i < d; ++i) rk_step();
+    };
+    std::function<void(int, int, int)> rev = [&](int a, int b, int free_slots) {
+      if (b <= a) return;
+      if (b - a == 1) { advance_from(a, 0); reverse_step(a, nullptr); return; }
+      if (free_slots == 0) {
+        for (int n = b - 1; n >= a; --n) { advance_from(a, n - a); reverse_step(n, nullptr); }
+        return;
+      }
+      const int d = binomial_split(b - a, free_slots);
+      int sl = snap_find(a + d);
+      if (sl < 0) { advance_from(a, d); sl = snap_take(a + d); }
+      rev(a + d, b, free_slots - 1);
+      adj.snap_pos[sl] = -1;
+      rev(a, a + d, free_slots);
+    };
     try {
-      for (int sg = nseg - 1; sg >= 0; --sg) {
-        const int n0 = sg * every, n1 = std::min(n0 + every, nsteps);
-        dcopy(w, adj.ckbuf.p + (size_t)sg * D, D);
-        time = adj.t0 + n0 * cfg.dt;
-        invalidate_guess();
-        for (int n = n0; n < n1; ++n) {
-          dcopy(adj.segbuf.p + (size_t)(n - n0) * D, w, D);
-          capture_psi = adj.segpsi.p + (size_t)(n - n0) * 4 * D;
-          rk_step();
-          capture_psi = nullptr;
-        }
-        for (int n = n1 - 1; n >= n0; --n) {
-          dcopy(w, adj.segbuf.p + (size_t)(n - n0) * D, D);
-          time = adj.t0 + n * cfg.dt;
-          adj_record_step(adj.segpsi.p + (size_t)(n - n0) * 4 * D);
-          HIPCHK(launch_adj_axpby(D, 1. / 6., adj.wbar, 0., adj.kbar[3], stream));
-          HIPCHK(launch_adj_axpby(D, 1. / 3., adj.wbar, 0., adj.kbar[2], stream));
-          HIPCHK(launch_adj_axpby(D, 1. / 3., adj.wbar, 0., adj.kbar[1], stream));
-          HIPCHK(launch_adj_axpby(D, 1. / 6., adj.wbar, 0., adj.kbar[0], stream));
-          for (int st = 3; st >= 0; --st) {
-            adj_stage(st, adj.kbar[st], adj.xbar, n + cst[st]);
-            HIPCHK(launch_adj_axpby(D, 1., adj.xbar, 1., adj.wbar, stream));
-            if (st > 0) HIPCHK(launch_adj_axpby(D, cn[st - 1], adj.xbar, 1., adj.kbar[st - 1], stream));
+      if (adj.sched == 0) {
+        const int nseg = (nsteps + every - 1) / every;
+        for (int sg = nseg - 1; sg >= 0; --sg) {
+          const int n0 = sg * every, n1 = std::min(n0 + every, nsteps);
+          dcopy(w, adj.ckbuf.p + (size_t)sg * D, D);
+          time = adj.t0 + n0 * cfg.dt;
+          invalidate_guess();
+          for (int n = n0; n < n1; ++n) {
+            dcopy(adj.segbuf.p + (size_t)(n - n0) * D, w, D);
+            capture_psi = adj.segpsi.p + (size_t)(n - n0) * 4 * D;
+            rk_step();
+            capture_psi = nullptr;
+          }
+          for (int n = n1 - 1; n >= n0; --n) {
+            dcopy(w, adj.segbuf.p + (size_t)(n - n0) * D, D);
+            reverse_step(n, adj.segpsi.p + (size_t)(n - n0) * 4 * D);
           }
         }
+      } else if (adj.sched == 1) {
+        for (int n = nsteps - 1; n >= 0; --n) {
+          dcopy(w, adj.trajw.p + (size_t)n * D, D);
+          reverse_step(n, adj.trajpsi.p + (size_t)n * 4 * D);
+        }
+      } else {
+        rev(0, nsteps, adj.nsnap - 1);
       }
       d2h(w0bar_h, adj.wbar.p, D * sizeof(double));
       if (pvbar_h && E > 0) d2h(pvbar_h, adj.pvbar.p, (size_t)E * K * sizeof(double));

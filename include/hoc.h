@@ -237,12 +237,16 @@ int hoc_set_wall_velocity(hoc_t h, int64_t len, const double* v);
  * callers).  Supported: FP64, one instance, problem_type 1 walls with Dirichlet stream
  * function, advection_type 1, time_integration_type 4, affine elements (symmetric Poisson
  * operator); otherwise HOC_EINVAL.
- * hoc_adjoint_forward advances nsteps RK4 steps like hoc_step, keeping the vorticity every
- * checkpoint_every steps on the device.  hoc_adjoint_backward takes the gradient of an objective
- * with respect to the final vorticity and returns its gradient with respect to the vorticity at
- * the start of that run and (optional, len_wall = E K) to the wall velocity BC_parl_vel.  It
- * recomputes segments of checkpoint_every steps from their checkpoint; the upwind switches of
- * the advective flux (|dw| > 1e-4) are taken as fixed at their recomputed state.  The handle's
+ * hoc_adjoint_forward advances nsteps RK4 steps like hoc_step, keeping on the device what the
+ * reverse sweep needs: checkpoint_every >= 1: the vorticity every checkpoint_every steps (the sweep
+ * recomputes each segment once); 0: every step's vorticity and stage stream functions when they
+ * fit in half the free device memory (no recompute), else binomial checkpointing with the
+ * vorticity snapshots that fit; < 0: binomial checkpointing with -checkpoint_every snapshots (the
+ * schedule of fewest step advances, each step then recorded once).  hoc_adjoint_backward takes
+ * the gradient of an objective with respect to the final vorticity and returns its gradient with
+ * respect to the vorticity at the start of that run and (optional, len_wall = E K) to the wall
+ * velocity BC_parl_vel; the upwind switches of
+ * the advective flux (|dw| > 1e-4) are taken as fixed at their recorded state.  The handle's
  * vorticity, stream function, velocity and time are left as hoc_adjoint_forward left them.
  * Poisson solves of the sweep count in hoc_stats. */
 int hoc_adjoint_forward(hoc_t h, int32_t nsteps, int32_t checkpoint_every);
