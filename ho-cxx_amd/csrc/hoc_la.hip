@@ -103,9 +103,10 @@ __global__ void k_zero_if_flag(size_t n, R* __restrict__ x, const int* __restric
 }
 
 // Scalar logic of the Krylov solvers.  pa: np partials of q0 then np of q1.  With mirror
-// set (pinned host memory), thread 0 finally publishes seq * 2 + (active flag) there with a
-// system-scope store, whether or not the phase ran, so the host can follow the iterations
-// without a copy (the host reads only this word, so the store needs no ordering).
+// set (pinned host memory), thread 0 finally publishes seq * 4 + near * 2 + (active flag) there
+// with a system-scope store, whether or not the phase ran, so the host can follow the iterations
+// without a copy (the host reads only this word, so the store needs no ordering); near: still
+// active with the residual within 8x of the tolerance, so the next iteration may be the last.
 __device__ void finalize_phase(int phase, double a, double b, double* __restrict__ S, int* __restrict__ F,
                                int maxiter);
 __global__ void k_finalize(int phase, const double* __restrict__ pa, int np, double* __restrict__ S,
@@ -122,7 +123,9 @@ __global__ void k_finalize(int phase, const double* __restrict__ pa, int np, dou
   }
   if (mirror && threadIdx.x == 0) {
     const int sq = seq >= 0 ? seq : ++F[F_SEQ];
-    __hip_atomic_store(mirror, sq * 2 + (F[F_ACTIVE] ? 1 : 0), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    const int act = F[F_ACTIVE] ? 1 : 0;
+    const int near = act && S[S_RR] <= 64. * S[S_TOL2] * S[S_BB];  // residual within 8x of the tolerance
+    __hip_atomic_store(mirror, sq * 4 + near * 2 + act, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
   }
 }
 __global__ void k_set_int(int* __restrict__ p, int v) { *p = v; }

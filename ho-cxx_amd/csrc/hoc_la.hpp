@@ -34,7 +34,8 @@ template <class R>
 void bicg_half(hipStream_t s, size_t n, R* x, const R* ph, const double* S, int* F);
 template <class R>
 void zero_if_flag(hipStream_t s, size_t n, R* x, const int* F);
-// mirror (optional): pinned host int that receives seq * 2 + (active flag) when the kernel ends;
+// mirror (optional): pinned host int that receives seq * 4 + near * 2 + (active flag) when the
+// kernel ends (near: active with the residual within 8x of the tolerance);
 // seq < 0: the next number of the device counter F[F_SEQ] (incremented) instead of seq.
 // maxiter < 0: the budget is read from F[F_MAXIT].
 void finalize(hipStream_t s, int phase, const double* part, int np, double* S, int* F, int maxiter,

@@ -143,7 +143,7 @@ struct Solver {
   Prec<float> P32;
   DBuf<double> part, S;
   DBuf<int> F;
-  // Krylov progress published by the end-of-iteration finalize kernel (seq * 2 + active), in
+  // Krylov progress published by the end-of-iteration finalize kernel (seq * 4 + near * 2 + active), in
   // pinned coherent host memory; seq counts iterations over the handle's lifetime.
   int* mirror = nullptr;
   int* mirror_dev = nullptr;
@@ -623,17 +623,24 @@ struct Solver {
   // seq_first: the first iteration of this solve (no wait before it has run).  The wait spins
   // on the pinned word and checks the stream every 4096 reads, so an idle stream without the
   // expected status (or a device error) ends it with an error instead of a hang.
-  bool poll(int seq, int seq_first, bool final_wait, int lag = 1) {
+  bool poll(int seq, int seq_first, bool final_wait, int lag = 1, bool* near = nullptr) {
     const int target = final_wait ? seq : seq - lag;
+    if (near) *near = false;
     if (target < seq_first) return true;
     for (unsigned spin = 1;; ++spin) {
       const int v = __atomic_load_n(mirror, __ATOMIC_ACQUIRE);
-      if ((v >> 1) >= target) return (v & 1) != 0;
+      if ((v >> 2) >= target) {
+        if (near) *near = (v & 2) != 0;
+        return (v & 1) != 0;
+      }
       if ((spin & 4095) == 0) {
         const hipError_t q = hipStreamQuery(stream);
         if (q == hipSuccess) {
           const int v2 = __atomic_load_n(mirror, __ATOMIC_ACQUIRE);
-          if ((v2 >> 1) >= target) return (v2 & 1) != 0;
+          if ((v2 >> 2) >= target) {
+            if (near) *near = (v2 & 2) != 0;
+            return (v2 & 1) != 0;
+          }
           throw HipError("Krylov status of an iteration was not published");
         }
         if (q != hipErrorNotReady) HIPCHK(q);
@@ -774,7 +781,11 @@ struct Solver {
     for (int it = 0;; ++it) {
       const int seq = seq_next++;
       krylov_iter(P, x, r, Sp, Fp, maxit, seq);
-      if (!poll(seq, seq0, it > maxit_host + 2) || it > maxit_host + 4) break;
+      bool near = false;
+      if (!poll(seq, seq0, it > maxit_host + 2, 1, &near) || it > maxit_host + 4) break;
+      // the previous iteration ended within 8x of the tolerance: the one just queued is likely
+      // the last, so wait for it instead of queuing one that would run predicated off
+      if (near && !poll(seq, seq0, true)) break;
     }
   }
 

@@ -23,6 +23,12 @@
  * configuration: HO_ORACLE_RELAX=0 (spai0), HO_ORACLE_UNSCALED=1,
  * HO_ORACLE_SA_RHO=1 (omega = 4/3 / rho), HO_ORACLE_CHEB_DEG.  A converged solve of
  * A psi = b does not depend on the preconditioner beyond the tolerance.
+ *
+ * Pins: every function is checked against what the mathematics fixes (tests/test_oracle_pins.py:
+ * tables, metrics, Laplacian, manufactured Poisson, RK4; tests/test_oracle_pins_flow.py: advective
+ * conservation, upwinding and polynomial exactness, the wall-circulation identity, Neumann data,
+ * inflow and hybrid boundary programmes) or against the reference's own src/Mesh.cpp (connectivity).
+ * Parity unpinned: only AMGCL's exact hierarchy, which a converged solve does not depend on.
  */
 #include "ho_oracle.h"
 

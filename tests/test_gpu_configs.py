@@ -3,9 +3,10 @@
 - Backward-facing step, 384 x 64 jittered grid (22 528 elements), K=5, Re=400, dt=5e-4:
   Poisson solve and stage residual against the oracle on the full mesh; FP64 vs mixed
   after 20 RK4 steps on the full mesh (size-independent agreement property).
-- Lid-driven cavity 32 x 32, K=4, Re=100: 100 RK4 steps against the oracle.  The
+- Lid-driven cavity 32 x 32, K=4, Re=100: 100 RK4 steps against the oracle run in the test, and
+  2000 steps (to t = 1, the configured end time) against the oracle's fingerprints.  The
   configured dt=1e-3 exceeds the explicit RK4 diffusion limit of this mesh (7.3e-4, from
-  rho(J^-1 A) = 3.83e5; the oracle itself diverges within 15 steps), so the run uses 5e-4.
+  rho(J^-1 A) = 3.83e5; the oracle itself diverges within 15 steps), so the runs use 5e-4.
 """
 import numpy as np
 import pytest
@@ -14,7 +15,7 @@ import hoc
 import pyoracle
 from hoc import meshes
 from pyoracle import Oracle
-from conftest import rel_l2
+from conftest import fixture_check, rel_l2
 
 pytestmark = pytest.mark.gpu
 
@@ -98,3 +99,14 @@ def test_cavity32_K4_100_steps(gpu_available):
     print("cavity 32x32 K=4, 100 steps: omega", ew, "psi", ep, "t", g.time)
     assert abs(g.time - 0.05) < 1e-12
     assert ew < 1e-11 and ep < 1e-11
+
+
+def test_cavity32_K4_2000_steps_vs_oracle(gpu_available):
+    m = meshes.cavity(32, 2)
+    g = hoc.HO2D(K=4, problem_type=1, Re=100.0, dt=5e-4, time_integration_type=4, advection_type=1, tol=1e-13,
+                 maxiter=500)
+    g.load_mesh(m)
+    g.setup()
+    g.step(2000)
+    assert abs(g.time - 1.0) < 1e-12
+    fixture_check(g, "cavity32_K4", 1e-11)

@@ -19,11 +19,10 @@ import pytest
 
 import hoc
 from hoc import meshes
-from conftest import rel_l2
+from conftest import fixture_check, rel_l2
 
 pytestmark = pytest.mark.gpu
 
-GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 
 def handle(m, K, **kw):
@@ -101,25 +100,6 @@ def test_bench_workload_circulation(gpu_available, precision):
         got = circ(g, g.get_vorticity())
         print("512x512 K=5 precision", precision, "step", s, "circulation", got, "expected", -1.0 + e)
         assert abs(got - (-1.0 + e)) < 1e-9
-
-
-def fixture_check(g, name, tol):
-    z = np.load(os.path.join(GOLD, f"flow_{name}.npz"))
-    assert abs(g.time - float(z["time"])) < 1e-12
-    idx = z["idx"]
-    out = {}
-    for fld, x in (("omega", g.get_vorticity()), ("psi", g.get_streamfunction())):
-        rng = np.random.default_rng(1234)
-        ridx = np.sort(rng.choice(len(x), size=min(4096, len(x)), replace=False))
-        assert np.array_equal(ridx, idx)
-        proj = np.array([rng.standard_normal(len(x)) @ x for _ in range(len(z[fld + "_proj"]))])
-        e_sample = rel_l2(x[idx], z[fld + "_sample"])
-        e_proj = float(np.sqrt(np.mean((proj - z[fld + "_proj"]) ** 2)) / z[fld + "_norm"])
-        e_norm = abs(np.linalg.norm(x) - float(z[fld + "_norm"])) / float(z[fld + "_norm"])
-        out[fld] = (e_sample, e_proj, e_norm)
-        print(name, fld, "sampled rel L2", e_sample, "projected estimate of rel L2", e_proj, "norm", e_norm)
-    for fld, (es, ep, en) in out.items():
-        assert es < tol and ep < tol and en < tol, (fld, es, ep, en)
 
 
 def test_bfs_full_10_steps_vs_oracle(gpu_available):
