@@ -2,6 +2,7 @@
 #include "hoc_la.hpp"
 
 #include <algorithm>
+#include <type_traits>
 
 namespace hoc {
 
@@ -438,30 +439,53 @@ void psi_combine(hipStream_t s, size_t n, int nterm, const double* a, const doub
 }
 constexpr int kProjBlocks = 1024;
 int proj_part_len(size_t n) { return (int)std::min<size_t>(kProjBlocks, (n + 255) / 256 + 1); }
-__global__ __launch_bounds__(256) void k_proj_dots(size_t n, int m, const double* __restrict__ X,
-                                                   const double* __restrict__ y, double* __restrict__ part) {
-  double s[kProjMax];
-#pragma unroll
-  for (int j = 0; j < kProjMax; ++j) s[j] = 0.;
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-    const double yi = y[i];
-#pragma unroll
-    for (int j = 0; j < kProjMax; ++j)
-      if (j < m) s[j] += X[(size_t)j * n + i] * yi;
+
+// The projection kernels take the basis size M as a template argument, so that the M + 2 loads
+// of one dof are issued back to back (one streaming pass over the basis at full bandwidth);
+// proj_dispatch maps the run-time m in [lo, kProjMax] to the instance.
+template <int M, int LO, class Fn>
+void proj_dispatch_impl(int m, Fn& fn) {
+  if constexpr (M >= LO) {
+    if (m == M) fn(std::integral_constant<int, M>{});
+    else proj_dispatch_impl<M - 1, LO>(m, fn);
   }
-  __shared__ double red[4][kProjMax];  // 4 wavefronts of 64
+}
+template <int LO, class Fn>
+void proj_dispatch(int m, Fn fn) { proj_dispatch_impl<kProjMax, LO>(m, fn); }
+
+// Sum over the block (256 threads) of NV per-thread values; thread k < NV writes sum k to out[k].
+template <int NV>
+__device__ __forceinline__ void block_sums(double (&v)[NV], double* __restrict__ out) {
+  __shared__ double red[4][NV];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll
-  for (int j = 0; j < kProjMax; ++j) {
-    double v = s[j];
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    if (lane == 0) red[wv][j] = v;
+  for (int k = 0; k < NV; ++k) {
+    const double w = wsum(v[k]);
+    if (lane == 0) red[wv][k] = w;
   }
   __syncthreads();
-  if (threadIdx.x < m) {
-    const int j = threadIdx.x;
-    part[(size_t)blockIdx.x * m + j] = red[0][j] + red[1][j] + red[2][j] + red[3][j];
+  if (threadIdx.x < NV) {
+    const int k = threadIdx.x;
+    out[k] = red[0][k] + red[1][k] + red[2][k] + red[3][k];
   }
+}
+// part[block][0, M) = Y_j . t,  part[block][M, 2M) = Y_j . b
+template <int M>
+__global__ __launch_bounds__(256) void k_proj_dots2(size_t n, const double* __restrict__ Y, const double* __restrict__ t,
+                                                    const double* __restrict__ b, double* __restrict__ part) {
+  double s[2 * M];
+#pragma unroll
+  for (int k = 0; k < 2 * M; ++k) s[k] = 0.;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    const double tv = t[i], bv = b[i];
+#pragma unroll
+    for (int j = 0; j < M; ++j) {
+      const double y = Y[(size_t)j * n + i];
+      s[j] += y * tv;
+      s[M + j] += y * bv;
+    }
+  }
+  block_sums<2 * M>(s, part + (size_t)blockIdx.x * 2 * M);
 }
 // one block per dot product: strided partial sums, then a fixed-order tree (deterministic)
 __global__ __launch_bounds__(256) void k_proj_fin(int nb, int m, const double* __restrict__ part,
@@ -475,77 +499,120 @@ __global__ __launch_bounds__(256) void k_proj_fin(int nb, int m, const double* _
   __syncthreads();
   if (threadIdx.x == 0) out[j] = red[0] + red[1] + red[2] + red[3];
 }
-__global__ void k_proj_combine(size_t n, int m, const double* __restrict__ X, const double* __restrict__ c,
-                               int accumulate, double* __restrict__ out) {
+// out += sum_j c_j X_j
+template <int M>
+__global__ __launch_bounds__(256) void k_proj_combine(size_t n, const double* __restrict__ X, const double* __restrict__ c,
+                                                      double* __restrict__ out) {
+  double cj[M];
+#pragma unroll
+  for (int j = 0; j < M; ++j) cj[j] = c[j];
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-    double v = 0.;
-    for (int j = 0; j < m; ++j) v += c[j] * X[(size_t)j * n + i];
-    out[i] = accumulate == 1 ? out[i] - v : accumulate == 2 ? out[i] + v : v;
+    double v = out[i];
+#pragma unroll
+    for (int j = 0; j < M; ++j) v += cj[j] * X[(size_t)j * n + i];
+    out[i] = v;
   }
 }
-// x -= sum_j c_j X_j and ax -= sum_j c_j AX_j in one pass (a Gram-Schmidt step on a vector and its image)
-__global__ void k_proj_combine2(size_t n, int m, const double* __restrict__ X, const double* __restrict__ AX,
-                                const double* __restrict__ c, double* __restrict__ x, double* __restrict__ ax) {
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-    double v = 0., w = 0.;
-    for (int j = 0; j < m; ++j) {
-      v += c[j] * X[(size_t)j * n + i];
-      w += c[j] * AX[(size_t)j * n + i];
-    }
-    x[i] -= v;
-    ax[i] -= w;
-  }
-}
-// x = psi, ax = b - r (As psi, r the solve's final residual), with per-block partials of
-// (mr ? ax : x) . ax in the k_proj_dots layout (m = 1)
-__global__ __launch_bounds__(256) void k_proj_start(size_t n, const double* __restrict__ psi, const double* __restrict__ b,
+// First Gram-Schmidt pass on a new solution, with the coefficients e computed before its solve:
+//   x = psi - sum_j e_j X_j,  ax = (b - r) - sum_j e_j AX_j,
+// and the per-block partials of f_j = Y_j . ax (the coefficients of a second pass), |x|^2 =
+// (mr ? ax : x) . ax and the norm before the pass (mr ? b - r : psi) . (b - r).
+template <int M>
+__global__ __launch_bounds__(256) void k_proj_ortho(size_t n, int mr, const double* __restrict__ X,
+                                                    const double* __restrict__ AX, const double* __restrict__ e,
+                                                    const double* __restrict__ psi, const double* __restrict__ b,
                                                     const double* __restrict__ r, double* __restrict__ x,
-                                                    double* __restrict__ ax, int mr, double* __restrict__ part) {
-  double s = 0.;
+                                                    double* __restrict__ ax, double* __restrict__ part) {
+  double ej[M > 0 ? M : 1], s[M + 2];
+#pragma unroll
+  for (int j = 0; j < M; ++j) ej[j] = e[j];
+#pragma unroll
+  for (int k = 0; k < M + 2; ++k) s[k] = 0.;
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-    const double xv = psi[i], av = b[i] - r[i];
+    const double p = psi[i], a0 = b[i] - r[i];
+    double xj[M > 0 ? M : 1], aj[M > 0 ? M : 1];
+    double xv = p, av = a0;
+#pragma unroll
+    for (int j = 0; j < M; ++j) {
+      xj[j] = X[(size_t)j * n + i];
+      aj[j] = AX[(size_t)j * n + i];
+    }
+#pragma unroll
+    for (int j = 0; j < M; ++j) {
+      xv -= ej[j] * xj[j];
+      av -= ej[j] * aj[j];
+    }
     x[i] = xv;
     ax[i] = av;
-    s += (mr ? av : xv) * av;
+#pragma unroll
+    for (int j = 0; j < M; ++j) s[j] += (mr ? aj[j] : xj[j]) * av;
+    s[M] += (mr ? av : xv) * av;
+    s[M + 1] += (mr ? a0 : p) * a0;
   }
-  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-  __shared__ double red[4];
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
-  __syncthreads();
-  if (threadIdx.x == 0) part[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+  block_sums<M + 2>(s, part + (size_t)blockIdx.x * (M + 2));
 }
-__global__ void k_proj_normalize(size_t n, const double* __restrict__ d, double* __restrict__ a, double* __restrict__ b) {
-  const double f = (d[0] > 1e-24 * fabs(d[1]) && d[0] > 0.) ? 1. / sqrt(d[0]) : 0.;
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-    a[i] *= f;
-    if (b) b[i] *= f;
+// Second Gram-Schmidt pass and normalisation: F = (f_0..f_{M-1}, |x|^2, norm before the first
+// pass).  The pass runs when the cosines left by the first exceed kProjReorth (sum f_j^2 >
+// kProjReorth^2 |x|^2); the squared norm after it is |x|^2 - sum f_j^2 (X orthonormal).  A vector
+// whose squared norm falls to 1e-24 of the first one (left in the span) becomes zero.
+constexpr double kProjReorth = 1e-7;
+template <int M>
+__global__ __launch_bounds__(256) void k_proj_fix(size_t n, const double* __restrict__ X, const double* __restrict__ AX,
+                                                  const double* __restrict__ F, double* __restrict__ x,
+                                                  double* __restrict__ ax) {
+  double fj[M > 0 ? M : 1], fsq = 0.;
+#pragma unroll
+  for (int j = 0; j < M; ++j) { fj[j] = F[j]; fsq += fj[j] * fj[j]; }
+  const double nn = F[M], n0 = F[M + 1];
+  const bool again = M > 0 && fsq > kProjReorth * kProjReorth * nn;
+  const double n2 = again ? nn - fsq : nn;
+  const double sc = (n2 > 1e-24 * fabs(n0) && n2 > 0.) ? 1. / sqrt(n2) : 0.;
+  if (again) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+      double xv = x[i], av = ax[i];
+#pragma unroll
+      for (int j = 0; j < M; ++j) {
+        xv -= fj[j] * X[(size_t)j * n + i];
+        av -= fj[j] * AX[(size_t)j * n + i];
+      }
+      x[i] = xv * sc;
+      ax[i] = av * sc;
+    }
+  } else {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+      x[i] *= sc;
+      ax[i] *= sc;
+    }
   }
 }
-void proj_dots(hipStream_t s, size_t n, int m, const double* X, const double* y, double* part, double* out) {
-  if (m <= 0 || n == 0) return;
+void proj_dots2(hipStream_t s, size_t n, int m, const double* Y, const double* t, const double* b, double* part,
+                double* out) {
+  if (m <= 0 || m > kProjMax || n == 0) return;
   const int nb = proj_part_len(n);
-  hipLaunchKernelGGL(k_proj_dots, dim3(nb), dim3(256), 0, s, n, m, X, y, part);
-  hipLaunchKernelGGL(k_proj_fin, dim3(m), dim3(256), 0, s, nb, m, part, out);
+  proj_dispatch<1>(m, [&](auto Mc) {
+    constexpr int M = decltype(Mc)::value;
+    hipLaunchKernelGGL(k_proj_dots2<M>, dim3(nb), dim3(256), 0, s, n, Y, t, b, part);
+  });
+  hipLaunchKernelGGL(k_proj_fin, dim3(2 * m), dim3(256), 0, s, nb, 2 * m, part, out);
 }
-void proj_combine(hipStream_t s, size_t n, int m, const double* X, const double* c, int accumulate, double* out) {
-  if (n == 0) return;
-  hipLaunchKernelGGL(k_proj_combine, dim3(vec_grid(n)), dim3(256), 0, s, n, m, X, c, accumulate, out);
+void proj_combine(hipStream_t s, size_t n, int m, const double* X, const double* c, double* out) {
+  if (m <= 0 || m > kProjMax || n == 0) return;
+  proj_dispatch<1>(m, [&](auto Mc) {
+    constexpr int M = decltype(Mc)::value;
+    hipLaunchKernelGGL(k_proj_combine<M>, dim3(vec_grid(n)), dim3(256), 0, s, n, X, c, out);
+  });
 }
-void proj_combine2(hipStream_t s, size_t n, int m, const double* X, const double* AX, const double* c, double* x,
-                   double* ax) {
-  if (n == 0 || m <= 0) return;
-  hipLaunchKernelGGL(k_proj_combine2, dim3(vec_grid(n)), dim3(256), 0, s, n, m, X, AX, c, x, ax);
-}
-void proj_start(hipStream_t s, size_t n, const double* psi, const double* b, const double* r, double* x, double* ax,
-                int mr, double* part, double* out) {
-  if (n == 0) return;
+void proj_ortho(hipStream_t s, size_t n, int m, int mr, const double* X, const double* AX, const double* e,
+                const double* psi, const double* b, const double* r, double* x, double* ax, double* part,
+                double* out) {
+  if (m < 0 || m > kProjMax || n == 0) return;
   const int nb = proj_part_len(n);
-  hipLaunchKernelGGL(k_proj_start, dim3(nb), dim3(256), 0, s, n, psi, b, r, x, ax, mr, part);
-  hipLaunchKernelGGL(k_proj_fin, dim3(1), dim3(256), 0, s, nb, 1, part, out);
-}
-void proj_normalize(hipStream_t s, size_t n, const double* d, double* a, double* b) {
-  if (n == 0) return;
-  hipLaunchKernelGGL(k_proj_normalize, dim3(vec_grid(n)), dim3(256), 0, s, n, d, a, b);
+  proj_dispatch<0>(m, [&](auto Mc) {
+    constexpr int M = decltype(Mc)::value;
+    hipLaunchKernelGGL(k_proj_ortho<M>, dim3(nb), dim3(256), 0, s, n, mr, X, AX, e, psi, b, r, x, ax, part);
+    hipLaunchKernelGGL(k_proj_fin, dim3(M + 2), dim3(256), 0, s, nb, M + 2, part, out);
+    hipLaunchKernelGGL(k_proj_fix<M>, dim3(vec_grid(n)), dim3(256), 0, s, n, X, AX, out, x, ax);
+  });
 }
 void psi_predict(hipStream_t s, size_t n, double f, int extrap, double* psi, double* prev) {
   if (n == 0) return;

@@ -67,22 +67,26 @@ void promote_dot(hipStream_t s, size_t n, const float* z32, double* z64, const d
 // and prev = psi (extrapolation and history shift in one pass).
 void psi_predict(hipStream_t s, size_t n, double f, int extrap, double* psi, double* prev);
 
-// Projection onto an As-orthonormal basis of earlier solutions (m <= kProjMax vectors X_j of
-// length n, stored contiguously, with AX_j = As X_j):
-//   proj_dots:    out[j] = X_j . y  (j < m), through per-block partials (part: proj_part_len(n) * m)
-//   proj_combine: out = sum_j c[j] X_j (accumulate 0), out - sum (1) or out + sum (2)
-//   proj_combine2: x -= sum_j c[j] X_j and ax -= sum_j c[j] AX_j in one pass
-//   proj_start:   x = psi, ax = b - r, out[0] = (mr ? ax : x) . ax (deterministic)
-//   proj_normalize: f = 1/sqrt(d[0]) when d[0] > tiny * d[1] (else 0); a *= f, b *= f (b may be null)
+// Projection onto a basis of earlier solutions (m <= kProjMax vectors X_j of length n, stored
+// contiguously, with AX_j = As X_j; Y = X for an As-orthonormal basis, Y = AX for an orthonormal
+// AX):
+//   proj_dots2:   out[j] = Y_j . t and out[m + j] = Y_j . b (j < m), through per-block partials
+//                 (part: proj_part_len(n) * 2m), deterministic
+//   proj_combine: out += sum_j c[j] X_j
+//   proj_ortho:   appends a new solution psi with As psi = b - r (r its solve's final residual):
+//                 x = psi - sum_j e[j] X_j and ax = (b - r) - sum_j e[j] AX_j (e = Y . b from
+//                 proj_dots2 before the solve: Y . As psi = Y . b - Y . r), a second pass with
+//                 the coefficients Y . ax when the first left cosines above 1e-7, and x, ax
+//                 scaled to unit norm (zero when x lies in the span); out: m + 2 values, part:
+//                 proj_part_len(n) * (m + 2)
 constexpr int kProjMax = 16;
 int proj_part_len(size_t n);
-void proj_dots(hipStream_t s, size_t n, int m, const double* X, const double* y, double* part, double* out);
-void proj_combine(hipStream_t s, size_t n, int m, const double* X, const double* c, int accumulate, double* out);
-void proj_combine2(hipStream_t s, size_t n, int m, const double* X, const double* AX, const double* c, double* x,
-                   double* ax);
-void proj_start(hipStream_t s, size_t n, const double* psi, const double* b, const double* r, double* x, double* ax,
-                int mr, double* part, double* out);
-void proj_normalize(hipStream_t s, size_t n, const double* d, double* a, double* b);
+void proj_dots2(hipStream_t s, size_t n, int m, const double* Y, const double* t, const double* b, double* part,
+                double* out);
+void proj_combine(hipStream_t s, size_t n, int m, const double* X, const double* c, double* out);
+void proj_ortho(hipStream_t s, size_t n, int m, int mr, const double* X, const double* AX, const double* e,
+                const double* psi, const double* b, const double* r, double* x, double* ax, double* part,
+                double* out);
 // psi' = sum_j a[j] x_j (x_j may be psi itself, read before the write), and save = psi (the old
 // value) when save is not null; nterm <= 8.
 void psi_combine(hipStream_t s, size_t n, int nterm, const double* a, const double* const* x, double* psi,

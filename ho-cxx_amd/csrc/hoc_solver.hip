@@ -963,53 +963,58 @@ struct Solver {
   //   symmetric As (PCG): X As-orthonormal, c_j = X_j . (b - As x0): the As-norm best correction;
   //   otherwise (BiCGSTAB): AX orthonormal, c_j = AX_j . (b - As x0): the least-squares residual.
   // Either is never worse than x0 in its norm.  The new solution x is then orthogonalised against
-  // the basis by classical Gram-Schmidt applied twice, using As x = b - r (r: the solve's final
-  // residual) and the stored AX, and appended; a full basis restarts from it.
+  // the basis by classical Gram-Schmidt and appended; a full basis restarts from it.  The first
+  // Gram-Schmidt coefficients Y . As x = Y . b - Y . r (Y = X, or AX for BiCGSTAB) are taken as
+  // Y . b, computed before the solve in the pass that forms c; the cosines this leaves (Y . r,
+  // within the solve's tolerance) are measured in the same pass that applies it, and a second
+  // pass runs only when they exceed 1e-7.
   static constexpr int kProjVectors = 16;
   struct ProjBasis {
     DBuf<double> X, AX;
-    int m = 0;  // vectors held
+    int m = 0;       // vectors held
+    int e_m = -1;    // basis size the coefficients Y . b of the solve in progress were taken for
   };
   ProjBasis proj_fwd, proj_adj;     // stage solves; adjoint (transposed) solves
   ProjBasis* proj_cur = &proj_fwd;  // the basis of the solve in progress
   bool proj_force = false;          // project a solve that is not a stage solve (adjoint solves)
-  DBuf<double> pcoef, ppart, ptmp;
+  DBuf<double> pcoef, ppart, ptmp;  // pcoef: c and e (2 kProjMax), then the appended vector's m + 2 values
   bool proj_on() const { return guess_mode() >= 5; }
   bool proj_mr() const { return krylov != HOC_KRYLOV_PCG; }  // minimal-residual form
+  void proj_alloc() {
+    if (ptmp.n < D) {  // t of proj_before; coefficients and norms; partial sums
+      ptmp.alloc(D);
+      pcoef.alloc(3 * kProjMax + 2); ppart.alloc((size_t)proj_part_len(D) * (2 * kProjMax + 2));
+    }
+  }
   void proj_before(const double* b) {
     ProjBasis& P = *proj_cur;
+    P.e_m = -1;
     if (P.m == 0) return;
+    proj_alloc();
     double* t = ptmp.p;
     HIPCHK(launch_trace(K, PRE_COPY, N, T, psi.p, nullptr, nullptr, nullptr, 0., nullptr, 0, nullptr, P64.tr.p, stream));
     lap<double>(P64, EPI_RESID, psi.p, t, b, nullptr, nullptr, 0., 0., nullptr, nullptr);  // t = b - As x0
-    proj_dots(stream, D, P.m, proj_mr() ? P.AX.p : P.X.p, t, ppart, pcoef);  // c_j
-    proj_combine(stream, D, P.m, P.X.p, pcoef, 2, psi);
+    proj_dots2(stream, D, P.m, proj_mr() ? P.AX.p : P.X.p, t, b, ppart, pcoef);  // c_j, e_j
+    proj_combine(stream, D, P.m, P.X.p, pcoef, psi);
+    P.e_m = P.m;
   }
   void proj_after(const double* b, const double* r) {
     constexpr int M = kProjVectors;
     ProjBasis& P = *proj_cur;
-    if (ptmp.n < D) {  // t of proj_before; coefficients and norms; partial sums
-      ptmp.alloc(D);
-      pcoef.alloc(kProjMax + 2); ppart.alloc((size_t)proj_part_len(D) * kProjMax);
-    }
+    proj_alloc();
     if (P.X.n < (size_t)M * D) {
       P.X.alloc((size_t)M * D); P.AX.alloc((size_t)M * D);
       P.m = 0;
     }
     if (P.m >= M) P.m = 0;  // restart from the newest solution
     const int j = P.m;
-    double* x = P.X.p + (size_t)j * D;   // the new vector is orthogonalised in its slot
-    double* ax = P.AX.p + (size_t)j * D;
-    const double* nx = proj_mr() ? ax : x;  // the norm: |x|_As^2 = x . As x, or |As x|^2
-    // x = psi, As x = b - r, and the norm before orthogonalisation
-    proj_start(stream, D, psi, b, r, x, ax, proj_mr() ? 1 : 0, ppart, pcoef.p + kProjMax + 1);
-    for (int pass = 0; pass < 2 && j > 0; ++pass) {
-      proj_dots(stream, D, j, proj_mr() ? P.AX.p : P.X.p, ax, ppart, pcoef);  // c_i
-      proj_combine2(stream, D, j, P.X.p, P.AX.p, pcoef, x, ax);
-    }
-    proj_dots(stream, D, 1, nx, ax, ppart, pcoef.p + kProjMax);
-    proj_normalize(stream, D, pcoef.p + kProjMax, x, ax);  // (a vector left in the span becomes zero)
+    const double* Y = proj_mr() ? P.AX.p : P.X.p;
+    if (j > 0 && P.e_m != j) proj_dots2(stream, D, j, Y, b, b, ppart, pcoef);  // e_j = Y_j . b
+    // orthogonalised and normalised into slot j (a vector left in the span becomes zero)
+    proj_ortho(stream, D, j, proj_mr() ? 1 : 0, P.X.p, P.AX.p, pcoef.p + j, psi, b, r, P.X.p + (size_t)j * D,
+               P.AX.p + (size_t)j * D, ppart, pcoef.p + 2 * kProjMax);
     P.m = j + 1;
+    P.e_m = -1;
   }
   // The adjoint's colouring, velocity map, buffers and recorded trajectory belong to one mesh and
   // one set-up: a new mesh or set-up rebuilds them on the next adjoint call.
