@@ -513,19 +513,26 @@ __global__ __launch_bounds__(256) void k_proj_combine(size_t n, const double* __
     out[i] = v;
   }
 }
-// First Gram-Schmidt pass on a new solution, with the coefficients e computed before its solve:
+// Gram-Schmidt step appending a new solution psi (As psi = b - r), with the coefficients
+// e = G^-1 Y . b computed before its solve:
 //   x = psi - sum_j e_j X_j,  ax = (b - r) - sum_j e_j AX_j,
-// and the per-block partials of f_j = Y_j . ax (the coefficients of a second pass), |x|^2 =
-// (mr ? ax : x) . ax and the norm before the pass (mr ? b - r : psi) . (b - r).
+// and the per-block partials of the new Gram column: Y_j . ax (j < M), (mr ? ax : x) . ax, and
+// the squared norm of the solution itself, (mr ? b - r : psi) . (b - r).  Block 0 writes psi's
+// coefficients in the basis with x appended: wcol = (e_0 .. e_{M-1}, 1, 0 ..), ldw values.
 template <int M>
 __global__ __launch_bounds__(256) void k_proj_ortho(size_t n, int mr, const double* __restrict__ X,
                                                     const double* __restrict__ AX, const double* __restrict__ e,
                                                     const double* __restrict__ psi, const double* __restrict__ b,
                                                     const double* __restrict__ r, double* __restrict__ x,
-                                                    double* __restrict__ ax, double* __restrict__ part) {
+                                                    double* __restrict__ ax, double* __restrict__ part,
+                                                    double* __restrict__ wcol, int ldw) {
   double ej[M > 0 ? M : 1], s[M + 2];
 #pragma unroll
   for (int j = 0; j < M; ++j) ej[j] = e[j];
+  if (blockIdx.x == 0 && (int)threadIdx.x < ldw) {
+    const int t = threadIdx.x;
+    wcol[t] = t < M ? e[t] : t == M ? 1. : 0.;
+  }
 #pragma unroll
   for (int k = 0; k < M + 2; ++k) s[k] = 0.;
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
@@ -551,39 +558,107 @@ __global__ __launch_bounds__(256) void k_proj_ortho(size_t n, int mr, const doub
   }
   block_sums<M + 2>(s, part + (size_t)blockIdx.x * (M + 2));
 }
-// Second Gram-Schmidt pass and normalisation: F = (f_0..f_{M-1}, |x|^2, norm before the first
-// pass).  The pass runs when the cosines left by the first exceed kProjReorth (sum f_j^2 >
-// kProjReorth^2 |x|^2); the squared norm after it is |x|^2 - sum f_j^2 (X orthonormal).  A vector
-// whose squared norm falls to 1e-24 of the first one (left in the span) becomes zero.
-constexpr double kProjReorth = 1e-7;
+// Restart of a full basis of M vectors: slots k < kk become sum_j C[k M + j] X_j, and the same
+// for AX.  In place: each dof's M values are read before the same thread writes its kk values.
 template <int M>
-__global__ __launch_bounds__(256) void k_proj_fix(size_t n, const double* __restrict__ X, const double* __restrict__ AX,
-                                                  const double* __restrict__ F, double* __restrict__ x,
-                                                  double* __restrict__ ax) {
-  double fj[M > 0 ? M : 1], fsq = 0.;
+__global__ __launch_bounds__(256) void k_proj_reseed(size_t n, int kk, const double* __restrict__ C, double* X,
+                                                     double* AX) {
+  __shared__ double Cs[kProjMax * kProjMax];
+  for (int t = threadIdx.x; t < kk * M; t += blockDim.x) Cs[t] = C[t];
+  __syncthreads();
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    double xv[M], av[M];
 #pragma unroll
-  for (int j = 0; j < M; ++j) { fj[j] = F[j]; fsq += fj[j] * fj[j]; }
-  const double nn = F[M], n0 = F[M + 1];
-  const bool again = M > 0 && fsq > kProjReorth * kProjReorth * nn;
-  const double n2 = again ? nn - fsq : nn;
-  const double sc = (n2 > 1e-24 * fabs(n0) && n2 > 0.) ? 1. / sqrt(n2) : 0.;
-  if (again) {
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-      double xv = x[i], av = ax[i];
+    for (int j = 0; j < M; ++j) {
+      xv[j] = X[(size_t)j * n + i];
+      av[j] = AX[(size_t)j * n + i];
+    }
+    for (int k = 0; k < kk; ++k) {
+      double sx = 0., sa = 0.;
 #pragma unroll
       for (int j = 0; j < M; ++j) {
-        xv -= fj[j] * X[(size_t)j * n + i];
-        av -= fj[j] * AX[(size_t)j * n + i];
+        sx += Cs[k * M + j] * xv[j];
+        sa += Cs[k * M + j] * av[j];
       }
-      x[i] = xv * sc;
-      ax[i] = av * sc;
-    }
-  } else {
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-      x[i] *= sc;
-      ax[i] *= sc;
+      X[(size_t)k * n + i] = sx;
+      AX[(size_t)k * n + i] = sa;
     }
   }
+}
+// The small system of a basis of m vectors: column j of G (at G + j ldg) holds G_ij (i < j),
+// G_jj, and in row j + 1 the squared norm of the solution the vector came from.  A vector with
+// G_jj <= 1e-24 of that norm (left in the span of the others), or whose Cholesky pivot falls
+// below 1e-8 G_jj, is left out.  v[0, m) and v[m, 2m) become G^-1 v on the kept vectors and 0
+// on the others: Cholesky in one thread (m <= kProjMax, about m^3 / 6 flops).
+__global__ __launch_bounds__(64) void k_proj_solve(int m, const double* __restrict__ G, int ldg,
+                                                   double* __restrict__ v) {
+  __shared__ double L[kProjMax][kProjMax];
+  __shared__ int ok[kProjMax];
+  if (threadIdx.x != 0) return;
+  for (int j = 0; j < m; ++j) {
+    const double gjj = G[(size_t)j * ldg + j];
+    ok[j] = gjj > 1e-24 * fabs(G[(size_t)j * ldg + j + 1]) && gjj > 0.;
+  }
+  for (int j = 0; j < m; ++j) {
+    if (!ok[j]) continue;
+    const double gjj = G[(size_t)j * ldg + j];
+    double d = gjj;
+    for (int k = 0; k < j; ++k)
+      if (ok[k]) d -= L[j][k] * L[j][k];
+    if (!(d > 1e-8 * gjj)) { ok[j] = 0; continue; }
+    d = sqrt(d);
+    L[j][j] = d;
+    for (int i = j + 1; i < m; ++i) {
+      if (!ok[i]) continue;
+      double t = G[(size_t)i * ldg + j];  // G_ji, j < i
+      for (int k = 0; k < j; ++k)
+        if (ok[k]) t -= L[i][k] * L[j][k];
+      L[i][j] = t / d;
+    }
+  }
+  for (int h = 0; h < 2; ++h) {
+    double* y = v + (size_t)h * m;
+    for (int j = 0; j < m; ++j) {
+      if (!ok[j]) { y[j] = 0.; continue; }
+      double t = y[j];
+      for (int k = 0; k < j; ++k)
+        if (ok[k]) t -= L[j][k] * y[k];
+      y[j] = t / L[j][j];
+    }
+    for (int j = m - 1; j >= 0; --j) {
+      if (!ok[j]) continue;
+      double t = y[j];
+      for (int k = j + 1; k < m; ++k)
+        if (ok[k]) t -= L[k][j] * y[k];
+      y[j] = t / L[j][j];
+    }
+  }
+}
+// Gram matrix G_ij = Y_i . AX_j of a basis of M vectors (M * M <= 256): per block, chunks of
+// 64 dofs of both sets staged in LDS, thread i M + j summing its pair; per-block partials
+// part[block][M * M].
+template <int M>
+__global__ __launch_bounds__(256) void k_proj_gram(size_t n, const double* __restrict__ Y, const double* __restrict__ AX,
+                                                   double* __restrict__ part) {
+  constexpr int CH = 64;
+  __shared__ double ys[M][CH + 1], as[M][CH + 1];  // (+1: rows on distinct banks)
+  const int t = threadIdx.x, pi = t / M, pj = t % M;
+  double s = 0.;
+  const size_t nch = (n + CH - 1) / CH;
+  for (size_t ch = blockIdx.x; ch < nch; ch += gridDim.x) {
+    const size_t i0 = ch * CH;
+    for (int q = t; q < M * CH; q += 256) {
+      const int v = q / CH, d = q % CH;
+      const size_t i = i0 + d;
+      ys[v][d] = i < n ? Y[(size_t)v * n + i] : 0.;
+      as[v][d] = i < n ? AX[(size_t)v * n + i] : 0.;
+    }
+    __syncthreads();
+    if (t < M * M)
+      for (int d = 0; d < CH; ++d) s += ys[pi][d] * as[pj][d];
+    __syncthreads();
+  }
+  if (t < M * M) part[(size_t)blockIdx.x * M * M + t] = s;
 }
 void proj_dots2(hipStream_t s, size_t n, int m, const double* Y, const double* t, const double* b, double* part,
                 double* out) {
@@ -602,16 +677,36 @@ void proj_combine(hipStream_t s, size_t n, int m, const double* X, const double*
     hipLaunchKernelGGL(k_proj_combine<M>, dim3(vec_grid(n)), dim3(256), 0, s, n, X, c, out);
   });
 }
+void proj_solve(hipStream_t s, int m, const double* G, int ldg, double* v) {
+  if (m <= 0 || m > kProjMax) return;
+  hipLaunchKernelGGL(k_proj_solve, dim3(1), dim3(64), 0, s, m, G, ldg, v);
+}
 void proj_ortho(hipStream_t s, size_t n, int m, int mr, const double* X, const double* AX, const double* e,
                 const double* psi, const double* b, const double* r, double* x, double* ax, double* part,
-                double* out) {
-  if (m < 0 || m > kProjMax || n == 0) return;
+                double* gcol, double* wcol, int ldw) {
+  if (m < 0 || m > kProjMax || n == 0 || ldw > 256) return;
   const int nb = proj_part_len(n);
   proj_dispatch<0>(m, [&](auto Mc) {
     constexpr int M = decltype(Mc)::value;
-    hipLaunchKernelGGL(k_proj_ortho<M>, dim3(nb), dim3(256), 0, s, n, mr, X, AX, e, psi, b, r, x, ax, part);
-    hipLaunchKernelGGL(k_proj_fin, dim3(M + 2), dim3(256), 0, s, nb, M + 2, part, out);
-    hipLaunchKernelGGL(k_proj_fix<M>, dim3(vec_grid(n)), dim3(256), 0, s, n, X, AX, out, x, ax);
+    hipLaunchKernelGGL(k_proj_ortho<M>, dim3(nb), dim3(256), 0, s, n, mr, X, AX, e, psi, b, r, x, ax, part, wcol,
+                       ldw);
+    hipLaunchKernelGGL(k_proj_fin, dim3(M + 2), dim3(256), 0, s, nb, M + 2, part, gcol);
+  });
+}
+void proj_gram(hipStream_t s, size_t n, int m, const double* Y, const double* AX, double* part, double* out) {
+  if (m <= 0 || m > kProjMax || m * m > 256 || n == 0) return;
+  const int nb = proj_part_len(n);
+  proj_dispatch<1>(m, [&](auto Mc) {
+    constexpr int M = decltype(Mc)::value;
+    if constexpr (M * M <= 256) hipLaunchKernelGGL(k_proj_gram<M>, dim3(nb), dim3(256), 0, s, n, Y, AX, part);
+  });
+  hipLaunchKernelGGL(k_proj_fin, dim3(m * m), dim3(256), 0, s, nb, m * m, part, out);
+}
+void proj_reseed(hipStream_t s, size_t n, int m, int kk, const double* C, double* X, double* AX) {
+  if (m <= 0 || m > kProjMax || kk <= 0 || kk > m || n == 0) return;
+  proj_dispatch<1>(m, [&](auto Mc) {
+    constexpr int M = decltype(Mc)::value;
+    hipLaunchKernelGGL(k_proj_reseed<M>, dim3(vec_grid(n)), dim3(256), 0, s, n, kk, C, X, AX);
   });
 }
 void psi_predict(hipStream_t s, size_t n, double f, int extrap, double* psi, double* prev) {

@@ -958,32 +958,41 @@ struct Solver {
   // Mode 5: projection for successive right-hand sides (P. F. Fischer, Comput. Methods Appl. Mech.
   // Engrg. 163 (1998) 193-204), applied to the correction of mode 4's guess x0 (with Euler and RK2,
   // whose stiff modes alternate in sign from step to step: of the previous solution).  A basis X of
-  // up to M = 16 earlier stage solutions with their images AX = As X gives the start
-  // x0 + sum_j c_j X_j with
-  //   symmetric As (PCG): X As-orthonormal, c_j = X_j . (b - As x0): the As-norm best correction;
-  //   otherwise (BiCGSTAB): AX orthonormal, c_j = AX_j . (b - As x0): the least-squares residual.
-  // Either is never worse than x0 in its norm.  The new solution x is then orthogonalised against
-  // the basis by classical Gram-Schmidt and appended; a full basis restarts from it.  The first
-  // Gram-Schmidt coefficients Y . As x = Y . b - Y . r (Y = X, or AX for BiCGSTAB) are taken as
-  // Y . b, computed before the solve in the pass that forms c; the cosines this leaves (Y . r,
-  // within the solve's tolerance) are measured in the same pass that applies it, and a second
-  // pass runs only when they exceed 1e-7.
+  // up to M = 16 vectors spanning earlier stage solutions, with their images AX = As X and Gram
+  // matrix G = Y^T AX, gives the start x0 + X c with c = G^-1 Y^T (b - As x0):
+  //   symmetric As (PCG): Y = X, the As-norm best correction in the span;
+  //   otherwise (BiCGSTAB): Y = AX, the least-squares residual.
+  // Either is never worse than x0 in its norm.  A new solution x is appended after its solve as
+  // x - X e with e = G^-1 Y^T As x = G^-1 (Y^T b - Y^T r); the Y^T b part is formed before the
+  // solve in the pass that forms c, and the Y^T r part (within the solve's tolerance) is left in
+  // place: the appending pass measures the new vector's Gram column exactly, so the basis is not
+  // kept orthogonal and needs no second pass or normalisation.  Each appended solution's
+  // coefficients in the basis are kept (W).  A full basis restarts from the span of its newest
+  // kProjKeep solutions: their coefficient columns, orthonormalised on the host in the G inner
+  // product, recombine the basis in one pass into G = I, so the start of every solve is
+  // projected on at least kProjKeep earlier solutions.
   static constexpr int kProjVectors = 16;
+  static constexpr int kProjKeep = 8;
+  static constexpr int kProjLdg = kProjVectors + 1;  // Gram column: M entries and the solution's norm
+  static_assert(2 * kProjKeep <= kProjVectors && kProjVectors <= kProjMax, "basis restart sizes");
   struct ProjBasis {
     DBuf<double> X, AX;
+    DBuf<double> G, W, C;  // Gram columns (M x kProjLdg); solution coefficients (M x M); restart combination
     int m = 0;       // vectors held
-    int e_m = -1;    // basis size the coefficients Y . b of the solve in progress were taken for
+    int e_m = -1;    // basis size the coefficients e of the solve in progress were formed for
+    int restarts = 0;
   };
   ProjBasis proj_fwd, proj_adj;     // stage solves; adjoint (transposed) solves
   ProjBasis* proj_cur = &proj_fwd;  // the basis of the solve in progress
   bool proj_force = false;          // project a solve that is not a stage solve (adjoint solves)
-  DBuf<double> pcoef, ppart, ptmp;  // pcoef: c and e (2 kProjMax), then the appended vector's m + 2 values
+  DBuf<double> pcoef, ppart, ptmp;  // pcoef: c (m) and e (m)
   bool proj_on() const { return guess_mode() >= 5; }
   bool proj_mr() const { return krylov != HOC_KRYLOV_PCG; }  // minimal-residual form
   void proj_alloc() {
-    if (ptmp.n < D) {  // t of proj_before; coefficients and norms; partial sums
+    if (ptmp.n < D) {  // t of proj_before; coefficients; partial sums
       ptmp.alloc(D);
-      pcoef.alloc(3 * kProjMax + 2); ppart.alloc((size_t)proj_part_len(D) * (2 * kProjMax + 2));
+      pcoef.alloc(2 * kProjMax);
+      ppart.alloc((size_t)proj_part_len(D) * std::max(2 * kProjMax + 2, kProjVectors * kProjVectors));
     }
   }
   void proj_before(const double* b) {
@@ -994,9 +1003,69 @@ struct Solver {
     double* t = ptmp.p;
     HIPCHK(launch_trace(K, PRE_COPY, N, T, psi.p, nullptr, nullptr, nullptr, 0., nullptr, 0, nullptr, P64.tr.p, stream));
     lap<double>(P64, EPI_RESID, psi.p, t, b, nullptr, nullptr, 0., 0., nullptr, nullptr);  // t = b - As x0
-    proj_dots2(stream, D, P.m, proj_mr() ? P.AX.p : P.X.p, t, b, ppart, pcoef);  // c_j, e_j
+    proj_dots2(stream, D, P.m, proj_mr() ? P.AX.p : P.X.p, t, b, ppart, pcoef);  // Y^T t, Y^T b
+    proj_solve(stream, P.m, P.G.p, kProjLdg, pcoef);                               // c, e
     proj_combine(stream, D, P.m, P.X.p, pcoef, psi);
     P.e_m = P.m;
+  }
+  // A full basis: keep the span of the newest kProjKeep appended solutions (columns M - kProjKeep
+  // .. M - 1 of W: with kProjKeep <= M / 2 these were all appended after the last restart).
+  void proj_restart(ProjBasis& P) {
+    constexpr int M = kProjVectors, KP = kProjKeep, LG = kProjLdg;
+    P.m = 0;
+    P.e_m = -1;
+    ++P.restarts;
+    if (KP == 0) return;
+    std::vector<double> w((size_t)M * M), gc((size_t)M * LG), g((size_t)M * M), c;
+    proj_gram(stream, D, M, proj_mr() ? P.AX.p : P.X.p, P.AX.p, ppart, P.C.p);  // G measured afresh
+    HIPCHK(hipStreamSynchronize(stream));
+    d2h(w.data(), P.W.p, w.size() * sizeof(double));
+    d2h(gc.data(), P.G.p, gc.size() * sizeof(double));
+    d2h(g.data(), P.C.p, g.size() * sizeof(double));
+    // a vector left out of the solves (see k_proj_solve) drops; G symmetrised
+    bool ok[M];
+    for (int j = 0; j < M; ++j) {
+      const double gjj = gc[(size_t)j * LG + j];
+      ok[j] = gjj > 1e-24 * std::fabs(gc[(size_t)j * LG + j + 1]) && gjj > 0.;
+    }
+    for (int j = 0; j < M; ++j)
+      for (int i = 0; i <= j; ++i) {
+        const double v = (ok[i] && ok[j]) ? 0.5 * (g[(size_t)i * M + j] + g[(size_t)j * M + i]) : 0.;
+        g[(size_t)i * M + j] = g[(size_t)j * M + i] = v;
+      }
+    auto gdot = [&](const double* u, const double* v) {
+      double d = 0.;
+      for (int i = 0; i < M; ++i) {
+        double gi = 0.;
+        for (int j = 0; j < M; ++j) gi += g[(size_t)i * M + j] * v[j];
+        d += u[i] * gi;
+      }
+      return d;
+    };
+    int kk = 0;
+    for (int col = M - KP; col < M; ++col) {  // modified Gram-Schmidt, twice, in the G inner product
+      double v[M];
+      for (int j = 0; j < M; ++j) v[j] = ok[j] ? w[(size_t)col * M + j] : 0.;
+      const double n0 = gdot(v, v);
+      for (int pass = 0; pass < 2; ++pass)
+        for (int k = 0; k < kk; ++k) {
+          const double d = gdot(&c[(size_t)k * M], v);
+          for (int j = 0; j < M; ++j) v[j] -= d * c[(size_t)k * M + j];
+        }
+      const double nv = gdot(v, v);
+      if (!(nv > 1e-24 * n0) || !(nv > 0.)) continue;  // in the span of the newer ones already kept
+      const double f = 1. / std::sqrt(nv);
+      for (int j = 0; j < M; ++j) c.push_back(v[j] * f);
+      ++kk;
+    }
+    if (kk == 0) return;
+    std::vector<double> gi((size_t)kk * LG, 0.);  // the new basis is G-orthonormal: identity columns
+    for (int j = 0; j < kk; ++j) { gi[(size_t)j * LG + j] = 1.; gi[(size_t)j * LG + j + 1] = 1.; }
+    HIPCHK(hipMemcpyAsync(P.C.p, c.data(), c.size() * sizeof(double), hipMemcpyHostToDevice, stream));
+    HIPCHK(hipMemcpyAsync(P.G.p, gi.data(), gi.size() * sizeof(double), hipMemcpyHostToDevice, stream));
+    proj_reseed(stream, D, M, kk, P.C.p, P.X.p, P.AX.p);
+    HIPCHK(hipStreamSynchronize(stream));  // (c and gi are locals)
+    P.m = kk;
   }
   void proj_after(const double* b, const double* r) {
     constexpr int M = kProjVectors;
@@ -1004,15 +1073,17 @@ struct Solver {
     proj_alloc();
     if (P.X.n < (size_t)M * D) {
       P.X.alloc((size_t)M * D); P.AX.alloc((size_t)M * D);
+      P.G.alloc((size_t)M * kProjLdg); P.W.alloc((size_t)M * M); P.C.alloc((size_t)M * M);
       P.m = 0;
     }
-    if (P.m >= M) P.m = 0;  // restart from the newest solution
+    if (P.m >= M) proj_restart(P);
     const int j = P.m;
-    const double* Y = proj_mr() ? P.AX.p : P.X.p;
-    if (j > 0 && P.e_m != j) proj_dots2(stream, D, j, Y, b, b, ppart, pcoef);  // e_j = Y_j . b
-    // orthogonalised and normalised into slot j (a vector left in the span becomes zero)
+    if (j > 0 && P.e_m != j) {  // e = G^-1 Y^T b
+      proj_dots2(stream, D, j, proj_mr() ? P.AX.p : P.X.p, b, b, ppart, pcoef);
+      proj_solve(stream, j, P.G.p, kProjLdg, pcoef);
+    }
     proj_ortho(stream, D, j, proj_mr() ? 1 : 0, P.X.p, P.AX.p, pcoef.p + j, psi, b, r, P.X.p + (size_t)j * D,
-               P.AX.p + (size_t)j * D, ppart, pcoef.p + 2 * kProjMax);
+               P.AX.p + (size_t)j * D, ppart, P.G.p + (size_t)j * kProjLdg, P.W.p + (size_t)j * M, M);
     P.m = j + 1;
     P.e_m = -1;
   }
