@@ -589,61 +589,76 @@ __global__ __launch_bounds__(256) void k_proj_reseed(size_t n, int kk, const dou
 // G_jj, and in row j + 1 the squared norm of the solution the vector came from.  A vector with
 // G_jj <= 1e-24 of that norm (left in the span of the others), or whose Cholesky pivot falls
 // below 1e-8 G_jj, is left out.  v[0, m) and v[m, 2m) become G^-1 v on the kept vectors and 0
-// on the others: Cholesky in one thread (m <= kProjMax, about m^3 / 6 flops).
+// on the others: Cholesky in LDS by one wavefront, lane i updating row i (m <= kProjMax <= 64).
 __global__ __launch_bounds__(64) void k_proj_solve(int m, const double* __restrict__ G, int ldg,
                                                    double* __restrict__ v) {
-  __shared__ double L[kProjMax][kProjMax];
+  __shared__ double A[kProjMax][kProjMax + 1], y[2][kProjMax], gd[kProjMax];
   __shared__ int ok[kProjMax];
-  if (threadIdx.x != 0) return;
-  for (int j = 0; j < m; ++j) {
-    const double gjj = G[(size_t)j * ldg + j];
-    ok[j] = gjj > 1e-24 * fabs(G[(size_t)j * ldg + j + 1]) && gjj > 0.;
+  const int i = threadIdx.x;
+  for (int q = i; q < m * m; q += 64) {
+    const int r = q / m, c = q % m;
+    A[r][c] = c >= r ? G[(size_t)c * ldg + r] : G[(size_t)r * ldg + c];
   }
-  for (int j = 0; j < m; ++j) {
-    if (!ok[j]) continue;
-    const double gjj = G[(size_t)j * ldg + j];
-    double d = gjj;
-    for (int k = 0; k < j; ++k)
-      if (ok[k]) d -= L[j][k] * L[j][k];
-    if (!(d > 1e-8 * gjj)) { ok[j] = 0; continue; }
-    d = sqrt(d);
-    L[j][j] = d;
-    for (int i = j + 1; i < m; ++i) {
-      if (!ok[i]) continue;
-      double t = G[(size_t)i * ldg + j];  // G_ji, j < i
-      for (int k = 0; k < j; ++k)
-        if (ok[k]) t -= L[i][k] * L[j][k];
-      L[i][j] = t / d;
-    }
+  if (i < m) {
+    const double gii = G[(size_t)i * ldg + i];
+    ok[i] = gii > 1e-24 * fabs(G[(size_t)i * ldg + i + 1]) && gii > 0.;
+    gd[i] = gii;
+    y[0][i] = v[i];
+    y[1][i] = v[m + i];
   }
-  for (int h = 0; h < 2; ++h) {
-    double* y = v + (size_t)h * m;
-    for (int j = 0; j < m; ++j) {
-      if (!ok[j]) { y[j] = 0.; continue; }
-      double t = y[j];
-      for (int k = 0; k < j; ++k)
-        if (ok[k]) t -= L[j][k] * y[k];
-      y[j] = t / L[j][j];
+  __syncthreads();
+  for (int j = 0; j < m; ++j) {  // right-looking: column j, then the trailing rows
+    const double d = A[j][j];
+    const bool okj = ok[j] && d > 1e-8 * gd[j];
+    __syncthreads();
+    if (!okj) {
+      if (i == 0) ok[j] = 0;
+      __syncthreads();
+      continue;
     }
-    for (int j = m - 1; j >= 0; --j) {
-      if (!ok[j]) continue;
-      double t = y[j];
-      for (int k = j + 1; k < m; ++k)
-        if (ok[k]) t -= L[k][j] * y[k];
-      y[j] = t / L[j][j];
+    const double l = sqrt(d);
+    if (i > j && i < m && ok[i]) A[i][j] /= l;
+    if (i == j) A[j][j] = l;
+    __syncthreads();
+    if (i > j && i < m && ok[i])
+      for (int k = j + 1; k <= i; ++k)
+        if (ok[k]) A[i][k] -= A[i][j] * A[k][j];
+    __syncthreads();
+  }
+  for (int j = 0; j < m; ++j) {  // L y = v
+    if (ok[j]) {
+      const double y0 = y[0][j] / A[j][j], y1 = y[1][j] / A[j][j];
+      __syncthreads();
+      if (i == j) { y[0][j] = y0; y[1][j] = y1; }
+      if (i > j && i < m && ok[i]) { y[0][i] -= A[i][j] * y0; y[1][i] -= A[i][j] * y1; }
     }
+    __syncthreads();
+  }
+  for (int j = m - 1; j >= 0; --j) {  // L^T x = y
+    if (ok[j]) {
+      const double x0 = y[0][j] / A[j][j], x1 = y[1][j] / A[j][j];
+      __syncthreads();
+      if (i == j) { y[0][j] = x0; y[1][j] = x1; }
+      if (i < j && ok[i]) { y[0][i] -= A[j][i] * x0; y[1][i] -= A[j][i] * x1; }
+    }
+    __syncthreads();
+  }
+  if (i < m) {
+    v[i] = ok[i] ? y[0][i] : 0.;
+    v[m + i] = ok[i] ? y[1][i] : 0.;
   }
 }
-// Gram matrix G_ij = Y_i . AX_j of a basis of M vectors (M * M <= 256): per block, chunks of
-// 64 dofs of both sets staged in LDS, thread i M + j summing its pair; per-block partials
-// part[block][M * M].
+// Gram matrix G_ij = Y_i . AX_j of a basis of M vectors: per block, chunks of 32 dofs of both
+// sets staged in LDS, thread t summing pairs t, t + 256, ..; per-block partials part[block][M * M].
 template <int M>
 __global__ __launch_bounds__(256) void k_proj_gram(size_t n, const double* __restrict__ Y, const double* __restrict__ AX,
                                                    double* __restrict__ part) {
-  constexpr int CH = 64;
+  constexpr int CH = 32, NP = (M * M + 255) / 256;
   __shared__ double ys[M][CH + 1], as[M][CH + 1];  // (+1: rows on distinct banks)
-  const int t = threadIdx.x, pi = t / M, pj = t % M;
-  double s = 0.;
+  const int t = threadIdx.x;
+  double s[NP];
+#pragma unroll
+  for (int k = 0; k < NP; ++k) s[k] = 0.;
   const size_t nch = (n + CH - 1) / CH;
   for (size_t ch = blockIdx.x; ch < nch; ch += gridDim.x) {
     const size_t i0 = ch * CH;
@@ -654,11 +669,19 @@ __global__ __launch_bounds__(256) void k_proj_gram(size_t n, const double* __res
       as[v][d] = i < n ? AX[(size_t)v * n + i] : 0.;
     }
     __syncthreads();
-    if (t < M * M)
-      for (int d = 0; d < CH; ++d) s += ys[pi][d] * as[pj][d];
+#pragma unroll
+    for (int k = 0; k < NP; ++k) {
+      const int pr = t + 256 * k;
+      if (pr < M * M) {
+        const int pi = pr / M, pj = pr % M;
+        for (int d = 0; d < CH; ++d) s[k] += ys[pi][d] * as[pj][d];
+      }
+    }
     __syncthreads();
   }
-  if (t < M * M) part[(size_t)blockIdx.x * M * M + t] = s;
+#pragma unroll
+  for (int k = 0; k < NP; ++k)
+    if (t + 256 * k < M * M) part[(size_t)blockIdx.x * M * M + t + 256 * k] = s[k];
 }
 void proj_dots2(hipStream_t s, size_t n, int m, const double* Y, const double* t, const double* b, double* part,
                 double* out) {
@@ -694,11 +717,11 @@ void proj_ortho(hipStream_t s, size_t n, int m, int mr, const double* X, const d
   });
 }
 void proj_gram(hipStream_t s, size_t n, int m, const double* Y, const double* AX, double* part, double* out) {
-  if (m <= 0 || m > kProjMax || m * m > 256 || n == 0) return;
+  if (m <= 0 || m > kProjMax || n == 0) return;
   const int nb = proj_part_len(n);
   proj_dispatch<1>(m, [&](auto Mc) {
     constexpr int M = decltype(Mc)::value;
-    if constexpr (M * M <= 256) hipLaunchKernelGGL(k_proj_gram<M>, dim3(nb), dim3(256), 0, s, n, Y, AX, part);
+    hipLaunchKernelGGL(k_proj_gram<M>, dim3(nb), dim3(256), 0, s, n, Y, AX, part);
   });
   hipLaunchKernelGGL(k_proj_fin, dim3(m * m), dim3(256), 0, s, nb, m * m, part, out);
 }

@@ -80,7 +80,7 @@ void psi_predict(hipStream_t s, size_t n, double f, int extrap, double* psi, dou
 //                 (m + 2 values: Y_j . ax, x's own entry, psi's squared norm); part:
 //                 proj_part_len(n) * (m + 2); wcol[0, ldw): psi's coefficients in the basis with x
 //                 appended
-//   proj_gram:    out[i m + j] = Y_i . AX_j (m * m <= 256; part: proj_part_len(n) * m * m)
+//   proj_gram:    out[i m + j] = Y_i . AX_j (part: proj_part_len(n) * m * m)
 //   proj_reseed:  slots k < kk of a basis of m vectors become sum_j C[k m + j] X_j (and AX alike)
 constexpr int kProjMax = 16;
 int proj_part_len(size_t n);
