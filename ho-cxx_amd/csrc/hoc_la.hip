@@ -437,8 +437,31 @@ void psi_combine(hipStream_t s, size_t n, int nterm, const double* a, const doub
   for (int j = 0; j < nterm; ++j) { C.a[j] = a[j]; C.x[j] = x[j]; }
   hipLaunchKernelGGL(k_psi_combine, dim3(vec_grid(n)), dim3(256), 0, s, n, nterm, C, psi, save);
 }
-constexpr int kProjBlocks = 1024;
-int proj_part_len(size_t n) { return (int)std::min<size_t>(kProjBlocks, (n + 255) / 256 + 1); }
+// ---- projection basis kernels -----------------------------------------------------------------
+// Every pass runs over the dof segments of a ProjSeg (one per ensemble instance): the blocks of
+// segment s are [bbeg[s], bbeg[s+1]) and stride over its dofs [sbeg[s], sbeg[s+1]); reductions
+// leave per-block partials that k_proj_fin sums per segment in block order (deterministic).
+struct SegRange {
+  size_t i0, i1, step;
+  int s, lb;
+};
+__device__ __forceinline__ SegRange seg_range(const ProjSeg& S) {
+  const int s = S.blk_seg[blockIdx.x];
+  const int lb = blockIdx.x - S.bbeg[s], nbs = S.bbeg[s + 1] - S.bbeg[s];
+  return {(size_t)S.sbeg[s] + (size_t)lb * blockDim.x + threadIdx.x, (size_t)S.sbeg[s + 1], (size_t)nbs * blockDim.x,
+          s, lb};
+}
+void proj_seg_layout(const std::vector<int64_t>& sbeg, std::vector<int32_t>& bbeg, std::vector<int32_t>& blk_seg) {
+  const int nseg = (int)sbeg.size() - 1;
+  bbeg.assign(nseg + 1, 0);
+  blk_seg.clear();
+  for (int s = 0; s < nseg; ++s) {  // about 8 dofs per thread, at least one block per segment
+    const int64_t n = sbeg[s + 1] - sbeg[s];
+    const int nb = (int)std::max<int64_t>(1, (n + 2047) / 2048);
+    bbeg[s + 1] = bbeg[s] + nb;
+    for (int k = 0; k < nb; ++k) blk_seg.push_back(s);
+  }
+}
 
 // The projection kernels take the basis size M as a template argument, so that the M + 2 loads
 // of one dof are issued back to back (one streaming pass over the basis at full bandwidth);
@@ -469,14 +492,16 @@ __device__ __forceinline__ void block_sums(double (&v)[NV], double* __restrict__
     out[k] = red[0][k] + red[1][k] + red[2][k] + red[3][k];
   }
 }
-// part[block][0, M) = Y_j . t,  part[block][M, 2M) = Y_j . b
+// part[block][0, M) = Y_j . t,  part[block][M, 2M) = Y_j . b over the block's segment
 template <int M>
-__global__ __launch_bounds__(256) void k_proj_dots2(size_t n, const double* __restrict__ Y, const double* __restrict__ t,
-                                                    const double* __restrict__ b, double* __restrict__ part) {
+__global__ __launch_bounds__(256) void k_proj_dots2(size_t n, ProjSeg S, const double* __restrict__ Y,
+                                                    const double* __restrict__ t, const double* __restrict__ b,
+                                                    double* __restrict__ part) {
   double s[2 * M];
 #pragma unroll
   for (int k = 0; k < 2 * M; ++k) s[k] = 0.;
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+  const SegRange R = seg_range(S);
+  for (size_t i = R.i0; i < R.i1; i += R.step) {
     const double tv = t[i], bv = b[i];
 #pragma unroll
     for (int j = 0; j < M; ++j) {
@@ -487,26 +512,29 @@ __global__ __launch_bounds__(256) void k_proj_dots2(size_t n, const double* __re
   }
   block_sums<2 * M>(s, part + (size_t)blockIdx.x * 2 * M);
 }
-// one block per dot product: strided partial sums, then a fixed-order tree (deterministic)
-__global__ __launch_bounds__(256) void k_proj_fin(int nb, int m, const double* __restrict__ part,
-                                                  double* __restrict__ out) {
-  const int j = blockIdx.x;
+// out[s * ostride + k] = sum of part[b * nv + k] over the blocks b of segment s (grid: nv x nseg),
+// strided partial sums, then a fixed-order tree
+__global__ __launch_bounds__(256) void k_proj_fin(ProjSeg S, int nv, const double* __restrict__ part,
+                                                  double* __restrict__ out, int ostride) {
+  const int k = blockIdx.x, sg = blockIdx.y;
   double s = 0.;
-  for (int b = threadIdx.x; b < nb; b += 256) s += part[(size_t)b * m + j];
-  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  for (int b = S.bbeg[sg] + threadIdx.x; b < S.bbeg[sg + 1]; b += 256) s += part[(size_t)b * nv + k];
+  s = wsum(s);
   __shared__ double red[4];
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
   __syncthreads();
-  if (threadIdx.x == 0) out[j] = red[0] + red[1] + red[2] + red[3];
+  if (threadIdx.x == 0) out[(size_t)sg * ostride + k] = red[0] + red[1] + red[2] + red[3];
 }
-// out += sum_j c_j X_j
+// out += sum_j c_j X_j, c of the block's segment at c + s cstride
 template <int M>
-__global__ __launch_bounds__(256) void k_proj_combine(size_t n, const double* __restrict__ X, const double* __restrict__ c,
+__global__ __launch_bounds__(256) void k_proj_combine(size_t n, ProjSeg S, const double* __restrict__ X,
+                                                      const double* __restrict__ c, int cstride,
                                                       double* __restrict__ out) {
+  const SegRange R = seg_range(S);
   double cj[M];
 #pragma unroll
-  for (int j = 0; j < M; ++j) cj[j] = c[j];
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+  for (int j = 0; j < M; ++j) cj[j] = c[(size_t)R.s * cstride + j];
+  for (size_t i = R.i0; i < R.i1; i += R.step) {
     double v = out[i];
 #pragma unroll
     for (int j = 0; j < M; ++j) v += cj[j] * X[(size_t)j * n + i];
@@ -514,28 +542,31 @@ __global__ __launch_bounds__(256) void k_proj_combine(size_t n, const double* __
   }
 }
 // Gram-Schmidt step appending a new solution psi (As psi = b - r), with the coefficients
-// e = G^-1 Y . b computed before its solve:
+// e = G^-1 Y . b computed before its solve (per segment, at e + s estride):
 //   x = psi - sum_j e_j X_j,  ax = (b - r) - sum_j e_j AX_j,
 // and the per-block partials of the new Gram column: Y_j . ax (j < M), (mr ? ax : x) . ax, and
-// the squared norm of the solution itself, (mr ? b - r : psi) . (b - r).  Block 0 writes psi's
-// coefficients in the basis with x appended: wcol = (e_0 .. e_{M-1}, 1, 0 ..), ldw values.
+// the squared norm of the solution itself, (mr ? b - r : psi) . (b - r).  The first block of
+// each segment writes psi's coefficients in the basis with x appended: wcol + s wstride =
+// (e_0 .. e_{M-1}, 1, 0 ..), ldw values.
 template <int M>
-__global__ __launch_bounds__(256) void k_proj_ortho(size_t n, int mr, const double* __restrict__ X,
+__global__ __launch_bounds__(256) void k_proj_ortho(size_t n, ProjSeg S, int mr, const double* __restrict__ X,
                                                     const double* __restrict__ AX, const double* __restrict__ e,
-                                                    const double* __restrict__ psi, const double* __restrict__ b,
-                                                    const double* __restrict__ r, double* __restrict__ x,
-                                                    double* __restrict__ ax, double* __restrict__ part,
-                                                    double* __restrict__ wcol, int ldw) {
+                                                    int estride, const double* __restrict__ psi,
+                                                    const double* __restrict__ b, const double* __restrict__ r,
+                                                    double* __restrict__ x, double* __restrict__ ax,
+                                                    double* __restrict__ part, double* __restrict__ wcol, int wstride,
+                                                    int ldw) {
+  const SegRange R = seg_range(S);
   double ej[M > 0 ? M : 1], s[M + 2];
 #pragma unroll
-  for (int j = 0; j < M; ++j) ej[j] = e[j];
-  if (blockIdx.x == 0 && (int)threadIdx.x < ldw) {
-    const int t = threadIdx.x;
-    wcol[t] = t < M ? e[t] : t == M ? 1. : 0.;
-  }
+  for (int j = 0; j < M; ++j) ej[j] = e[(size_t)R.s * estride + j];
 #pragma unroll
   for (int k = 0; k < M + 2; ++k) s[k] = 0.;
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+  if (R.lb == 0 && (int)threadIdx.x < ldw) {
+    const int t = threadIdx.x;
+    wcol[(size_t)R.s * wstride + t] = t < M ? e[(size_t)R.s * estride + t] : t == M ? 1. : 0.;
+  }
+  for (size_t i = R.i0; i < R.i1; i += R.step) {
     const double p = psi[i], a0 = b[i] - r[i];
     double xj[M > 0 ? M : 1], aj[M > 0 ? M : 1];
     double xv = p, av = a0;
@@ -558,15 +589,17 @@ __global__ __launch_bounds__(256) void k_proj_ortho(size_t n, int mr, const doub
   }
   block_sums<M + 2>(s, part + (size_t)blockIdx.x * (M + 2));
 }
-// Restart of a full basis of M vectors: slots k < kk become sum_j C[k M + j] X_j, and the same
-// for AX.  In place: each dof's M values are read before the same thread writes its kk values.
+// Restart of a full basis of M vectors: slots k < kk become sum_j C[k M + j] X_j (C of the
+// block's segment at C + s cstride), and the same for AX.  In place: each dof's M values are
+// read before the same thread writes its kk values.
 template <int M>
-__global__ __launch_bounds__(256) void k_proj_reseed(size_t n, int kk, const double* __restrict__ C, double* X,
-                                                     double* AX) {
+__global__ __launch_bounds__(256) void k_proj_reseed(size_t n, ProjSeg S, int kk, const double* __restrict__ C,
+                                                     int cstride, double* X, double* AX) {
+  const SegRange R = seg_range(S);
   __shared__ double Cs[kProjMax * kProjMax];
-  for (int t = threadIdx.x; t < kk * M; t += blockDim.x) Cs[t] = C[t];
+  for (int t = threadIdx.x; t < kk * M; t += blockDim.x) Cs[t] = C[(size_t)R.s * cstride + t];
   __syncthreads();
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+  for (size_t i = R.i0; i < R.i1; i += R.step) {
     double xv[M], av[M];
 #pragma unroll
     for (int j = 0; j < M; ++j) {
@@ -585,13 +618,16 @@ __global__ __launch_bounds__(256) void k_proj_reseed(size_t n, int kk, const dou
     }
   }
 }
-// The small system of a basis of m vectors: column j of G (at G + j ldg) holds G_ij (i < j),
-// G_jj, and in row j + 1 the squared norm of the solution the vector came from.  A vector with
-// G_jj <= 1e-24 of that norm (left in the span of the others), or whose Cholesky pivot falls
-// below 1e-8 G_jj, is left out.  v[0, m) and v[m, 2m) become G^-1 v on the kept vectors and 0
-// on the others: Cholesky in LDS by one wavefront, lane i updating row i (m <= kProjMax <= 64).
-__global__ __launch_bounds__(64) void k_proj_solve(int m, const double* __restrict__ G, int ldg,
-                                                   double* __restrict__ v) {
+// The small system of a basis of m vectors, one per segment (grid: nseg; G + s gstride,
+// v + s vstride): column j of G (at G + j ldg) holds G_ij (i < j), G_jj, and in row j + 1 the
+// squared norm of the solution the vector came from.  A vector with G_jj <= 1e-24 of that norm
+// (left in the span of the others), or whose Cholesky pivot falls below 1e-8 G_jj, is left out.
+// v[0, m) and v[m, 2m) become G^-1 v on the kept vectors and 0 on the others: Cholesky in LDS by
+// one wavefront, lane i updating row i (m <= kProjMax <= 64).
+__global__ __launch_bounds__(64) void k_proj_solve(int m, const double* __restrict__ Gall, int ldg, int gstride,
+                                                   double* __restrict__ vall, int vstride) {
+  const double* G = Gall + (size_t)blockIdx.x * gstride;
+  double* v = vall + (size_t)blockIdx.x * vstride;
   __shared__ double A[kProjMax][kProjMax + 1], y[2][kProjMax], gd[kProjMax];
   __shared__ int ok[kProjMax];
   const int i = threadIdx.x;
@@ -648,25 +684,27 @@ __global__ __launch_bounds__(64) void k_proj_solve(int m, const double* __restri
     v[m + i] = ok[i] ? y[1][i] : 0.;
   }
 }
-// Gram matrix G_ij = Y_i . AX_j of a basis of M vectors: per block, chunks of 32 dofs of both
-// sets staged in LDS, thread t summing pairs t, t + 256, ..; per-block partials part[block][M * M].
+// Gram matrix G_ij = Y_i . AX_j of a basis of M vectors over each segment: per block, chunks of
+// 32 dofs of both sets staged in LDS, thread t summing pairs t, t + 256, ..; per-block partials
+// part[block][M * M].
 template <int M>
-__global__ __launch_bounds__(256) void k_proj_gram(size_t n, const double* __restrict__ Y, const double* __restrict__ AX,
-                                                   double* __restrict__ part) {
+__global__ __launch_bounds__(256) void k_proj_gram(size_t n, ProjSeg S, const double* __restrict__ Y,
+                                                   const double* __restrict__ AX, double* __restrict__ part) {
   constexpr int CH = 32, NP = (M * M + 255) / 256;
   __shared__ double ys[M][CH + 1], as[M][CH + 1];  // (+1: rows on distinct banks)
   const int t = threadIdx.x;
+  const int sg = S.blk_seg[blockIdx.x];
+  const int lb = blockIdx.x - S.bbeg[sg], nbs = S.bbeg[sg + 1] - S.bbeg[sg];
+  const size_t d0 = S.sbeg[sg], d1 = S.sbeg[sg + 1];
   double s[NP];
 #pragma unroll
   for (int k = 0; k < NP; ++k) s[k] = 0.;
-  const size_t nch = (n + CH - 1) / CH;
-  for (size_t ch = blockIdx.x; ch < nch; ch += gridDim.x) {
-    const size_t i0 = ch * CH;
+  for (size_t c0 = d0 + (size_t)lb * CH; c0 < d1; c0 += (size_t)nbs * CH) {
     for (int q = t; q < M * CH; q += 256) {
       const int v = q / CH, d = q % CH;
-      const size_t i = i0 + d;
-      ys[v][d] = i < n ? Y[(size_t)v * n + i] : 0.;
-      as[v][d] = i < n ? AX[(size_t)v * n + i] : 0.;
+      const size_t i = c0 + d;
+      ys[v][d] = i < d1 ? Y[(size_t)v * n + i] : 0.;
+      as[v][d] = i < d1 ? AX[(size_t)v * n + i] : 0.;
     }
     __syncthreads();
 #pragma unroll
@@ -683,53 +721,54 @@ __global__ __launch_bounds__(256) void k_proj_gram(size_t n, const double* __res
   for (int k = 0; k < NP; ++k)
     if (t + 256 * k < M * M) part[(size_t)blockIdx.x * M * M + t + 256 * k] = s[k];
 }
-void proj_dots2(hipStream_t s, size_t n, int m, const double* Y, const double* t, const double* b, double* part,
-                double* out) {
-  if (m <= 0 || m > kProjMax || n == 0) return;
-  const int nb = proj_part_len(n);
-  proj_dispatch<1>(m, [&](auto Mc) {
-    constexpr int M = decltype(Mc)::value;
-    hipLaunchKernelGGL(k_proj_dots2<M>, dim3(nb), dim3(256), 0, s, n, Y, t, b, part);
-  });
-  hipLaunchKernelGGL(k_proj_fin, dim3(2 * m), dim3(256), 0, s, nb, 2 * m, part, out);
-}
-void proj_combine(hipStream_t s, size_t n, int m, const double* X, const double* c, double* out) {
+void proj_dots2(hipStream_t s, size_t n, const ProjSeg& S, int m, const double* Y, const double* t, const double* b,
+                double* part, double* out, int ostride) {
   if (m <= 0 || m > kProjMax || n == 0) return;
   proj_dispatch<1>(m, [&](auto Mc) {
     constexpr int M = decltype(Mc)::value;
-    hipLaunchKernelGGL(k_proj_combine<M>, dim3(vec_grid(n)), dim3(256), 0, s, n, X, c, out);
+    hipLaunchKernelGGL(k_proj_dots2<M>, dim3(S.nb), dim3(256), 0, s, n, S, Y, t, b, part);
   });
+  hipLaunchKernelGGL(k_proj_fin, dim3(2 * m, S.nseg), dim3(256), 0, s, S, 2 * m, part, out, ostride);
 }
-void proj_solve(hipStream_t s, int m, const double* G, int ldg, double* v) {
+void proj_solve(hipStream_t s, const ProjSeg& S, int m, const double* G, int ldg, int gstride, double* v,
+                int vstride) {
   if (m <= 0 || m > kProjMax) return;
-  hipLaunchKernelGGL(k_proj_solve, dim3(1), dim3(64), 0, s, m, G, ldg, v);
+  hipLaunchKernelGGL(k_proj_solve, dim3(S.nseg), dim3(64), 0, s, m, G, ldg, gstride, v, vstride);
 }
-void proj_ortho(hipStream_t s, size_t n, int m, int mr, const double* X, const double* AX, const double* e,
-                const double* psi, const double* b, const double* r, double* x, double* ax, double* part,
-                double* gcol, double* wcol, int ldw) {
+void proj_combine(hipStream_t s, size_t n, const ProjSeg& S, int m, const double* X, const double* c, int cstride,
+                  double* out) {
+  if (m <= 0 || m > kProjMax || n == 0) return;
+  proj_dispatch<1>(m, [&](auto Mc) {
+    constexpr int M = decltype(Mc)::value;
+    hipLaunchKernelGGL(k_proj_combine<M>, dim3(S.nb), dim3(256), 0, s, n, S, X, c, cstride, out);
+  });
+}
+void proj_ortho(hipStream_t s, size_t n, const ProjSeg& S, int m, int mr, const double* X, const double* AX,
+                const double* e, int estride, const double* psi, const double* b, const double* r, double* x,
+                double* ax, double* part, double* gcol, int gstride, double* wcol, int wstride, int ldw) {
   if (m < 0 || m > kProjMax || n == 0 || ldw > 256) return;
-  const int nb = proj_part_len(n);
   proj_dispatch<0>(m, [&](auto Mc) {
     constexpr int M = decltype(Mc)::value;
-    hipLaunchKernelGGL(k_proj_ortho<M>, dim3(nb), dim3(256), 0, s, n, mr, X, AX, e, psi, b, r, x, ax, part, wcol,
-                       ldw);
-    hipLaunchKernelGGL(k_proj_fin, dim3(M + 2), dim3(256), 0, s, nb, M + 2, part, gcol);
+    hipLaunchKernelGGL(k_proj_ortho<M>, dim3(S.nb), dim3(256), 0, s, n, S, mr, X, AX, e, estride, psi, b, r, x, ax,
+                       part, wcol, wstride, ldw);
+    hipLaunchKernelGGL(k_proj_fin, dim3(M + 2, S.nseg), dim3(256), 0, s, S, M + 2, part, gcol, gstride);
   });
 }
-void proj_gram(hipStream_t s, size_t n, int m, const double* Y, const double* AX, double* part, double* out) {
+void proj_gram(hipStream_t s, size_t n, const ProjSeg& S, int m, const double* Y, const double* AX, double* part,
+               double* out) {
   if (m <= 0 || m > kProjMax || n == 0) return;
-  const int nb = proj_part_len(n);
   proj_dispatch<1>(m, [&](auto Mc) {
     constexpr int M = decltype(Mc)::value;
-    hipLaunchKernelGGL(k_proj_gram<M>, dim3(nb), dim3(256), 0, s, n, Y, AX, part);
+    hipLaunchKernelGGL(k_proj_gram<M>, dim3(S.nb), dim3(256), 0, s, n, S, Y, AX, part);
   });
-  hipLaunchKernelGGL(k_proj_fin, dim3(m * m), dim3(256), 0, s, nb, m * m, part, out);
+  hipLaunchKernelGGL(k_proj_fin, dim3(m * m, S.nseg), dim3(256), 0, s, S, m * m, part, out, m * m);
 }
-void proj_reseed(hipStream_t s, size_t n, int m, int kk, const double* C, double* X, double* AX) {
+void proj_reseed(hipStream_t s, size_t n, const ProjSeg& S, int m, int kk, const double* C, int cstride, double* X,
+                 double* AX) {
   if (m <= 0 || m > kProjMax || kk <= 0 || kk > m || n == 0) return;
   proj_dispatch<1>(m, [&](auto Mc) {
     constexpr int M = decltype(Mc)::value;
-    hipLaunchKernelGGL(k_proj_reseed<M>, dim3(vec_grid(n)), dim3(256), 0, s, n, kk, C, X, AX);
+    hipLaunchKernelGGL(k_proj_reseed<M>, dim3(S.nb), dim3(256), 0, s, n, S, kk, C, cstride, X, AX);
   });
 }
 void psi_predict(hipStream_t s, size_t n, double f, int extrap, double* psi, double* prev) {

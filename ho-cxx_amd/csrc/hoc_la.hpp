@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 #include <cstddef>
 #include <cstdint>
+#include <vector>
 
 #include "hoc_kernels.hpp"
 
@@ -69,30 +70,42 @@ void psi_predict(hipStream_t s, size_t n, double f, int extrap, double* psi, dou
 
 // Projection onto a basis of earlier solutions (m <= kProjMax vectors X_j of length n, stored
 // contiguously, with AX_j = As X_j; Y = X for the As inner product, Y = AX for the Euclidean one
-// of the images; the basis need not be orthogonal: its Gram matrix G = Y^T AX is kept):
-//   proj_dots2:   out[j] = Y_j . t and out[m + j] = Y_j . b (j < m), through per-block partials
-//                 (part: proj_part_len(n) * 2m), deterministic
+// of the images; the basis need not be orthogonal: its Gram matrix G = Y^T AX is kept).  Every
+// coefficient is per segment of a ProjSeg (an ensemble instance: its own c, e, G), at a stride
+// per segment; reductions are deterministic (per-block partials summed in block order).
+//   proj_dots2:   out[j] = Y_j . t and out[m + j] = Y_j . b (j < m); part: S.nb * 2m
 //   proj_solve:   v[0, m) and v[m, 2m) <- G^-1 v (G in columns of ldg values, see k_proj_solve)
 //   proj_combine: out += sum_j c[j] X_j
 //   proj_ortho:   appends a new solution psi with As psi = b - r (r its solve's final residual):
 //                 x = psi - sum_j e[j] X_j and ax = (b - r) - sum_j e[j] AX_j (e = G^-1 Y . b
 //                 from before the solve: Y . As psi = Y . b - Y . r), and its Gram column gcol
-//                 (m + 2 values: Y_j . ax, x's own entry, psi's squared norm); part:
-//                 proj_part_len(n) * (m + 2); wcol[0, ldw): psi's coefficients in the basis with x
-//                 appended
-//   proj_gram:    out[i m + j] = Y_i . AX_j (part: proj_part_len(n) * m * m)
+//                 (m + 2 values: Y_j . ax, x's own entry, psi's squared norm); part: S.nb * (m + 2);
+//                 wcol[0, ldw): psi's coefficients in the basis with x appended
+//   proj_gram:    out[i m + j] = Y_i . AX_j (part: S.nb * m * m; out stride m * m)
 //   proj_reseed:  slots k < kk of a basis of m vectors become sum_j C[k m + j] X_j (and AX alike)
 constexpr int kProjMax = 16;
-int proj_part_len(size_t n);
-void proj_dots2(hipStream_t s, size_t n, int m, const double* Y, const double* t, const double* b, double* part,
-                double* out);
-void proj_solve(hipStream_t s, int m, const double* G, int ldg, double* v);
-void proj_combine(hipStream_t s, size_t n, int m, const double* X, const double* c, double* out);
-void proj_ortho(hipStream_t s, size_t n, int m, int mr, const double* X, const double* AX, const double* e,
-                const double* psi, const double* b, const double* r, double* x, double* ax, double* part,
-                double* gcol, double* wcol, int ldw);
-void proj_gram(hipStream_t s, size_t n, int m, const double* Y, const double* AX, double* part, double* out);
-void proj_reseed(hipStream_t s, size_t n, int m, int kk, const double* C, double* X, double* AX);
+struct ProjSeg {
+  int nseg = 0, nb = 0;              // segments; blocks of 256 threads over all of them
+  const int32_t* blk_seg = nullptr;  // [nb] the segment of each block (device)
+  const int32_t* bbeg = nullptr;     // [nseg + 1] first block of each segment (device)
+  const int64_t* sbeg = nullptr;     // [nseg + 1] first dof of each segment (device)
+};
+// Host side of a ProjSeg: blocks per segment (about 8 dofs per thread, at least one) from the
+// segments' first dofs.
+void proj_seg_layout(const std::vector<int64_t>& sbeg, std::vector<int32_t>& bbeg, std::vector<int32_t>& blk_seg);
+void proj_dots2(hipStream_t s, size_t n, const ProjSeg& S, int m, const double* Y, const double* t, const double* b,
+                double* part, double* out, int ostride);
+void proj_solve(hipStream_t s, const ProjSeg& S, int m, const double* G, int ldg, int gstride, double* v,
+                int vstride);
+void proj_combine(hipStream_t s, size_t n, const ProjSeg& S, int m, const double* X, const double* c, int cstride,
+                  double* out);
+void proj_ortho(hipStream_t s, size_t n, const ProjSeg& S, int m, int mr, const double* X, const double* AX,
+                const double* e, int estride, const double* psi, const double* b, const double* r, double* x,
+                double* ax, double* part, double* gcol, int gstride, double* wcol, int wstride, int ldw);
+void proj_gram(hipStream_t s, size_t n, const ProjSeg& S, int m, const double* Y, const double* AX, double* part,
+               double* out);
+void proj_reseed(hipStream_t s, size_t n, const ProjSeg& S, int m, int kk, const double* C, int cstride, double* X,
+                 double* AX);
 // psi' = sum_j a[j] x_j (x_j may be psi itself, read before the write), and save = psi (the old
 // value) when save is not null; nterm <= 8.
 void psi_combine(hipStream_t s, size_t n, int nterm, const double* a, const double* const* x, double* psi,

@@ -974,25 +974,50 @@ struct Solver {
   static constexpr int kProjVectors = 16;
   static constexpr int kProjKeep = 8;
   static constexpr int kProjLdg = kProjVectors + 1;  // Gram column: M entries and the solution's norm
+  static constexpr int kProjCoef = 2 * kProjMax;      // c and e of one segment
   static_assert(2 * kProjKeep <= kProjVectors && kProjVectors <= kProjMax, "basis restart sizes");
   struct ProjBasis {
     DBuf<double> X, AX;
-    DBuf<double> G, W, C;  // Gram columns (M x kProjLdg); solution coefficients (M x M); restart combination
+    // per segment: Gram columns (M x kProjLdg), solution coefficients (M x M), restart combination
+    // or measured Gram matrix (M x M)
+    DBuf<double> G, W, C;
     int m = 0;       // vectors held
     int e_m = -1;    // basis size the coefficients e of the solve in progress were formed for
+    int nseg = 0;    // segments the per-segment arrays were sized for
     int restarts = 0;
   };
   ProjBasis proj_fwd, proj_adj;     // stage solves; adjoint (transposed) solves
   ProjBasis* proj_cur = &proj_fwd;  // the basis of the solve in progress
   bool proj_force = false;          // project a solve that is not a stage solve (adjoint solves)
-  DBuf<double> pcoef, ppart, ptmp;  // pcoef: c (m) and e (m)
+  DBuf<double> pcoef, ppart, ptmp;  // pcoef: per segment c (m) and e (m)
+  // The projection's dof segments: one per ensemble instance (each projects on its own
+  // coefficients), one for a single problem.
+  ProjSeg pseg;
+  DBuf<int32_t> pseg_blk, pseg_bbeg;
+  DBuf<int64_t> pseg_sbeg;
+  std::vector<int64_t> pseg_key;  // the segments' first dofs the map was built for
   bool proj_on() const { return guess_mode() >= 5; }
   bool proj_mr() const { return krylov != HOC_KRYLOV_PCG; }  // minimal-residual form
   void proj_alloc() {
+    std::vector<int64_t> sb;
+    if (n_inst > 1 && (int)inst_off.size() == n_inst + 1)
+      for (int32_t e : inst_off) sb.push_back((int64_t)e * K2);
+    else
+      sb = {0, (int64_t)D};
+    if (sb != pseg_key) {
+      std::vector<int32_t> bbeg, blk;
+      proj_seg_layout(sb, bbeg, blk);
+      pseg_sbeg.upload(sb); pseg_bbeg.upload(bbeg); pseg_blk.upload(blk);
+      pseg.nseg = (int)sb.size() - 1;
+      pseg.nb = (int)blk.size();
+      pseg.blk_seg = pseg_blk.p; pseg.bbeg = pseg_bbeg.p; pseg.sbeg = pseg_sbeg.p;
+      pseg_key = sb;
+      ptmp.alloc(0);
+    }
     if (ptmp.n < D) {  // t of proj_before; coefficients; partial sums
       ptmp.alloc(D);
-      pcoef.alloc(2 * kProjMax);
-      ppart.alloc((size_t)proj_part_len(D) * std::max(2 * kProjMax + 2, kProjVectors * kProjVectors));
+      pcoef.alloc((size_t)pseg.nseg * kProjCoef);
+      ppart.alloc((size_t)pseg.nb * std::max(2 * kProjMax + 2, kProjVectors * kProjVectors));
     }
   }
   void proj_before(const double* b) {
@@ -1000,90 +1025,116 @@ struct Solver {
     P.e_m = -1;
     if (P.m == 0) return;
     proj_alloc();
+    if (P.nseg != pseg.nseg) { P.m = 0; return; }
     double* t = ptmp.p;
     HIPCHK(launch_trace(K, PRE_COPY, N, T, psi.p, nullptr, nullptr, nullptr, 0., nullptr, 0, nullptr, P64.tr.p, stream));
     lap<double>(P64, EPI_RESID, psi.p, t, b, nullptr, nullptr, 0., 0., nullptr, nullptr);  // t = b - As x0
-    proj_dots2(stream, D, P.m, proj_mr() ? P.AX.p : P.X.p, t, b, ppart, pcoef);  // Y^T t, Y^T b
-    proj_solve(stream, P.m, P.G.p, kProjLdg, pcoef);                               // c, e
-    proj_combine(stream, D, P.m, P.X.p, pcoef, psi);
+    proj_dots2(stream, D, pseg, P.m, proj_mr() ? P.AX.p : P.X.p, t, b, ppart, pcoef, kProjCoef);  // Y^T t, Y^T b
+    proj_solve(stream, pseg, P.m, P.G.p, kProjLdg, kProjVectors * kProjLdg, pcoef, kProjCoef);    // c, e
+    proj_combine(stream, D, pseg, P.m, P.X.p, pcoef, kProjCoef, psi);
     P.e_m = P.m;
   }
-  // A full basis: keep the span of the newest kProjKeep appended solutions (columns M - kProjKeep
-  // .. M - 1 of W: with kProjKeep <= M / 2 these were all appended after the last restart).
+  // A full basis: keep, per segment, the span of the newest kProjKeep appended solutions
+  // (columns M - kProjKeep .. M - 1 of W: with kProjKeep <= M / 2 these were all appended after
+  // the last restart).  The new basis is orthonormalised in the inner product of a freshly
+  // measured Gram matrix; a segment whose newest solutions span fewer than the most vectors kept
+  // by any segment gets zero vectors in the remaining slots (left out of its solves).
   void proj_restart(ProjBasis& P) {
     constexpr int M = kProjVectors, KP = kProjKeep, LG = kProjLdg;
+    const int ns = pseg.nseg;
     P.m = 0;
     P.e_m = -1;
     ++P.restarts;
     if (KP == 0) return;
-    std::vector<double> w((size_t)M * M), gc((size_t)M * LG), g((size_t)M * M), c;
-    proj_gram(stream, D, M, proj_mr() ? P.AX.p : P.X.p, P.AX.p, ppart, P.C.p);  // G measured afresh
+    std::vector<double> w((size_t)ns * M * M), gc((size_t)ns * M * LG), gm((size_t)ns * M * M);
+    proj_gram(stream, D, pseg, M, proj_mr() ? P.AX.p : P.X.p, P.AX.p, ppart, P.C.p);  // G measured afresh
     HIPCHK(hipStreamSynchronize(stream));
     d2h(w.data(), P.W.p, w.size() * sizeof(double));
     d2h(gc.data(), P.G.p, gc.size() * sizeof(double));
-    d2h(g.data(), P.C.p, g.size() * sizeof(double));
-    // a vector left out of the solves (see k_proj_solve) drops; G symmetrised
-    bool ok[M];
-    for (int j = 0; j < M; ++j) {
-      const double gjj = gc[(size_t)j * LG + j];
-      ok[j] = gjj > 1e-24 * std::fabs(gc[(size_t)j * LG + j + 1]) && gjj > 0.;
-    }
-    for (int j = 0; j < M; ++j)
-      for (int i = 0; i <= j; ++i) {
-        const double v = (ok[i] && ok[j]) ? 0.5 * (g[(size_t)i * M + j] + g[(size_t)j * M + i]) : 0.;
-        g[(size_t)i * M + j] = g[(size_t)j * M + i] = v;
-      }
-    auto gdot = [&](const double* u, const double* v) {
-      double d = 0.;
-      for (int i = 0; i < M; ++i) {
-        double gi = 0.;
-        for (int j = 0; j < M; ++j) gi += g[(size_t)i * M + j] * v[j];
-        d += u[i] * gi;
-      }
-      return d;
-    };
+    d2h(gm.data(), P.C.p, gm.size() * sizeof(double));
+    std::vector<std::vector<double>> cs(ns);
     int kk = 0;
-    for (int col = M - KP; col < M; ++col) {  // modified Gram-Schmidt, twice, in the G inner product
-      double v[M];
-      for (int j = 0; j < M; ++j) v[j] = ok[j] ? w[(size_t)col * M + j] : 0.;
-      const double n0 = gdot(v, v);
-      for (int pass = 0; pass < 2; ++pass)
-        for (int k = 0; k < kk; ++k) {
-          const double d = gdot(&c[(size_t)k * M], v);
-          for (int j = 0; j < M; ++j) v[j] -= d * c[(size_t)k * M + j];
+    for (int sg = 0; sg < ns; ++sg) {
+      const double* W = w.data() + (size_t)sg * M * M;
+      const double* Gc = gc.data() + (size_t)sg * M * LG;
+      double* g = gm.data() + (size_t)sg * M * M;
+      std::vector<double>& c = cs[sg];
+      // a vector left out of the solves (see k_proj_solve) drops; G symmetrised
+      bool ok[M];
+      for (int j = 0; j < M; ++j) {
+        const double gjj = Gc[(size_t)j * LG + j];
+        ok[j] = gjj > 1e-24 * std::fabs(Gc[(size_t)j * LG + j + 1]) && gjj > 0.;
+      }
+      for (int j = 0; j < M; ++j)
+        for (int i = 0; i <= j; ++i) {
+          const double v = (ok[i] && ok[j]) ? 0.5 * (g[(size_t)i * M + j] + g[(size_t)j * M + i]) : 0.;
+          g[(size_t)i * M + j] = g[(size_t)j * M + i] = v;
         }
-      const double nv = gdot(v, v);
-      if (!(nv > 1e-24 * n0) || !(nv > 0.)) continue;  // in the span of the newer ones already kept
-      const double f = 1. / std::sqrt(nv);
-      for (int j = 0; j < M; ++j) c.push_back(v[j] * f);
-      ++kk;
+      auto gdot = [&](const double* u, const double* v) {
+        double d = 0.;
+        for (int i = 0; i < M; ++i) {
+          double gi = 0.;
+          for (int j = 0; j < M; ++j) gi += g[(size_t)i * M + j] * v[j];
+          d += u[i] * gi;
+        }
+        return d;
+      };
+      int k = 0;
+      for (int col = M - KP; col < M; ++col) {  // modified Gram-Schmidt, twice, in the G inner product
+        double v[M];
+        for (int j = 0; j < M; ++j) v[j] = ok[j] ? W[(size_t)col * M + j] : 0.;
+        const double n0 = gdot(v, v);
+        for (int pass = 0; pass < 2; ++pass)
+          for (int q = 0; q < k; ++q) {
+            const double d = gdot(&c[(size_t)q * M], v);
+            for (int j = 0; j < M; ++j) v[j] -= d * c[(size_t)q * M + j];
+          }
+        const double nv = gdot(v, v);
+        if (!(nv > 1e-24 * n0) || !(nv > 0.)) continue;  // in the span of the newer ones already kept
+        const double f = 1. / std::sqrt(nv);
+        for (int j = 0; j < M; ++j) c.push_back(v[j] * f);
+        ++k;
+      }
+      kk = std::max(kk, k);
     }
     if (kk == 0) return;
-    std::vector<double> gi((size_t)kk * LG, 0.);  // the new basis is G-orthonormal: identity columns
-    for (int j = 0; j < kk; ++j) { gi[(size_t)j * LG + j] = 1.; gi[(size_t)j * LG + j + 1] = 1.; }
-    HIPCHK(hipMemcpyAsync(P.C.p, c.data(), c.size() * sizeof(double), hipMemcpyHostToDevice, stream));
+    // combinations (zero rows pad a segment to kk) and the new Gram columns: G = I on the kept
+    // vectors, 0 (left out) on the padding
+    std::vector<double> ca((size_t)ns * M * M, 0.), gi((size_t)ns * M * LG, 0.);
+    for (int sg = 0; sg < ns; ++sg) {
+      const int k = (int)cs[sg].size() / M;
+      std::copy(cs[sg].begin(), cs[sg].end(), ca.begin() + (size_t)sg * M * M);
+      for (int j = 0; j < kk; ++j) {
+        gi[(size_t)sg * M * LG + (size_t)j * LG + j] = j < k ? 1. : 0.;
+        gi[(size_t)sg * M * LG + (size_t)j * LG + j + 1] = 1.;
+      }
+    }
+    HIPCHK(hipMemcpyAsync(P.C.p, ca.data(), ca.size() * sizeof(double), hipMemcpyHostToDevice, stream));
     HIPCHK(hipMemcpyAsync(P.G.p, gi.data(), gi.size() * sizeof(double), hipMemcpyHostToDevice, stream));
-    proj_reseed(stream, D, M, kk, P.C.p, P.X.p, P.AX.p);
-    HIPCHK(hipStreamSynchronize(stream));  // (c and gi are locals)
+    proj_reseed(stream, D, pseg, M, kk, P.C.p, M * M, P.X.p, P.AX.p);
+    HIPCHK(hipStreamSynchronize(stream));  // (ca and gi are locals)
     P.m = kk;
   }
   void proj_after(const double* b, const double* r) {
     constexpr int M = kProjVectors;
     ProjBasis& P = *proj_cur;
     proj_alloc();
-    if (P.X.n < (size_t)M * D) {
+    if (P.X.n < (size_t)M * D || P.nseg != pseg.nseg) {
       P.X.alloc((size_t)M * D); P.AX.alloc((size_t)M * D);
-      P.G.alloc((size_t)M * kProjLdg); P.W.alloc((size_t)M * M); P.C.alloc((size_t)M * M);
+      const size_t ns = pseg.nseg;
+      P.G.alloc(ns * M * kProjLdg); P.W.alloc(ns * M * M); P.C.alloc(ns * M * M);
+      P.nseg = pseg.nseg;
       P.m = 0;
     }
     if (P.m >= M) proj_restart(P);
     const int j = P.m;
     if (j > 0 && P.e_m != j) {  // e = G^-1 Y^T b
-      proj_dots2(stream, D, j, proj_mr() ? P.AX.p : P.X.p, b, b, ppart, pcoef);
-      proj_solve(stream, j, P.G.p, kProjLdg, pcoef);
+      proj_dots2(stream, D, pseg, j, proj_mr() ? P.AX.p : P.X.p, b, b, ppart, pcoef, kProjCoef);
+      proj_solve(stream, pseg, j, P.G.p, kProjLdg, M * kProjLdg, pcoef, kProjCoef);
     }
-    proj_ortho(stream, D, j, proj_mr() ? 1 : 0, P.X.p, P.AX.p, pcoef.p + j, psi, b, r, P.X.p + (size_t)j * D,
-               P.AX.p + (size_t)j * D, ppart, P.G.p + (size_t)j * kProjLdg, P.W.p + (size_t)j * M, M);
+    proj_ortho(stream, D, pseg, j, proj_mr() ? 1 : 0, P.X.p, P.AX.p, pcoef.p + j, kProjCoef, psi, b, r,
+               P.X.p + (size_t)j * D, P.AX.p + (size_t)j * D, ppart, P.G.p + (size_t)j * kProjLdg, M * kProjLdg,
+               P.W.p + (size_t)j * M, M * M, M);
     P.m = j + 1;
     P.e_m = -1;
   }
