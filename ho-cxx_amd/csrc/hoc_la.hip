@@ -109,30 +109,45 @@ __global__ void k_zero_if_flag(size_t n, R* __restrict__ x, const int* __restric
 // without a copy (the host reads only this word, so the store needs no ordering); near: still
 // active with the residual within 8x of the tolerance, so the next iteration may be the last.
 __device__ void finalize_phase(int phase, double a, double b, double* __restrict__ S, int* __restrict__ F,
-                               int maxiter);
+                               int maxiter, const double* __restrict__ irr, const double* __restrict__ ibb,
+                               int ninst);
 __global__ void k_finalize(int phase, const double* __restrict__ pa, int np, double* __restrict__ S,
-                           int* __restrict__ F, int maxiter, int* __restrict__ mirror, int seq) {
+                           int* __restrict__ F, int maxiter, int* __restrict__ mirror, int seq,
+                           const double* __restrict__ irr, const double* __restrict__ ibb, int ninst) {
   const bool live = phase == PH_INIT || phase == PH_BB || F[F_ACTIVE];
   const bool half = live && phase >= PH_OMEGA && F[F_HALF];  // BiCGSTAB converged at the half step
   if (live && !half) {  // uniform over the block
     double a = 0., b = 0.;
     for (int i = threadIdx.x; i < np; i += blockDim.x) { a += pa[i]; b += pa[np + i]; }
     block_sum2(a, b);
-    if (threadIdx.x == 0) finalize_phase(phase, a, b, S, F, maxiter >= 0 ? maxiter : F[F_MAXIT]);
+    if (threadIdx.x == 0) finalize_phase(phase, a, b, S, F, maxiter >= 0 ? maxiter : F[F_MAXIT], irr, ibb, ninst);
   } else if (half && phase == PH_RNORM && threadIdx.x == 0) {
     F[F_ACTIVE] = 0;
   }
   if (mirror && threadIdx.x == 0) {
     const int sq = seq >= 0 ? seq : ++F[F_SEQ];
     const int act = F[F_ACTIVE] ? 1 : 0;
-    const int near = act && S[S_RR] <= 64. * S[S_TOL2] * S[S_BB];  // residual within 8x of the tolerance
+    const int near = act && (irr ? S[S_IMAX] <= 64. : S[S_RR] <= 64. * S[S_TOL2] * S[S_BB]);  // within 8x of the tolerance
     __hip_atomic_store(mirror, sq * 4 + near * 2 + act, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
   }
 }
 __global__ void k_set_int(int* __restrict__ p, int v) { *p = v; }
 __device__ void finalize_phase(int phase, double a, double b, double* __restrict__ S, int* __restrict__ F,
-                               int maxiter) {
+                               int maxiter, const double* __restrict__ irr, const double* __restrict__ ibb,
+                               int ninst) {
   const double tol2bb = S[S_TOL2] * S[S_BB];
+  // the convergence test of PH_INIT / PH_UPD: the joint residual, or every instance's own
+  bool above = b > tol2bb;
+  if (irr && (phase == PH_INIT || phase == PH_UPD)) {
+    double bmax = 0., rmax = 0.;
+    for (int i = 0; i < ninst; ++i) bmax = fmax(bmax, ibb[i]);
+    for (int i = 0; i < ninst; ++i) {
+      const double d = S[S_TOL2] * (ibb[i] > 0. ? ibb[i] : bmax);
+      rmax = fmax(rmax, d > 0. ? irr[i] / d : (irr[i] > 0. ? HUGE_VAL : 0.));
+    }
+    S[S_IMAX] = rmax;
+    above = rmax > 1. || !isfinite(rmax);
+  }
   switch (phase) {
     case PH_BB: S[S_BB] = a; break;
     case PH_INIT: {
@@ -142,7 +157,7 @@ __device__ void finalize_phase(int phase, double a, double b, double* __restrict
       F[F_ZERO] = zero;
       F[F_ITER] = 0;
       F[F_HALF] = 0;
-      F[F_ACTIVE] = (!zero && b > S[S_TOL2] * S[S_BB] && maxiter > 0) ? 1 : 0;
+      F[F_ACTIVE] = (!zero && above && maxiter > 0) ? 1 : 0;
       S[S_RZ] = 0.; S[S_RHO] = 1.; S[S_ALPHA] = 1.; S[S_OMEGA] = 1.; S[S_BETA] = 0.;
     } break;
     case PH_RZ: {  // breakdown (z = 0 or an indefinite preconditioner) ends the solve
@@ -159,7 +174,7 @@ __device__ void finalize_phase(int phase, double a, double b, double* __restrict
     case PH_UPD: {
       S[S_RR] = b;
       F[F_ITER] += 1;
-      F[F_ACTIVE] = (b > tol2bb && F[F_ITER] < maxiter && isfinite(b)) ? 1 : 0;
+      F[F_ACTIVE] = (above && F[F_ITER] < maxiter && isfinite(b)) ? 1 : 0;
     } break;
     case PH_RHO: {
       if (a == 0. || !isfinite(a)) { F[F_ACTIVE] = 0; break; }
@@ -771,6 +786,23 @@ void proj_reseed(hipStream_t s, size_t n, const ProjSeg& S, int m, int kk, const
     hipLaunchKernelGGL(k_proj_reseed<M>, dim3(S.nb), dim3(256), 0, s, n, S, kk, C, cstride, X, AX);
   });
 }
+__global__ __launch_bounds__(256) void k_seg_norm2(ProjSeg S, VecArgs A, const double* __restrict__ r,
+                                                   const int* __restrict__ F, double* __restrict__ part) {
+  if (F && !F[F_ACTIVE]) return;
+  const SegRange R = seg_range(S);
+  DofSlot ds(A, R.i0, R.step);
+  double p[1] = {0.};
+  for (size_t v = R.i0; v < R.i1; v += R.step, ds.next()) {
+    const double u = r[v] * A.winv[ds.m];
+    p[0] += u * u;
+  }
+  block_sums<1>(p, part + blockIdx.x);
+}
+void seg_norm2(hipStream_t s, const ProjSeg& S, const VecArgs& A, const double* r, const int* F, double* part,
+               double* out) {
+  hipLaunchKernelGGL(k_seg_norm2, dim3(S.nb), dim3(256), 0, s, S, A, r, F, part);
+  hipLaunchKernelGGL(k_proj_fin, dim3(1, S.nseg), dim3(256), 0, s, S, 1, part, out, 1);
+}
 void psi_predict(hipStream_t s, size_t n, double f, int extrap, double* psi, double* prev) {
   if (n == 0) return;
   hipLaunchKernelGGL(k_psi_predict, dim3(vec_grid(n)), dim3(256), 0, s, n, f, extrap, psi, prev);
@@ -781,8 +813,9 @@ void zero_if_flag(hipStream_t s, size_t n, R* x, const int* F) {
 }
 void set_int(hipStream_t s, int* p, int v) { hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, s, p, v); }
 void finalize(hipStream_t s, int phase, const double* part, int np, double* S, int* F, int maxiter, int* mirror,
-              int seq) {
-  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, phase, part, np, S, F, maxiter, mirror, seq);
+              int seq, const double* irr, const double* ibb, int ninst) {
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, s, phase, part, np, S, F, maxiter, mirror, seq, irr, ibb,
+                     ninst);
 }
 template <class R>
 void csr(hipStream_t s, int mode, int n, const int32_t* ptr, const int32_t* col, const R* val, const R* x, const R* f,

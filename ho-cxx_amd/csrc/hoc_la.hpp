@@ -15,7 +15,7 @@ namespace hoc {
 
 // Device scalar slots (double) and flags (int) of the Krylov solvers.
 // F_MAXIT: iteration budget read by finalize kernels launched with maxiter < 0.
-enum { S_BB = 0, S_RR, S_RZ, S_PQ, S_ALPHA, S_BETA, S_OMEGA, S_RHO, S_TOL2, S_COUNT = 16 };
+enum { S_BB = 0, S_RR, S_RZ, S_PQ, S_ALPHA, S_BETA, S_OMEGA, S_RHO, S_TOL2, S_IMAX, S_COUNT = 16 };
 enum { F_ACTIVE = 0, F_ITER, F_ZERO, F_HALF, F_MAXIT, F_SEQ, F_COUNT = 8 };
 enum { PH_BB = 0, PH_INIT, PH_RZ, PH_PQ, PH_UPD, PH_RHO, PH_ALPHA, PH_SNORM, PH_OMEGA, PH_RNORM };
 enum { VOP_PCG_UPD = 0, VOP_BICG_P, VOP_BICG_S, VOP_DOT2, VOP_COPY, VOP_BICG_X };
@@ -39,8 +39,13 @@ void zero_if_flag(hipStream_t s, size_t n, R* x, const int* F);
 // kernel ends (near: active with the residual within 8x of the tolerance);
 // seq < 0: the next number of the device counter F[F_SEQ] (incremented) instead of seq.
 // maxiter < 0: the budget is read from F[F_MAXIT].
+// irr, ibb (optional, ninst values): per-instance squared residual and right-hand-side norms of
+// an ensemble; PH_INIT and PH_UPD then continue while some instance has irr > tol^2 ibb (tol^2 =
+// S[S_TOL2]; an instance with ibb = 0 is held to the largest ibb), and S[S_IMAX] keeps the
+// largest irr / (tol^2 ibb).
 void finalize(hipStream_t s, int phase, const double* part, int np, double* S, int* F, int maxiter,
-              int* mirror = nullptr, int seq = 0);
+              int* mirror = nullptr, int seq = 0, const double* irr = nullptr, const double* ibb = nullptr,
+              int ninst = 0);
 // *p = v, in stream order (a kernel: usable where a memset or copy may not be recorded).
 void set_int(hipStream_t s, int* p, int v);
 template <class R>
@@ -106,6 +111,10 @@ void proj_gram(hipStream_t s, size_t n, const ProjSeg& S, int m, const double* Y
                double* out);
 void proj_reseed(hipStream_t s, size_t n, const ProjSeg& S, int m, int kk, const double* C, int cstride, double* X,
                  double* AX);
+// out[s] = sum over segment s of (r winv)^2 (weighted squared norms per ensemble instance);
+// skipped when F (may be null) has its active flag clear.  part: S.nb values.
+void seg_norm2(hipStream_t s, const ProjSeg& S, const VecArgs& A, const double* r, const int* F, double* part,
+               double* out);
 // psi' = sum_j a[j] x_j (x_j may be psi itself, read before the write), and save = psi (the old
 // value) when save is not null; nterm <= 8.
 void psi_combine(hipStream_t s, size_t n, int nterm, const double* a, const double* const* x, double* psi,

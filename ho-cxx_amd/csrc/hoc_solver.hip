@@ -341,6 +341,7 @@ struct Solver {
     proj_fwd.X.alloc(0); proj_fwd.AX.alloc(0); proj_fwd.m = 0;
     proj_adj.X.alloc(0); proj_adj.AX.alloc(0); proj_adj.m = 0;
     reset_adjoint();
+    ensure_segments();  // (the timed iterations of an ensemble run its per-instance test)
     tune_launch_mode();
     invalidate_guess();
     time = 0.;
@@ -714,6 +715,13 @@ struct Solver {
       lap(P, EPI_STORE, P.p.p, P.q.p, (const R*)nullptr, (const R*)nullptr, (R*)nullptr, 0., 0., part.p, Fp);
       finalize(stream, PH_PQ, part, lb, Sp, Fp, maxit);
       vec<R>(stream, VOP_PCG_UPD, VA, D, x, r, P.p, P.q, nullptr, Sp, Fp, part);
+      if constexpr (std::is_same<R, double>::value) {
+        if (ens_pcg()) {  // every instance against its own right-hand side
+          seg_norm2(stream, pseg, VA, r, Fp, ens_part.p, ens_rr.p);
+          finalize(stream, PH_UPD, part, vg, Sp, Fp, maxit, mirror_dev, seq, ens_rr.p, ens_bb.p, pseg.nseg);
+          return;
+        }
+      }
       finalize(stream, PH_UPD, part, vg, Sp, Fp, maxit, mirror_dev, seq);
     } else {
       const R* none = nullptr;
@@ -839,7 +847,12 @@ struct Solver {
     const int lb = lap_blocks(N);
     HIPCHK(launch_trace(K, PRE_COPY, N, T, psi.p, nullptr, nullptr, nullptr, 0., nullptr, 0, nullptr, P64.tr.p, stream));
     lap<double>(P64, EPI_RESID, psi.p, P64.r.p, bs.p, nullptr, nullptr, 0., 0., part.p, nullptr);
-    finalize(stream, PH_INIT, part, lb, S, F, cfg.maxiter);
+    if (ens_pcg()) {
+      seg_norm2(stream, pseg, VA, P64.r.p, nullptr, ens_part.p, ens_rr.p);
+      finalize(stream, PH_INIT, part, lb, S, F, cfg.maxiter, nullptr, 0, ens_rr.p, ens_bb.p, pseg.nseg);
+    } else {
+      finalize(stream, PH_INIT, part, lb, S, F, cfg.maxiter);
+    }
   }
 
   // solve_Poisson (src/HO_2D.cpp:1841-1977) for omega at the current boundary values.
@@ -998,22 +1011,29 @@ struct Solver {
   std::vector<int64_t> pseg_key;  // the segments' first dofs the map was built for
   bool proj_on() const { return guess_mode() >= 5; }
   bool proj_mr() const { return krylov != HOC_KRYLOV_PCG; }  // minimal-residual form
-  void proj_alloc() {
+  // The segment map (rebuilt when the mesh or the instances change) and the per-instance norms
+  // of an ensemble's convergence test.
+  DBuf<double> ens_part, ens_rr, ens_bb;
+  bool ens_pcg() const { return n_inst > 1 && krylov == HOC_KRYLOV_PCG; }
+  void ensure_segments() {
     std::vector<int64_t> sb;
     if (n_inst > 1 && (int)inst_off.size() == n_inst + 1)
       for (int32_t e : inst_off) sb.push_back((int64_t)e * K2);
     else
       sb = {0, (int64_t)D};
-    if (sb != pseg_key) {
-      std::vector<int32_t> bbeg, blk;
-      proj_seg_layout(sb, bbeg, blk);
-      pseg_sbeg.upload(sb); pseg_bbeg.upload(bbeg); pseg_blk.upload(blk);
-      pseg.nseg = (int)sb.size() - 1;
-      pseg.nb = (int)blk.size();
-      pseg.blk_seg = pseg_blk.p; pseg.bbeg = pseg_bbeg.p; pseg.sbeg = pseg_sbeg.p;
-      pseg_key = sb;
-      ptmp.alloc(0);
-    }
+    if (sb == pseg_key) return;
+    std::vector<int32_t> bbeg, blk;
+    proj_seg_layout(sb, bbeg, blk);
+    pseg_sbeg.upload(sb); pseg_bbeg.upload(bbeg); pseg_blk.upload(blk);
+    pseg.nseg = (int)sb.size() - 1;
+    pseg.nb = (int)blk.size();
+    pseg.blk_seg = pseg_blk.p; pseg.bbeg = pseg_bbeg.p; pseg.sbeg = pseg_sbeg.p;
+    pseg_key = sb;
+    ptmp.alloc(0);
+    ens_part.alloc(pseg.nb); ens_rr.alloc(pseg.nseg); ens_bb.alloc(pseg.nseg);
+  }
+  void proj_alloc() {
+    ensure_segments();
     if (ptmp.n < D) {  // t of proj_before; coefficients; partial sums
       ptmp.alloc(D);
       pcoef.alloc((size_t)pseg.nseg * kProjCoef);
@@ -1186,8 +1206,12 @@ struct Solver {
     const bool proj = proj_on() && (!std::isnan(stage_tau) || proj_force);
     if (proj) proj_before(bs);
     finalize(stream, PH_BB, part, lb, S, F, maxit);
-    if (n_inst > 1) {
-      // ||b - A psi|| <= tol min_i ||b_i|| bounds every instance's own relative residual
+    if (ens_pcg()) {
+      // PCG: the iterations continue while some instance has ||r_i|| > tol ||b_i||
+      ensure_segments();
+      seg_norm2(stream, pseg, VA, bs, nullptr, ens_part.p, ens_bb.p);
+    } else if (n_inst > 1) {
+      // BiCGSTAB: ||b - A psi|| <= tol min_i ||b_i|| bounds every instance's own relative residual
       // (instances with a zero right-hand side are left out of the minimum)
       inst_norms(stream, VA, n_inst, d_ioff, K2, bs, d_inorm);
       std::vector<double> bi(n_inst);
