@@ -158,7 +158,7 @@ def roofline(prof, suffix="", noop=0):
 
 # Every PROFILE_EVERY-th fine-operator launch of the timed steps is bracketed by HIP events: an
 # event pair around every launch costs ~5% (FP64) to ~7% (mixed) of a step, one in eight costs
-# < 1%; 8 is coprime with the 7 operator launches of a PCG iteration, so all variants and all
+# < 1%; 8 is coprime with the 7 (FP64) and 9 (mixed) operator launches of a PCG iteration, so all variants and all
 # positions in the V-cycle are sampled.
 PROFILE_EVERY = 8
 

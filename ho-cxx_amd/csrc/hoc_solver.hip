@@ -524,7 +524,9 @@ struct Solver {
     const double lmax = 1.1 * lam, lmin = lmax * (cfg.cheb_lower > 0. ? cfg.cheb_lower : 1. / 60.);
     const double theta = 0.5 * (lmax + lmin), delta = 0.5 * (lmax - lmin);
     const double sigma = theta / delta;
-    cheb_m = cfg.cheb_degree > 0 ? cfg.cheb_degree : 3;
+    // default degree: 3 in FP64; 4 for the FP32 cycle of the mixed-precision solve, where the two
+    // extra operator applies cost less than the residual reduction they add
+    cheb_m = cfg.cheb_degree > 0 ? cfg.cheb_degree : mixed ? 4 : 3;
     cheb_theta = theta;
     double rho = 1. / sigma;
     for (int k = 1; k < cheb_m; ++k) {
