@@ -4,7 +4,8 @@ Metric: Mdof-steps/s per RK4 step including the four Poisson solves, where
 Mdof = N_el * K^2 / 1e6.  Workload: lid-driven cavity on the unit square, 512 x 512 = 2^18 bilinear quads stored as
 unstructured element/edge lists, K = 5 (P = 4), Re = 100, zero initial
 vorticity, RK4 with dt = half the measured explicit-RK4 diffusion limit, Krylov
-relative tolerance 1e-10.  Inputs live in HBM for the whole timed region.
+relative tolerance 1e-10; by default 5 warm-up steps, then steps 6-25 timed.  Inputs live in
+HBM for the whole timed region.
 
 Prints ONE JSON line.  `value` is the plain FP64 path; by default the same
 workload is then run in mixed precision (the FP64 Krylov method with its V-cycle
@@ -200,8 +201,8 @@ def run(hoc, m, n, K, Re, dt, tol, precision, warmup, steps):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=20)   # the workload's protocol: steps 6-25 timed
+    ap.add_argument("--warmup", type=int, default=5)   # after 5 warm-up steps
     ap.add_argument("--n", type=int, default=512, help="cells per side of the cavity")
     ap.add_argument("--K", type=int, default=5)
     ap.add_argument("--Re", type=float, default=100.0)
