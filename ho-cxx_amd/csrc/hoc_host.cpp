@@ -970,4 +970,46 @@ void write_vtk_sps(const std::string& path, int K, int n_elem, const double* pts
   if (!out) throw std::runtime_error("vtk: write failed for " + path);
 }
 
+int proj_restart_basis(int m, int keep, const double* gcols, int ldg, double* g, const double* w,
+                       std::vector<double>& c) {
+  if (m <= 0 || keep < 0 || keep > m || ldg < m + 1) throw std::invalid_argument("proj_restart_basis: sizes");
+  std::vector<char> ok(m);
+  for (int j = 0; j < m; ++j) {
+    const double gjj = gcols[(size_t)j * ldg + j];
+    ok[j] = gjj > 1e-24 * std::fabs(gcols[(size_t)j * ldg + j + 1]) && gjj > 0.;
+  }
+  for (int j = 0; j < m; ++j)
+    for (int i = 0; i <= j; ++i) {
+      const double v = (ok[i] && ok[j]) ? 0.5 * (g[(size_t)i * m + j] + g[(size_t)j * m + i]) : 0.;
+      g[(size_t)i * m + j] = g[(size_t)j * m + i] = v;
+    }
+  auto gdot = [&](const double* u, const double* v) {
+    double d = 0.;
+    for (int i = 0; i < m; ++i) {
+      double gi = 0.;
+      for (int j = 0; j < m; ++j) gi += g[(size_t)i * m + j] * v[j];
+      d += u[i] * gi;
+    }
+    return d;
+  };
+  c.clear();
+  int kk = 0;
+  std::vector<double> v(m);
+  for (int col = m - keep; col < m; ++col) {
+    for (int j = 0; j < m; ++j) v[j] = ok[j] ? w[(size_t)col * m + j] : 0.;
+    const double n0 = gdot(v.data(), v.data());
+    for (int pass = 0; pass < 2; ++pass)
+      for (int q = 0; q < kk; ++q) {
+        const double d = gdot(&c[(size_t)q * m], v.data());
+        for (int j = 0; j < m; ++j) v[j] -= d * c[(size_t)q * m + j];
+      }
+    const double nv = gdot(v.data(), v.data());
+    if (!(nv > 1e-24 * n0) || !(nv > 0.)) continue;  // in the span of those already kept
+    const double f = 1. / std::sqrt(nv);
+    for (int j = 0; j < m; ++j) c.push_back(v[j] * f);
+    ++kk;
+  }
+  return kk;
+}
+
 }  // namespace hoc

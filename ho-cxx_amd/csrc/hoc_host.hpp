@@ -142,6 +142,17 @@ struct AMGHost {
   std::vector<AMGLevelHost> levels;  // last level: coarsest (A only)
   std::vector<double> coarse_inv;    // dense inverse of the coarsest matrix (row-major)
 };
+// Restart combination of a projection basis of m vectors X_j.  gcols: the basis's recorded Gram
+// columns (column j at gcols + j ldg: G_ij for i < j, G_jj, then in row j + 1 the squared norm of
+// the solution vector j came from; a vector with G_jj <= 1e-24 of that norm takes no part).  g:
+// the measured Gram matrix (m x m, row-major), symmetrised in place with the non-participating
+// rows and columns cleared.  w: column-major, column q the coefficients in the basis of the q-th
+// appended solution.  The newest keep columns (m - keep .. m - 1) are orthonormalised in the g
+// inner product by modified Gram-Schmidt applied twice, oldest first; a column whose remaining
+// norm falls to 1e-24 of its own is left out.  c receives the combinations, kk rows of m (new
+// vector k = sum_j c[k m + j] X_j); returns kk.
+int proj_restart_basis(int m, int keep, const double* gcols, int ldg, double* g, const double* w,
+                       std::vector<double>& c);
 void build_sa_amg(const CSR& A, int coarse_enough, AMGHost& h);
 
 // Throws std::invalid_argument unless A is a well-formed n x m CSR matrix (row pointers from

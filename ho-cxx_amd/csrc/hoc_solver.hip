@@ -1080,43 +1080,7 @@ struct Solver {
       const double* W = w.data() + (size_t)sg * M * M;
       const double* Gc = gc.data() + (size_t)sg * M * LG;
       double* g = gm.data() + (size_t)sg * M * M;
-      std::vector<double>& c = cs[sg];
-      // a vector left out of the solves (see k_proj_solve) drops; G symmetrised
-      bool ok[M];
-      for (int j = 0; j < M; ++j) {
-        const double gjj = Gc[(size_t)j * LG + j];
-        ok[j] = gjj > 1e-24 * std::fabs(Gc[(size_t)j * LG + j + 1]) && gjj > 0.;
-      }
-      for (int j = 0; j < M; ++j)
-        for (int i = 0; i <= j; ++i) {
-          const double v = (ok[i] && ok[j]) ? 0.5 * (g[(size_t)i * M + j] + g[(size_t)j * M + i]) : 0.;
-          g[(size_t)i * M + j] = g[(size_t)j * M + i] = v;
-        }
-      auto gdot = [&](const double* u, const double* v) {
-        double d = 0.;
-        for (int i = 0; i < M; ++i) {
-          double gi = 0.;
-          for (int j = 0; j < M; ++j) gi += g[(size_t)i * M + j] * v[j];
-          d += u[i] * gi;
-        }
-        return d;
-      };
-      int k = 0;
-      for (int col = M - KP; col < M; ++col) {  // modified Gram-Schmidt, twice, in the G inner product
-        double v[M];
-        for (int j = 0; j < M; ++j) v[j] = ok[j] ? W[(size_t)col * M + j] : 0.;
-        const double n0 = gdot(v, v);
-        for (int pass = 0; pass < 2; ++pass)
-          for (int q = 0; q < k; ++q) {
-            const double d = gdot(&c[(size_t)q * M], v);
-            for (int j = 0; j < M; ++j) v[j] -= d * c[(size_t)q * M + j];
-          }
-        const double nv = gdot(v, v);
-        if (!(nv > 1e-24 * n0) || !(nv > 0.)) continue;  // in the span of the newer ones already kept
-        const double f = 1. / std::sqrt(nv);
-        for (int j = 0; j < M; ++j) c.push_back(v[j] * f);
-        ++k;
-      }
+      const int k = proj_restart_basis(M, KP, Gc, LG, g, W, cs[sg]);
       kk = std::max(kk, k);
     }
     if (kk == 0) return;
