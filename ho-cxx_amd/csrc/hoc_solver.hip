@@ -521,20 +521,6 @@ struct Solver {
       h2d(P64.z.p, hx.data(), D * sizeof(double));
     }
     lam_max = lam;
-    const double lmax = 1.1 * lam, lmin = lmax * (cfg.cheb_lower > 0. ? cfg.cheb_lower : 1. / 60.);
-    const double theta = 0.5 * (lmax + lmin), delta = 0.5 * (lmax - lmin);
-    const double sigma = theta / delta;
-    // default degree: 3 in FP64; 4 for the FP32 cycle of the mixed-precision solve, where the two
-    // extra operator applies cost less than the residual reduction they add
-    cheb_m = cfg.cheb_degree > 0 ? cfg.cheb_degree : mixed ? 4 : 3;
-    cheb_theta = theta;
-    double rho = 1. / sigma;
-    for (int k = 1; k < cheb_m; ++k) {
-      const double rho_new = 1. / (2. * sigma - rho);
-      cheb_c1[k] = rho_new * rho;
-      cheb_c2[k] = 2. * rho_new / delta;
-      rho = rho_new;
-    }
     // symmetry of As decides the Krylov method
     krylov = cfg.krylov;
     if (krylov == HOC_KRYLOV_AUTO) {
@@ -550,6 +536,23 @@ struct Solver {
       double a = 0., b = 0., na = 0., nb = 0.;
       for (size_t i = 0; i < D; ++i) { a += hx2[i] * hAx[i]; b += hx[i] * hAy[i]; na += std::fabs(hx2[i] * hAx[i]); nb += std::fabs(hx[i] * hAy[i]); }
       krylov = std::fabs(a - b) <= 1e-12 * std::max(na, nb) ? HOC_KRYLOV_PCG : HOC_KRYLOV_BICGSTAB;
+    }
+    const double lmax = 1.1 * lam, lmin = lmax * (cfg.cheb_lower > 0. ? cfg.cheb_lower : 1. / 60.);
+    const double theta = 0.5 * (lmax + lmin), delta = 0.5 * (lmax - lmin);
+    const double sigma = theta / delta;
+    // default degree 3; 4 for the FP32 cycle of a mixed-precision PCG, where the two extra operator
+    // applies cost less than the residual reduction they add.  BiCGSTAB (non-affine meshes) keeps 3
+    // in both precisions: there degree 4 did not pay at the cylinder, and at the step it made the
+    // two precisions stop after different iteration counts, moving the mixed fields away from the
+    // FP64 ones by more than the 1e-8 budget at a Krylov tolerance of 1e-12.
+    cheb_m = cfg.cheb_degree > 0 ? cfg.cheb_degree : (mixed && krylov == HOC_KRYLOV_PCG) ? 4 : 3;
+    cheb_theta = theta;
+    double rho = 1. / sigma;
+    for (int k = 1; k < cheb_m; ++k) {
+      const double rho_new = 1. / (2. * sigma - rho);
+      cheb_c1[k] = rho_new * rho;
+      cheb_c2[k] = 2. * rho_new / delta;
+      rho = rho_new;
     }
     for (size_t i = 0; i < D; ++i) hd[i] = 1. / hd[i];
     h2d(P64.dinv.p, hd.data(), D * sizeof(double));

@@ -55,7 +55,7 @@ typedef struct {
   int32_t fail_on_noconv;         /* nonzero: return HOC_ENOCONV when tol is not met     */
   double mixed_inner_tol;         /* unused (kept for layout); must lie in [0, 1)       */
   int32_t cheb_degree;            /* fine-level Chebyshev smoother degree, pre and post
-                                     (0 = 3, or 4 with HOC_MIXED; at most 8)              */
+                                     (0 = 3, or 4 with HOC_MIXED and PCG; at most 8)      */
   int32_t psi_guess;              /* initial guess of the stage Poisson solves: 1 = the
                                      previous solution (the reference's warm start),
                                      2 = linear extrapolation in the stage time from the
