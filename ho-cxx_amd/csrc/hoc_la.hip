@@ -556,53 +556,53 @@ __global__ __launch_bounds__(256) void k_proj_combine(size_t n, ProjSeg S, const
     out[i] = v;
   }
 }
-// Gram-Schmidt step appending a new solution psi (As psi = b - r), with the coefficients
-// e = G^-1 Y . b computed before its solve (per segment, at e + s estride):
-//   x = psi - sum_j e_j X_j,  ax = (b - r) - sum_j e_j AX_j,
-// and the per-block partials of the new Gram column: Y_j . ax (j < M), (mr ? ax : x) . ax, and
-// the squared norm of the solution itself, (mr ? b - r : psi) . (b - r).  The first block of
+// Gram-Schmidt step appending a new solution psi, with the coefficients e = G^-1 Y . b computed
+// before its solve (per segment, at e + s estride): x = psi - sum_j e_j X_j, and the per-block
+// partials of psi's squared norm (mr ? b . b : psi . b; the scale of the in-span test).  Its
+// image As x is then applied exactly and k_proj_gcol forms its Gram column.  The first block of
 // each segment writes psi's coefficients in the basis with x appended: wcol + s wstride =
 // (e_0 .. e_{M-1}, 1, 0 ..), ldw values.
 template <int M>
 __global__ __launch_bounds__(256) void k_proj_ortho(size_t n, ProjSeg S, int mr, const double* __restrict__ X,
-                                                    const double* __restrict__ AX, const double* __restrict__ e,
-                                                    int estride, const double* __restrict__ psi,
-                                                    const double* __restrict__ b, const double* __restrict__ r,
-                                                    double* __restrict__ x, double* __restrict__ ax,
-                                                    double* __restrict__ part, double* __restrict__ wcol, int wstride,
-                                                    int ldw) {
+                                                    const double* __restrict__ e, int estride,
+                                                    const double* __restrict__ psi, const double* __restrict__ b,
+                                                    double* __restrict__ x, double* __restrict__ part,
+                                                    double* __restrict__ wcol, int wstride, int ldw) {
   const SegRange R = seg_range(S);
-  double ej[M > 0 ? M : 1], s[M + 2];
+  double ej[M > 0 ? M : 1], s[1] = {0.};
 #pragma unroll
   for (int j = 0; j < M; ++j) ej[j] = e[(size_t)R.s * estride + j];
-#pragma unroll
-  for (int k = 0; k < M + 2; ++k) s[k] = 0.;
   if (R.lb == 0 && (int)threadIdx.x < ldw) {
     const int t = threadIdx.x;
     wcol[(size_t)R.s * wstride + t] = t < M ? e[(size_t)R.s * estride + t] : t == M ? 1. : 0.;
   }
   for (size_t i = R.i0; i < R.i1; i += R.step) {
-    const double p = psi[i], a0 = b[i] - r[i];
-    double xj[M > 0 ? M : 1], aj[M > 0 ? M : 1];
-    double xv = p, av = a0;
+    const double p = psi[i], bv = b[i];
+    double xv = p;
 #pragma unroll
-    for (int j = 0; j < M; ++j) {
-      xj[j] = X[(size_t)j * n + i];
-      aj[j] = AX[(size_t)j * n + i];
-    }
-#pragma unroll
-    for (int j = 0; j < M; ++j) {
-      xv -= ej[j] * xj[j];
-      av -= ej[j] * aj[j];
-    }
+    for (int j = 0; j < M; ++j) xv -= ej[j] * X[(size_t)j * n + i];
     x[i] = xv;
-    ax[i] = av;
-#pragma unroll
-    for (int j = 0; j < M; ++j) s[j] += (mr ? aj[j] : xj[j]) * av;
-    s[M] += (mr ? av : xv) * av;
-    s[M + 1] += (mr ? a0 : p) * a0;
+    s[0] += (mr ? bv : p) * bv;
   }
-  block_sums<M + 2>(s, part + (size_t)blockIdx.x * (M + 2));
+  block_sums<1>(s, part + blockIdx.x);
+}
+// Gram column of an appended vector x from its image ax = As x: Y_i . ax (i < M), then
+// (mr ? ax : x) . ax; per-block partials.
+template <int M>
+__global__ __launch_bounds__(256) void k_proj_gcol(size_t n, ProjSeg S, int mr, const double* __restrict__ Y,
+                                                   const double* __restrict__ x, const double* __restrict__ ax,
+                                                   double* __restrict__ part) {
+  const SegRange R = seg_range(S);
+  double s[M + 1];
+#pragma unroll
+  for (int k = 0; k < M + 1; ++k) s[k] = 0.;
+  for (size_t i = R.i0; i < R.i1; i += R.step) {
+    const double a = ax[i];
+#pragma unroll
+    for (int j = 0; j < M; ++j) s[j] += Y[(size_t)j * n + i] * a;
+    s[M] += (mr ? a : x[i]) * a;
+  }
+  block_sums<M + 1>(s, part + (size_t)blockIdx.x * (M + 1));
 }
 // Restart of a full basis of M vectors: slots k < kk become sum_j C[k M + j] X_j (C of the
 // block's segment at C + s cstride), and the same for AX.  In place: each dof's M values are
@@ -758,15 +758,24 @@ void proj_combine(hipStream_t s, size_t n, const ProjSeg& S, int m, const double
     hipLaunchKernelGGL(k_proj_combine<M>, dim3(S.nb), dim3(256), 0, s, n, S, X, c, cstride, out);
   });
 }
-void proj_ortho(hipStream_t s, size_t n, const ProjSeg& S, int m, int mr, const double* X, const double* AX,
-                const double* e, int estride, const double* psi, const double* b, const double* r, double* x,
-                double* ax, double* part, double* gcol, int gstride, double* wcol, int wstride, int ldw) {
+void proj_ortho(hipStream_t s, size_t n, const ProjSeg& S, int m, int mr, const double* X, const double* e,
+                int estride, const double* psi, const double* b, double* x, double* part, double* n0, int n0stride,
+                double* wcol, int wstride, int ldw) {
   if (m < 0 || m > kProjMax || n == 0 || ldw > 256) return;
   proj_dispatch<0>(m, [&](auto Mc) {
     constexpr int M = decltype(Mc)::value;
-    hipLaunchKernelGGL(k_proj_ortho<M>, dim3(S.nb), dim3(256), 0, s, n, S, mr, X, AX, e, estride, psi, b, r, x, ax,
-                       part, wcol, wstride, ldw);
-    hipLaunchKernelGGL(k_proj_fin, dim3(M + 2, S.nseg), dim3(256), 0, s, S, M + 2, part, gcol, gstride);
+    hipLaunchKernelGGL(k_proj_ortho<M>, dim3(S.nb), dim3(256), 0, s, n, S, mr, X, e, estride, psi, b, x, part, wcol,
+                       wstride, ldw);
+  });
+  hipLaunchKernelGGL(k_proj_fin, dim3(1, S.nseg), dim3(256), 0, s, S, 1, part, n0, n0stride);
+}
+void proj_gcol(hipStream_t s, size_t n, const ProjSeg& S, int m, int mr, const double* Y, const double* x,
+               const double* ax, double* part, double* gcol, int gstride) {
+  if (m < 0 || m > kProjMax || n == 0) return;
+  proj_dispatch<0>(m, [&](auto Mc) {
+    constexpr int M = decltype(Mc)::value;
+    hipLaunchKernelGGL(k_proj_gcol<M>, dim3(S.nb), dim3(256), 0, s, n, S, mr, Y, x, ax, part);
+    hipLaunchKernelGGL(k_proj_fin, dim3(M + 1, S.nseg), dim3(256), 0, s, S, M + 1, part, gcol, gstride);
   });
 }
 void proj_gram(hipStream_t s, size_t n, const ProjSeg& S, int m, const double* Y, const double* AX, double* part,

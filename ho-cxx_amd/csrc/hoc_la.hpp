@@ -81,11 +81,13 @@ void psi_predict(hipStream_t s, size_t n, double f, int extrap, double* psi, dou
 //   proj_dots2:   out[j] = Y_j . t and out[m + j] = Y_j . b (j < m); part: S.nb * 2m
 //   proj_solve:   v[0, m) and v[m, 2m) <- G^-1 v (G in columns of ldg values, see k_proj_solve)
 //   proj_combine: out += sum_j c[j] X_j
-//   proj_ortho:   appends a new solution psi with As psi = b - r (r its solve's final residual):
-//                 x = psi - sum_j e[j] X_j and ax = (b - r) - sum_j e[j] AX_j (e = G^-1 Y . b
-//                 from before the solve: Y . As psi = Y . b - Y . r), and its Gram column gcol
-//                 (m + 2 values: Y_j . ax, x's own entry, psi's squared norm); part: S.nb * (m + 2);
-//                 wcol[0, ldw): psi's coefficients in the basis with x appended
+//   proj_ortho:   appends a new solution psi: x = psi - sum_j e[j] X_j (e = G^-1 Y . b from
+//                 before the solve: Y . As psi = Y . b up to the solve's residual), psi's
+//                 squared norm (mr ? b . b : psi . b) into n0 (the scale of the in-span test),
+//                 and wcol[0, ldw): psi's coefficients in the basis with x appended; part: S.nb
+//   proj_gcol:    the Gram column of the appended x from its image ax = As x (applied by the
+//                 operator, not inferred from the solve's recursive residual): Y_j . ax (j < m)
+//                 and (mr ? ax : x) . ax; part: S.nb * (m + 1)
 //   proj_gram:    out[i m + j] = Y_i . AX_j (part: S.nb * m * m; out stride m * m)
 //   proj_reseed:  slots k < kk of a basis of m vectors become sum_j C[k m + j] X_j (and AX alike)
 constexpr int kProjMax = 16;
@@ -104,13 +106,15 @@ void proj_solve(hipStream_t s, const ProjSeg& S, int m, const double* G, int ldg
                 int vstride);
 void proj_combine(hipStream_t s, size_t n, const ProjSeg& S, int m, const double* X, const double* c, int cstride,
                   double* out);
-void proj_ortho(hipStream_t s, size_t n, const ProjSeg& S, int m, int mr, const double* X, const double* AX,
-                const double* e, int estride, const double* psi, const double* b, const double* r, double* x,
-                double* ax, double* part, double* gcol, int gstride, double* wcol, int wstride, int ldw);
+void proj_ortho(hipStream_t s, size_t n, const ProjSeg& S, int m, int mr, const double* X, const double* e,
+                int estride, const double* psi, const double* b, double* x, double* part, double* n0, int n0stride,
+                double* wcol, int wstride, int ldw);
 void proj_gram(hipStream_t s, size_t n, const ProjSeg& S, int m, const double* Y, const double* AX, double* part,
                double* out);
 void proj_reseed(hipStream_t s, size_t n, const ProjSeg& S, int m, int kk, const double* C, int cstride, double* X,
                  double* AX);
+void proj_gcol(hipStream_t s, size_t n, const ProjSeg& S, int m, int mr, const double* Y, const double* x,
+               const double* ax, double* part, double* gcol, int gstride);
 // out[s] = sum over segment s of (r winv)^2 (weighted squared norms per ensemble instance);
 // skipped when F (may be null) has its active flag clear.  part: S.nb values.
 void seg_norm2(hipStream_t s, const ProjSeg& S, const VecArgs& A, const double* r, const int* F, double* part,

@@ -983,12 +983,14 @@ struct Solver {
   // Either is never worse than x0 in its norm.  A new solution x is appended after its solve as
   // x - X e with e = G^-1 Y^T As x = G^-1 (Y^T b - Y^T r); the Y^T b part is formed before the
   // solve in the pass that forms c, and the Y^T r part (within the solve's tolerance) is left in
-  // place: the appending pass measures the new vector's Gram column exactly, so the basis is not
-  // kept orthogonal and needs no second pass or normalisation.  Each appended solution's
-  // coefficients in the basis are kept (W).  A full basis restarts from the span of its newest
-  // kProjKeep solutions: their coefficient columns, orthonormalised on the host in the G inner
-  // product, recombine the basis in one pass into G = I, so the start of every solve is
-  // projected on at least kProjKeep earlier solutions.
+  // place.  The appended vector's image is applied by the operator (one apply per solve): images
+  // inferred from the solve's recursive residual, b - r - AX e, carried the residual's drift, and
+  // the projection built on them degraded as the basis aged.  Its Gram column is then measured
+  // exactly, so the basis is not kept orthogonal and needs no second pass or normalisation.
+  // Each appended solution's coefficients in the basis are kept (W).  A full basis restarts from
+  // the span of its newest kProjKeep solutions: their coefficient columns, orthonormalised on the
+  // host in the inner product of a freshly measured G, recombine the basis in one pass into
+  // G = I, so the start of every solve is projected on at least kProjKeep earlier solutions.
   static constexpr int kProjVectors = 16;
   static constexpr int kProjKeep = 8;
   static constexpr int kProjLdg = kProjVectors + 1;  // Gram column: M entries and the solution's norm
@@ -1104,7 +1106,7 @@ struct Solver {
     HIPCHK(hipStreamSynchronize(stream));  // (ca and gi are locals)
     P.m = kk;
   }
-  void proj_after(const double* b, const double* r) {
+  void proj_after(const double* b) {
     constexpr int M = kProjVectors;
     ProjBasis& P = *proj_cur;
     proj_alloc();
@@ -1121,9 +1123,16 @@ struct Solver {
       proj_dots2(stream, D, pseg, j, proj_mr() ? P.AX.p : P.X.p, b, b, ppart, pcoef, kProjCoef);
       proj_solve(stream, pseg, j, P.G.p, kProjLdg, M * kProjLdg, pcoef, kProjCoef);
     }
-    proj_ortho(stream, D, pseg, j, proj_mr() ? 1 : 0, P.X.p, P.AX.p, pcoef.p + j, kProjCoef, psi, b, r,
-               P.X.p + (size_t)j * D, P.AX.p + (size_t)j * D, ppart, P.G.p + (size_t)j * kProjLdg, M * kProjLdg,
-               P.W.p + (size_t)j * M, M * M, M);
+    // x = psi - X e into slot j, its exact image As x, and its Gram column
+    double* xj = P.X.p + (size_t)j * D;
+    double* axj = P.AX.p + (size_t)j * D;
+    double* gcol = P.G.p + (size_t)j * kProjLdg;
+    proj_ortho(stream, D, pseg, j, proj_mr() ? 1 : 0, P.X.p, pcoef.p + j, kProjCoef, psi, b, xj, ppart, gcol + j + 1,
+               M * kProjLdg, P.W.p + (size_t)j * M, M * M, M);
+    HIPCHK(launch_trace(K, PRE_COPY, N, T, xj, nullptr, nullptr, nullptr, 0., nullptr, 0, nullptr, P64.tr.p, stream));
+    lap<double>(P64, EPI_STORE, xj, axj, nullptr, nullptr, nullptr, 0., 0., nullptr, nullptr);
+    proj_gcol(stream, D, pseg, j, proj_mr() ? 1 : 0, proj_mr() ? P.AX.p : P.X.p, xj, axj, ppart, gcol,
+              M * kProjLdg);
     P.m = j + 1;
     P.e_m = -1;
   }
@@ -1220,7 +1229,7 @@ struct Solver {
     if (hf[F_ZERO]) last_rel = 0.;
     total_iters += last_iters;
     ++n_solves;
-    if (proj && !hf[F_ZERO]) proj_after(bs, P64.r);
+    if (proj && !hf[F_ZERO]) proj_after(bs);
     guess_after_solve();
     if (!std::isfinite(last_rel)) throw StatusError(HOC_ENAN, "NaN/Inf in the Poisson solve");
     if (cfg.fail_on_noconv && last_rel > cfg.tol)
