@@ -240,7 +240,10 @@ __device__ __forceinline__ void second_order_core(const TabT<R>& T, const R (&u)
 //   PRE_COPY   u = x
 //   PRE_CHEB0  u = r * dinv * coef (coef = 1/theta); writes x = d = u   (Chebyshev step from zero)
 //   PRE_PROL   u = x + Pc xc (bilinear from the corner vertices); writes x = u
-//   PRE_PUPD   u = z + beta * p; writes p = u                     (PCG direction)
+//   PRE_PUPD   u = z + beta * p; writes p = u                     (PCG direction; beta =
+//              scal[scal_idx], or with scal_idx < 0 scal[einst[element]]: per ensemble instance)
+// einst, iact (may be null): a block inside one ensemble instance whose flag in iact is 0
+// returns at once.
 // RA: the type of a (R, or float for the FP64 direction update of the mixed-precision solve,
 // whose V-cycle leaves z in FP32).
 // ==========================================================================================
@@ -248,16 +251,21 @@ template <int K, int PRE, class R, class RA = R>
 __global__ __launch_bounds__(kWG) void k_trace(int N, TabT<R> T, R* __restrict__ x, const RA* __restrict__ a,
                                                 const R* __restrict__ b, R* __restrict__ d, R coef,
                                                 const double* __restrict__ scal, int scal_idx, const int* __restrict__ active,
-                                                R* __restrict__ tr, const int32_t* __restrict__ ev) {
+                                                R* __restrict__ tr, const int32_t* __restrict__ ev,
+                                                const int32_t* __restrict__ einst, const int* __restrict__ iact) {
   if (active && !*active) return;
   constexpr int K2 = K * K;
   __shared__ __align__(16) R lds[kWG * K2];  // one tile: two inputs are staged in turn
   const int e0 = blockIdx.x * kWG;
+  if (iact) {  // a block inside one stopped ensemble instance
+    const int i0 = einst[e0];
+    if (i0 == einst[min(e0 + kWG, N) - 1] && !iact[i0]) return;
+  }
   const int el = e0 + threadIdx.x;
   const bool live = el < N;
   R u[K2];
   R beta = 0;
-  if (PRE == PRE_PUPD) beta = (R)scal[scal_idx];
+  if (PRE == PRE_PUPD) beta = (R)(scal_idx >= 0 ? scal[scal_idx] : live ? scal[einst[el]] : 0.);
   R xc[4] = {0, 0, 0, 0};
   if (PRE == PRE_PROL && live)
 #pragma unroll
@@ -324,6 +332,10 @@ __global__ __launch_bounds__(kWG, (AFF && sizeof(R) == 4 && K <= 5) ? 4 : 1) voi
   constexpr int K2 = K * K;
   __shared__ __align__(16) R lds[kWG * K2];
   const int e0 = blockIdx.x * kWG;
+  if (G.iact) {  // a block inside one stopped ensemble instance
+    const int i0 = G.inst[e0];
+    if (i0 == G.inst[min(e0 + kWG, N) - 1] && !G.iact[i0]) return;
+  }
   const int el = e0 + threadIdx.x;
   const bool live = el < N;
   R u[K2], ga[K2], gb[K2], gc[K2];
@@ -1006,9 +1018,9 @@ static inline int nblk(int N) { return (N + kWG - 1) / kWG; }
 template <class R>
 static hipError_t launch_trace_t(int K, int pre, int N, const TabT<R>& T, R* x, const R* a, const R* b, R* d,
                                  double coef, const double* scal, int scal_idx, const int* active, R* tr, hipStream_t s,
-                                 const int32_t* ev) {
+                                 const int32_t* ev, const int32_t* einst, const int* iact) {
 #define LT(P) hipLaunchKernelGGL((k_trace<KK, P, R>), dim3(nblk(N)), dim3(kWG), 0, s, N, T, x, a, b, d, (R)coef, scal, \
-                                 scal_idx, active, tr, ev)
+                                 scal_idx, active, tr, ev, einst, iact)
   HOC_K_SWITCH(K, {
     switch (pre) {
       case PRE_COPY: LT(PRE_COPY); break;
@@ -1051,19 +1063,20 @@ static hipError_t launch_lap_t(int K, int epi, int N, const TabT<R>& T, const Ge
 
 hipError_t launch_trace(int K, int pre, int N, const TabT<double>& T, double* x, const double* a, const double* b,
                         double* d, double coef, const double* scal, int scal_idx, const int* active, double* tr,
-                        hipStream_t s, const int32_t* ev) {
-  return launch_trace_t<double>(K, pre, N, T, x, a, b, d, coef, scal, scal_idx, active, tr, s, ev);
+                        hipStream_t s, const int32_t* ev, const int32_t* einst, const int* iact) {
+  return launch_trace_t<double>(K, pre, N, T, x, a, b, d, coef, scal, scal_idx, active, tr, s, ev, einst, iact);
 }
 hipError_t launch_trace(int K, int pre, int N, const TabT<float>& T, float* x, const float* a, const float* b,
                         float* d, double coef, const double* scal, int scal_idx, const int* active, float* tr,
-                        hipStream_t s, const int32_t* ev) {
-  return launch_trace_t<float>(K, pre, N, T, x, a, b, d, coef, scal, scal_idx, active, tr, s, ev);
+                        hipStream_t s, const int32_t* ev, const int32_t* einst, const int* iact) {
+  return launch_trace_t<float>(K, pre, N, T, x, a, b, d, coef, scal, scal_idx, active, tr, s, ev, einst, iact);
 }
 hipError_t launch_trace_pupd_f32z(int K, int N, const TabT<double>& T, double* p, const float* z, const double* scal,
-                                  int scal_idx, const int* active, double* tr, hipStream_t s) {
+                                  int scal_idx, const int* active, double* tr, hipStream_t s, const int32_t* einst,
+                                  const int* iact) {
   HOC_K_SWITCH(K, hipLaunchKernelGGL((k_trace<KK, PRE_PUPD, double, float>), dim3(nblk(N)), dim3(kWG), 0, s, N, T, p,
                                      z, (const double*)nullptr, (double*)nullptr, 0., scal, scal_idx, active, tr,
-                                     (const int32_t*)nullptr));
+                                     (const int32_t*)nullptr, einst, iact));
   return hipGetLastError();
 }
 hipError_t launch_lap(int K, int epi, int N, const TabT<double>& T, const GeoT<double>& G, double* x, const double* tr,

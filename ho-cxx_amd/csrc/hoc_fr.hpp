@@ -32,16 +32,21 @@ struct VelBatch {
   size_t psi = 0, trpsi = 0, omega = 0, vel = 0, U = 0, trw = 0, tru = 0, vjump = 0, diff = 0, pvel = 0;
 };
 
-// Poisson-operator kernels in FP64 or (mixed-precision preconditioner) FP32.
+// Poisson-operator kernels in FP64 or (mixed-precision preconditioner) FP32.  einst, iact (may
+// be null): element -> ensemble instance and the instances' live flags; blocks of one stopped
+// instance return at once, and PRE_PUPD with scal_idx < 0 takes beta per instance, scal[einst[el]].
 hipError_t launch_trace(int K, int pre, int N, const TabT<double>& T, double* x, const double* a, const double* b,
                         double* d, double coef, const double* scal, int scal_idx, const int* active, double* tr,
-                        hipStream_t s, const int32_t* ev = nullptr);
+                        hipStream_t s, const int32_t* ev = nullptr, const int32_t* einst = nullptr,
+                        const int* iact = nullptr);
 hipError_t launch_trace(int K, int pre, int N, const TabT<float>& T, float* x, const float* a, const float* b,
                         float* d, double coef, const double* scal, int scal_idx, const int* active, float* tr,
-                        hipStream_t s, const int32_t* ev = nullptr);
+                        hipStream_t s, const int32_t* ev = nullptr, const int32_t* einst = nullptr,
+                        const int* iact = nullptr);
 // PRE_PUPD of the FP64 Krylov method with the FP32 V-cycle: p = z + beta p with z in FP32.
 hipError_t launch_trace_pupd_f32z(int K, int N, const TabT<double>& T, double* p, const float* z, const double* scal,
-                                  int scal_idx, const int* active, double* tr, hipStream_t s);
+                                  int scal_idx, const int* active, double* tr, hipStream_t s,
+                                  const int32_t* einst = nullptr, const int* iact = nullptr);
 // tr: traces of x (input); tr_out (EPI_CHEB/EPI_CHEBF only, may be null): traces of the
 // updated x, written for the next application.  tr and tr_out must differ.  ran (may be null):
 // set to 1 when the launch worked, 0 when the active flag made it return at once.

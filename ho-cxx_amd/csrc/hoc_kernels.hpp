@@ -58,6 +58,8 @@ struct GeoT {
   const int8_t* no_slip;                   // [E]
   const R* esign;                          // [E] 2*(ijp%2)-1 of the owner face
   const int32_t* inst;                     // [N] ensemble instance of each element, or null
+  const int* iact = nullptr;               // [instances] active flags: a Poisson-operator block whose
+                                           // elements all belong to one inactive instance returns at once
 };
 using Geo = GeoT<double>;
 

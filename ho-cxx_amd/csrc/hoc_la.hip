@@ -30,12 +30,19 @@ __device__ __forceinline__ void block_sum2(double& a, double& b) {
 // inverse weight 1/(w_j w_i) of that position is A.winv[m].
 struct DofSlot {
   int m, step, K2;
+  size_t el, estep;  // the element of the dof
   __device__ DofSlot(const VecArgs& A, size_t v0, size_t stride) {
     K2 = A.K * A.K;
     m = (int)(v0 % (size_t)K2);
     step = (int)(stride % (size_t)K2);
+    el = v0 / (size_t)K2;
+    estep = stride / (size_t)K2;
   }
-  __device__ void next() { m += step; if (m >= K2) m -= K2; }
+  __device__ void next() {
+    m += step;
+    el += estep;
+    if (m >= K2) { m -= K2; ++el; }
+  }
 };
 
 // Elementwise Krylov updates with fused partial norms.  Scalars come from S.
@@ -51,7 +58,8 @@ __global__ void k_vec(int op, VecArgs A, size_t n, R* __restrict__ a, R* __restr
   for (size_t v = v0; v < n; v += stride, ds.next()) {
     switch (op) {
       case VOP_PCG_UPD: {  // x += alpha p; r -= alpha q      a=x b=r c=p d=q
-        const R al = (R)S[S_ALPHA];
+        if (A.ialpha && A.ialpha[A.einst[ds.el]] == 0.) break;  // a stopped ensemble instance
+        const R al = (R)(A.ialpha ? A.ialpha[A.einst[ds.el]] : S[S_ALPHA]);
         a[v] += al * c[v];
         const R r = b[v] - al * d[v];
         b[v] = r;
@@ -811,6 +819,115 @@ void seg_norm2(hipStream_t s, const ProjSeg& S, const VecArgs& A, const double* 
                double* out) {
   hipLaunchKernelGGL(k_seg_norm2, dim3(S.nb), dim3(256), 0, s, S, A, r, F, part);
   hipLaunchKernelGGL(k_proj_fin, dim3(1, S.nseg), dim3(256), 0, s, S, 1, part, out, 1);
+}
+template <class RB>
+__global__ __launch_bounds__(256) void k_seg_dot(ProjSeg S, const double* __restrict__ a, const RB* __restrict__ b,
+                                                 const int* __restrict__ F, double* __restrict__ part) {
+  if (F && !F[F_ACTIVE]) return;
+  const SegRange R = seg_range(S);
+  double p[1] = {0.};
+  for (size_t v = R.i0; v < R.i1; v += R.step) p[0] += a[v] * (double)b[v];
+  block_sums<1>(p, part + blockIdx.x);
+}
+template <class RB>
+void seg_dot(hipStream_t s, const ProjSeg& S, const double* a, const RB* b, const int* F, double* part,
+             double* out) {
+  hipLaunchKernelGGL(k_seg_dot<RB>, dim3(S.nb), dim3(256), 0, s, S, a, b, F, part);
+  hipLaunchKernelGGL(k_proj_fin, dim3(1, S.nseg), dim3(256), 0, s, S, 1, part, out, 1);
+}
+template void seg_dot<double>(hipStream_t, const ProjSeg&, const double*, const double*, const int*, double*,
+                              double*);
+template void seg_dot<float>(hipStream_t, const ProjSeg&, const double*, const float*, const int*, double*, double*);
+
+// One block: thread t handles instances t, t + 256, ..; the block then reduces "any active", the
+// largest residual ratio of the active instances and the sum of the residual norms.
+__global__ __launch_bounds__(256) void k_finalize_ens(int phase, EnsKry E, double* __restrict__ S,
+                                                      int* __restrict__ F, int maxiter, int* __restrict__ mirror,
+                                                      int seq) {
+  const bool live = phase == ENS_INIT || F[F_ACTIVE];
+  __shared__ int sany[256];
+  __shared__ double smax[256], ssum[256];
+  int any = 0;
+  double rmax = 0., rsum = 0.;
+  if (live) {
+    const double tol2 = S[S_TOL2];
+    double bmax = 0.;
+    if (phase == ENS_INIT || phase == ENS_UPD)
+      for (int i = 0; i < E.n; ++i) bmax = fmax(bmax, E.bb[i]);
+    for (int i = threadIdx.x; i < E.n; i += blockDim.x) {
+      const double v = E.in[i];
+      int a = E.act[i];
+      switch (phase) {
+        case ENS_INIT: {
+          const double d = tol2 * (E.bb[i] > 0. ? E.bb[i] : bmax);
+          a = F[F_ACTIVE] && (d > 0. ? v > d : v > 0.);
+          E.rz[i] = 0.;
+          E.alpha[i] = 0.;
+          E.beta[i] = 0.;
+        } break;
+        case ENS_RZ: {
+          double be = 0.;
+          if (a) {
+            if (!(v > 0.) || !isfinite(v)) a = 0;
+            else { be = F[F_ITER] > 0 ? v / E.rz[i] : 0.; E.rz[i] = v; }
+          }
+          E.beta[i] = a ? be : 0.;
+        } break;
+        case ENS_PQ: {
+          double al = 0.;
+          if (a) {
+            al = E.rz[i] / v;
+            if (!(v > 0.) || !isfinite(al)) { a = 0; al = 0.; }
+          }
+          E.alpha[i] = al;
+        } break;
+        case ENS_UPD: {
+          const double d = tol2 * (E.bb[i] > 0. ? E.bb[i] : bmax);
+          const double ratio = d > 0. ? v / d : (v > 0. ? HUGE_VAL : 0.);
+          if (a) a = ratio > 1. && isfinite(v);
+          if (a) rmax = fmax(rmax, ratio);
+          rsum += v;
+          if (!a) E.alpha[i] = 0.;
+        } break;
+      }
+      E.act[i] = a;
+      any |= a;
+    }
+  }
+  sany[threadIdx.x] = any;
+  smax[threadIdx.x] = rmax;
+  ssum[threadIdx.x] = rsum;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      sany[threadIdx.x] |= sany[threadIdx.x + w];
+      smax[threadIdx.x] = fmax(smax[threadIdx.x], smax[threadIdx.x + w]);
+      ssum[threadIdx.x] += ssum[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x != 0) return;
+  if (live) {
+    if (phase == ENS_UPD) {
+      F[F_ITER] += 1;
+      S[S_RR] = ssum[0];
+      S[S_IMAX] = smax[0];
+      F[F_ACTIVE] = (sany[0] && F[F_ITER] < (maxiter >= 0 ? maxiter : F[F_MAXIT])) ? 1 : 0;
+    } else if (phase == ENS_INIT) {
+      F[F_ACTIVE] = F[F_ACTIVE] && sany[0];
+    } else {
+      F[F_ACTIVE] = sany[0] ? 1 : 0;
+    }
+  }
+  if (mirror) {
+    const int sq = seq >= 0 ? seq : ++F[F_SEQ];
+    const int act = F[F_ACTIVE] ? 1 : 0;
+    const int near = act && S[S_IMAX] <= 64.;
+    __hip_atomic_store(mirror, sq * 4 + near * 2 + act, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  }
+}
+void finalize_ens(hipStream_t s, int phase, const EnsKry& E, double* S, int* F, int maxiter, int* mirror, int seq) {
+  hipLaunchKernelGGL(k_finalize_ens, dim3(1), dim3(256), 0, s, phase, E, S, F, maxiter, mirror, seq);
 }
 void psi_predict(hipStream_t s, size_t n, double f, int extrap, double* psi, double* prev) {
   if (n == 0) return;

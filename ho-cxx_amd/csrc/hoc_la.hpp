@@ -25,6 +25,10 @@ enum { VOP_PCG_UPD = 0, VOP_BICG_P, VOP_BICG_S, VOP_DOT2, VOP_COPY, VOP_BICG_X }
 struct VecArgs {
   int K;
   const double* winv;
+  // ensembles with a step length per instance: VOP_PCG_UPD takes alpha = ialpha[einst[element]]
+  // and skips the dofs of an instance whose alpha is 0 (stopped)
+  const double* ialpha = nullptr;
+  const int32_t* einst = nullptr;
 };
 
 int vec_grid(size_t n);
@@ -115,6 +119,32 @@ void proj_reseed(hipStream_t s, size_t n, const ProjSeg& S, int m, int kk, const
                  double* AX);
 void proj_gcol(hipStream_t s, size_t n, const ProjSeg& S, int m, int mr, const double* Y, const double* x,
                const double* ax, double* part, double* gcol, int gstride);
+// PCG over an ensemble with a step length per instance: every instance has its own alpha, beta
+// and active flag, so each converges as if solved alone, to its own tolerance; the shared
+// kernels run while any instance is active, and an instance that has stopped takes alpha =
+// beta = 0, so its x and r stay as they are (its Poisson-operator blocks return at once).
+struct EnsKry {
+  double* in;        // the per-instance sums of the phase (r.z, p.q or the weighted ||r||^2)
+  double* rz;        // r.z of the last direction update
+  double* alpha;     // step lengths
+  double* beta;      // direction-update coefficients
+  const double* bb;  // weighted ||b_i||^2
+  int* act;          // per-instance active flags
+  int n;
+};
+enum { ENS_INIT = 0, ENS_RZ, ENS_PQ, ENS_UPD };
+// ENS_INIT after the joint PH_INIT finalize (E.in = ||r_i||^2): each instance active when the
+// solve is and its residual is above tol^2 bb_i.  ENS_RZ: beta_i (a breakdown stops the
+// instance); ENS_PQ: alpha_i; ENS_UPD: the instances' tests, the iteration count, F_ACTIVE =
+// any instance active within maxiter, S[S_RR] = sum ||r_i||^2, S[S_IMAX], and the status word as
+// finalize publishes it.  An instance with bb_i = 0 is held to the largest bb.
+void finalize_ens(hipStream_t s, int phase, const EnsKry& E, double* S, int* F, int maxiter, int* mirror = nullptr,
+                  int seq = 0);
+// out[s] = sum over segment s of a . b (a FP64, b FP64 or FP32); skipped when F's active flag is
+// clear (F may be null).  part: S.nb values.
+template <class RB>
+void seg_dot(hipStream_t s, const ProjSeg& S, const double* a, const RB* b, const int* F, double* part,
+             double* out);
 // out[s] = sum over segment s of (r winv)^2 (weighted squared norms per ensemble instance);
 // skipped when F (may be null) has its active flag clear.  part: S.nb values.
 void seg_norm2(hipStream_t s, const ProjSeg& S, const VecArgs& A, const double* r, const int* F, double* part,

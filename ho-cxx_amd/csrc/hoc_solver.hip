@@ -77,6 +77,10 @@ struct Prec {
   DBuf<R> coarse_inv;
   int coarse_n = 0;
   int prof_off = 0;  // offset of this precision's entries in the live kernel profile
+  // set only while the iterations of a batched ensemble PCG are queued: element -> instance and
+  // the instances' live flags; fine-level blocks of a stopped instance then return at once
+  const int32_t* einst = nullptr;
+  const int* iact = nullptr;
   void alloc_vectors(size_t D, int N, int K) {
     for (auto* f : {&r, &z, &p, &q, &rh, &v, &ph, &sh, &t, &x, &dch, &dch2, &dinv}) f->alloc(D);
     tr.alloc((size_t)N * 8 * K); tr2.alloc((size_t)N * 8 * K);
@@ -358,7 +362,9 @@ struct Solver {
            const int* active, const R* tin = nullptr, R* tout = nullptr) {
     const bool rec = prof.on && (prof.seen++ % prof.every) == 0;
     if (rec) HIPCHK(hipEventRecord(prof.e0[prof.n], stream));
-    HIPCHK(launch_lap(K, epi, N, P.T, P.G, x, tin ? tin : P.tr.p, y, rr, dg, d, c1, c2, partp, active, stream, tout,
+    GeoT<R> G = P.G;
+    if (P.iact) { G.inst = P.einst; G.iact = P.iact; }
+    HIPCHK(launch_lap(K, epi, N, P.T, G, x, tin ? tin : P.tr.p, y, rr, dg, d, c1, c2, partp, active, stream, tout,
                       rec ? prof.ran.p + prof.n : nullptr));
     if (rec) {
       HIPCHK(hipEventRecord(prof.e1[prof.n], stream));
@@ -454,6 +460,13 @@ struct Solver {
         for (int a = 0; a < 4; ++a) vinc[pos[ev[(size_t)e * 4 + a]]++] = e * 4 + a;
     }
     d_ev.upload(ev); d_vptr.upload(vptr); d_vinc.upload(vinc);
+    inst_coupled = false;  // an ensemble whose instances share a corner vertex
+    if (n_inst > 1 && (int)inst_off.size() == n_inst + 1) {
+      auto inst_of = [&](int e) { return std::upper_bound(inst_off.begin(), inst_off.end(), e) - inst_off.begin() - 1; };
+      for (int v = 0; v < nv && !inst_coupled; ++v)
+        for (int k = vptr[v] + 1; k < vptr[v + 1]; ++k)
+          if (inst_of(vinc[k] / 4) != inst_of(vinc[vptr[v]] / 4)) { inst_coupled = true; break; }
+    }
     std::vector<std::vector<std::pair<int, double>>> rows(nv);
     auto add = [&](int r, int c, double val) {
       for (auto& pr : rows[r]) if (pr.first == c) { pr.second += val; return; }
@@ -578,7 +591,8 @@ struct Solver {
     // directions ping-pong between dch and dch2; the last step of a sweep stores none
     R* din = P.dch.p;
     R* dout = P.dch2.p;
-    HIPCHK(launch_trace(K, PRE_CHEB0, N, P.T, zout, rr, P.dinv.p, din, 1. / cheb_theta, nullptr, 0, Fp, cur, stream));
+    HIPCHK(launch_trace(K, PRE_CHEB0, N, P.T, zout, rr, P.dinv.p, din, 1. / cheb_theta, nullptr, 0, Fp, cur, stream,
+                        nullptr, P.einst, P.iact));
     for (int k = 1; k < cheb_m; ++k) {
       lap(P, EPI_CHEB, zout, k + 1 < cheb_m ? dout : (R*)nullptr, rr, P.dinv.p, din, cheb_c1[k], cheb_c2[k], nullptr,
           Fp, cur, nxt);
@@ -592,7 +606,7 @@ struct Solver {
     // prolongate and post-smooth: Chebyshev(m) from the corrected iterate
     cur = P.tr.p; nxt = P.tr2.p;
     HIPCHK(launch_trace(K, PRE_PROL, N, P.T, zout, P.xc.p, (const R*)nullptr, (R*)nullptr, 0., nullptr, 0, Fp, cur,
-                        stream, d_ev.p));
+                        stream, d_ev.p, P.einst, P.iact));
     din = P.dch.p;
     dout = P.dch2.p;
     lap(P, EPI_CHEBF, zout, cheb_m > 1 ? din : (R*)nullptr, rr, P.dinv.p, (R*)nullptr, 0., 1. / cheb_theta,
@@ -700,6 +714,12 @@ struct Solver {
   void krylov_iter(Prec<R>& P, R* x, R* r, double* Sp, int* Fp, int maxit, int seq) {
     const int lb = lap_blocks(N);
     const int vg = vec_grid(D);
+    if constexpr (std::is_same<R, double>::value) {
+      if (ens_batched()) {
+        ens_iter(x, r, Sp, Fp, maxit, seq);
+        return;
+      }
+    }
     if (krylov == HOC_KRYLOV_PCG) {
       bool zf = false;  // z left in FP32 by the mixed-precision V-cycle
       if constexpr (std::is_same<R, double>::value) {
@@ -749,6 +769,50 @@ struct Solver {
       vec<R>(stream, VOP_BICG_X, VA, D, x, r, P.ph, P.sh, P.t, Sp, Fp, part);
       finalize(stream, PH_RNORM, part, vg, Sp, Fp, maxit, mirror_dev, seq);
     }
+  }
+  // One iteration of the batched ensemble PCG (ens_batched): PCG with a step length and a
+  // direction coefficient per instance (r_i.z_i, p_i.q_i and ||r_i|| by segment), each instance
+  // stopping on its own test; while an instance is stopped, the fine-level kernels skip the
+  // blocks that hold only its elements, and its x and r stay as they are.  The coarse levels run
+  // over every instance: with no Q1 vertex shared between instances they stay block-diagonal.
+  void ens_iter(double* x, double* r, double* Sp, int* Fp, int maxit, int seq) {
+    struct Mask {  // the masks hold for this iteration's launches only
+      Solver& s;
+      explicit Mask(Solver& so) : s(so) {
+        s.P64.einst = s.P32.einst = s.d_inst.p;
+        s.P64.iact = s.P32.iact = s.ens_act.p;
+      }
+      ~Mask() {
+        s.P64.einst = s.P32.einst = nullptr;
+        s.P64.iact = s.P32.iact = nullptr;
+      }
+    } mask(*this);
+    Prec<double>& P = P64;
+    const EnsKry E = ensk();
+    VecArgs va = VA;  // x += alpha_i p, r -= alpha_i q
+    va.ialpha = ens_alpha.p;
+    va.einst = d_inst.p;
+    if (mixed) {
+      demote(stream, D, r, P32.r.p, Fp);
+      vcycle(P32, P32.r.p, P32.z.p, nullptr, Fp);
+      seg_dot<float>(stream, pseg, r, P32.z.p, Fp, ens_part.p, ens_in.p);
+      finalize_ens(stream, ENS_RZ, E, Sp, Fp, maxit);
+      HIPCHK(launch_trace_pupd_f32z(K, N, P.T, P.p.p, P32.z.p, ens_beta.p, -1, Fp, P.tr.p, stream, d_inst.p,
+                                    ens_act.p));
+    } else {
+      vcycle(P, r, P.z.p, nullptr, Fp);
+      seg_dot<double>(stream, pseg, r, P.z.p, Fp, ens_part.p, ens_in.p);
+      finalize_ens(stream, ENS_RZ, E, Sp, Fp, maxit);
+      HIPCHK(launch_trace(K, PRE_PUPD, N, P.T, P.p.p, P.z.p, nullptr, nullptr, 0., ens_beta.p, -1, Fp, P.tr.p, stream,
+                          nullptr, d_inst.p, ens_act.p));
+    }
+    lap(P, EPI_STORE, P.p.p, P.q.p, (const double*)nullptr, (const double*)nullptr, (double*)nullptr, 0., 0., nullptr,
+        Fp);
+    seg_dot<double>(stream, pseg, P.p.p, P.q.p, Fp, ens_part.p, ens_in.p);
+    finalize_ens(stream, ENS_PQ, E, Sp, Fp, maxit);
+    vec<double>(stream, VOP_PCG_UPD, va, D, x, r, P.p, P.q, nullptr, Sp, Fp, nullptr);
+    seg_norm2(stream, pseg, VA, r, Fp, ens_part.p, ens_in.p);
+    finalize_ens(stream, ENS_UPD, E, Sp, Fp, maxit, mirror_dev, seq);
   }
   // The instantiated graph of kGraphChunk iterations on these buffers (captured on first use).
   template <class R>
@@ -855,6 +919,11 @@ struct Solver {
     if (ens_pcg()) {
       seg_norm2(stream, pseg, VA, P64.r.p, nullptr, ens_part.p, ens_rr.p);
       finalize(stream, PH_INIT, part, lb, S, F, cfg.maxiter, nullptr, 0, ens_rr.p, ens_bb.p, pseg.nseg);
+      if (ens_batched()) {
+        EnsKry E = ensk();
+        E.in = ens_rr.p;
+        finalize_ens(stream, ENS_INIT, E, S, F, cfg.maxiter);
+      }
     } else {
       finalize(stream, PH_INIT, part, lb, S, F, cfg.maxiter);
     }
@@ -1022,6 +1091,15 @@ struct Solver {
   // of an ensemble's convergence test.
   DBuf<double> ens_part, ens_rr, ens_bb;
   bool ens_pcg() const { return n_inst > 1 && krylov == HOC_KRYLOV_PCG; }
+  // The batched ensemble PCG (ens_iter) when no Q1 vertex is shared between instances (else the
+  // joint PCG with the per-instance stopping test).
+  bool inst_coupled = false;
+  bool ens_batched() const { return ens_pcg() && !inst_coupled; }
+  DBuf<double> ens_in, ens_rz, ens_alpha, ens_beta;
+  DBuf<int> ens_act;
+  EnsKry ensk() {
+    return EnsKry{ens_in.p, ens_rz.p, ens_alpha.p, ens_beta.p, ens_bb.p, ens_act.p, pseg.nseg};
+  }
   void ensure_segments() {
     std::vector<int64_t> sb;
     if (n_inst > 1 && (int)inst_off.size() == n_inst + 1)
@@ -1038,6 +1116,10 @@ struct Solver {
     pseg_key = sb;
     ptmp.alloc(0);
     ens_part.alloc(pseg.nb); ens_rr.alloc(pseg.nseg); ens_bb.alloc(pseg.nseg);
+    for (auto* b : {&ens_in, &ens_rz, &ens_alpha, &ens_beta}) b->alloc(pseg.nseg);
+    ens_act.alloc(pseg.nseg);
+    HIPCHK(hipMemsetAsync(ens_alpha.p, 0, pseg.nseg * sizeof(double), stream));
+    HIPCHK(hipMemsetAsync(ens_act.p, 0, pseg.nseg * sizeof(int), stream));
   }
   void proj_alloc() {
     ensure_segments();

@@ -124,9 +124,11 @@ int hoc_load_gmsh(hoc_t h, const char* path, int32_t occ);
  * loaded mesh is the union of n_inst independent meshes, instance i owning the elements
  * [elem_offsets[i], elem_offsets[i+1]) (instances may not share faces).  Every instance
  * advances with its own Re and dt (and its own clock for time-dependent boundary values)
- * in the same kernel launches; each Poisson solve stops when every instance has
- * ||b_i - A psi_i|| <= tol * ||b_i|| (PCG; with BiCGSTAB, on the joint bound
- * ||b - A psi|| <= tol * min_i ||b_i||, which implies it).  Call after hoc_load_mesh /
+ * in the same kernel launches.  With PCG every instance iterates with its own step lengths and
+ * stops on its own ||b_i - A psi_i|| <= tol * ||b_i||, as if solved alone (instances that touch
+ * at a corner share one iteration and stop when every instance meets that test); with
+ * BiCGSTAB the instances share one iteration and stop on the joint bound
+ * ||b - A psi|| <= tol * min_i ||b_i||, which implies it.  Call after hoc_load_mesh /
  * hoc_load_gmsh and before hoc_setup; hoc_solveto is rejected for an ensemble (it sets one dt
  * and Re). */
 int hoc_set_instances(hoc_t h, int32_t n_inst, const int32_t* elem_offsets, const double* Re, const double* dt);
