@@ -125,7 +125,7 @@ int hoc_load_gmsh(hoc_t h, const char* path, int32_t occ);
  * [elem_offsets[i], elem_offsets[i+1]) (instances may not share faces).  Every instance
  * advances with its own Re and dt (and its own clock for time-dependent boundary values)
  * in the same kernel launches.  With PCG every instance iterates with its own step lengths and
- * stops on its own ||b_i - A psi_i|| <= tol * ||b_i||, as if solved alone (instances that touch
+ * stops on its own ||b_i - A psi_i|| <= tol * ||b_i|| (instances that touch
  * at a corner share one iteration and stop when every instance meets that test); with
  * BiCGSTAB the instances share one iteration and stop on the joint bound
  * ||b - A psi|| <= tol * min_i ||b_i||, which implies it.  Call after hoc_load_mesh /
