@@ -552,17 +552,55 @@ __global__ __launch_bounds__(256) void k_proj_fin(ProjSeg S, int nv, const doubl
 template <int M>
 __global__ __launch_bounds__(256) void k_proj_combine(size_t n, ProjSeg S, const double* __restrict__ X,
                                                       const double* __restrict__ c, int cstride,
-                                                      double* __restrict__ out) {
+                                                      double* __restrict__ out, double* __restrict__ v) {
   const SegRange R = seg_range(S);
   double cj[M];
 #pragma unroll
   for (int j = 0; j < M; ++j) cj[j] = c[(size_t)R.s * cstride + j];
-  for (size_t i = R.i0; i < R.i1; i += R.step) {
-    double v = out[i];
+  if (v) {  // uniform: also v = -sum_j e_j X_j with e = c + M, from the same loads of X
+    double ej[M];
 #pragma unroll
-    for (int j = 0; j < M; ++j) v += cj[j] * X[(size_t)j * n + i];
-    out[i] = v;
+    for (int j = 0; j < M; ++j) ej[j] = c[(size_t)R.s * cstride + M + j];
+    for (size_t i = R.i0; i < R.i1; i += R.step) {
+      double o = out[i], w = 0.;
+#pragma unroll
+      for (int j = 0; j < M; ++j) {
+        const double xj = X[(size_t)j * n + i];
+        o += cj[j] * xj;
+        w -= ej[j] * xj;
+      }
+      out[i] = o;
+      v[i] = w;
+    }
+    return;
   }
+  for (size_t i = R.i0; i < R.i1; i += R.step) {
+    double o = out[i];
+#pragma unroll
+    for (int j = 0; j < M; ++j) o += cj[j] * X[(size_t)j * n + i];
+    out[i] = o;
+  }
+}
+// The append of proj_ortho when k_proj_combine has left v = -sum_j e_j X_j before the solve (e
+// for a basis of m vectors): x = psi + v (x holds v on entry), with n0's partials and wcol as
+// k_proj_ortho writes them; reads 3 vectors instead of m + 2.
+__global__ __launch_bounds__(256) void k_proj_append(size_t n, ProjSeg S, int m, int mr,
+                                                     const double* __restrict__ e, int estride,
+                                                     const double* __restrict__ psi, const double* __restrict__ b,
+                                                     double* __restrict__ x, double* __restrict__ part,
+                                                     double* __restrict__ wcol, int wstride, int ldw) {
+  const SegRange R = seg_range(S);
+  double s[1] = {0.};
+  if (R.lb == 0 && (int)threadIdx.x < ldw) {
+    const int t = threadIdx.x;
+    wcol[(size_t)R.s * wstride + t] = t < m ? e[(size_t)R.s * estride + t] : t == m ? 1. : 0.;
+  }
+  for (size_t i = R.i0; i < R.i1; i += R.step) {
+    const double p = psi[i], bv = b[i];
+    x[i] = p + x[i];
+    s[0] += (mr ? bv : p) * bv;
+  }
+  block_sums<1>(s, part + blockIdx.x);
 }
 // Gram-Schmidt step appending a new solution psi, with the coefficients e = G^-1 Y . b computed
 // before its solve (per segment, at e + s estride): x = psi - sum_j e_j X_j, and the per-block
@@ -759,12 +797,20 @@ void proj_solve(hipStream_t s, const ProjSeg& S, int m, const double* G, int ldg
   hipLaunchKernelGGL(k_proj_solve, dim3(S.nseg), dim3(64), 0, s, m, G, ldg, gstride, v, vstride);
 }
 void proj_combine(hipStream_t s, size_t n, const ProjSeg& S, int m, const double* X, const double* c, int cstride,
-                  double* out) {
-  if (m <= 0 || m > kProjMax || n == 0) return;
+                  double* out, double* v) {
+  if (m <= 0 || m > kProjMax || n == 0 || (v && 2 * m > cstride)) return;
   proj_dispatch<1>(m, [&](auto Mc) {
     constexpr int M = decltype(Mc)::value;
-    hipLaunchKernelGGL(k_proj_combine<M>, dim3(S.nb), dim3(256), 0, s, n, S, X, c, cstride, out);
+    hipLaunchKernelGGL(k_proj_combine<M>, dim3(S.nb), dim3(256), 0, s, n, S, X, c, cstride, out, v);
   });
+}
+void proj_append(hipStream_t s, size_t n, const ProjSeg& S, int m, int mr, const double* e, int estride,
+                 const double* psi, const double* b, double* x, double* part, double* n0, int n0stride,
+                 double* wcol, int wstride, int ldw) {
+  if (m < 0 || m > kProjMax || n == 0 || ldw > 256) return;
+  hipLaunchKernelGGL(k_proj_append, dim3(S.nb), dim3(256), 0, s, n, S, m, mr, e, estride, psi, b, x, part, wcol,
+                     wstride, ldw);
+  hipLaunchKernelGGL(k_proj_fin, dim3(1, S.nseg), dim3(256), 0, s, S, 1, part, n0, n0stride);
 }
 void proj_ortho(hipStream_t s, size_t n, const ProjSeg& S, int m, int mr, const double* X, const double* e,
                 int estride, const double* psi, const double* b, double* x, double* part, double* n0, int n0stride,

@@ -84,7 +84,10 @@ void psi_predict(hipStream_t s, size_t n, double f, int extrap, double* psi, dou
 // per segment; reductions are deterministic (per-block partials summed in block order).
 //   proj_dots2:   out[j] = Y_j . t and out[m + j] = Y_j . b (j < m); part: S.nb * 2m
 //   proj_solve:   v[0, m) and v[m, 2m) <- G^-1 v (G in columns of ldg values, see k_proj_solve)
-//   proj_combine: out += sum_j c[j] X_j
+//   proj_combine: out += sum_j c[j] X_j; with v set, also v = -sum_j c[m + j] X_j (the e of a
+//                 later append, from the same pass over X)
+//   proj_append:  proj_ortho's append when proj_combine has left v = -X e in x: x = psi + x,
+//                 with n0 and wcol as proj_ortho writes them
 //   proj_ortho:   appends a new solution psi: x = psi - sum_j e[j] X_j (e = G^-1 Y . b from
 //                 before the solve: Y . As psi = Y . b up to the solve's residual), psi's
 //                 squared norm (mr ? b . b : psi . b) into n0 (the scale of the in-span test),
@@ -109,7 +112,10 @@ void proj_dots2(hipStream_t s, size_t n, const ProjSeg& S, int m, const double* 
 void proj_solve(hipStream_t s, const ProjSeg& S, int m, const double* G, int ldg, int gstride, double* v,
                 int vstride);
 void proj_combine(hipStream_t s, size_t n, const ProjSeg& S, int m, const double* X, const double* c, int cstride,
-                  double* out);
+                  double* out, double* v = nullptr);
+void proj_append(hipStream_t s, size_t n, const ProjSeg& S, int m, int mr, const double* e, int estride,
+                 const double* psi, const double* b, double* x, double* part, double* n0, int n0stride,
+                 double* wcol, int wstride, int ldw);
 void proj_ortho(hipStream_t s, size_t n, const ProjSeg& S, int m, int mr, const double* X, const double* e,
                 int estride, const double* psi, const double* b, double* x, double* part, double* n0, int n0stride,
                 double* wcol, int wstride, int ldw);

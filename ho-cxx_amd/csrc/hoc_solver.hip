@@ -1072,6 +1072,7 @@ struct Solver {
     DBuf<double> G, W, C;
     int m = 0;       // vectors held
     int e_m = -1;    // basis size the coefficients e of the solve in progress were formed for
+    int v_slot = -1; // the slot holding -X e for the append after the solve in progress (or -1)
     int nseg = 0;    // segments the per-segment arrays were sized for
     int restarts = 0;
   };
@@ -1132,6 +1133,7 @@ struct Solver {
   void proj_before(const double* b) {
     ProjBasis& P = *proj_cur;
     P.e_m = -1;
+    P.v_slot = -1;
     if (P.m == 0) return;
     proj_alloc();
     if (P.nseg != pseg.nseg) { P.m = 0; return; }
@@ -1140,8 +1142,11 @@ struct Solver {
     lap<double>(P64, EPI_RESID, psi.p, t, b, nullptr, nullptr, 0., 0., nullptr, nullptr);  // t = b - As x0
     proj_dots2(stream, D, pseg, P.m, proj_mr() ? P.AX.p : P.X.p, t, b, ppart, pcoef, kProjCoef);  // Y^T t, Y^T b
     proj_solve(stream, pseg, P.m, P.G.p, kProjLdg, kProjVectors * kProjLdg, pcoef, kProjCoef);    // c, e
-    proj_combine(stream, D, pseg, P.m, P.X.p, pcoef, kProjCoef, psi);
+    // with a free slot, the same pass leaves -X e there for the append after this solve
+    const bool pre = P.m < kProjVectors;
+    proj_combine(stream, D, pseg, P.m, P.X.p, pcoef, kProjCoef, psi, pre ? P.X.p + (size_t)P.m * D : nullptr);
     P.e_m = P.m;
+    P.v_slot = pre ? P.m : -1;
   }
   // A full basis: keep, per segment, the span of the newest kProjKeep appended solutions
   // (columns M - kProjKeep .. M - 1 of W: with kProjKeep <= M / 2 these were all appended after
@@ -1153,6 +1158,7 @@ struct Solver {
     const int ns = pseg.nseg;
     P.m = 0;
     P.e_m = -1;
+    P.v_slot = -1;
     ++P.restarts;
     if (KP == 0) return;
     std::vector<double> w((size_t)ns * M * M), gc((size_t)ns * M * LG), gm((size_t)ns * M * M);
@@ -1209,14 +1215,19 @@ struct Solver {
     double* xj = P.X.p + (size_t)j * D;
     double* axj = P.AX.p + (size_t)j * D;
     double* gcol = P.G.p + (size_t)j * kProjLdg;
-    proj_ortho(stream, D, pseg, j, proj_mr() ? 1 : 0, P.X.p, pcoef.p + j, kProjCoef, psi, b, xj, ppart, gcol + j + 1,
-               M * kProjLdg, P.W.p + (size_t)j * M, M * M, M);
+    if (P.v_slot == j && P.e_m == j)  // slot j holds -X e from proj_before's pass
+      proj_append(stream, D, pseg, j, proj_mr() ? 1 : 0, pcoef.p + j, kProjCoef, psi, b, xj, ppart, gcol + j + 1,
+                  M * kProjLdg, P.W.p + (size_t)j * M, M * M, M);
+    else
+      proj_ortho(stream, D, pseg, j, proj_mr() ? 1 : 0, P.X.p, pcoef.p + j, kProjCoef, psi, b, xj, ppart,
+                 gcol + j + 1, M * kProjLdg, P.W.p + (size_t)j * M, M * M, M);
     HIPCHK(launch_trace(K, PRE_COPY, N, T, xj, nullptr, nullptr, nullptr, 0., nullptr, 0, nullptr, P64.tr.p, stream));
     lap<double>(P64, EPI_STORE, xj, axj, nullptr, nullptr, nullptr, 0., 0., nullptr, nullptr);
     proj_gcol(stream, D, pseg, j, proj_mr() ? 1 : 0, proj_mr() ? P.AX.p : P.X.p, xj, axj, ppart, gcol,
               M * kProjLdg);
     P.m = j + 1;
     P.e_m = -1;
+    P.v_slot = -1;
   }
   // The adjoint's colouring, velocity map, buffers and recorded trajectory belong to one mesh and
   // one set-up: a new mesh or set-up rebuilds them on the next adjoint call.

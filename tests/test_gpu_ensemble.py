@@ -43,6 +43,41 @@ def test_uneven_cavity_ensemble_vs_oracle(gpu_available):
         assert ew < 1e-10 and ep < 1e-10
 
 
+def test_corner_coupled_ensemble_vs_oracle(gpu_available):
+    """Two cavities that share one corner vertex (the second translated by (1, 1), its corner
+    node merged with the first's): the Q1 coarse space couples them, so the instances run one
+    joint PCG with the per-instance stopping test instead of the batched per-instance PCG.
+    Each instance against its own FP64 oracle run."""
+    a = meshes.cavity(6, 2)
+    b0 = meshes.cavity(8, 2)
+    b = meshes.QuadMesh(b0.lnod, b0.xy + 1.0, b0.elem, b0.bedges, b0.bedge_bdry, list(b0.bnames))
+    m, off = meshes.concat([a, b])
+    ia = int(np.flatnonzero(np.all(np.abs(a.xy - 1.0) < 1e-14, axis=1))[0])
+    ib = int(np.flatnonzero(np.all(np.abs(b0.xy) < 1e-14, axis=1))[0]) + a.n_nodes
+    m.elem[m.elem == ib] = ia
+    m.bedges[m.bedges == ib] = ia
+    assert np.array_equal(m.xy[ia], m.xy[ib])
+    K = 4
+    Re = np.array([100.0, 400.0])
+    dt = np.array([2e-4, 1e-4])
+    steps = 3
+    g = hoc.HO2D(K=K, problem_type=1, Re=Re[0], dt=dt[0], time_integration_type=4, advection_type=1,
+                 tol=1e-13, maxiter=800)
+    g.load_mesh(m)
+    g.set_instances(off, Re, dt)
+    g.setup()
+    g.step(steps)
+    w, p = g.get_vorticity(), g.get_streamfunction()
+    for i, part in enumerate([a, b]):
+        o = Oracle(part, K, problem_type=1, Re=Re[i], dt=dt[i], time_integration_type=4, advection_type=1,
+                   tol=1e-13, maxiter=800)
+        o.step(steps)
+        sl = slice(off[i] * K * K, off[i + 1] * K * K)
+        ew, ep = rel_l2(w[sl], o.vorticity()), rel_l2(p[sl], o.streamfunction())
+        print("corner-coupled instance", i, "omega", ew, "psi", ep)
+        assert ew < 1e-10 and ep < 1e-10
+
+
 def test_ensemble_with_ramped_channels(gpu_available):
     """Backward-facing steps with time-dependent inflow: each instance ramps on its own clock,
     from t = 20/3 where the ramp 2|sin(pi t/40)| (src/HO_2D.cpp:2014) is 1."""
