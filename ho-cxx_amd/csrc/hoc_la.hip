@@ -535,6 +535,29 @@ __global__ __launch_bounds__(256) void k_proj_dots2(size_t n, ProjSeg S, const d
   }
   block_sums<2 * M>(s, part + (size_t)blockIdx.x * 2 * M);
 }
+// k_proj_dots2 fused with the Gram column of the newest basis vector: Y_j . t, Y_j . b and
+// Y_j . ax (j < M; ax the image of the newest vector, whose Gram column k_proj_gcol would give
+// from the same loads of Y); partials at part + b 3M.
+template <int M>
+__global__ __launch_bounds__(256) void k_proj_dots3(size_t n, ProjSeg S, const double* __restrict__ Y,
+                                                    const double* __restrict__ ax, const double* __restrict__ t,
+                                                    const double* __restrict__ b, double* __restrict__ part) {
+  double s[3 * M];
+#pragma unroll
+  for (int k = 0; k < 3 * M; ++k) s[k] = 0.;
+  const SegRange R = seg_range(S);
+  for (size_t i = R.i0; i < R.i1; i += R.step) {
+    const double tv = t[i], bv = b[i], av = ax[i];
+#pragma unroll
+    for (int j = 0; j < M; ++j) {
+      const double y = Y[(size_t)j * n + i];
+      s[j] += y * tv;
+      s[M + j] += y * bv;
+      s[2 * M + j] += y * av;
+    }
+  }
+  block_sums<3 * M>(s, part + (size_t)blockIdx.x * 3 * M);
+}
 // out[s * ostride + k] = sum of part[b * nv + k] over the blocks b of segment s (grid: nv x nseg),
 // strided partial sums, then a fixed-order tree
 __global__ __launch_bounds__(256) void k_proj_fin(ProjSeg S, int nv, const double* __restrict__ part,
@@ -790,6 +813,16 @@ void proj_dots2(hipStream_t s, size_t n, const ProjSeg& S, int m, const double* 
     hipLaunchKernelGGL(k_proj_dots2<M>, dim3(S.nb), dim3(256), 0, s, n, S, Y, t, b, part);
   });
   hipLaunchKernelGGL(k_proj_fin, dim3(2 * m, S.nseg), dim3(256), 0, s, S, 2 * m, part, out, ostride);
+}
+void proj_dots3(hipStream_t s, size_t n, const ProjSeg& S, int m, const double* Y, const double* ax, const double* t,
+                const double* b, double* part, double* out, int ostride, double* gcol, int gstride) {
+  if (m <= 0 || m > kProjMax || n == 0) return;
+  proj_dispatch<1>(m, [&](auto Mc) {
+    constexpr int M = decltype(Mc)::value;
+    hipLaunchKernelGGL(k_proj_dots3<M>, dim3(S.nb), dim3(256), 0, s, n, S, Y, ax, t, b, part);
+  });
+  hipLaunchKernelGGL(k_proj_fin, dim3(2 * m, S.nseg), dim3(256), 0, s, S, 3 * m, part, out, ostride);
+  hipLaunchKernelGGL(k_proj_fin, dim3(m, S.nseg), dim3(256), 0, s, S, 3 * m, part + 2 * m, gcol, gstride);
 }
 void proj_solve(hipStream_t s, const ProjSeg& S, int m, const double* G, int ldg, int gstride, double* v,
                 int vstride) {

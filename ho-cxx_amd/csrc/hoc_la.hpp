@@ -83,6 +83,9 @@ void psi_predict(hipStream_t s, size_t n, double f, int extrap, double* psi, dou
 // coefficient is per segment of a ProjSeg (an ensemble instance: its own c, e, G), at a stride
 // per segment; reductions are deterministic (per-block partials summed in block order).
 //   proj_dots2:   out[j] = Y_j . t and out[m + j] = Y_j . b (j < m); part: S.nb * 2m
+//   proj_dots3:   proj_dots2 and, from the same pass over Y, gcol[j] = Y_j . ax (j < m; ax the
+//                 image of the newest vector Y_{m-1}: its Gram column, as proj_gcol gives it);
+//                 part: S.nb * 3m
 //   proj_solve:   v[0, m) and v[m, 2m) <- G^-1 v (G in columns of ldg values, see k_proj_solve)
 //   proj_combine: out += sum_j c[j] X_j; with v set, also v = -sum_j c[m + j] X_j (the e of a
 //                 later append, from the same pass over X)
@@ -109,6 +112,8 @@ struct ProjSeg {
 void proj_seg_layout(const std::vector<int64_t>& sbeg, std::vector<int32_t>& bbeg, std::vector<int32_t>& blk_seg);
 void proj_dots2(hipStream_t s, size_t n, const ProjSeg& S, int m, const double* Y, const double* t, const double* b,
                 double* part, double* out, int ostride);
+void proj_dots3(hipStream_t s, size_t n, const ProjSeg& S, int m, const double* Y, const double* ax, const double* t,
+                const double* b, double* part, double* out, int ostride, double* gcol, int gstride);
 void proj_solve(hipStream_t s, const ProjSeg& S, int m, const double* G, int ldg, int gstride, double* v,
                 int vstride);
 void proj_combine(hipStream_t s, size_t n, const ProjSeg& S, int m, const double* X, const double* c, int cstride,

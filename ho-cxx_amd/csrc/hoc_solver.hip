@@ -1073,6 +1073,7 @@ struct Solver {
     int m = 0;       // vectors held
     int e_m = -1;    // basis size the coefficients e of the solve in progress were formed for
     int v_slot = -1; // the slot holding -X e for the append after the solve in progress (or -1)
+    int gcol_pending = -1;  // the newest slot, whose Gram column the next projected solve forms
     int nseg = 0;    // segments the per-segment arrays were sized for
     int restarts = 0;
   };
@@ -1140,7 +1141,15 @@ struct Solver {
     double* t = ptmp.p;
     HIPCHK(launch_trace(K, PRE_COPY, N, T, psi.p, nullptr, nullptr, nullptr, 0., nullptr, 0, nullptr, P64.tr.p, stream));
     lap<double>(P64, EPI_RESID, psi.p, t, b, nullptr, nullptr, 0., 0., nullptr, nullptr);  // t = b - As x0
-    proj_dots2(stream, D, pseg, P.m, proj_mr() ? P.AX.p : P.X.p, t, b, ppart, pcoef, kProjCoef);  // Y^T t, Y^T b
+    const double* Y = proj_mr() ? P.AX.p : P.X.p;
+    if (P.gcol_pending >= 0 && P.gcol_pending == P.m - 1) {  // Y^T t, Y^T b and the newest Gram column
+      const int j = P.m - 1;
+      proj_dots3(stream, D, pseg, P.m, Y, P.AX.p + (size_t)j * D, t, b, ppart, pcoef, kProjCoef,
+                 P.G.p + (size_t)j * kProjLdg, kProjVectors * kProjLdg);
+    } else {
+      proj_dots2(stream, D, pseg, P.m, Y, t, b, ppart, pcoef, kProjCoef);  // Y^T t, Y^T b
+    }
+    P.gcol_pending = -1;
     proj_solve(stream, pseg, P.m, P.G.p, kProjLdg, kProjVectors * kProjLdg, pcoef, kProjCoef);    // c, e
     // with a free slot, the same pass leaves -X e there for the append after this solve
     const bool pre = P.m < kProjVectors;
@@ -1159,6 +1168,7 @@ struct Solver {
     P.m = 0;
     P.e_m = -1;
     P.v_slot = -1;
+    P.gcol_pending = -1;
     ++P.restarts;
     if (KP == 0) return;
     std::vector<double> w((size_t)ns * M * M), gc((size_t)ns * M * LG), gm((size_t)ns * M * M);
@@ -1205,6 +1215,7 @@ struct Solver {
       P.nseg = pseg.nseg;
       P.m = 0;
     }
+    proj_flush_gcol(P);
     if (P.m >= M) proj_restart(P);
     const int j = P.m;
     if (j > 0 && P.e_m != j) {  // e = G^-1 Y^T b
@@ -1223,11 +1234,22 @@ struct Solver {
                  gcol + j + 1, M * kProjLdg, P.W.p + (size_t)j * M, M * M, M);
     HIPCHK(launch_trace(K, PRE_COPY, N, T, xj, nullptr, nullptr, nullptr, 0., nullptr, 0, nullptr, P64.tr.p, stream));
     lap<double>(P64, EPI_STORE, xj, axj, nullptr, nullptr, nullptr, 0., 0., nullptr, nullptr);
-    proj_gcol(stream, D, pseg, j, proj_mr() ? 1 : 0, proj_mr() ? P.AX.p : P.X.p, xj, axj, ppart, gcol,
-              M * kProjLdg);
+    // its Gram column (Y_i . axj, i <= j) comes from the next projected solve's pass over Y
+    // (proj_dots3), or from proj_flush_gcol when the basis is used otherwise first
     P.m = j + 1;
     P.e_m = -1;
     P.v_slot = -1;
+    P.gcol_pending = j;
+  }
+  // The Gram column of the newest vector when it is still pending (its vector and image are in
+  // their slots); a pending column of a basis since reset is dropped.
+  void proj_flush_gcol(ProjBasis& P) {
+    const int j = P.gcol_pending;
+    P.gcol_pending = -1;
+    if (j < 0 || j != P.m - 1) return;
+    constexpr int M = kProjVectors;
+    proj_gcol(stream, D, pseg, j, proj_mr() ? 1 : 0, proj_mr() ? P.AX.p : P.X.p, P.X.p + (size_t)j * D,
+              P.AX.p + (size_t)j * D, ppart, P.G.p + (size_t)j * kProjLdg, M * kProjLdg);
   }
   // The adjoint's colouring, velocity map, buffers and recorded trajectory belong to one mesh and
   // one set-up: a new mesh or set-up rebuilds them on the next adjoint call.
