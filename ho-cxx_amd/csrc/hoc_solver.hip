@@ -1058,8 +1058,10 @@ struct Solver {
   // exactly, so the basis is not kept orthogonal and needs no second pass or normalisation.
   // Each appended solution's coefficients in the basis are kept (W).  A full basis restarts from
   // the span of its newest kProjKeep solutions: their coefficient columns, orthonormalised on the
-  // host in the inner product of a freshly measured G, recombine the basis in one pass into
-  // G = I, so the start of every solve is projected on at least kProjKeep earlier solutions.
+  // host in the inner product of G (every column of it measured: at an append, or after the last
+  // restart), recombine the basis in one pass; a pass over the recombined vectors then measures
+  // their Gram matrix (about I), so the start of every solve is projected on at least kProjKeep
+  // earlier solutions.
   static constexpr int kProjVectors = 16;
   static constexpr int kProjKeep = 8;
   static constexpr int kProjLdg = kProjVectors + 1;  // Gram column: M entries and the solution's norm
@@ -1159,9 +1161,10 @@ struct Solver {
   }
   // A full basis: keep, per segment, the span of the newest kProjKeep appended solutions
   // (columns M - kProjKeep .. M - 1 of W: with kProjKeep <= M / 2 these were all appended after
-  // the last restart).  The new basis is orthonormalised in the inner product of a freshly
-  // measured Gram matrix; a segment whose newest solutions span fewer than the most vectors kept
-  // by any segment gets zero vectors in the remaining slots (left out of its solves).
+  // the last restart).  The new basis is orthonormalised in the inner product of the Gram matrix
+  // held in G's columns, and the new vectors' Gram matrix, measured after the recombination,
+  // replaces it; a segment whose newest solutions span fewer than the most vectors kept by any
+  // segment gets zero vectors in the remaining slots (left out of its solves).
   void proj_restart(ProjBasis& P) {
     constexpr int M = kProjVectors, KP = kProjKeep, LG = kProjLdg;
     const int ns = pseg.nseg;
@@ -1171,37 +1174,39 @@ struct Solver {
     P.gcol_pending = -1;
     ++P.restarts;
     if (KP == 0) return;
-    std::vector<double> w((size_t)ns * M * M), gc((size_t)ns * M * LG), gm((size_t)ns * M * M);
-    proj_gram(stream, D, pseg, M, proj_mr() ? P.AX.p : P.X.p, P.AX.p, ppart, P.C.p);  // G measured afresh
+    std::vector<double> w((size_t)ns * M * M), gc((size_t)ns * M * LG), gm((size_t)M * M);
     HIPCHK(hipStreamSynchronize(stream));
     d2h(w.data(), P.W.p, w.size() * sizeof(double));
     d2h(gc.data(), P.G.p, gc.size() * sizeof(double));
-    d2h(gm.data(), P.C.p, gm.size() * sizeof(double));
     std::vector<std::vector<double>> cs(ns);
     int kk = 0;
     for (int sg = 0; sg < ns; ++sg) {
       const double* W = w.data() + (size_t)sg * M * M;
       const double* Gc = gc.data() + (size_t)sg * M * LG;
-      double* g = gm.data() + (size_t)sg * M * M;
-      const int k = proj_restart_basis(M, KP, Gc, LG, g, W, cs[sg]);
+      for (int j = 0; j < M; ++j)  // the symmetric Gram matrix from its columns (upper triangle)
+        for (int i = 0; i <= j; ++i) gm[(size_t)i * M + j] = gm[(size_t)j * M + i] = Gc[(size_t)j * LG + i];
+      const int k = proj_restart_basis(M, KP, Gc, LG, gm.data(), W, cs[sg]);
       kk = std::max(kk, k);
     }
     if (kk == 0) return;
-    // combinations (zero rows pad a segment to kk) and the new Gram columns: G = I on the kept
-    // vectors, 0 (left out) on the padding
-    std::vector<double> ca((size_t)ns * M * M, 0.), gi((size_t)ns * M * LG, 0.);
-    for (int sg = 0; sg < ns; ++sg) {
-      const int k = (int)cs[sg].size() / M;
-      std::copy(cs[sg].begin(), cs[sg].end(), ca.begin() + (size_t)sg * M * M);
-      for (int j = 0; j < kk; ++j) {
-        gi[(size_t)sg * M * LG + (size_t)j * LG + j] = j < k ? 1. : 0.;
-        gi[(size_t)sg * M * LG + (size_t)j * LG + j + 1] = 1.;
-      }
-    }
+    // combinations (zero rows pad a segment to kk), then the recombined vectors' Gram matrix
+    std::vector<double> ca((size_t)ns * M * M, 0.), gn((size_t)ns * kk * kk), gi((size_t)ns * M * LG, 0.);
+    for (int sg = 0; sg < ns; ++sg) std::copy(cs[sg].begin(), cs[sg].end(), ca.begin() + (size_t)sg * M * M);
     HIPCHK(hipMemcpyAsync(P.C.p, ca.data(), ca.size() * sizeof(double), hipMemcpyHostToDevice, stream));
-    HIPCHK(hipMemcpyAsync(P.G.p, gi.data(), gi.size() * sizeof(double), hipMemcpyHostToDevice, stream));
     proj_reseed(stream, D, pseg, M, kk, P.C.p, M * M, P.X.p, P.AX.p);
-    HIPCHK(hipStreamSynchronize(stream));  // (ca and gi are locals)
+    proj_gram(stream, D, pseg, kk, proj_mr() ? P.AX.p : P.X.p, P.AX.p, ppart, P.C.p);  // kk x kk per segment
+    HIPCHK(hipStreamSynchronize(stream));  // (ca is a local)
+    d2h(gn.data(), P.C.p, gn.size() * sizeof(double));
+    // the Gram columns of the new vectors, each with a unit solution norm (zero vectors padding a
+    // segment have G_jj = 0 and are left out)
+    for (int sg = 0; sg < ns; ++sg)
+      for (int j = 0; j < kk; ++j) {
+        double* col = gi.data() + (size_t)sg * M * LG + (size_t)j * LG;
+        for (int i = 0; i <= j; ++i) col[i] = gn[(size_t)sg * kk * kk + (size_t)i * kk + j];
+        col[j + 1] = 1.;
+      }
+    HIPCHK(hipMemcpyAsync(P.G.p, gi.data(), gi.size() * sizeof(double), hipMemcpyHostToDevice, stream));
+    HIPCHK(hipStreamSynchronize(stream));  // (gi is a local)
     P.m = kk;
   }
   void proj_after(const double* b) {
