@@ -100,7 +100,7 @@ void psi_predict(hipStream_t s, size_t n, double f, int extrap, double* psi, dou
 //                 and (mr ? ax : x) . ax; part: S.nb * (m + 1)
 //   proj_gram:    out[i m + j] = Y_i . AX_j (part: S.nb * m * m; out stride m * m)
 //   proj_reseed:  slots k < kk of a basis of m vectors become sum_j C[k m + j] X_j (and AX alike)
-constexpr int kProjMax = 16;
+constexpr int kProjMax = 24;
 struct ProjSeg {
   int nseg = 0, nb = 0;              // segments; blocks of 256 threads over all of them
   const int32_t* blk_seg = nullptr;  // [nb] the segment of each block (device)

@@ -66,7 +66,7 @@ typedef struct {
                                      stage increment corrected by the same stage
                                      difference one step earlier;  5 = mode 4's guess (with
                                      Euler and RK2 the previous solution) plus its best
-                                     correction in the span of 8 to 16 recent stage
+                                     correction in the span of 12 to 24 recent stage
                                      solutions: in the As-norm for a symmetric operator,
                                      least-squares residual otherwise  (0 = 5)            */
   double cheb_lower;              /* smoothing interval [cheb_lower, 1.1] x lambda_max(D^-1 As)
