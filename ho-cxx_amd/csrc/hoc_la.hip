@@ -706,66 +706,78 @@ __global__ __launch_bounds__(256) void k_proj_reseed(size_t n, ProjSeg S, int kk
 // v + s vstride): column j of G (at G + j ldg) holds G_ij (i < j), G_jj, and in row j + 1 the
 // squared norm of the solution the vector came from.  A vector with G_jj <= 1e-24 of that norm
 // (left in the span of the others), or whose Cholesky pivot falls below 1e-8 G_jj, is left out.
-// v[0, m) and v[m, 2m) become G^-1 v on the kept vectors and 0 on the others: Cholesky in LDS by
-// one wavefront, lane i updating row i (m <= kProjMax <= 64).
+// v[0, m) and v[m, 2m) become G^-1 v on the kept vectors and 0 on the others: Cholesky by one
+// wavefront, lane i holding row i of L in registers (m <= kProjMax <= 64); each column of L is
+// broadcast by lane shuffles, and L goes to LDS once for the back substitution, which reads it
+// by columns.
 __global__ __launch_bounds__(64) void k_proj_solve(int m, const double* __restrict__ Gall, int ldg, int gstride,
                                                    double* __restrict__ vall, int vstride) {
+  constexpr int MX = kProjMax;
+  static_assert(MX <= 64, "one lane per row");
   const double* G = Gall + (size_t)blockIdx.x * gstride;
   double* v = vall + (size_t)blockIdx.x * vstride;
-  __shared__ double A[kProjMax][kProjMax + 1], y[2][kProjMax], gd[kProjMax];
-  __shared__ int ok[kProjMax];
+  __shared__ double Ls[MX][MX + 1];
   const int i = threadIdx.x;
-  for (int q = i; q < m * m; q += 64) {
-    const int r = q / m, c = q % m;
-    A[r][c] = c >= r ? G[(size_t)c * ldg + r] : G[(size_t)r * ldg + c];
+  const bool row = i < m;
+  double a[MX];  // row i of the lower triangle: G_ik (k <= i), the upper half of column i
+#pragma unroll
+  for (int k = 0; k < MX; ++k) a[k] = (row && k <= i) ? G[(size_t)i * ldg + k] : 0.;
+  const double gd = row ? G[(size_t)i * ldg + i] : 0.;
+  int ok = row && gd > 1e-24 * fabs(G[(size_t)i * ldg + i + 1]) && gd > 0.;
+  double y0 = row ? v[i] : 0., y1 = row ? v[m + i] : 0.;
+#pragma unroll
+  for (int j = 0; j < MX; ++j) {  // right-looking: column j, then the trailing rows
+    const double d = __shfl(a[j], j);
+    const bool okj = j < m && __shfl(ok, j) && d > 1e-8 * __shfl(gd, j);  // the same on every lane
+    if (okj) {
+      const double l = sqrt(d);
+      if (i == j) a[j] = l;
+      else if (i > j && ok) a[j] /= l;
+#pragma unroll
+      for (int k = j + 1; k < MX; ++k) {
+        const double lkj = __shfl(a[j], k);
+        const int okk = __shfl(ok, k);
+        if (k < m && k <= i && i > j && ok && okk) a[k] -= a[j] * lkj;
+      }
+    } else if (i == j) {
+      ok = 0;
+    }
   }
-  if (i < m) {
-    const double gii = G[(size_t)i * ldg + i];
-    ok[i] = gii > 1e-24 * fabs(G[(size_t)i * ldg + i + 1]) && gii > 0.;
-    gd[i] = gii;
-    y[0][i] = v[i];
-    y[1][i] = v[m + i];
+#pragma unroll
+  for (int j = 0; j < MX; ++j) {  // L y = v
+    const double ljj = __shfl(a[j], j);
+    const double y0j = __shfl(y0, j) / ljj, y1j = __shfl(y1, j) / ljj;
+    if (j < m && __shfl(ok, j)) {
+      if (i == j) {
+        y0 = y0j;
+        y1 = y1j;
+      } else if (i > j && ok) {
+        y0 -= a[j] * y0j;
+        y1 -= a[j] * y1j;
+      }
+    }
+  }
+  if (row) {
+#pragma unroll
+    for (int k = 0; k < MX; ++k) Ls[i][k] = a[k];
   }
   __syncthreads();
-  for (int j = 0; j < m; ++j) {  // right-looking: column j, then the trailing rows
-    const double d = A[j][j];
-    const bool okj = ok[j] && d > 1e-8 * gd[j];
-    __syncthreads();
-    if (!okj) {
-      if (i == 0) ok[j] = 0;
-      __syncthreads();
-      continue;
-    }
-    const double l = sqrt(d);
-    if (i > j && i < m && ok[i]) A[i][j] /= l;
-    if (i == j) A[j][j] = l;
-    __syncthreads();
-    if (i > j && i < m && ok[i])
-      for (int k = j + 1; k <= i; ++k)
-        if (ok[k]) A[i][k] -= A[i][j] * A[k][j];
-    __syncthreads();
-  }
-  for (int j = 0; j < m; ++j) {  // L y = v
-    if (ok[j]) {
-      const double y0 = y[0][j] / A[j][j], y1 = y[1][j] / A[j][j];
-      __syncthreads();
-      if (i == j) { y[0][j] = y0; y[1][j] = y1; }
-      if (i > j && i < m && ok[i]) { y[0][i] -= A[i][j] * y0; y[1][i] -= A[i][j] * y1; }
-    }
-    __syncthreads();
-  }
   for (int j = m - 1; j >= 0; --j) {  // L^T x = y
-    if (ok[j]) {
-      const double x0 = y[0][j] / A[j][j], x1 = y[1][j] / A[j][j];
-      __syncthreads();
-      if (i == j) { y[0][j] = x0; y[1][j] = x1; }
-      if (i < j && ok[i]) { y[0][i] -= A[j][i] * x0; y[1][i] -= A[j][i] * x1; }
+    if (!__shfl(ok, j)) continue;
+    const double ljj = Ls[j][j];
+    const double x0 = __shfl(y0, j) / ljj, x1 = __shfl(y1, j) / ljj;
+    if (i == j) {
+      y0 = x0;
+      y1 = x1;
+    } else if (i < j && ok) {
+      const double lji = Ls[j][i];
+      y0 -= lji * x0;
+      y1 -= lji * x1;
     }
-    __syncthreads();
   }
-  if (i < m) {
-    v[i] = ok[i] ? y[0][i] : 0.;
-    v[m + i] = ok[i] ? y[1][i] : 0.;
+  if (row) {
+    v[i] = ok ? y0 : 0.;
+    v[m + i] = ok ? y1 : 0.;
   }
 }
 // Gram matrix G_ij = Y_i . AX_j of a basis of M vectors over each segment: per block, chunks of
