@@ -894,10 +894,26 @@ void proj_reseed(hipStream_t s, size_t n, const ProjSeg& S, int m, int kk, const
     hipLaunchKernelGGL(k_proj_reseed<M>, dim3(S.nb), dim3(256), 0, s, n, S, kk, C, cstride, X, AX);
   });
 }
+// The per-segment sum of one partial per block (k_proj_fin with nv = 1), leaving out[s] as it is
+// for a segment whose flag in iact (may be null) is 0.
+__global__ __launch_bounds__(256) void k_seg_fin(ProjSeg S, const double* __restrict__ part, double* __restrict__ out,
+                                                 const int* __restrict__ iact) {
+  const int sg = blockIdx.y;
+  if (iact && !iact[sg]) return;
+  double s = 0.;
+  for (int b = S.bbeg[sg] + threadIdx.x; b < S.bbeg[sg + 1]; b += 256) s += part[b];
+  s = wsum(s);
+  __shared__ double red[4];
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) out[sg] = red[0] + red[1] + red[2] + red[3];
+}
 __global__ __launch_bounds__(256) void k_seg_norm2(ProjSeg S, VecArgs A, const double* __restrict__ r,
-                                                   const int* __restrict__ F, double* __restrict__ part) {
+                                                   const int* __restrict__ F, double* __restrict__ part,
+                                                   const int* __restrict__ iact) {
   if (F && !F[F_ACTIVE]) return;
   const SegRange R = seg_range(S);
+  if (iact && !iact[R.s]) return;  // a stopped instance: its sum is left as it was
   DofSlot ds(A, R.i0, R.step);
   double p[1] = {0.};
   for (size_t v = R.i0; v < R.i1; v += R.step, ds.next()) {
@@ -907,28 +923,31 @@ __global__ __launch_bounds__(256) void k_seg_norm2(ProjSeg S, VecArgs A, const d
   block_sums<1>(p, part + blockIdx.x);
 }
 void seg_norm2(hipStream_t s, const ProjSeg& S, const VecArgs& A, const double* r, const int* F, double* part,
-               double* out) {
-  hipLaunchKernelGGL(k_seg_norm2, dim3(S.nb), dim3(256), 0, s, S, A, r, F, part);
-  hipLaunchKernelGGL(k_proj_fin, dim3(1, S.nseg), dim3(256), 0, s, S, 1, part, out, 1);
+               double* out, const int* iact) {
+  hipLaunchKernelGGL(k_seg_norm2, dim3(S.nb), dim3(256), 0, s, S, A, r, F, part, iact);
+  hipLaunchKernelGGL(k_seg_fin, dim3(1, S.nseg), dim3(256), 0, s, S, part, out, iact);
 }
 template <class RB>
 __global__ __launch_bounds__(256) void k_seg_dot(ProjSeg S, const double* __restrict__ a, const RB* __restrict__ b,
-                                                 const int* __restrict__ F, double* __restrict__ part) {
+                                                 const int* __restrict__ F, double* __restrict__ part,
+                                                 const int* __restrict__ iact) {
   if (F && !F[F_ACTIVE]) return;
   const SegRange R = seg_range(S);
+  if (iact && !iact[R.s]) return;
   double p[1] = {0.};
   for (size_t v = R.i0; v < R.i1; v += R.step) p[0] += a[v] * (double)b[v];
   block_sums<1>(p, part + blockIdx.x);
 }
 template <class RB>
-void seg_dot(hipStream_t s, const ProjSeg& S, const double* a, const RB* b, const int* F, double* part,
-             double* out) {
-  hipLaunchKernelGGL(k_seg_dot<RB>, dim3(S.nb), dim3(256), 0, s, S, a, b, F, part);
-  hipLaunchKernelGGL(k_proj_fin, dim3(1, S.nseg), dim3(256), 0, s, S, 1, part, out, 1);
+void seg_dot(hipStream_t s, const ProjSeg& S, const double* a, const RB* b, const int* F, double* part, double* out,
+             const int* iact) {
+  hipLaunchKernelGGL(k_seg_dot<RB>, dim3(S.nb), dim3(256), 0, s, S, a, b, F, part, iact);
+  hipLaunchKernelGGL(k_seg_fin, dim3(1, S.nseg), dim3(256), 0, s, S, part, out, iact);
 }
-template void seg_dot<double>(hipStream_t, const ProjSeg&, const double*, const double*, const int*, double*,
-                              double*);
-template void seg_dot<float>(hipStream_t, const ProjSeg&, const double*, const float*, const int*, double*, double*);
+template void seg_dot<double>(hipStream_t, const ProjSeg&, const double*, const double*, const int*, double*, double*,
+                              const int*);
+template void seg_dot<float>(hipStream_t, const ProjSeg&, const double*, const float*, const int*, double*, double*,
+                             const int*);
 
 // One block: thread t handles instances t, t + 256, ..; the block then reduces "any active", the
 // largest residual ratio of the active instances and the sum of the residual norms.

@@ -152,14 +152,16 @@ enum { ENS_INIT = 0, ENS_RZ, ENS_PQ, ENS_UPD };
 void finalize_ens(hipStream_t s, int phase, const EnsKry& E, double* S, int* F, int maxiter, int* mirror = nullptr,
                   int seq = 0);
 // out[s] = sum over segment s of a . b (a FP64, b FP64 or FP32); skipped when F's active flag is
-// clear (F may be null).  part: S.nb values.
+// clear (F may be null), and out[s] left as it is for a segment whose flag in iact (may be null)
+// is 0.  part: S.nb values.
 template <class RB>
-void seg_dot(hipStream_t s, const ProjSeg& S, const double* a, const RB* b, const int* F, double* part,
-             double* out);
+void seg_dot(hipStream_t s, const ProjSeg& S, const double* a, const RB* b, const int* F, double* part, double* out,
+             const int* iact = nullptr);
 // out[s] = sum over segment s of (r winv)^2 (weighted squared norms per ensemble instance);
-// skipped when F (may be null) has its active flag clear.  part: S.nb values.
+// skipped when F (may be null) has its active flag clear, and out[s] left as it is for a segment
+// whose flag in iact (may be null) is 0.  part: S.nb values.
 void seg_norm2(hipStream_t s, const ProjSeg& S, const VecArgs& A, const double* r, const int* F, double* part,
-               double* out);
+               double* out, const int* iact = nullptr);
 // psi' = sum_j a[j] x_j (x_j may be psi itself, read before the write), and save = psi (the old
 // value) when save is not null; nterm <= 8.
 void psi_combine(hipStream_t s, size_t n, int nterm, const double* a, const double* const* x, double* psi,

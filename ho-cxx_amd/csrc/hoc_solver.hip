@@ -795,24 +795,27 @@ struct Solver {
     if (mixed) {
       demote(stream, D, r, P32.r.p, Fp);
       vcycle(P32, P32.r.p, P32.z.p, nullptr, Fp);
-      seg_dot<float>(stream, pseg, r, P32.z.p, Fp, ens_part.p, ens_in.p);
+      seg_dot<float>(stream, pseg, r, P32.z.p, Fp, ens_part.p, ens_in.p, ens_act.p);
       finalize_ens(stream, ENS_RZ, E, Sp, Fp, maxit);
       HIPCHK(launch_trace_pupd_f32z(K, N, P.T, P.p.p, P32.z.p, ens_beta.p, -1, Fp, P.tr.p, stream, d_inst.p,
                                     ens_act.p));
     } else {
       vcycle(P, r, P.z.p, nullptr, Fp);
-      seg_dot<double>(stream, pseg, r, P.z.p, Fp, ens_part.p, ens_in.p);
+      seg_dot<double>(stream, pseg, r, P.z.p, Fp, ens_part.p, ens_in.p, ens_act.p);
       finalize_ens(stream, ENS_RZ, E, Sp, Fp, maxit);
       HIPCHK(launch_trace(K, PRE_PUPD, N, P.T, P.p.p, P.z.p, nullptr, nullptr, 0., ens_beta.p, -1, Fp, P.tr.p, stream,
                           nullptr, d_inst.p, ens_act.p));
     }
     lap(P, EPI_STORE, P.p.p, P.q.p, (const double*)nullptr, (const double*)nullptr, (double*)nullptr, 0., 0., nullptr,
         Fp);
-    seg_dot<double>(stream, pseg, P.p.p, P.q.p, Fp, ens_part.p, ens_in.p);
+    seg_dot<double>(stream, pseg, P.p.p, P.q.p, Fp, ens_part.p, ens_in.p, ens_act.p);
     finalize_ens(stream, ENS_PQ, E, Sp, Fp, maxit);
     vec<double>(stream, VOP_PCG_UPD, va, D, x, r, P.p, P.q, nullptr, Sp, Fp, nullptr);
-    seg_norm2(stream, pseg, VA, r, Fp, ens_part.p, ens_in.p);
-    finalize_ens(stream, ENS_UPD, E, Sp, Fp, maxit, mirror_dev, seq);
+    // ||r_i||^2 into ens_rr, where a stopped instance keeps its last value (its r no longer changes)
+    seg_norm2(stream, pseg, VA, r, Fp, ens_part.p, ens_rr.p, ens_act.p);
+    EnsKry Eu = E;
+    Eu.in = ens_rr.p;
+    finalize_ens(stream, ENS_UPD, Eu, Sp, Fp, maxit, mirror_dev, seq);
   }
   // The instantiated graph of kGraphChunk iterations on these buffers (captured on first use).
   template <class R>
