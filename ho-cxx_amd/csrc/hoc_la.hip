@@ -453,6 +453,51 @@ __global__ void k_psi_combine(size_t n, int nterm, Comb5 C, double* __restrict__
     if (save) save[i] = old;
   }
 }
+// The RK4 guess history at the first stage of a step, in one pass (T1 = psi on entry, the
+// first stage solution of the new step; S1..S4 the previous step's stage solutions, Q4 its fourth
+// stage difference): psi = 4 T1 - 2 S3 - S4 - Q4/2 (qok) or 4 T1 + S1 - S2 - 2 S3 - S4; then
+// Q1 = 2 S2 - 2 S1, Q2 = 2 S3 - 2 S1, Q3 = S4 - S1, Q4 = 6 T1 + 2 S1 - 2 S2 - 4 S3 - 2 S4, S1 = T1.
+__global__ void k_rk4_history(size_t n, int qok, double* __restrict__ psi, double* __restrict__ s1,
+                              const double* __restrict__ s2, const double* __restrict__ s3,
+                              const double* __restrict__ s4, double* __restrict__ q1, double* __restrict__ q2,
+                              double* __restrict__ q3, double* __restrict__ q4) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    const double t = psi[i], a = s1[i], b = s2[i], c = s3[i], d = s4[i];
+    double g = 4. * t;
+    if (qok) {
+      g += -2. * c;
+      g += -1. * d;
+      g += -0.5 * q4[i];
+    } else {
+      g += 1. * a;
+      g += -1. * b;
+      g += -2. * c;
+      g += -1. * d;
+    }
+    psi[i] = g;
+    double v = 2. * b;
+    v += -2. * a;
+    q1[i] = v;
+    v = 2. * c;
+    v += -2. * a;
+    q2[i] = v;
+    v = 6. * t;
+    v += 2. * a;
+    v += -2. * b;
+    v += -4. * c;
+    v += -2. * d;
+    q4[i] = v;
+    v = 1. * d;
+    v += -1. * a;
+    q3[i] = v;
+    s1[i] = t;
+  }
+}
+void rk4_history(hipStream_t s, size_t n, int qok, double* psi, double* const* S, double* const* Q) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_rk4_history, dim3(vec_grid(n)), dim3(256), 0, s, n, qok, psi, S[0], S[1], S[2], S[3], Q[0],
+                     Q[1], Q[2], Q[3]);
+}
 void psi_combine(hipStream_t s, size_t n, int nterm, const double* a, const double* const* x, double* psi,
                  double* save) {
   if (n == 0 || nterm < 1 || nterm > 8) return;

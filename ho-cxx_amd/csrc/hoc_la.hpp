@@ -162,6 +162,10 @@ void seg_dot(hipStream_t s, const ProjSeg& S, const double* a, const RB* b, cons
 // whose flag in iact (may be null) is 0.  part: S.nb values.
 void seg_norm2(hipStream_t s, const ProjSeg& S, const VecArgs& A, const double* r, const int* F, double* part,
                double* out, const int* iact = nullptr);
+// The RK4 guess and stage-difference history at the first stage of a step, in one pass: psi (the
+// new step's first stage solution T1 on entry) becomes the second stage's guess, Q[0..3] the
+// completed step's stage differences and S[0] = T1 (see k_rk4_history).
+void rk4_history(hipStream_t s, size_t n, int qok, double* psi, double* const* S, double* const* Q);
 // psi' = sum_j a[j] x_j (x_j may be psi itself, read before the write), and save = psi (the old
 // value) when save is not null; nterm <= 8.
 void psi_combine(hipStream_t s, size_t n, int nterm, const double* a, const double* const* x, double* psi,

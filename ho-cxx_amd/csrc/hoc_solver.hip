@@ -981,23 +981,8 @@ struct Solver {
       if (rk_full) {
         // T1 = psi; guess 4 T1 - 2 S3 - S4 - Q4/2 (Q4 of the older step), then the Q history of the
         // completed step: Q1 = 2 (S2 - S1), Q2 = 2 (S3 - S1), Q3 = S4 - S1, Q4 = 6 T1 + 2 S1 - 2 S2 - 4 S3 - 2 S4
-        dcopy(adj_tmp(), psi, D);  // T1
-        if (rk_qok) {
-          const double a[4] = {4., -2., -1., -0.5};
-          const double* x[4] = {psi, S[2], S[3], Q[3]};
-          psi_combine(stream, D, 4, a, x, psi, nullptr);
-        } else {  // mode 3's guess
-          const double a[5] = {4., 1., -1., -2., -1.};
-          const double* x[5] = {psi, S[0], S[1], S[2], S[3]};
-          psi_combine(stream, D, 5, a, x, psi, nullptr);
-        }
-        const double* T1 = adj_tmp();
-        { const double b[2] = {2., -2.}; const double* y[2] = {S[1], S[0]}; psi_combine(stream, D, 2, b, y, Q[0], nullptr); }
-        { const double b[2] = {2., -2.}; const double* y[2] = {S[2], S[0]}; psi_combine(stream, D, 2, b, y, Q[1], nullptr); }
-        { const double b[5] = {6., 2., -2., -4., -2.}; const double* y[5] = {T1, S[0], S[1], S[2], S[3]};
-          psi_combine(stream, D, 5, b, y, Q[3], nullptr); }
-        { const double b[2] = {1., -1.}; const double* y[2] = {S[3], S[0]}; psi_combine(stream, D, 2, b, y, Q[2], nullptr); }
-        dcopy(S[0], T1, D);
+        // (mode 3's guess 4 T1 + S1 - S2 - 2 S3 - S4 while Q holds no complete step), and S1 = T1, in one pass
+        rk4_history(stream, D, rk_qok ? 1 : 0, psi, S, Q);
         rk_qok = true;
       } else {
         dcopy(rkS[0], psi, D);
@@ -1011,11 +996,6 @@ struct Solver {
       const double* x[4] = {psi, rkS[0], Q[2], Q[1]};
       psi_combine(stream, D, rk_qok ? 4 : 2, a, x, psi, nullptr);
     }
-  }
-  DBuf<double> guess_tmp;
-  double* adj_tmp() {
-    if (guess_tmp.n != D) guess_tmp.alloc(D);
-    return guess_tmp.p;
   }
   void rk_guess_before() {
     if (guess_mode() >= 4) { rk4_guess_before(); return; }
